@@ -1,0 +1,235 @@
+"""Python mirror of eT's CCSD wavefunction interface over libetgpu.so.
+
+The method names and argument meanings follow the type-bound procedures of
+eT's ``ccsd`` class that the ground-state solvers call
+(src/wavefunctions/ccsd/ccsd_class.F90:101-160, 232-292); errors raise
+``ETGError`` where eT would stop with ``output%error_msg``.  Arrays cross the
+boundary in eT's Fortran layouts (see include/etgpu.h).  There is no CPU
+fallback: constructing a wavefunction without the built library raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, 'libetgpu.so')
+
+FP64 = 0
+FP32 = 1
+PRECISIONS = {'fp64': FP64, 'fp32': FP32}
+
+
+class ETGError(RuntimeError):
+    pass
+
+
+_lib = None
+
+_c_dbl_p = ctypes.POINTER(ctypes.c_double)
+_i64 = ctypes.c_int64
+
+# name -> argtypes (restype is int status unless listed in _RESTYPE)
+_SIGNATURES = {
+    'etg_create': [_i64, _i64, _i64, ctypes.POINTER(ctypes.c_void_p)],
+    'etg_destroy': [ctypes.c_void_p],
+    'etg_last_error': [],
+    'etg_set_cholesky': [ctypes.c_void_p, _c_dbl_p],
+    'etg_set_oei': [ctypes.c_void_p, _c_dbl_p],
+    'etg_set_orbital_energies': [ctypes.c_void_p, _c_dbl_p],
+    'etg_set_amplitudes': [ctypes.c_void_p, _c_dbl_p],
+    'etg_get_amplitudes': [ctypes.c_void_p, _c_dbl_p],
+    'etg_set_t2_to_mp2_guess': [ctypes.c_void_p],
+    'etg_construct_t1_cholesky': [ctypes.c_void_p],
+    'etg_construct_fock': [ctypes.c_void_p, _c_dbl_p],
+    'etg_calculate_energy': [ctypes.c_void_p, _c_dbl_p],
+    'etg_construct_omega': [ctypes.c_void_p, _c_dbl_p, ctypes.c_int],
+    'etg_get_omega': [ctypes.c_void_p, _c_dbl_p],
+    'etg_omega_norm': [ctypes.c_void_p, _c_dbl_p],
+    'etg_get_orbital_differences': [ctypes.c_void_p, _c_dbl_p, _i64],
+    'etg_set_ladder_batch': [ctypes.c_void_p, _i64],
+    'etg_synchronize': [ctypes.c_void_p],
+    'etg_last_omega_flops': [ctypes.c_void_p, _c_dbl_p],
+    'etg_test_gemm': [ctypes.c_char, ctypes.c_char, _i64, _i64, _i64, ctypes.c_double, _c_dbl_p,
+                      _i64, _c_dbl_p, _i64, ctypes.c_double, _c_dbl_p, _i64, ctypes.c_int],
+    'etg_bench_gemm': [ctypes.c_char, ctypes.c_char, _i64, _i64, _i64, ctypes.c_int, ctypes.c_int,
+                       _c_dbl_p],
+}
+_RESTYPE = {'etg_last_error': ctypes.c_char_p}
+
+
+def load_library(path=LIB_PATH):
+    """Loads libetgpu.so (raises ETGError if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ETGError('libetgpu.so not found at %s: run `python -m et_amd.build`' % path)
+    lib = ctypes.CDLL(path)
+    for name, args in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = _RESTYPE.get(name, ctypes.c_int)
+    _lib = lib
+    return lib
+
+
+def _check(status):
+    if status != 0:
+        raise ETGError(_lib.etg_last_error().decode())
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_c_dbl_p)
+
+
+def device_gemm(ta, tb, alpha, A, B, beta, C, precision=FP64):
+    """C <- alpha op(A) op(B) + beta C on the matrix-core kernel (diagnostic entry)."""
+    lib = load_library()
+    A = np.asfortranarray(A, dtype=np.float64)
+    B = np.asfortranarray(B, dtype=np.float64)
+    C = np.array(C, dtype=np.float64, order='F', copy=True)
+    m, n = C.shape
+    k = A.shape[0] if ta == 'T' else A.shape[1]
+    _check(lib.etg_test_gemm(ta.encode(), tb.encode(), m, n, k, alpha, _ptr(A), max(1, A.shape[0]),
+                             _ptr(B), max(1, B.shape[0]), beta, _ptr(C), max(1, m), precision))
+    return C
+
+
+def bench_gemm(m, n, k, precision=FP64, reps=5, ta='N', tb='N'):
+    """Seconds per m x n x k product on device-resident operands."""
+    lib = load_library()
+    out = ctypes.c_double()
+    _check(lib.etg_bench_gemm(ta.encode(), tb.encode(), m, n, k, precision, reps, ctypes.byref(out)))
+    return out.value
+
+
+class CCSD:
+    """Device-resident CCSD wavefunction (eT ``ccsd`` type-bound procedures).
+
+    Sizes: n_t1 = n_o n_v, n_t2 = n_t1 (n_t1 + 1) / 2.
+    """
+
+    def __init__(self, n_o, n_v, n_J):
+        self._lib = load_library()
+        self.n_o, self.n_v, self.n_J = int(n_o), int(n_v), int(n_J)
+        self.n_mo = self.n_o + self.n_v
+        self.n_t1 = self.n_o * self.n_v
+        self.n_t2 = self.n_t1 * (self.n_t1 + 1) // 2
+        self.n_gs_amplitudes = self.n_t1 + self.n_t2
+        h = ctypes.c_void_p()
+        _check(self._lib.etg_create(self.n_o, self.n_v, self.n_J, ctypes.byref(h)))
+        self._h = h
+        self.hf_energy = 0.0
+        self.correlation_energy = 0.0
+        self.energy = 0.0
+
+    def close(self):
+        if getattr(self, '_h', None) is not None and self._h.value:
+            _check(self._lib.etg_destroy(self._h))
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ---- inputs ---------------------------------------------------------
+    def set_cholesky(self, L):
+        L = np.asfortranarray(L, dtype=np.float64)
+        if L.shape != (self.n_J, self.n_mo, self.n_mo):
+            raise ETGError('L must have shape (n_J, n_mo, n_mo)')
+        _check(self._lib.etg_set_cholesky(self._h, _ptr(L)))
+
+    def set_oei(self, h):
+        h = np.asfortranarray(h, dtype=np.float64)
+        if h.shape != (self.n_mo, self.n_mo):
+            raise ETGError('h must have shape (n_mo, n_mo)')
+        _check(self._lib.etg_set_oei(self._h, _ptr(h)))
+
+    def set_orbital_energies(self, eps):
+        eps = np.ascontiguousarray(eps, dtype=np.float64)
+        if eps.shape != (self.n_mo,):
+            raise ETGError('orbital energies must have length n_mo')
+        _check(self._lib.etg_set_orbital_energies(self._h, _ptr(eps)))
+
+    def set_ladder_batch(self, batch):
+        _check(self._lib.etg_set_ladder_batch(self._h, int(batch)))
+
+    # ---- amplitudes -----------------------------------------------------
+    def set_amplitudes(self, t):
+        t = np.ascontiguousarray(t, dtype=np.float64)
+        if t.shape != (self.n_gs_amplitudes,):
+            raise ETGError('amplitudes must have length n_t1 + n_t2')
+        _check(self._lib.etg_set_amplitudes(self._h, _ptr(t)))
+
+    def get_amplitudes(self):
+        t = np.empty(self.n_gs_amplitudes)
+        _check(self._lib.etg_get_amplitudes(self._h, _ptr(t)))
+        return t
+
+    def set_t2_to_cc2_guess(self):
+        _check(self._lib.etg_set_t2_to_mp2_guess(self._h))
+
+    def set_initial_amplitudes_guess(self):
+        """t1 = 0, L_t1 = L_mo, t2 = MP2 guess (ccsd_class.F90:357-420, no restart)."""
+        t = np.zeros(self.n_gs_amplitudes)
+        self.set_amplitudes(t)
+        self.construct_t1_cholesky()
+        self.set_t2_to_cc2_guess()
+
+    # ---- residual path --------------------------------------------------
+    def construct_t1_cholesky(self):
+        _check(self._lib.etg_construct_t1_cholesky(self._h))
+
+    def construct_fock(self, task=None):
+        """All four T1-Fock blocks (CCSD ignores task, fock_ccsd.F90:34-79); returns F."""
+        F = np.empty((self.n_mo, self.n_mo), order='F')
+        _check(self._lib.etg_construct_fock(self._h, _ptr(F)))
+        return F
+
+    def calculate_energy(self):
+        e = ctypes.c_double()
+        _check(self._lib.etg_calculate_energy(self._h, ctypes.byref(e)))
+        self.correlation_energy = e.value
+        self.energy = self.hf_energy + e.value
+        return e.value
+
+    def construct_omega(self, precision=FP64, copy_out=True):
+        if copy_out:
+            om = np.empty(self.n_gs_amplitudes)
+            _check(self._lib.etg_construct_omega(self._h, _ptr(om), int(precision)))
+            return om
+        _check(self._lib.etg_construct_omega(self._h, None, int(precision)))
+        return None
+
+    def get_omega(self):
+        om = np.empty(self.n_gs_amplitudes)
+        _check(self._lib.etg_get_omega(self._h, _ptr(om)))
+        return om
+
+    def omega_norm(self):
+        x = ctypes.c_double()
+        _check(self._lib.etg_omega_norm(self._h, ctypes.byref(x)))
+        return x.value
+
+    def get_orbital_differences(self, n=None):
+        n = self.n_gs_amplitudes if n is None else int(n)
+        x = np.empty(n)
+        _check(self._lib.etg_get_orbital_differences(self._h, _ptr(x), n))
+        return x
+
+    def synchronize(self):
+        _check(self._lib.etg_synchronize(self._h))
+
+    def last_omega_flops(self):
+        x = ctypes.c_double()
+        _check(self._lib.etg_last_omega_flops(self._h, ctypes.byref(x)))
+        return x.value

@@ -1,0 +1,231 @@
+// extern "C" entry points of libetgpu.so (declared in include/etgpu.h).
+// Each entry catches every C++ exception and turns it into a status code
+// plus a thread-local message: nothing but plain C types crosses the ABI.
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "../../include/etgpu.h"
+#include "ccsd.hpp"
+
+struct etg_handle {
+  etg::CCSD* wf;
+};
+
+namespace {
+thread_local std::string g_last_error;
+
+template <typename F>
+int guarded(F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "host allocation failed";
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+  } catch (...) {
+    g_last_error = "unknown error";
+  }
+  return 1;
+}
+
+etg::CCSD& wf_of(etg_handle* h) {
+  if (h == nullptr || h->wf == nullptr) etg::fail("null etg handle");
+  return *h->wf;
+}
+
+etg::Prec prec_of(int p) {
+  switch (p) {
+    case ETG_FP64: return etg::Prec::F64;
+    case ETG_FP32: return etg::Prec::F32;
+    default: etg::fail("unknown precision " + std::to_string(p));
+  }
+}
+}  // namespace
+
+extern "C" {
+
+int etg_create(int64_t n_o, int64_t n_v, int64_t n_J, etg_handle** handle) {
+  return guarded([&] {
+    if (handle == nullptr) etg::fail("etg_create: null handle pointer");
+    *handle = nullptr;
+    etg::CCSD* wf = new etg::CCSD(n_o, n_v, n_J);
+    *handle = new etg_handle{wf};
+  });
+}
+
+int etg_destroy(etg_handle* handle) {
+  return guarded([&] {
+    if (handle == nullptr) return;
+    delete handle->wf;
+    delete handle;
+  });
+}
+
+const char* etg_last_error(void) { return g_last_error.c_str(); }
+
+int etg_set_cholesky(etg_handle* h, const double* L) {
+  return guarded([&] { wf_of(h).set_cholesky(L); });
+}
+
+int etg_set_oei(etg_handle* h, const double* oei) {
+  return guarded([&] { wf_of(h).set_oei(oei); });
+}
+
+int etg_set_orbital_energies(etg_handle* h, const double* eps) {
+  return guarded([&] { wf_of(h).set_orbital_energies(eps); });
+}
+
+int etg_set_amplitudes(etg_handle* h, const double* t) {
+  return guarded([&] { wf_of(h).set_amplitudes(t); });
+}
+
+int etg_get_amplitudes(etg_handle* h, double* t) {
+  return guarded([&] { wf_of(h).get_amplitudes(t); });
+}
+
+int etg_set_t2_to_mp2_guess(etg_handle* h) {
+  return guarded([&] { wf_of(h).set_t2_to_mp2_guess(); });
+}
+
+int etg_construct_t1_cholesky(etg_handle* h) {
+  return guarded([&] {
+    wf_of(h).construct_t1_cholesky();
+    wf_of(h).synchronize();
+  });
+}
+
+int etg_construct_fock(etg_handle* h, double* F) {
+  return guarded([&] {
+    etg::CCSD& wf = wf_of(h);
+    wf.construct_fock();
+    if (F) wf.get_fock(F);
+    wf.synchronize();
+  });
+}
+
+int etg_calculate_energy(etg_handle* h, double* e) {
+  return guarded([&] {
+    if (e == nullptr) etg::fail("etg_calculate_energy: null output");
+    *e = wf_of(h).calculate_energy();
+  });
+}
+
+int etg_construct_omega(etg_handle* h, double* omega, int precision) {
+  return guarded([&] {
+    etg::CCSD& wf = wf_of(h);
+    wf.construct_omega(prec_of(precision));
+    if (omega) wf.get_omega(omega);
+    else wf.synchronize();
+  });
+}
+
+int etg_get_omega(etg_handle* h, double* omega) {
+  return guarded([&] { wf_of(h).get_omega(omega); });
+}
+
+int etg_omega_norm(etg_handle* h, double* norm) {
+  return guarded([&] {
+    if (norm == nullptr) etg::fail("etg_omega_norm: null output");
+    *norm = wf_of(h).omega_norm();
+  });
+}
+
+int etg_get_orbital_differences(etg_handle* h, double* x, int64_t n) {
+  return guarded([&] {
+    if (x == nullptr) etg::fail("etg_get_orbital_differences: null output");
+    wf_of(h).get_orbital_differences(x, n);
+  });
+}
+
+int etg_set_ladder_batch(etg_handle* h, int64_t batch) {
+  return guarded([&] { wf_of(h).set_ladder_batch(batch); });
+}
+
+int etg_synchronize(etg_handle* h) {
+  return guarded([&] { wf_of(h).synchronize(); });
+}
+
+int etg_last_omega_flops(etg_handle* h, double* flops) {
+  return guarded([&] {
+    if (flops == nullptr) etg::fail("etg_last_omega_flops: null output");
+    *flops = wf_of(h).last_omega_flops;
+  });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Diagnostics (not part of the reference interface): direct access to the
+// matrix-core product for tests and roofline measurements.
+// ---------------------------------------------------------------------------
+#include "gemm.hpp"
+
+namespace {
+struct DeviceArrays {
+  etg::DeviceBuffer a, b, c;
+};
+}  // namespace
+
+extern "C" {
+
+int etg_test_gemm(char ta, char tb, int64_t m, int64_t n, int64_t k, double alpha, const double* A,
+                  int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc,
+                  int precision) {
+  return guarded([&] {
+    const int64_t colsA = (ta == 'T' || ta == 't') ? m : k;
+    const int64_t colsB = (tb == 'T' || tb == 't') ? k : n;
+    DeviceArrays d;
+    d.a.resize(sizeof(double) * std::max<int64_t>(1, lda * colsA));
+    d.b.resize(sizeof(double) * std::max<int64_t>(1, ldb * colsB));
+    d.c.resize(sizeof(double) * std::max<int64_t>(1, ldc * n));
+    ETG_HIP(hipMemcpy(d.a.as<double>(), A, sizeof(double) * lda * colsA, hipMemcpyHostToDevice));
+    ETG_HIP(hipMemcpy(d.b.as<double>(), B, sizeof(double) * ldb * colsB, hipMemcpyHostToDevice));
+    ETG_HIP(hipMemcpy(d.c.as<double>(), C, sizeof(double) * ldc * n, hipMemcpyHostToDevice));
+    etg::GemmWorkspace ws;
+    etg::gemm(nullptr, ta, tb, m, n, k, alpha, d.a.as<double>(), lda, d.b.as<double>(), ldb, beta,
+              d.c.as<double>(), ldc, prec_of(precision), &ws);
+    ETG_HIP(hipDeviceSynchronize());
+    ETG_HIP(hipMemcpy(C, d.c.as<double>(), sizeof(double) * ldc * n, hipMemcpyDeviceToHost));
+  });
+}
+
+// Times `reps` products C = A B (op N,N unless ta/tb say otherwise) on
+// device-resident operands filled with a fixed pattern; returns seconds per product.
+int etg_bench_gemm(char ta, char tb, int64_t m, int64_t n, int64_t k, int precision, int reps,
+                   double* seconds) {
+  return guarded([&] {
+    if (seconds == nullptr || reps < 1) etg::fail("etg_bench_gemm: bad arguments");
+    const bool tA = (ta == 'T' || ta == 't'), tB = (tb == 'T' || tb == 't');
+    const int64_t lda = tA ? k : m, ldb = tB ? n : k;
+    DeviceArrays d;
+    d.a.resize(sizeof(double) * m * k);
+    d.b.resize(sizeof(double) * k * n);
+    d.c.resize(sizeof(double) * m * n);
+    ETG_HIP(hipMemset(d.a.as<double>(), 0, sizeof(double) * m * k));
+    ETG_HIP(hipMemset(d.b.as<double>(), 0, sizeof(double) * k * n));
+    etg::GemmWorkspace ws;
+    auto run = [&] {
+      etg::gemm(nullptr, ta, tb, m, n, k, 1.0, d.a.as<double>(), lda, d.b.as<double>(), ldb, 0.0,
+                d.c.as<double>(), m, prec_of(precision), &ws);
+    };
+    run();
+    ETG_HIP(hipDeviceSynchronize());
+    hipEvent_t e0, e1;
+    ETG_HIP(hipEventCreate(&e0));
+    ETG_HIP(hipEventCreate(&e1));
+    ETG_HIP(hipEventRecord(e0, nullptr));
+    for (int r = 0; r < reps; ++r) run();
+    ETG_HIP(hipEventRecord(e1, nullptr));
+    ETG_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    ETG_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *seconds = double(ms) * 1e-3 / reps;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,86 @@
+// Device-resident state of one CCSD wavefunction and the operations eT's
+// ground-state solvers call on it (reference: src/wavefunctions/ccsd/ccsd_class.F90).
+#pragma once
+
+#include <vector>
+
+#include "common.hpp"
+#include "gemm.hpp"
+
+namespace etg {
+
+struct Timers;
+
+class CCSD {
+ public:
+  CCSD(int64_t n_o, int64_t n_v, int64_t n_J);
+  ~CCSD();
+
+  int64_t n_o, n_v, n_J, n_mo, n_t1, n_t2;
+
+  // inputs
+  void set_cholesky(const double* L_host);        // L(J, p, q), full MO range
+  void set_oei(const double* h_host);             // h(p, q)
+  void set_orbital_energies(const double* eps);   // eps(p)
+  void set_amplitudes(const double* t_host);      // t1 (v,o) then packed t2
+  void get_amplitudes(double* t_host);
+  void set_ladder_batch(int64_t nb);              // virtual batch length of the ladder term
+
+  // reference-interface operations
+  void construct_t1_cholesky();                   // L_t1 from L_mo and t1
+  void construct_fock();                          // T1-transformed Fock matrix
+  double calculate_energy();                      // correlation energy
+  void construct_omega(Prec prec);                // residual into device buffer
+  void get_omega(double* host);
+  void get_fock(double* host);
+  void set_t2_to_mp2_guess();
+  void get_orbital_differences(double* host, int64_t n);
+  double omega_norm();                            // ||omega||_2 (FP64 reduction)
+
+  // device pointers (for callers that keep data in HBM)
+  double* d_omega() { return omega_.as<double>(); }
+  double* d_amplitudes() { return t_.as<double>(); }
+  hipStream_t stream() const { return stream_; }
+  void synchronize();
+
+  // flop ledger of the last residual evaluation (sum of 2 m n k over products)
+  double last_omega_flops = 0.0;
+
+ private:
+  void require(bool ok, const char* what) const;
+  void gemm(char ta, char tb, int64_t m, int64_t n, int64_t k, double alpha, const double* A,
+            int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc,
+            Prec prec);
+  void gemm_batched(char ta, char tb, int64_t m, int64_t n, int64_t k, double alpha,
+                    const double* A, int64_t lda, int64_t sA, const double* B, int64_t ldb,
+                    int64_t sB, double beta, double* C, int64_t ldc, int64_t sC, int64_t batch,
+                    Prec prec);
+  void build_ladder_plan();
+  void omega_singles(Prec prec);
+  void omega_doubles_e2_b2(Prec prec);
+  void omega_c2_d2(Prec prec);
+  void omega_a2(Prec prec);
+  double* work(int slot);
+
+  hipStream_t stream_ = nullptr;
+  GemmWorkspace gws_;
+
+  DeviceBuffer Lmo_oo_, Lmo_ov_, Lmo_vo_, Lmo_vv_;
+  DeviceBuffer Lt_oo_, Lt_vo_, Lt_vv_, Lt_vvT_;   // L_t1 ov block equals L_mo ov
+  DeviceBuffer h_, eps_, F_, t_, omega_, diff_;
+  DeviceBuffer g_ovov_;                           // independent of t1 (ov block is not dressed)
+  DeviceBuffer red_partial_, red_result_;
+  DeviceBuffer tmp_small_[4];
+  std::vector<DeviceBuffer> work_;                // (ov)^2 scratch slots
+
+  // ladder plan
+  int64_t ladder_nb_ = 0;
+  struct Block { int64_t a0, la, b0, lb, col0, npairs; };
+  std::vector<Block> blocks_;
+  DeviceBuffer pair_a_, pair_b_, pair_col_;
+  DeviceBuffer tp_, tm_, wp_, wm_, gblk_, gp_, gm_;
+
+  bool have_L_ = false, have_h_ = false, have_eps_ = false, have_t_ = false, have_F_ = false;
+};
+
+}  // namespace etg

@@ -1,0 +1,340 @@
+// Matrix-core products for gfx950 (CDNA4).
+//
+// One workgroup of 256 threads (4 wave64s in a 2 x 2 arrangement) computes a
+// BM x BN tile of C.  Operand tiles of BK columns are staged through LDS with
+// double buffering: the global loads for step k+1 are issued before the
+// matrix-core work of step k and written to the other LDS buffer after it, so
+// a single barrier per step suffices.  Each wave owns a (BM/2) x (BN/2)
+// sub-tile made of 16 x 16 MFMA blocks.
+//
+// MFMA 16x16x4 operand layout (one element per lane, f64 and f32 alike):
+//   A[i][k] in lane i + 16 k,  B[k][j] in lane j + 16 k,
+//   D[4 (lane/16) + r][lane % 16] in element r of the lane's 4-vector.
+//
+// Tile order: workgroups are dispatched round-robin over the 8 XCDs, so the
+// linear block id is first remapped to give each XCD a contiguous range of
+// tiles, and that range is walked in column groups of GROUP_M tile-rows so
+// that tiles sharing A rows / B columns run close together in time on the
+// same L2.
+#include "gemm.hpp"
+
+#include <algorithm>
+
+namespace etg {
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+constexpr int kThreads = 256;
+constexpr int kXcds = 8;
+constexpr int kGroupM = 8;
+
+struct Params {
+  int64_t m, n, k;
+  double alpha, beta;
+  const double* A;
+  int64_t lda, strideA;
+  const double* B;
+  int64_t ldb, strideB;
+  double* C;
+  int64_t ldc, strideC;
+  // split-K: partial sums go to W (m x n per split, ld = m) when W != nullptr
+  double* W;
+  int64_t k_per_split;
+  int tiles_m, tiles_n;
+};
+
+__device__ inline void map_tile(int bid, int tiles_m, int tiles_n, int& tm, int& tn) {
+  const int nb = tiles_m * tiles_n;
+  const int per = nb / kXcds, rem = nb % kXcds;
+  const int xcd = bid % kXcds, idx = bid / kXcds;
+  const int logical = (xcd < rem) ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
+  const int group = kGroupM * tiles_n;
+  const int first_m = (logical / group) * kGroupM;
+  const int gsize = min(tiles_m - first_m, kGroupM);
+  tm = first_m + (logical % group) % gsize;
+  tn = (logical % group) / gsize;
+}
+
+template <typename T>
+struct Mfma;
+
+template <>
+struct Mfma<double> {
+  typedef d4 acc_t;
+  static __device__ inline acc_t run(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ inline double cvt(double x) { return x; }
+};
+
+template <>
+struct Mfma<float> {
+  typedef f4 acc_t;
+  static __device__ inline acc_t run(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ inline float cvt(double x) { return static_cast<float>(x); }
+};
+
+// LDS tile orientation follows the operand's contiguous dimension in HBM so
+// that both the global loads and the LDS stores are unit-stride:
+//   contiguous along k  -> tile[mn][k], leading dim BK + 1 (odd stride: the 16
+//                          lanes of a fragment read hit distinct banks);
+//   contiguous along mn -> tile[k][mn], leading dim BM + 128 B (adjacent k rows of
+//                          a fragment read land on opposite bank halves).
+template <typename T, bool KCONTIG, int BMN, int BK>
+struct TileLayout {
+  static constexpr int LD = KCONTIG ? BK + 1 : BMN + int(128 / sizeof(T));
+  static constexpr int SIZE = KCONTIG ? BMN * LD : BK * LD;
+  static __device__ inline int at(int mn, int k) { return KCONTIG ? mn * LD + k : k * LD + mn; }
+};
+
+// T: arithmetic type of the products (double or float); operands are FP64 in memory.
+// Workgroups per CU the register budget is sized for: the FP64 128 x 128 tile
+// keeps 128 accumulator registers per lane and runs one wave per SIMD.
+template <typename T, int BM, int BN>
+constexpr int min_blocks() { return (sizeof(T) == 8 && BM * BN >= 128 * 128) ? 1 : 2; }
+
+template <typename T, bool TA, bool TB, int BM, int BN, int BK>
+__global__ __launch_bounds__(kThreads, (min_blocks<T, BM, BN>())) void gemm_mfma16(Params p) {
+  using M = Mfma<T>;
+  using acc_t = typename M::acc_t;
+  constexpr int WM = BM / 2, WN = BN / 2;  // wave sub-tile
+  constexpr int TMs = WM / 16, TNs = WN / 16;
+  // op(A) = A^T means A is stored k x m: contiguous along k.  op(B) = B means k x n: along k.
+  using LA = TileLayout<T, TA, BM, BK>;
+  using LB = TileLayout<T, !TB, BN, BK>;
+  constexpr int A_PER = BM * BK / kThreads, B_PER = BN * BK / kThreads;
+  static_assert(A_PER * kThreads == BM * BK && B_PER * kThreads == BN * BK, "tile/threads");
+
+  __shared__ T As[2][LA::SIZE];
+  __shared__ T Bs[2][LB::SIZE];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  int tile_m, tile_n;
+  map_tile(blockIdx.x, p.tiles_m, p.tiles_n, tile_m, tile_n);
+  const int64_t m0 = int64_t(tile_m) * BM, n0 = int64_t(tile_n) * BN;
+  const int64_t bz = blockIdx.z;
+  const int64_t split = blockIdx.y;
+  const double* A = p.A + bz * p.strideA;
+  const double* B = p.B + bz * p.strideB;
+  const int64_t kbeg = split * p.k_per_split;
+  const int64_t kend = min(p.k, kbeg + p.k_per_split);
+
+  T ra[A_PER], rb[B_PER];
+
+  // element r of this thread's share of the tile: (mn, k) inside the tile
+  auto a_coord = [&](int r, int& mm, int& kk) {
+    const int idx = tid + r * kThreads;
+    if (TA) { kk = idx % BK; mm = idx / BK; } else { mm = idx % BM; kk = idx / BM; }
+  };
+  auto b_coord = [&](int r, int& nn, int& kk) {
+    const int idx = tid + r * kThreads;
+    if (TB) { nn = idx % BN; kk = idx / BN; } else { kk = idx % BK; nn = idx / BK; }
+  };
+  auto load = [&](int64_t k0) {
+#pragma unroll
+    for (int r = 0; r < A_PER; ++r) {
+      int mm, kk;
+      a_coord(r, mm, kk);
+      const int64_t gm = m0 + mm, gk = k0 + kk;
+      double v = 0.0;
+      if (gm < p.m && gk < kend) v = TA ? A[gk + gm * p.lda] : A[gm + gk * p.lda];
+      ra[r] = M::cvt(v);
+    }
+#pragma unroll
+    for (int r = 0; r < B_PER; ++r) {
+      int nn, kk;
+      b_coord(r, nn, kk);
+      const int64_t gn = n0 + nn, gk = k0 + kk;
+      double v = 0.0;
+      if (gn < p.n && gk < kend) v = TB ? B[gn + gk * p.ldb] : B[gk + gn * p.ldb];
+      rb[r] = M::cvt(v);
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int r = 0; r < A_PER; ++r) {
+      int mm, kk;
+      a_coord(r, mm, kk);
+      As[buf][LA::at(mm, kk)] = ra[r];
+    }
+#pragma unroll
+    for (int r = 0; r < B_PER; ++r) {
+      int nn, kk;
+      b_coord(r, nn, kk);
+      Bs[buf][LB::at(nn, kk)] = rb[r];
+    }
+  };
+
+  acc_t acc[TMs][TNs];
+#pragma unroll
+  for (int i = 0; i < TMs; ++i)
+#pragma unroll
+    for (int j = 0; j < TNs; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+
+  const int64_t nsteps = (kend > kbeg) ? (kend - kbeg + BK - 1) / BK : 0;
+  if (nsteps > 0) {
+    load(kbeg);
+    store(0);
+  }
+  __syncthreads();
+  const int lrow = lane >> 4, lcol = lane & 15;
+  for (int64_t s = 0; s < nsteps; ++s) {
+    const int buf = int(s & 1);
+    if (s + 1 < nsteps) load(kbeg + (s + 1) * BK);
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      T a[TMs], b[TNs];
+#pragma unroll
+      for (int i = 0; i < TMs; ++i) a[i] = As[buf][LA::at(wm * WM + i * 16 + lcol, kk + lrow)];
+#pragma unroll
+      for (int j = 0; j < TNs; ++j) b[j] = Bs[buf][LB::at(wn * WN + j * 16 + lcol, kk + lrow)];
+#pragma unroll
+      for (int i = 0; i < TMs; ++i)
+#pragma unroll
+        for (int j = 0; j < TNs; ++j) acc[i][j] = M::run(a[i], b[j], acc[i][j]);
+    }
+    if (s + 1 < nsteps) store(buf ^ 1);
+    __syncthreads();
+  }
+
+  // epilogue
+  if (p.W) {
+    double* W = p.W + (split + bz * gridDim.y) * p.m * p.n;
+#pragma unroll
+    for (int i = 0; i < TMs; ++i)
+#pragma unroll
+      for (int j = 0; j < TNs; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int64_t row = m0 + wm * WM + i * 16 + lrow * 4 + r;
+          const int64_t col = n0 + wn * WN + j * 16 + lcol;
+          if (row < p.m && col < p.n) W[row + col * p.m] = double(acc[i][j][r]);
+        }
+    return;
+  }
+  double* C = p.C + bz * p.strideC;
+#pragma unroll
+  for (int i = 0; i < TMs; ++i)
+#pragma unroll
+    for (int j = 0; j < TNs; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t row = m0 + wm * WM + i * 16 + lrow * 4 + r;
+        const int64_t col = n0 + wn * WN + j * 16 + lcol;
+        if (row < p.m && col < p.n) {
+          double* c = C + row + col * p.ldc;
+          const double v = p.alpha * double(acc[i][j][r]);
+          *c = (p.beta == 0.0) ? v : v + p.beta * *c;
+        }
+      }
+}
+
+// C = beta * C (a product with k = 0).
+__global__ void scale_matrix(double* C, int64_t m, int64_t n, int64_t ldc, int64_t strideC,
+                             double beta) {
+  const int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (idx >= m * n) return;
+  double* c = C + int64_t(blockIdx.z) * strideC + (idx % m) + (idx / m) * ldc;
+  *c = (beta == 0.0) ? 0.0 : beta * *c;
+}
+
+// C = alpha * sum_s W_s + beta * C for split-K partials.
+__global__ void splitk_reduce(const double* __restrict__ W, int splits, int64_t m, int64_t n,
+                              double alpha, double beta, double* C, int64_t ldc, int64_t strideC) {
+  const int64_t bz = blockIdx.z;
+  const int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (idx >= m * n) return;
+  const int64_t row = idx % m, col = idx / m;
+  const double* w = W + bz * splits * m * n + idx;
+  double s = 0.0;
+  for (int q = 0; q < splits; ++q) s += w[int64_t(q) * m * n];
+  double* c = C + bz * strideC + row + col * ldc;
+  const double v = alpha * s;
+  *c = (beta == 0.0) ? v : v + beta * *c;
+}
+
+template <typename T, int BM, int BN, int BK>
+void launch_cfg(hipStream_t stream, const Gemm& g, Params p, int splits) {
+  p.tiles_m = int(ceil_div(g.m, BM));
+  p.tiles_n = int(ceil_div(g.n, BN));
+  dim3 grid(p.tiles_m * p.tiles_n, splits, unsigned(g.batch));
+  if (!g.ta && !g.tb) gemm_mfma16<T, false, false, BM, BN, BK><<<grid, kThreads, 0, stream>>>(p);
+  else if (!g.ta && g.tb) gemm_mfma16<T, false, true, BM, BN, BK><<<grid, kThreads, 0, stream>>>(p);
+  else if (g.ta && !g.tb) gemm_mfma16<T, true, false, BM, BN, BK><<<grid, kThreads, 0, stream>>>(p);
+  else gemm_mfma16<T, true, true, BM, BN, BK><<<grid, kThreads, 0, stream>>>(p);
+  ETG_LAUNCH("gemm_mfma16");
+}
+
+template <typename T>
+void launch_typed(hipStream_t stream, const Gemm& g, GemmWorkspace* ws) {
+  Params p{};
+  p.m = g.m; p.n = g.n; p.k = g.k;
+  p.alpha = g.alpha; p.beta = g.beta;
+  p.A = g.A; p.lda = g.lda; p.strideA = g.strideA;
+  p.B = g.B; p.ldb = g.ldb; p.strideB = g.strideB;
+  p.C = g.C; p.ldc = g.ldc; p.strideC = g.strideC;
+  p.W = nullptr;
+  p.k_per_split = g.k;
+
+  // Tile size: large tiles when the output is large enough to fill the chip.
+  const bool big = (ceil_div(g.m, 128) * ceil_div(g.n, 128) * g.batch) >= 512;
+  const int64_t BMc = big ? 128 : 64, BNc = big ? 128 : 64;
+  const int64_t tiles = ceil_div(g.m, BMc) * ceil_div(g.n, BNc) * g.batch;
+  // Split K when there are too few output tiles and K is long.
+  int splits = 1;
+  const int64_t target = 1024;
+  if (tiles < target && g.k >= 2048 && ws != nullptr) {
+    splits = int(std::min<int64_t>(std::max<int64_t>(1, target / tiles), g.k / 512));
+    splits = std::max(1, std::min(splits, 256));
+  }
+  if (splits > 1) {
+    const int64_t kps = ceil_div(ceil_div(g.k, splits), 16) * 16;
+    splits = int(ceil_div(g.k, kps));
+    p.k_per_split = kps;
+    const size_t need = size_t(splits) * size_t(g.m) * size_t(g.n) * size_t(g.batch) * sizeof(double);
+    ws->buf.resize(need);
+    p.W = ws->buf.as<double>();
+  }
+  if (big) launch_cfg<T, 128, 128, 16>(stream, g, p, splits);
+  else launch_cfg<T, 64, 64, 16>(stream, g, p, splits);
+  if (splits > 1) {
+    const int64_t mn = g.m * g.n;
+    dim3 grid(unsigned(ceil_div(mn, 256)), 1, unsigned(g.batch));
+    splitk_reduce<<<grid, 256, 0, stream>>>(p.W, splits, g.m, g.n, g.alpha, g.beta, g.C, g.ldc,
+                                            g.strideC);
+    ETG_LAUNCH("splitk_reduce");
+  }
+}
+
+}  // namespace
+
+void gemm(hipStream_t stream, const Gemm& g, Prec prec, GemmWorkspace* ws) {
+  if (g.m < 0 || g.n < 0 || g.k < 0 || g.batch < 1) fail("gemm: negative dimension");
+  if (g.m == 0 || g.n == 0) return;
+  const int64_t rowsA = g.ta ? g.k : g.m;
+  const int64_t rowsB = g.tb ? g.n : g.k;
+  if (g.lda < std::max<int64_t>(1, rowsA)) fail("gemm: lda too small");
+  if (g.ldb < std::max<int64_t>(1, rowsB)) fail("gemm: ldb too small");
+  if (g.ldc < std::max<int64_t>(1, g.m)) fail("gemm: ldc too small");
+  if (g.batch > 65535) fail("gemm: batch too large");
+  if (g.k == 0) {
+    const int64_t mn = g.m * g.n;
+    dim3 grid(unsigned(ceil_div(mn, 256)), 1, unsigned(g.batch));
+    scale_matrix<<<grid, 256, 0, stream>>>(g.C, g.m, g.n, g.ldc, g.strideC, g.beta);
+    ETG_LAUNCH("scale_matrix");
+    return;
+  }
+  switch (prec) {
+    case Prec::F64: launch_typed<double>(stream, g, ws); break;
+    case Prec::F32: launch_typed<float>(stream, g, ws); break;
+    case Prec::BF16X3: fail("gemm: split-bf16 products are not available in this build");
+  }
+}
+
+}  // namespace etg

@@ -1,0 +1,61 @@
+// Dense matrix products on CDNA4 matrix cores.
+//
+// Column-major (Fortran) convention throughout, as the reference's BLAS
+// calls use it: op(A) is m x k, op(B) is k x n, C is m x n,
+//   C = alpha * op(A) * op(B) + beta * C,
+// op(X) = X ("N") or X^T ("T").  Batched products take element strides.
+#pragma once
+
+#include "common.hpp"
+
+namespace etg {
+
+// Arithmetic used for the products.
+//  F64  : FP64 operands, v_mfma_f64_16x16x4 (exact FP64 arithmetic).
+//  F32  : FP64 operands rounded to FP32 on load, v_mfma_f32_16x16x4, FP32 accumulation.
+//  BF16X3: FP64 operands split on load into bf16 hi + lo (x ~ hi + lo); three
+//         products hi*hi + hi*lo + lo*hi per term with v_mfma_f32_16x16x32_bf16,
+//         FP32 accumulation.
+enum class Prec : int { F64 = 0, F32 = 1, BF16X3 = 2 };
+
+struct Gemm {
+  bool ta = false, tb = false;
+  int64_t m = 0, n = 0, k = 0;
+  double alpha = 1.0, beta = 0.0;
+  const double* A = nullptr;
+  int64_t lda = 0, strideA = 0;
+  const double* B = nullptr;
+  int64_t ldb = 0, strideB = 0;
+  double* C = nullptr;
+  int64_t ldc = 0, strideC = 0;
+  int64_t batch = 1;
+};
+
+// Scratch used for split-K partial sums; grown on demand.
+struct GemmWorkspace {
+  DeviceBuffer buf;
+};
+
+// Launches the product on `stream`.  Validates shapes and leading dimensions
+// on the host (throws etg::Error on inconsistent arguments).
+void gemm(hipStream_t stream, const Gemm& g, Prec prec, GemmWorkspace* ws);
+
+// Convenience: C = alpha op(A) op(B) + beta C, no batching.
+inline void gemm(hipStream_t s, char ta, char tb, int64_t m, int64_t n, int64_t k, double alpha,
+                 const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
+                 double* C, int64_t ldc, Prec prec, GemmWorkspace* ws) {
+  Gemm g;
+  g.ta = (ta == 'T' || ta == 't');
+  g.tb = (tb == 'T' || tb == 't');
+  g.m = m; g.n = n; g.k = k;
+  g.alpha = alpha; g.beta = beta;
+  g.A = A; g.lda = lda; g.B = B; g.ldb = ldb; g.C = C; g.ldc = ldc;
+  gemm(s, g, prec, ws);
+}
+
+// Floating-point operations the product performs (2 m n k per batch entry).
+inline double gemm_flops(const Gemm& g) {
+  return 2.0 * double(g.m) * double(g.n) * double(g.k) * double(g.batch);
+}
+
+}  // namespace etg

@@ -1,0 +1,75 @@
+// Bandwidth-bound kernels of the CCSD residual: index permutations,
+// (un)packing of the symmetric doubles vector, the +/- combinations of the
+// ladder term, and deterministic FP64 reductions.  All arrays are
+// column-major; "packed" means the eT doubles layout
+//   idx(X, Y) = max(X,Y) (max(X,Y) + 1) / 2 + min(X,Y),   X = a + n_v i.
+#pragma once
+
+#include "common.hpp"
+
+namespace etg {
+
+// y[o0,o1,o2,o3] = alpha * x[...] + beta * y, where output index k runs over
+// dims out_dim[k] and reads x at sum_k o_k * src_stride[k].
+void permute4(hipStream_t s, const double* x, double* y, const int64_t out_dim[4],
+              const int64_t src_stride[4], double alpha, double beta);
+
+// Helper: y = alpha * permute(x) + beta * y for x of dims d (column-major) and
+// y laid out in the order `perm` (1-based digits, e.g. "1324": y(p,r,q,s) = x(p,q,r,s)).
+void sort4(hipStream_t s, const double* x, const int64_t d[4], const char* perm, double* y,
+           double alpha = 1.0, double beta = 0.0);
+
+// full N x N symmetric matrix from packed storage
+void squareup(hipStream_t s, const double* packed, double* full, int64_t N);
+
+// packed[idx(i,j)] = alpha * full(i,j) for i >= j (+ beta * packed), diagonal times diag_scale
+void packin(hipStream_t s, const double* full, double* packed, int64_t N, double alpha, double beta,
+            double diag_scale);
+
+// y = x + x^T (N x N), out of place
+void symmetric_sum(hipStream_t s, const double* x, double* y, int64_t N);
+
+// y (packed) += alpha (1 - delta/2) [f1 x(p,q,r,s) + f2 x(p,s,r,q) + f1 x(r,s,p,q) + f2 x(r,q,p,s)]
+// for x a full (v,o,v,o) array; f1 = 1, f2 = 0 gives the plain symmetrisation.
+// Diagonal elements receive alpha * x(p,q,p,q).
+void symmetrize_and_pack(hipStream_t s, const double* x, double* packed, int64_t n_v, int64_t n_o,
+                         double alpha, double f1, double f2);
+
+// Ladder (+/-) helpers ------------------------------------------------------
+// t+ (o+ x v+): (1 - d_cd/2)(t_cidj + t_cjdi), rows ij (i <= j), cols cd (c <= d)
+// t- (o- x v-): t_cidj - t_cjdi, rows i < j, cols c < d
+void ladder_t_pm(hipStream_t s, const double* t2, double* tp, double* tm, int64_t n_o, int64_t n_v);
+
+// From G(c, a', d, b') = sum_J L_ac L_bd of one (A, B) block build
+// g+ (v+ x npairs) = G(c,a,d,b) + G(d,a,c,b) and g- (v- x npairs) = G(c,a,d,b) - G(d,a,c,b)
+// for the block's (a, b) pairs (pair_a, pair_b are local indices into A and B).
+void ladder_g_pm(hipStream_t s, const double* G, int64_t n_v, int64_t lenA, int64_t lenB,
+                 const int32_t* pair_a, const int32_t* pair_b, int64_t npairs, double* gp, double* gm);
+
+// packed Omega += (1 - delta/2) W_abij with W_abij = w+ +- w- looked up through
+// col[a + n_v b] (pair column for a >= b).
+void ladder_gather(hipStream_t s, const double* wp, const double* wm, const int32_t* col,
+                   double* omega2, int64_t n_o, int64_t n_v);
+
+// Reductions (deterministic two-pass, FP64) ---------------------------------
+// sum_aibj (t_aibj + t_ai t_bj)(2 g_iajb - g_ibja); t2 packed, g_ovov (o,v,o,v)
+void ccsd_energy(hipStream_t s, const double* t1, const double* t2, const double* g_ovov,
+                 int64_t n_o, int64_t n_v, double* partial, double* result);
+// ||x||_2^2 into result
+void sum_squares(hipStream_t s, const double* x, int64_t n, double* partial, double* result);
+// result = x . y
+void dot(hipStream_t s, const double* x, const double* y, int64_t n, double* partial, double* result);
+constexpr int kReduceBlocks = 1024;
+
+// Amplitude-space helpers ----------------------------------------------------
+// MP2 guess: t2[aibj] = g_vovo(a,i,b,j) / (e_i - e_a + e_j - e_b)
+void mp2_guess(hipStream_t s, const double* g_vovo, const double* eps, double* t2, int64_t n_o,
+               int64_t n_v);
+// orbital differences for singles (v,o) and packed doubles
+void orbital_differences(hipStream_t s, const double* eps, double* x, int64_t n_o, int64_t n_v,
+                         int64_t n);
+// x (+)= alpha * y
+void axpy(hipStream_t s, int64_t n, double alpha, const double* x, double* y);
+void fill(hipStream_t s, double* x, int64_t n, double value);
+
+}  // namespace etg

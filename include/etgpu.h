@@ -1,0 +1,136 @@
+/*
+ * etgpu.h - C ABI of the MI355X CCSD residual library (libetgpu.so).
+ *
+ * The entry points replace the type-bound procedures that eT's ground-state
+ * solvers call on a `ccsd` wavefunction.  Each one is a thin, synchronous
+ * call: host arrays are borrowed for the duration of the call only, all
+ * tensors (Cholesky vectors, amplitudes, Fock matrix, residual) live in GPU
+ * memory owned by the handle.  Array layouts are Fortran (column-major) and
+ * exactly those of eT:
+ *
+ *   L(n_J, n_mo, n_mo)   Cholesky vectors, J fastest, occupied orbitals first
+ *   t1(n_v, n_o)         singles,   compound index ai = a + n_v*i (0-based)
+ *   t2(n_t2)             doubles packed as idx(ai,bj) = max*(max+1)/2 + min,
+ *                        n_t2 = n_o*n_v*(n_o*n_v+1)/2
+ *   amplitude / residual vectors are [t1 | t2] of length n_t1 + n_t2.
+ *
+ * Sizes are int64_t (eT is built with 8-byte default integers,
+ * cmake/Fortran_flags.cmake:92).  Every function returns 0 on success and a
+ * non-zero status otherwise; etg_last_error() then returns a message
+ * (thread-local, valid until the next failing call on the same thread).
+ * No exception crosses the ABI.  A handle is not re-entrant: call it from one
+ * thread at a time (eT's solver loops are single threaded).
+ */
+#ifndef ETGPU_H
+#define ETGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct etg_handle etg_handle;
+
+/* Arithmetic of the bulk contractions in etg_construct_omega. */
+enum etg_precision {
+  ETG_FP64 = 0,   /* FP64 matrix cores throughout */
+  ETG_FP32 = 1    /* FP32 matrix-core products, FP32 accumulation */
+};
+
+/* Creates the device state for a CCSD wavefunction with n_o occupied and
+ * n_v virtual orbitals and n_J Cholesky vectors, on the current HIP device.
+ * Replaces: ccsd%initialize (src/wavefunctions/ccsd/ccsd_class.F90:323-354). */
+int etg_create(int64_t n_o, int64_t n_v, int64_t n_J, etg_handle** handle);
+
+/* Releases all device memory of the handle.  Replaces: ccsd%cleanup. */
+int etg_destroy(etg_handle* handle);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* etg_last_error(void);
+
+/* Uploads the MO Cholesky vectors L(n_J, n_mo, n_mo) and T1-dresses them with
+ * the current singles (zero unless amplitudes were set before).
+ * Replaces: the L_mo producer hand-over (src/engines/cc/tasks/
+ * cholesky_decomposition_task_class.F90:127-134, L_t1 <- L_mo :133). */
+int etg_set_cholesky(etg_handle* handle, const double* L);
+
+/* MO one-electron Hamiltonian h(n_mo, n_mo).
+ * Replaces: the MO-transformed integrals of get_t1_oei('hamiltonian')
+ * (src/wavefunctions/ccs/oei_ccs.F90:35-97) before the T1 transformation. */
+int etg_set_oei(etg_handle* handle, const double* h);
+
+/* Orbital energies eps(n_mo).  Replaces: wf%orbital_energies
+ * (src/wavefunctions/wavefunction_class.F90:35-60). */
+int etg_set_orbital_energies(etg_handle* handle, const double* eps);
+
+/* Amplitudes [t1 | t2].  Replaces: set_amplitudes / get_amplitudes
+ * (src/wavefunctions/ccsd/set_get_ccsd.F90). */
+int etg_set_amplitudes(etg_handle* handle, const double* t);
+int etg_get_amplitudes(etg_handle* handle, double* t);
+
+/* t1 = 0 is not implied: sets t2 to the MP2 guess from the current
+ * T1-dressed integrals and orbital energies.
+ * Replaces: set_t2_to_cc2_guess (src/wavefunctions/ccsd/ccsd_class.F90:478-533). */
+int etg_set_t2_to_mp2_guess(etg_handle* handle);
+
+/* L_t1 = X L_mo Y^T from the current singles.
+ * Replaces: construct_t1_cholesky (src/wavefunctions/ccs/t1_ccs.F90:208-246). */
+int etg_construct_t1_cholesky(etg_handle* handle);
+
+/* T1-transformed Fock matrix, all four blocks; F (n_mo, n_mo) is copied out
+ * when non-NULL.  Replaces: construct_fock (src/wavefunctions/ccsd/fock_ccsd.F90:34-79). */
+int etg_construct_fock(etg_handle* handle, double* F);
+
+/* Correlation energy sum (t_aibj + t_ai t_bj)(2 g_iajb - g_ibja).
+ * Replaces: calculate_energy (src/wavefunctions/ccsd/mean_value_ccsd.F90:36-99);
+ * the caller adds E_HF as eT does. */
+int etg_calculate_energy(etg_handle* handle, double* e_corr);
+
+/* Residual Omega(n_t1 + n_t2) for the current amplitudes, in eT's packed
+ * convention (doubles diagonal carries the factor 1/2).  Copied to `omega`
+ * when non-NULL, otherwise left in device memory (see etg_get_omega).
+ * Replaces: construct_omega (src/wavefunctions/ccsd/omega_ccsd.F90:39-110). */
+int etg_construct_omega(etg_handle* handle, double* omega, int precision);
+
+/* Copies the last residual to the host. */
+int etg_get_omega(etg_handle* handle, double* omega);
+
+/* ||Omega||_2 of the last residual (FP64 reduction on the device).
+ * Replaces: get_l2_norm(omega) in run_diis_cc_gs (src/solvers/cc/diis_cc_gs_class.F90:260). */
+int etg_omega_norm(etg_handle* handle, double* norm);
+
+/* eps_a - eps_i (+ eps_b - eps_j) for n = n_t1 or n_t1 + n_t2 entries.
+ * Replaces: get_orbital_differences (src/wavefunctions/doubles/doubles_class.F90:421-470). */
+int etg_get_orbital_differences(etg_handle* handle, double* x, int64_t n);
+
+/* Virtual batch length of the ladder term's g_acbd blocks (memory/launch
+ * trade-off; results do not depend on it beyond rounding). */
+int etg_set_ladder_batch(etg_handle* handle, int64_t batch);
+
+/* Waits for all work queued on the handle's stream. */
+int etg_synchronize(etg_handle* handle);
+
+/* Floating-point operations issued by the last etg_construct_omega (sum of
+ * 2 m n k over the products it ran). */
+int etg_last_omega_flops(etg_handle* handle, double* flops);
+
+/* ---- Diagnostics: no counterpart in the reference interface; used by the
+ * tests and the roofline measurement of the matrix-core product kernel. ---- */
+
+/* C = alpha op(A) op(B) + beta C on host arrays (copied to the device and
+ * back), column-major, op = 'N' or 'T', with the given precision. */
+int etg_test_gemm(char ta, char tb, int64_t m, int64_t n, int64_t k, double alpha,
+                  const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
+                  double* C, int64_t ldc, int precision);
+
+/* Seconds per product for `reps` back-to-back m x n x k products on
+ * device-resident operands. */
+int etg_bench_gemm(char ta, char tb, int64_t m, int64_t n, int64_t k, int precision, int reps,
+                   double* seconds);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ETGPU_H */

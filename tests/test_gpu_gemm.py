@@ -1,0 +1,47 @@
+"""Matrix-core product kernel against numpy (FP64 exact to round-off, FP32 to its tolerance)."""
+import numpy as np
+import pytest
+
+from et_amd import ccsd
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [
+    (1, 1, 1), (16, 16, 4), (17, 33, 5), (64, 64, 16), (100, 70, 37), (128, 128, 16),
+    (129, 257, 31), (300, 200, 500), (48, 48, 4000),     # split-K
+    (1000, 1, 1000),                                       # GEMV-shaped
+    (640, 520, 130),
+]
+
+
+@pytest.mark.parametrize('ta', ['N', 'T'])
+@pytest.mark.parametrize('tb', ['N', 'T'])
+@pytest.mark.parametrize('m,n,k', SHAPES)
+def test_fp64_gemm(ta, tb, m, n, k):
+    rng = np.random.default_rng(m * 131 + n * 7 + k)
+    A = rng.normal(size=(k, m) if ta == 'T' else (m, k))
+    B = rng.normal(size=(n, k) if tb == 'T' else (k, n))
+    C = rng.normal(size=(m, n))
+    out = ccsd.device_gemm(ta, tb, 0.7, A, B, -0.3, C)
+    ref = 0.7 * (A.T if ta == 'T' else A) @ (B.T if tb == 'T' else B) - 0.3 * C
+    scale = np.sqrt(k) + 1.0
+    assert np.abs(out - ref).max() < 1e-14 * scale * 4
+
+
+@pytest.mark.parametrize('m,n,k', [(100, 70, 37), (300, 200, 500), (48, 48, 4000)])
+def test_fp32_gemm(m, n, k):
+    rng = np.random.default_rng(5)
+    A = rng.normal(size=(m, k))
+    B = rng.normal(size=(k, n))
+    C = np.zeros((m, n))
+    out = ccsd.device_gemm('N', 'N', 1.0, A, B, 0.0, C, precision=ccsd.FP32)
+    ref = A @ B
+    # FP32 rounding of the operands and FP32 accumulation: ~ sqrt(k) 2^-24 relative per term
+    tol = 4.0 * np.sqrt(k) * 2.0 ** -24 * np.abs(A).max() * np.abs(B).max() * np.sqrt(k)
+    assert np.abs(out - ref).max() < tol
+
+
+def test_zero_k_scales_c():
+    C = np.arange(12.0).reshape(3, 4)
+    out = ccsd.device_gemm('N', 'N', 1.0, np.zeros((3, 0)), np.zeros((0, 4)), 2.0, C)
+    np.testing.assert_array_equal(out, 2.0 * C)
