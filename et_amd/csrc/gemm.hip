@@ -8,8 +8,9 @@
 // sub-tile made of 16 x 16 MFMA blocks.
 //
 // MFMA 16x16x4 operand layout (one element per lane, f64 and f32 alike):
-//   A[i][k] in lane i + 16 k,  B[k][j] in lane j + 16 k,
-//   D[4 (lane/16) + r][lane % 16] in element r of the lane's 4-vector.
+//   A[i][k] in lane i + 16 k,  B[k][j] in lane j + 16 k.
+// Result D (element r of the lane's 4-vector, column lane % 16) differs:
+//   f32: row 4 (lane/16) + r        f64: row (lane/16) + 4 r
 //
 // Tile order: workgroups are dispatched round-robin over the 8 XCDs, so the
 // linear block id is first remapped to give each XCD a contiguous range of
@@ -67,6 +68,7 @@ struct Mfma<double> {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
   static __device__ inline double cvt(double x) { return x; }
+  static __device__ inline int row(int lrow, int r) { return lrow + 4 * r; }
 };
 
 template <>
@@ -76,6 +78,7 @@ struct Mfma<float> {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
   static __device__ inline float cvt(double x) { return static_cast<float>(x); }
+  static __device__ inline int row(int lrow, int r) { return 4 * lrow + r; }
 };
 
 // LDS tile orientation follows the operand's contiguous dimension in HBM so
@@ -212,7 +215,7 @@ __global__ __launch_bounds__(kThreads, (min_blocks<T, BM, BN>())) void gemm_mfma
       for (int j = 0; j < TNs; ++j)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int64_t row = m0 + wm * WM + i * 16 + lrow * 4 + r;
+          const int64_t row = m0 + wm * WM + i * 16 + M::row(lrow, r);
           const int64_t col = n0 + wn * WN + j * 16 + lcol;
           if (row < p.m && col < p.n) W[row + col * p.m] = double(acc[i][j][r]);
         }
@@ -225,7 +228,7 @@ __global__ __launch_bounds__(kThreads, (min_blocks<T, BM, BN>())) void gemm_mfma
     for (int j = 0; j < TNs; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int64_t row = m0 + wm * WM + i * 16 + lrow * 4 + r;
+        const int64_t row = m0 + wm * WM + i * 16 + M::row(lrow, r);
         const int64_t col = n0 + wn * WN + j * 16 + lcol;
         if (row < p.m && col < p.n) {
           double* c = C + row + col * p.ldc;
