@@ -1,0 +1,188 @@
+#!/usr/bin/env python3
+"""CCSD residual throughput on one MI355X.
+
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--precision fp64] [--no-cpu]
+
+Workload: the synthetic omega-contraction case o = 48, v = 400, n_J = 1600,
+seed 0 (Cholesky vectors 0.5 exp(-J/400) N(0,1), symmetric in p,q; packed
+t2 ~ 1e-3 N(0,1); t1 = 0; canonical orbital energies).  A step is one full
+residual evaluation (T1 Fock, singles and doubles terms, the O^2V^4 ladder
+with g_acbd generated on the fly) with all inputs resident in HBM and the
+result left in HBM.  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from et_amd import ccsd  # noqa: E402
+from et_amd import flops as flopcount  # noqa: E402
+from et_amd import synthetic as syn  # noqa: E402
+
+METRIC = 'CCSD omega evals/s & time-to-solution, mixed vs FP64 GPU at |ΔE|≤1e-8 Eh'
+PEAKS = {'fp64': 78.6e12, 'fp32': 157.3e12}
+WORKLOAD = dict(n_o=48, n_v=400, n_J=1600, seed=0)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_system(n_o, n_v, n_J, seed):
+    t0 = time.time()
+    L = syn.cholesky_vectors(seed, n_o + n_v, n_J)
+    eps = syn.orbital_energies(seed, n_o, n_v)
+    h = syn.canonical_core_hamiltonian(L, eps, n_o)
+    t2 = syn.doubles_amplitudes(seed, n_o, n_v, sigma=1e-3)
+    log('inputs generated in %.1f s' % (time.time() - t0))
+    return L, h, eps, t2
+
+
+def cpu_baseline(n_o, n_v, n_J, cols=4800):
+    """Oracle FP64 product of the C2/D2 (ov) x (ov) shape on a column slice, host BLAS.
+
+    CPU omega/s = measured FLOP rate / F_omega: every eT term is credited with
+    the rate of its largest DGEMM, so this bounds eT's CPU throughput from above.
+    """
+    from oracle import ccsd as O
+    ov = n_o * n_v
+    rng = np.random.default_rng(1)
+    A = rng.standard_normal((ov, ov))
+    B = rng.standard_normal((ov, cols))
+    O.symmetric_product(1.0, A[:512], B[:, :64])          # warm the BLAS thread pool
+    t0 = time.perf_counter()
+    O.symmetric_product(-0.5, A, B)
+    dt = time.perf_counter() - t0
+    rate = 2.0 * ov * ov * cols / dt
+    try:
+        from threadpoolctl import threadpool_info
+        threads = max(int(p.get('num_threads', 1)) for p in threadpool_info()) if threadpool_info() else 1
+    except Exception:
+        threads = int(os.environ.get('OMP_NUM_THREADS', os.cpu_count() or 1))
+    f_omega = flopcount.omega_flops(n_o, n_v, n_J)
+    return {
+        'value': rate / f_omega,
+        'unit': 'omega evals/s',
+        'cores': threads,
+        'kind': 'port',
+        'sample': ('oracle FP64 C2/D2 product (ov=%d)x(ov) times (ov)x%d on host BLAS, '
+                   '%.2e FLOP in %.2f s (%.1f GFLOP/s); omega/s = rate / F_omega(%.3e)'
+                   % (ov, cols, 2.0 * ov * ov * cols, dt, rate / 1e9, f_omega)),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=3)
+    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--precision', choices=['fp64', 'fp32'], default='fp64')
+    ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-mixed', action='store_true')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print('bench.py measures one GPU: every box in this setup has a single MI355X, so '
+              '--gpus %d is not supported (the multi-GPU sharding is described in DESIGN.md).'
+              % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    w = WORKLOAD
+    n_o, n_v, n_J = w['n_o'], w['n_v'], w['n_J']
+    L, h, eps, t2 = build_system(n_o, n_v, n_J, w['seed'])
+    wf = ccsd.CCSD(n_o, n_v, n_J)
+    t0 = time.time()
+    wf.set_cholesky(L)
+    wf.set_oei(h)
+    wf.set_orbital_energies(eps)
+    t = np.zeros(wf.n_gs_amplitudes)
+    t[wf.n_t1:] = t2
+    wf.set_amplitudes(t)
+    wf.construct_t1_cholesky()
+    wf.construct_fock()
+    del L
+    log('uploaded in %.1f s' % (time.time() - t0))
+
+    prec = ccsd.PRECISIONS[args.precision]
+    for _ in range(args.warmup):
+        wf.construct_omega(prec, copy_out=False)
+    wf.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        wf.construct_omega(prec, copy_out=False)
+        log('step %d done' % k)
+    wf.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+
+    # per-kernel ledger over one more evaluation: events on the library's stream
+    wf.set_kernel_ledger(True)
+    wf.construct_omega(prec, copy_out=False)
+    ledger = wf.kernel_ledger()
+    wf.set_kernel_ledger(False)
+    dom = max(ledger, key=lambda e: e[2])
+    peak = PEAKS[args.precision]
+    f_omega = flopcount.omega_flops(n_o, n_v, n_J)
+    roofline = {
+        'bound': 'mfma',
+        'kernel': dom[0],
+        'launches': dom[1],
+        'achieved': dom[3] / dom[2] / 1e12,
+        'peak': peak / 1e12,
+        'unit': 'TFLOP/s',
+        'frac': dom[3] / dom[2] / peak,
+        'traffic': None,
+        'avg_launch_ms': dom[2] / dom[1] * 1e3,
+        'omega_achieved': f_omega / dt / 1e12,
+        'omega_frac': f_omega / dt / peak,
+    }
+    log('ledger:', json.dumps([(e[0], e[1], round(e[2], 4), '%.3e' % e[3]) for e in ledger]))
+
+    extra = {}
+    if not args.no_mixed and args.precision == 'fp64':
+        ref = wf.get_omega()
+        t0 = time.perf_counter()
+        wf.construct_omega(ccsd.FP32, copy_out=False)
+        wf.synchronize()
+        dt32 = time.perf_counter() - t0
+        om32 = wf.get_omega()
+        extra['mixed'] = {
+            'precision': 'fp32-mfma',
+            'omega_evals_per_s': 1.0 / dt32,
+            'max_abs_err_vs_fp64': float(np.abs(om32 - ref).max()),
+            'rel_err_vs_fp64': float(np.abs(om32 - ref).max() / np.abs(ref).max()),
+        }
+    wf.close()
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(n_o, n_v, n_J)
+
+    out = {
+        'metric': METRIC,
+        'value': 1.0 / dt,
+        'unit': 'omega evals/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': dt * 1e3,
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f64' if args.precision == 'fp64' else 'f32',
+        'data': 'synthetic',
+        'config': {'workload': 'omega_o48_v400_J1600_seed0', 'n_o': n_o, 'n_v': n_v, 'n_J': n_J,
+                   'seed': w['seed'], 'precision': args.precision, 'F_omega': f_omega},
+        'roofline': roofline,
+        'cpu_baseline': cpu,
+    }
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == '__main__':
+    main()

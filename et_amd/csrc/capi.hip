@@ -155,6 +155,28 @@ int etg_last_omega_flops(etg_handle* h, double* flops) {
   });
 }
 
+int etg_set_kernel_ledger(etg_handle* h, int on) {
+  return guarded([&] { wf_of(h).set_ledger(on != 0); });
+}
+
+int etg_kernel_ledger(etg_handle* h, int64_t index, char* name, int64_t name_len,
+                      int64_t* launches, double* seconds, double* flops, int64_t* count) {
+  return guarded([&] {
+    const auto entries = wf_of(h).ledger();
+    if (count) *count = int64_t(entries.size());
+    if (index < 0) return;
+    if (index >= int64_t(entries.size())) etg::fail("etg_kernel_ledger: index out of range");
+    const auto& e = entries[size_t(index)];
+    if (name && name_len > 0) {
+      std::strncpy(name, e.kernel.c_str(), size_t(name_len - 1));
+      name[name_len - 1] = '\0';
+    }
+    if (launches) *launches = e.launches;
+    if (seconds) *seconds = e.seconds;
+    if (flops) *flops = e.flops;
+  });
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------------------------

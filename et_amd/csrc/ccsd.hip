@@ -58,6 +58,11 @@ CCSD::~CCSD() {
     (void)hipStreamSynchronize(stream_);
     (void)hipStreamDestroy(stream_);
   }
+  for (auto& r : ledger_recs_) {
+    (void)hipEventDestroy(r.start);
+    (void)hipEventDestroy(r.stop);
+  }
+  for (auto e : event_pool_) (void)hipEventDestroy(e);
 }
 
 void CCSD::require(bool ok, const char* what) const {
@@ -73,11 +78,65 @@ double* CCSD::work(int slot) {
   return work_[slot].as<double>();
 }
 
+hipEvent_t CCSD::take_event() {
+  if (!event_pool_.empty()) {
+    hipEvent_t e = event_pool_.back();
+    event_pool_.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  ETG_HIP(hipEventCreate(&e));
+  return e;
+}
+
+void CCSD::record_gemm(const GemmLaunch& info, double flops, hipEvent_t a, hipEvent_t b) {
+  ETG_HIP(hipEventRecord(b, stream_));
+  ledger_recs_.push_back(LedgerRec{info.kernel, flops, a, b});
+}
+
+void CCSD::set_ledger(bool on) {
+  synchronize();
+  for (auto& r : ledger_recs_) {
+    event_pool_.push_back(r.start);
+    event_pool_.push_back(r.stop);
+  }
+  ledger_recs_.clear();
+  ledger_on_ = on;
+}
+
+std::vector<CCSD::LedgerEntry> CCSD::ledger() {
+  synchronize();
+  std::vector<LedgerEntry> out;
+  for (const auto& r : ledger_recs_) {
+    float ms = 0.f;
+    ETG_HIP(hipEventElapsedTime(&ms, r.start, r.stop));
+    auto it = std::find_if(out.begin(), out.end(),
+                           [&](const LedgerEntry& e) { return e.kernel == r.kernel; });
+    if (it == out.end()) {
+      out.push_back(LedgerEntry{r.kernel, 0, 0.0, 0.0});
+      it = out.end() - 1;
+    }
+    it->launches += 1;
+    it->seconds += double(ms) * 1e-3;
+    it->flops += r.flops;
+  }
+  return out;
+}
+
 void CCSD::gemm(char ta, char tb, int64_t m, int64_t n, int64_t k, double alpha, const double* A,
                 int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc,
                 Prec prec) {
-  etg::gemm(stream_, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, prec, &gws_);
-  last_omega_flops += 2.0 * double(m) * double(n) * double(k);
+  const double flops = 2.0 * double(m) * double(n) * double(k);
+  hipEvent_t a = nullptr, b = nullptr;
+  if (ledger_on_) {
+    a = take_event();
+    b = take_event();
+    ETG_HIP(hipEventRecord(a, stream_));
+  }
+  const GemmLaunch info =
+      etg::gemm(stream_, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, prec, &gws_);
+  if (ledger_on_) record_gemm(info, flops, a, b);
+  last_omega_flops += flops;
 }
 
 void CCSD::gemm_batched(char ta, char tb, int64_t m, int64_t n, int64_t k, double alpha,
@@ -93,7 +152,14 @@ void CCSD::gemm_batched(char ta, char tb, int64_t m, int64_t n, int64_t k, doubl
   g.B = B; g.ldb = ldb; g.strideB = sB;
   g.C = C; g.ldc = ldc; g.strideC = sC;
   g.batch = batch;
-  etg::gemm(stream_, g, prec, &gws_);
+  hipEvent_t a = nullptr, b = nullptr;
+  if (ledger_on_) {
+    a = take_event();
+    b = take_event();
+    ETG_HIP(hipEventRecord(a, stream_));
+  }
+  const GemmLaunch info = etg::gemm(stream_, g, prec, &gws_);
+  if (ledger_on_) record_gemm(info, gemm_flops(g), a, b);
   last_omega_flops += gemm_flops(g);
 }
 

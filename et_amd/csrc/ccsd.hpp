@@ -2,6 +2,7 @@
 // ground-state solvers call on it (reference: src/wavefunctions/ccsd/ccsd_class.F90).
 #pragma once
 
+#include <string>
 #include <vector>
 
 #include "common.hpp"
@@ -46,6 +47,16 @@ class CCSD {
   // flop ledger of the last residual evaluation (sum of 2 m n k over products)
   double last_omega_flops = 0.0;
 
+  // Per-kernel ledger: when on, every product is bracketed by HIP events on
+  // the handle's stream; entries are aggregated by kernel instantiation.
+  struct LedgerEntry {
+    std::string kernel;
+    int64_t launches = 0;
+    double seconds = 0.0, flops = 0.0;
+  };
+  void set_ledger(bool on);
+  std::vector<LedgerEntry> ledger();
+
  private:
   void require(bool ok, const char* what) const;
   void gemm(char ta, char tb, int64_t m, int64_t n, int64_t k, double alpha, const double* A,
@@ -79,6 +90,17 @@ class CCSD {
   std::vector<Block> blocks_;
   DeviceBuffer pair_a_, pair_b_, pair_col_;
   DeviceBuffer tp_, tm_, wp_, wm_, gblk_, gp_, gm_;
+
+  struct LedgerRec {
+    const char* kernel;
+    double flops;
+    hipEvent_t start, stop;
+  };
+  bool ledger_on_ = false;
+  std::vector<LedgerRec> ledger_recs_;
+  std::vector<hipEvent_t> event_pool_;
+  hipEvent_t take_event();
+  void record_gemm(const GemmLaunch& info, double flops, hipEvent_t a, hipEvent_t b);
 
   bool have_L_ = false, have_h_ = false, have_eps_ = false, have_t_ = false, have_F_ = false;
 };

@@ -20,6 +20,7 @@
 #include "gemm.hpp"
 
 #include <algorithm>
+#include <string>
 
 namespace etg {
 namespace {
@@ -262,20 +263,39 @@ __global__ void splitk_reduce(const double* __restrict__ W, int splits, int64_t 
   *c = (beta == 0.0) ? v : v + beta * *c;
 }
 
+template <typename T, bool TA, bool TB, int BM, int BN>
+const char* kernel_name() {
+  static const std::string name = std::string("gemm_mfma16<") + (sizeof(T) == 8 ? "f64," : "f32,") +
+                                  (TA ? "T" : "N") + (TB ? "T" : "N") + "," + std::to_string(BM) +
+                                  "x" + std::to_string(BN) + ">";
+  return name.c_str();
+}
+
 template <typename T, int BM, int BN, int BK>
-void launch_cfg(hipStream_t stream, const Gemm& g, Params p, int splits) {
+const char* launch_cfg(hipStream_t stream, const Gemm& g, Params p, int splits) {
   p.tiles_m = int(ceil_div(g.m, BM));
   p.tiles_n = int(ceil_div(g.n, BN));
   dim3 grid(p.tiles_m * p.tiles_n, splits, unsigned(g.batch));
-  if (!g.ta && !g.tb) gemm_mfma16<T, false, false, BM, BN, BK><<<grid, kThreads, 0, stream>>>(p);
-  else if (!g.ta && g.tb) gemm_mfma16<T, false, true, BM, BN, BK><<<grid, kThreads, 0, stream>>>(p);
-  else if (g.ta && !g.tb) gemm_mfma16<T, true, false, BM, BN, BK><<<grid, kThreads, 0, stream>>>(p);
-  else gemm_mfma16<T, true, true, BM, BN, BK><<<grid, kThreads, 0, stream>>>(p);
+  const char* name;
+  if (!g.ta && !g.tb) {
+    gemm_mfma16<T, false, false, BM, BN, BK><<<grid, kThreads, 0, stream>>>(p);
+    name = kernel_name<T, false, false, BM, BN>();
+  } else if (!g.ta && g.tb) {
+    gemm_mfma16<T, false, true, BM, BN, BK><<<grid, kThreads, 0, stream>>>(p);
+    name = kernel_name<T, false, true, BM, BN>();
+  } else if (g.ta && !g.tb) {
+    gemm_mfma16<T, true, false, BM, BN, BK><<<grid, kThreads, 0, stream>>>(p);
+    name = kernel_name<T, true, false, BM, BN>();
+  } else {
+    gemm_mfma16<T, true, true, BM, BN, BK><<<grid, kThreads, 0, stream>>>(p);
+    name = kernel_name<T, true, true, BM, BN>();
+  }
   ETG_LAUNCH("gemm_mfma16");
+  return name;
 }
 
 template <typename T>
-void launch_typed(hipStream_t stream, const Gemm& g, GemmWorkspace* ws) {
+GemmLaunch launch_typed(hipStream_t stream, const Gemm& g, GemmWorkspace* ws) {
   Params p{};
   p.m = g.m; p.n = g.n; p.k = g.k;
   p.alpha = g.alpha; p.beta = g.beta;
@@ -304,8 +324,10 @@ void launch_typed(hipStream_t stream, const Gemm& g, GemmWorkspace* ws) {
     ws->buf.resize(need);
     p.W = ws->buf.as<double>();
   }
-  if (big) launch_cfg<T, 128, 128, 16>(stream, g, p, splits);
-  else launch_cfg<T, 64, 64, 16>(stream, g, p, splits);
+  GemmLaunch info;
+  info.splits = splits;
+  if (big) info.kernel = launch_cfg<T, 128, 128, 16>(stream, g, p, splits);
+  else info.kernel = launch_cfg<T, 64, 64, 16>(stream, g, p, splits);
   if (splits > 1) {
     const int64_t mn = g.m * g.n;
     dim3 grid(unsigned(ceil_div(mn, 256)), 1, unsigned(g.batch));
@@ -313,13 +335,14 @@ void launch_typed(hipStream_t stream, const Gemm& g, GemmWorkspace* ws) {
                                             g.strideC);
     ETG_LAUNCH("splitk_reduce");
   }
+  return info;
 }
 
 }  // namespace
 
-void gemm(hipStream_t stream, const Gemm& g, Prec prec, GemmWorkspace* ws) {
+GemmLaunch gemm(hipStream_t stream, const Gemm& g, Prec prec, GemmWorkspace* ws) {
   if (g.m < 0 || g.n < 0 || g.k < 0 || g.batch < 1) fail("gemm: negative dimension");
-  if (g.m == 0 || g.n == 0) return;
+  if (g.m == 0 || g.n == 0) return GemmLaunch{};
   const int64_t rowsA = g.ta ? g.k : g.m;
   const int64_t rowsB = g.tb ? g.n : g.k;
   if (g.lda < std::max<int64_t>(1, rowsA)) fail("gemm: lda too small");
@@ -331,13 +354,14 @@ void gemm(hipStream_t stream, const Gemm& g, Prec prec, GemmWorkspace* ws) {
     dim3 grid(unsigned(ceil_div(mn, 256)), 1, unsigned(g.batch));
     scale_matrix<<<grid, 256, 0, stream>>>(g.C, g.m, g.n, g.ldc, g.strideC, g.beta);
     ETG_LAUNCH("scale_matrix");
-    return;
+    return GemmLaunch{"scale_matrix", 1};
   }
   switch (prec) {
-    case Prec::F64: launch_typed<double>(stream, g, ws); break;
-    case Prec::F32: launch_typed<float>(stream, g, ws); break;
-    case Prec::BF16X3: fail("gemm: split-bf16 products are not available in this build");
+    case Prec::F64: return launch_typed<double>(stream, g, ws);
+    case Prec::F32: return launch_typed<float>(stream, g, ws);
+    case Prec::BF16X3: break;
   }
+  fail("gemm: split-bf16 products are not available in this build");
 }
 
 }  // namespace etg

@@ -36,21 +36,27 @@ struct GemmWorkspace {
   DeviceBuffer buf;
 };
 
+// Which kernel instantiation a product ran on (for the per-kernel ledger).
+struct GemmLaunch {
+  const char* kernel = "";   // e.g. "gemm_mfma16<f64,NN,128x128>"
+  int splits = 1;            // > 1: split-K partial sums + reduction kernel
+};
+
 // Launches the product on `stream`.  Validates shapes and leading dimensions
 // on the host (throws etg::Error on inconsistent arguments).
-void gemm(hipStream_t stream, const Gemm& g, Prec prec, GemmWorkspace* ws);
+GemmLaunch gemm(hipStream_t stream, const Gemm& g, Prec prec, GemmWorkspace* ws);
 
 // Convenience: C = alpha op(A) op(B) + beta C, no batching.
-inline void gemm(hipStream_t s, char ta, char tb, int64_t m, int64_t n, int64_t k, double alpha,
-                 const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
-                 double* C, int64_t ldc, Prec prec, GemmWorkspace* ws) {
+inline GemmLaunch gemm(hipStream_t s, char ta, char tb, int64_t m, int64_t n, int64_t k,
+                       double alpha, const double* A, int64_t lda, const double* B, int64_t ldb,
+                       double beta, double* C, int64_t ldc, Prec prec, GemmWorkspace* ws) {
   Gemm g;
   g.ta = (ta == 'T' || ta == 't');
   g.tb = (tb == 'T' || tb == 't');
   g.m = m; g.n = n; g.k = k;
   g.alpha = alpha; g.beta = beta;
   g.A = A; g.lda = lda; g.B = B; g.ldb = ldb; g.C = C; g.ldc = ldc;
-  gemm(s, g, prec, ws);
+  return gemm(s, g, prec, ws);
 }
 
 // Floating-point operations the product performs (2 m n k per batch entry).

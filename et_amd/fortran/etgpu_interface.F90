@@ -1,0 +1,140 @@
+!
+!  ISO_C_BINDING interface to libetgpu.so (include/etgpu.h).
+!
+!  This is the binding a `ccsd_gpu` wavefunction (extends eT's `ccsd`) uses
+!  to forward its type-bound procedures to the GPU: every routine is a
+!  bind(C) function returning 0 on success; etg_error_message converts the
+!  C error string for output%error_msg.  Sizes are integer(c_int64_t) (eT's
+!  default 8-byte integers), arrays are passed by address in eT's layouts.
+!
+module etgpu_interface
+!
+   use iso_c_binding, only: c_int, c_int64_t, c_double, c_ptr, c_char, c_null_char, &
+                            c_f_pointer, c_associated
+!
+   implicit none
+!
+   integer(c_int), parameter :: etg_fp64 = 0
+   integer(c_int), parameter :: etg_fp32 = 1
+!
+   interface
+!
+      integer(c_int) function etg_create(n_o, n_v, n_J, handle) bind(C, name='etg_create')
+         import :: c_int, c_int64_t, c_ptr
+         integer(c_int64_t), value :: n_o, n_v, n_J
+         type(c_ptr), intent(out) :: handle
+      end function etg_create
+!
+      integer(c_int) function etg_destroy(handle) bind(C, name='etg_destroy')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: handle
+      end function etg_destroy
+!
+      type(c_ptr) function etg_last_error() bind(C, name='etg_last_error')
+         import :: c_ptr
+      end function etg_last_error
+!
+      integer(c_int) function etg_set_cholesky(handle, L) bind(C, name='etg_set_cholesky')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: handle
+         real(c_double), dimension(*), intent(in) :: L
+      end function etg_set_cholesky
+!
+      integer(c_int) function etg_set_oei(handle, h) bind(C, name='etg_set_oei')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: handle
+         real(c_double), dimension(*), intent(in) :: h
+      end function etg_set_oei
+!
+      integer(c_int) function etg_set_orbital_energies(handle, eps) &
+                                                   bind(C, name='etg_set_orbital_energies')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: handle
+         real(c_double), dimension(*), intent(in) :: eps
+      end function etg_set_orbital_energies
+!
+      integer(c_int) function etg_set_amplitudes(handle, t) bind(C, name='etg_set_amplitudes')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: handle
+         real(c_double), dimension(*), intent(in) :: t
+      end function etg_set_amplitudes
+!
+      integer(c_int) function etg_get_amplitudes(handle, t) bind(C, name='etg_get_amplitudes')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: handle
+         real(c_double), dimension(*), intent(out) :: t
+      end function etg_get_amplitudes
+!
+      integer(c_int) function etg_set_t2_to_mp2_guess(handle) bind(C, name='etg_set_t2_to_mp2_guess')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: handle
+      end function etg_set_t2_to_mp2_guess
+!
+      integer(c_int) function etg_construct_t1_cholesky(handle) &
+                                                   bind(C, name='etg_construct_t1_cholesky')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: handle
+      end function etg_construct_t1_cholesky
+!
+      integer(c_int) function etg_construct_fock(handle, F) bind(C, name='etg_construct_fock')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: handle
+         real(c_double), dimension(*), intent(out) :: F
+      end function etg_construct_fock
+!
+      integer(c_int) function etg_calculate_energy(handle, e_corr) bind(C, name='etg_calculate_energy')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: handle
+         real(c_double), intent(out) :: e_corr
+      end function etg_calculate_energy
+!
+      integer(c_int) function etg_construct_omega(handle, omega, precision) &
+                                                   bind(C, name='etg_construct_omega')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: handle
+         real(c_double), dimension(*), intent(out) :: omega
+         integer(c_int), value :: precision
+      end function etg_construct_omega
+!
+      integer(c_int) function etg_omega_norm(handle, norm) bind(C, name='etg_omega_norm')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: handle
+         real(c_double), intent(out) :: norm
+      end function etg_omega_norm
+!
+      integer(c_int) function etg_get_orbital_differences(handle, x, n) &
+                                                   bind(C, name='etg_get_orbital_differences')
+         import :: c_int, c_ptr, c_double, c_int64_t
+         type(c_ptr), value :: handle
+         real(c_double), dimension(*), intent(out) :: x
+         integer(c_int64_t), value :: n
+      end function etg_get_orbital_differences
+!
+   end interface
+!
+contains
+!
+   function etg_error_message() result(message)
+!!
+!!    Copies the library's last error string into a Fortran character variable.
+!!
+      implicit none
+!
+      character(len=:), allocatable :: message
+      character(kind=c_char), dimension(:), pointer :: chars
+      integer :: n
+!
+      message = ''
+      if (.not. c_associated(etg_last_error())) return
+      call c_f_pointer(etg_last_error(), chars, [4096])
+      n = 0
+      do while (n < 4096)
+         if (chars(n + 1) == c_null_char) exit
+         n = n + 1
+      enddo
+      allocate(character(len=n) :: message)
+      message = transfer(chars(1:n), message)
+!
+   end function etg_error_message
+!
+end module etgpu_interface

@@ -118,6 +118,17 @@ int etg_last_omega_flops(etg_handle* handle, double* flops);
 /* ---- Diagnostics: no counterpart in the reference interface; used by the
  * tests and the roofline measurement of the matrix-core product kernel. ---- */
 
+/* Turns the per-kernel ledger on (1) or off (0) and clears it.  While on,
+ * every matrix-core product is bracketed by HIP events on the handle's
+ * stream; entries aggregate launches, seconds and 2mnk flops per kernel
+ * instantiation. */
+int etg_set_kernel_ledger(etg_handle* handle, int on);
+
+/* Entry `index` of the ledger (name copied into `name`, NUL-terminated);
+ * `count` receives the number of entries.  index < 0 only returns count. */
+int etg_kernel_ledger(etg_handle* handle, int64_t index, char* name, int64_t name_len,
+                      int64_t* launches, double* seconds, double* flops, int64_t* count);
+
 /* C = alpha op(A) op(B) + beta C on host arrays (copied to the device and
  * back), column-major, op = 'N' or 'T', with the given precision. */
 int etg_test_gemm(char ta, char tb, int64_t m, int64_t n, int64_t k, double alpha,

@@ -252,6 +252,11 @@ def symmetrize_and_pack(x_mat, n_v, n_o, contra=False, psrq=False):
     return packin(s)
 
 
+def symmetric_product(alpha, A, B):
+    """alpha * A @ B, the (ov) x (ov) DSYMM products of omega_ccsd_c2_d2 (omega_ccsd.F90:647-688)."""
+    return alpha * (A @ B)
+
+
 def omega_c2_d2(L_t1, t2, n_o, n_v):
     """omega_ccsd_c2_d2 (omega_ccsd.F90:586-709): the four (ov)x(ov) DSYMMs.
 
@@ -272,13 +277,13 @@ def omega_c2_d2(L_t1, t2, n_o, n_v):
     # G[(c,k),(d,l)] = g_lckd  (sort_1234_to_2341 of g_ovov)   (:643-644)
     g_ovov = eri(L_t1, o, v, o, v)
     G = aibj_to_mat(g_ovov.transpose(1, 2, 3, 0))              # G(c,l,d,k) = g_kcld
-    C += -0.5 * G @ T                                          # dsymm R,U (:648)
+    C += symmetric_product(-0.5, G, T)                         # dsymm R,U (:648)
     x = mat_to_aibj(G, n_v, n_o)
     Gc = aibj_to_mat(2.0 * x.transpose(0, 3, 2, 1) - x)       # contra, psrq (:661)
-    D += 0.5 * Gc @ U                                          # dsymm R,L (:663)
-    W = -1.5 * T @ C                                           # dsymm L,U (:675)
+    D += symmetric_product(0.5, Gc, U)                         # dsymm R,L (:663)
+    W = symmetric_product(-1.5, T, C)                          # dsymm L,U (:675)
     out = symmetrize_and_pack(W, n_v, n_o, contra=True, psrq=True)
-    V = 0.5 * U @ D                                            # dsymm L,L (:689)
+    V = symmetric_product(0.5, U, D)                           # dsymm L,L (:689)
     out += symmetrize_and_pack(V, n_v, n_o)
     return out
 
