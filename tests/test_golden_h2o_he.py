@@ -1,0 +1,79 @@
+"""eT's own golden values for H2O-He / cc-pVDZ (tests/ccsd_gs_energy in the reference).
+
+The fixture tests/golden/h2o_he_ccpvdz.json holds the basis, geometry and the
+reference outputs (energies, the 19-iteration DIIS table, excitation energies)
+with their citations; tests/golden/make_h2o_he_fixture.py wrote it from eT's
+test files.  The molecular inputs come from oracle/frontend.py (integrals ->
+RHF -> pivoted Cholesky at eT's threshold 1e-12).
+
+CPU: the oracle reproduces E_nuc, E_HF, the number of Cholesky vectors, every
+row of eT's DIIS iteration table and E_CCSD.
+GPU: the same solve through the C ABI reproduces E_CCSD and the table.
+"""
+import functools
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import ccsd as O
+from oracle import frontend as fe
+
+FIXTURE = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden', 'h2o_he_ccpvdz.json')
+
+
+@functools.lru_cache(maxsize=1)
+def fixture():
+    return json.load(open(FIXTURE))
+
+
+@functools.lru_cache(maxsize=1)
+def inputs():
+    return fe.molecular_inputs(fixture())
+
+
+def test_reference_energies_and_cholesky_count():
+    g = fixture()['golden']
+    m = inputs()
+    assert abs(m['e_nuc'] - g['nuclear_repulsion']) < 1e-11
+    assert abs(m['e_hf'] - g['hf_energy']) < 1e-10
+    assert m['n_J'] == g['n_cholesky']
+    F0 = O.fock_t1(m['h_mo'], m['L_mo'], np.zeros((fixture()['n_v'], m['n_o'])), m['n_o'])
+    assert np.abs(F0 - np.diag(m['eps'])).max() < 1e-10      # canonical RHF through the Cholesky vectors
+
+
+def test_oracle_reproduces_et_diis_trajectory():
+    g = fixture()['golden']
+    m = inputs()
+    t1, t2, e, hist = O.solve_ground_state(m['L_mo'], m['h_mo'], m['eps'], m['n_o'],
+                                           residual_threshold=1e-10)
+    table = g['ccsd_diis_iterations']
+    assert len(hist) == len(table)
+    for (it, e_ref, norm_ref, _), (e_corr, norm) in zip(table, hist):
+        assert abs(m['e_hf'] + e_corr - e_ref) < 1e-10, it
+        # eT prints |omega| with 4 significant digits
+        assert abs(norm - norm_ref) <= 0.51e-4 * 10 ** np.floor(np.log10(norm_ref)) * 10, it
+    assert abs(m['e_hf'] + e - g['ccsd_energy']) < 1e-10
+    assert abs(e - g['ccsd_correlation_energy']) < 1e-10
+
+
+@pytest.mark.gpu
+def test_gpu_ground_state_reproduces_et():
+    from et_amd import ccsd, solvers
+    g = fixture()['golden']
+    m = inputs()
+    n_o = m['n_o']
+    n_v = m['L_mo'].shape[1] - n_o
+    with ccsd.CCSD(n_o, n_v, m['n_J']) as wf:
+        wf.set_cholesky(m['L_mo'])
+        wf.set_oei(m['h_mo'])
+        wf.set_orbital_energies(m['eps'])
+        wf.hf_energy = m['e_hf']
+        solver = solvers.DiisCCGroundState(wf, residual_threshold=1e-10)
+        e = solver.run()
+    table = g['ccsd_diis_iterations']
+    assert len(solver.history) == len(table)
+    for (it, e_ref, norm_ref, _), (e_it, norm, _) in zip(table, solver.history):
+        assert abs(e_it - e_ref) < 1e-10, it
+    assert abs(e - g['ccsd_energy']) < 1e-10
