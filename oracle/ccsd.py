@@ -39,7 +39,7 @@ def packin(full):
     """packin_real (reordering.F90:4267): lower triangle full[i, j], i >= j -> packed."""
     n = full.shape[0]
     i, j = np.tril_indices(n)
-    out = np.empty(n * (n + 1) // 2)
+    out = np.empty(n * (n + 1) // 2, dtype=full.dtype)
     out[i * (i + 1) // 2 + j] = full[i, j]
     return out
 
@@ -87,8 +87,9 @@ def eri(L, s1, s2, s3, s4):
 def t1_transform(Z, t1, n_o):
     """t1_transform_ccs (t1_ccs.F90:34-95): Z <- X Z Y^T, X = I - t1, Y = I + t1^T."""
     n = Z.shape[0]
-    X = np.eye(n)
-    Y = np.eye(n)
+    dt = np.result_type(Z, t1)
+    X = np.eye(n, dtype=dt)
+    Y = np.eye(n, dtype=dt)
     X[n_o:, :n_o] = -t1
     Y[:n_o, n_o:] = t1.T
     return X @ Z @ Y.T
@@ -104,7 +105,7 @@ def t1_cholesky(L_mo, t1, n_o):
     """
     o = slice(0, n_o)
     v = slice(n_o, L_mo.shape[1])
-    L = L_mo.copy()
+    L = L_mo.astype(np.result_type(L_mo, t1))
     L[:, o, o] = L_mo[:, o, o] + np.einsum('Jia,aj->Jij', L_mo[:, o, v], t1)
     Lp_ji = L_mo[:, o, o] + np.einsum('Jjb,bi->Jji', L_mo[:, o, v], t1)
     L[:, v, o] = (L_mo[:, v, o] + np.einsum('Jab,bi->Jai', L_mo[:, v, v], t1)
@@ -171,6 +172,12 @@ def orbital_differences(eps, n_o):
     d1 = (e_v[:, None] - e_o[None, :])
     d2 = d1[:, :, None, None] + d1[None, None, :, :]
     return np.concatenate([d1.ravel(order='F'), pack_aibj(d2)])
+
+
+def doubles_diagonal(n_t1):
+    """Positions of the doubles diagonal (ai = bj) in [t1 | packed t2]."""
+    d = np.arange(n_t1)
+    return n_t1 + d * (d + 1) // 2 + d
 
 
 def scale_doubles_diagonal(x, n_t1, factor=2.0):
@@ -333,10 +340,10 @@ def omega_a2_plus_minus(L_t1, t2, n_o, n_v):
     gm = (gA[c_m[:, None], d_m[:, None], c_m[None, :], d_m[None, :]]
           - gA[d_m[:, None], c_m[:, None], c_m[None, :], d_m[None, :]])
     wp = 0.5 * tp @ gp                                         # omega+_{ij,ab}
-    wm = 0.5 * tm @ gm if tm.size and gm.size else np.zeros((len(i_m), len(c_m)))
+    wm = 0.5 * tm @ gm if tm.size and gm.size else np.zeros((len(i_m), len(c_m)), dtype=tp.dtype)
     wp *= np.where(c_p == d_p, 0.5, 1.0)[None, :]              # a = b columns
     # add_double_packed_plus_minus: (a<=b, i<=j); omega(ai,bj) += w+ + w-, omega(aj,bi) += w+ - w-
-    out = np.zeros(n_t2(n_o, n_v))
+    out = np.zeros(n_t2(n_o, n_v), dtype=wp.dtype)
     ov = n_o * n_v
     P = packed_index(ov)
     a_, b_ = c_p[None, :], d_p[None, :]
@@ -460,3 +467,86 @@ def solve_ground_state(L_mo, h_mo, eps, n_o, residual_threshold=1e-10, max_itera
         t2 = amps[n_t1:]
         L_t1 = t1_cholesky(L_mo, t1, n_o)
     raise RuntimeError('Did not converge in the max number of iterations.')
+
+
+# ----------------------------------------------------------------------------
+# Jacobian transformations (jacobian_ccsd.F90:74-178, jacobian_transpose_ccsd.F90:129-236)
+#
+#   A_mu,nu = < mu~ | exp(-T) [H, tau_nu] exp(T) | R >,  tau_aibj = E_ai E_bj,
+#
+# so A = J D2 with J = d Omega / d t in eT's packed amplitude basis (Omega with
+# its 1/2 doubles diagonal, t2 packed with T2 = 1/2 sum t_aibj E_ai E_bj) and
+# D2 = diag(1, ..., 2 on the doubles diagonal) (eT scales c_aiai by two,
+# jacobian_ccsd.F90:123).  The right transform is rho = A c, the left sigma^T = b^T A.
+# Omega is a polynomial in t, so the complex step i h d/dt is exact to rounding.
+# ----------------------------------------------------------------------------
+
+def residual(L_mo, h_mo, t1, t2, n_o):
+    """Omega(t1, t2) including the T1 dressing and Fock matrix of the current t1."""
+    Lt = t1_cholesky(L_mo, t1, n_o)
+    F = fock_t1(h_mo, Lt, t1, n_o)
+    return omega(L_mo, Lt, F, t1, t2, n_o)
+
+
+def jacobian_transform(L_mo, h_mo, t1, t2, n_o, c, step=1e-30):
+    """rho = A c by complex-step differentiation of the residual along D2 c."""
+    n_v = L_mo.shape[1] - n_o
+    n_t1 = n_o * n_v
+    c = np.asarray(c, dtype=float).copy()
+    c[doubles_diagonal(n_t1)] *= 2.0
+    t1c = t1 + 1j * step * c[:n_t1].reshape(n_v, n_o, order='F')
+    t2c = t2 + 1j * step * c[n_t1:]
+    return residual(L_mo, h_mo, t1c, t2c, n_o).imag / step
+
+
+def jacobian_matrix(L_mo, h_mo, t1, t2, n_o):
+    """Full A (n_t1 + n_t2 square) column by column (tiny systems only)."""
+    n_v = L_mo.shape[1] - n_o
+    n = n_o * n_v + n_t2(n_o, n_v)
+    A = np.empty((n, n))
+    e = np.zeros(n)
+    for k in range(n):
+        e[k] = 1.0
+        A[:, k] = jacobian_transform(L_mo, h_mo, t1, t2, n_o, e)
+        e[k] = 0.0
+    return A
+
+
+def davidson_lowest(transform, diagonal, n_roots, tol=1e-10, max_iter=100):
+    """Lowest eigenvalues of a non-symmetric operator by a Davidson iteration
+    (solver_tools/davidson: trial vectors orthonormalised, reduced problem by
+    a dense eigensolver, residuals preconditioned by the orbital differences)."""
+    n = diagonal.size
+    start = np.argsort(diagonal)[:n_roots]
+    V = np.zeros((n, 0))
+    AV = np.zeros((n, 0))
+    new = []
+    for k in start:
+        e = np.zeros(n)
+        e[k] = 1.0
+        new.append(e)
+    for it in range(max_iter):
+        for x in new:
+            for _ in range(2):
+                x = x - V @ (V.T @ x)
+            nx = np.linalg.norm(x)
+            if nx < 1e-10:
+                continue
+            x = x / nx
+            V = np.column_stack([V, x])
+            AV = np.column_stack([AV, transform(x)])
+        M = V.T @ AV
+        w, S = np.linalg.eig(M)
+        order = np.argsort(w.real)[:n_roots]
+        w, S = w[order].real, S[:, order].real
+        new = []
+        done = True
+        for r in range(n_roots):
+            x = V @ S[:, r]
+            res = AV @ S[:, r] - w[r] * x
+            if np.linalg.norm(res) > tol:
+                done = False
+                new.append(res / (w[r] - diagonal + 1e-12))
+        if done:
+            return w
+    raise RuntimeError('Davidson did not converge')

@@ -77,3 +77,24 @@ def test_gpu_ground_state_reproduces_et():
     for (it, e_ref, norm_ref, _), (e_it, norm, _) in zip(table, solver.history):
         assert abs(e_it - e_ref) < 1e-10, it
     assert abs(e - g['ccsd_energy']) < 1e-10
+
+
+@functools.lru_cache(maxsize=1)
+def ground_state():
+    m = inputs()
+    t1, t2, e, hist = O.solve_ground_state(m['L_mo'], m['h_mo'], m['eps'], m['n_o'],
+                                           residual_threshold=1e-11)
+    return t1, t2
+
+
+def test_oracle_jacobian_reproduces_et_excitation_energies():
+    """Lowest eigenvalues of the oracle's A (complex-step derivative of Omega along D2 c)
+    equal eT's EOM-CCSD excitation energies (right and left transformations share A)."""
+    g = fixture()['golden']
+    m = inputs()
+    t1, t2 = ground_state()
+    diag = O.orbital_differences(m['eps'], m['n_o'])
+    w = O.davidson_lowest(lambda x: O.jacobian_transform(m['L_mo'], m['h_mo'], t1, t2, m['n_o'], x),
+                          diag, 2, tol=1e-9)
+    np.testing.assert_allclose(w, g['right_excitation_energies'], rtol=0, atol=1e-10)
+    np.testing.assert_allclose(w, g['left_excitation_energies'], rtol=0, atol=1e-10)

@@ -126,3 +126,39 @@ def test_ground_state_converges_to_determinant_space_stationary_point():
     assert np.abs(R1).max() < 1e-10 and np.abs(R2).max() < 1e-10
     assert abs(E - bf.nuclear_free_reference_energy(h, g, n_o) - e) < 1e-12
     assert hist[0][1] > hist[-1][1]
+
+
+def test_complex_step_jacobian_equals_central_differences():
+    n_o, n_v = 2, 3
+    L, h, eps, t1, t2 = tiny(41, n_o, n_v, 6, t_sigma=0.05)
+    n = n_o * n_v + O.n_t2(n_o, n_v)
+    c = np.random.default_rng(3).normal(size=n)
+    rho = O.jacobian_transform(L, h, t1, t2, n_o, c)
+    cs = c.copy()
+    cs[O.doubles_diagonal(n_o * n_v)] *= 2.0
+    hstep = 1e-4
+    n_t1 = n_o * n_v
+
+    def om(x):
+        return O.residual(L, h, t1 + x * cs[:n_t1].reshape(n_v, n_o, order='F'), t2 + x * cs[n_t1:], n_o)
+    # fourth-order central difference (Omega is a low-degree polynomial in x)
+    fd = (8 * (om(hstep) - om(-hstep)) - (om(2 * hstep) - om(-2 * hstep))) / (12 * hstep)
+    np.testing.assert_allclose(rho, fd, rtol=0, atol=1e-9 * np.abs(rho).max())
+
+
+def test_jacobian_diagonal_convention_on_determinant_space():
+    """A_mu,nu = <mu~|[Hbar, E_ck E_dl]|HF>: for a doubles-diagonal nu (ck = dl) the operator
+    E_ck E_ck is twice the amplitude's T2 weight, hence the factor 2 on c's diagonal."""
+    n_o, n_v = 2, 2
+    L, h, eps, t1, t2 = tiny(43, n_o, n_v, 5)
+    n_t1 = n_o * n_v
+    n = n_t1 + O.n_t2(n_o, n_v)
+    e = np.zeros(n)
+    k = O.doubles_diagonal(n_t1)[1]
+    e[k] = 1.0
+    rho = O.jacobian_transform(L, h, t1, t2, n_o, e)
+    step = 1e-30
+    t2c = t2 + 0j
+    t2c[k - n_t1] += 2j * step       # amplitude change 2 eps <-> operator E_ai E_ai with weight eps
+    direct = O.residual(L, h, t1 + 0j, t2c, n_o).imag / step
+    np.testing.assert_allclose(rho, direct, rtol=0, atol=1e-13)
