@@ -50,6 +50,9 @@ _SIGNATURES = {
     'etg_set_ladder_batch': [ctypes.c_void_p, _i64],
     'etg_synchronize': [ctypes.c_void_p],
     'etg_last_omega_flops': [ctypes.c_void_p, _c_dbl_p],
+    'etg_prepare_for_jacobian': [ctypes.c_void_p],
+    'etg_jacobian_transformation': [ctypes.c_void_p, _c_dbl_p, _c_dbl_p],
+    'etg_jacobian_transpose_transformation': [ctypes.c_void_p, _c_dbl_p, _c_dbl_p],
     'etg_set_kernel_ledger': [ctypes.c_void_p, ctypes.c_int],
     'etg_kernel_ledger': [ctypes.c_void_p, _i64, ctypes.c_char_p, _i64, ctypes.POINTER(_i64),
                           _c_dbl_p, _c_dbl_p, ctypes.POINTER(_i64)],
@@ -231,6 +234,29 @@ class CCSD:
 
     def synchronize(self):
         _check(self._lib.etg_synchronize(self._h))
+
+    # ---- Jacobian transformations -------------------------------------------
+    def prepare_for_jacobian(self, side='right'):
+        """prepare_for_jacobian(_transpose) (jacobian_ccsd.F90:46-71): records the residual program."""
+        _check(self._lib.etg_prepare_for_jacobian(self._h))
+
+    def jacobian_transformation(self, c):
+        """rho = A c (jacobian_ccsd.F90:74-178)."""
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        if c.shape != (self.n_gs_amplitudes,):
+            raise ETGError('trial vector must have length n_t1 + n_t2')
+        rho = np.empty(self.n_gs_amplitudes)
+        _check(self._lib.etg_jacobian_transformation(self._h, _ptr(c), _ptr(rho)))
+        return rho
+
+    def jacobian_transpose_transformation(self, b):
+        """sigma^T = b^T A (jacobian_transpose_ccsd.F90:129-236)."""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        if b.shape != (self.n_gs_amplitudes,):
+            raise ETGError('trial vector must have length n_t1 + n_t2')
+        sigma = np.empty(self.n_gs_amplitudes)
+        _check(self._lib.etg_jacobian_transpose_transformation(self._h, _ptr(b), _ptr(sigma)))
+        return sigma
 
     def set_kernel_ledger(self, on):
         _check(self._lib.etg_set_kernel_ledger(self._h, 1 if on else 0))

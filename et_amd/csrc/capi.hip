@@ -155,6 +155,18 @@ int etg_last_omega_flops(etg_handle* h, double* flops) {
   });
 }
 
+int etg_prepare_for_jacobian(etg_handle* h) {
+  return guarded([&] { wf_of(h).prepare_for_jacobian(); });
+}
+
+int etg_jacobian_transformation(etg_handle* h, const double* c, double* rho) {
+  return guarded([&] { wf_of(h).jacobian_transform(c, rho); });
+}
+
+int etg_jacobian_transpose_transformation(etg_handle* h, const double* b, double* sigma) {
+  return guarded([&] { wf_of(h).jacobian_transpose_transform(b, sigma); });
+}
+
 int etg_set_kernel_ledger(etg_handle* h, int on) {
   return guarded([&] { wf_of(h).set_ledger(on != 0); });
 }

@@ -1,9 +1,12 @@
-// CCSD ground-state residual, energy, T1 transformation and Fock matrix on
-// one MI355X.  The sequence of contractions follows eT's factorisation
-// (src/wavefunctions/ccsd/omega_ccsd.F90 and the doubles/ccs submodules it
-// calls); every product runs on the matrix-core kernels in gemm.hip and every
-// reorder on the bandwidth kernels in kernels.hip.  All tensors stay in HBM
-// between calls; host pointers are only read/written at the boundary.
+// CCSD ground-state residual, energy, T1 transformation, Fock matrix and the
+// Jacobian transformations on one MI355X.  The sequence of contractions
+// follows eT's factorisation (src/wavefunctions/ccsd/omega_ccsd.F90 and the
+// doubles/ccs submodules it calls); every product runs on the matrix-core
+// kernels in gemm.hip and every reorder on the bandwidth kernels in
+// kernels.hip.  The pipeline (T1 dressing -> Fock -> Omega) is written once
+// against the Ops interface: run directly it is the residual; recorded on a
+// Tape it is replayed forward for rho = A c and backward for sigma = A^T b.
+// All tensors stay in HBM between calls.
 #include "ccsd.hpp"
 
 #include <algorithm>
@@ -13,17 +16,13 @@
 
 namespace etg {
 
-namespace {
-enum Slot { S_T = 0, S_U, S_A, S_B, S_C, S_D, S_E, S_F, kSlots };
-}
-
 CCSD::CCSD(int64_t no, int64_t nv, int64_t nJ) : n_o(no), n_v(nv), n_J(nJ) {
   if (no < 1 || nv < 1 || nJ < 1) fail("etg_create: n_o, n_v and n_J must be positive");
   n_mo = n_o + n_v;
   n_t1 = n_o * n_v;
   n_t2 = n_t1 * (n_t1 + 1) / 2;
   ETG_HIP(hipStreamCreate(&stream_));
-  const int64_t J = n_J, o = n_o, v = n_v;
+  const int64_t J = n_J, o = n_o, v = n_v, N = n_t1, n = n_mo;
   Lmo_oo_.resize(sizeof(double) * J * o * o);
   Lmo_ov_.resize(sizeof(double) * J * o * v);
   Lmo_vo_.resize(sizeof(double) * J * v * o);
@@ -32,21 +31,20 @@ CCSD::CCSD(int64_t no, int64_t nv, int64_t nJ) : n_o(no), n_v(nv), n_J(nJ) {
   Lt_vo_.resize(sizeof(double) * J * v * o);
   Lt_vv_.resize(sizeof(double) * J * v * v);
   Lt_vvT_.resize(sizeof(double) * J * v * v);
-  h_.resize(sizeof(double) * n_mo * n_mo);
-  eps_.resize(sizeof(double) * n_mo);
-  F_.resize(sizeof(double) * n_mo * n_mo);
+  h_.resize(sizeof(double) * n * n);
+  eps_.resize(sizeof(double) * n);
+  F_.resize(sizeof(double) * n * n);
   t_.resize(sizeof(double) * (n_t1 + n_t2));
   omega_.resize(sizeof(double) * (n_t1 + n_t2));
-  g_ovov_.resize(sizeof(double) * n_t1 * n_t1);
+  g_ovov_.resize(sizeof(double) * N * N);
   red_partial_.resize(sizeof(double) * kReduceBlocks);
   red_result_.resize(sizeof(double) * 8);
-  work_.resize(kSlots);
-  // small scratch, sized once for every use below
-  const int64_t N = n_t1, n = n_mo;
-  tmp_small_[0].resize(sizeof(double) * n * n);
-  tmp_small_[1].resize(sizeof(double) * n * n);
-  tmp_small_[2].resize(sizeof(double) * std::max(std::max(n * n, J), o * o * o * o));
-  tmp_small_[3].resize(sizeof(double) * std::max(std::max(J * N, J * o * o), o * o * o * o));
+  work_.resize(kBigSlots);
+  // small scratch, sized once for every use in the programs below
+  small_[0].resize(sizeof(double) * n * n);
+  small_[1].resize(sizeof(double) * n * n);
+  small_[2].resize(sizeof(double) * std::max(std::max(n * n, J), o * o * o * o));
+  small_[3].resize(sizeof(double) * std::max(std::max(J * N, J * o * o), o * o * o * o));
   // default ladder batch: G blocks of at most ~12 GB
   const double budget = 12e9 / 8.0;
   int64_t nb = int64_t(std::sqrt(budget) / double(v));
@@ -71,12 +69,22 @@ void CCSD::require(bool ok, const char* what) const {
 
 void CCSD::synchronize() { ETG_HIP(hipStreamSynchronize(stream_)); }
 
-double* CCSD::work(int slot) {
-  // (ov)^2 elements, or more when n_o > n_v (the o^3 v and o^4 integral blocks share the slots)
-  const int64_t elems = std::max(n_t1 * n_t1, n_o * n_o * n_o * std::max(n_o, n_v));
-  work_[slot].resize(sizeof(double) * elems);
-  return work_[slot].as<double>();
+double* CCSD::scratch(int slot, int64_t elems, const char* name) {
+  if (slot < kBigSlots) {
+    // (ov)^2 elements, or more when n_o > n_v (o^3 v and o^4 blocks share the slots)
+    const int64_t cap = std::max(n_t1 * n_t1, n_o * n_o * n_o * std::max(n_o, n_v));
+    if (elems > cap) fail(std::string("scratch slot too small for ") + name);
+    work_[size_t(slot)].resize(sizeof(double) * cap);
+    return work_[size_t(slot)].as<double>();
+  }
+  DeviceBuffer& b = small_[slot - SL_SMALL0];
+  if (size_t(elems) * sizeof(double) > b.bytes()) fail(std::string("small scratch too small for ") + name);
+  return b.as<double>();
 }
+
+// ---------------------------------------------------------------------------
+// products and the per-kernel ledger
+// ---------------------------------------------------------------------------
 
 hipEvent_t CCSD::take_event() {
   if (!event_pool_.empty()) {
@@ -87,11 +95,6 @@ hipEvent_t CCSD::take_event() {
   hipEvent_t e;
   ETG_HIP(hipEventCreate(&e));
   return e;
-}
-
-void CCSD::record_gemm(const GemmLaunch& info, double flops, hipEvent_t a, hipEvent_t b) {
-  ETG_HIP(hipEventRecord(b, stream_));
-  ledger_recs_.push_back(LedgerRec{info.kernel, flops, a, b});
 }
 
 void CCSD::set_ledger(bool on) {
@@ -123,35 +126,7 @@ std::vector<CCSD::LedgerEntry> CCSD::ledger() {
   return out;
 }
 
-void CCSD::gemm(char ta, char tb, int64_t m, int64_t n, int64_t k, double alpha, const double* A,
-                int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc,
-                Prec prec) {
-  const double flops = 2.0 * double(m) * double(n) * double(k);
-  hipEvent_t a = nullptr, b = nullptr;
-  if (ledger_on_) {
-    a = take_event();
-    b = take_event();
-    ETG_HIP(hipEventRecord(a, stream_));
-  }
-  const GemmLaunch info =
-      etg::gemm(stream_, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, prec, &gws_);
-  if (ledger_on_) record_gemm(info, flops, a, b);
-  last_omega_flops += flops;
-}
-
-void CCSD::gemm_batched(char ta, char tb, int64_t m, int64_t n, int64_t k, double alpha,
-                        const double* A, int64_t lda, int64_t sA, const double* B, int64_t ldb,
-                        int64_t sB, double beta, double* C, int64_t ldc, int64_t sC, int64_t batch,
-                        Prec prec) {
-  Gemm g;
-  g.ta = (ta == 'T');
-  g.tb = (tb == 'T');
-  g.m = m; g.n = n; g.k = k;
-  g.alpha = alpha; g.beta = beta;
-  g.A = A; g.lda = lda; g.strideA = sA;
-  g.B = B; g.ldb = ldb; g.strideB = sB;
-  g.C = C; g.ldc = ldc; g.strideC = sC;
-  g.batch = batch;
+void CCSD::run_gemm(const Gemm& g, Prec prec) {
   hipEvent_t a = nullptr, b = nullptr;
   if (ledger_on_) {
     a = take_event();
@@ -159,7 +134,10 @@ void CCSD::gemm_batched(char ta, char tb, int64_t m, int64_t n, int64_t k, doubl
     ETG_HIP(hipEventRecord(a, stream_));
   }
   const GemmLaunch info = etg::gemm(stream_, g, prec, &gws_);
-  if (ledger_on_) record_gemm(info, gemm_flops(g), a, b);
+  if (ledger_on_) {
+    ETG_HIP(hipEventRecord(b, stream_));
+    ledger_recs_.push_back(LedgerRec{info.kernel, gemm_flops(g), a, b});
+  }
   last_omega_flops += gemm_flops(g);
 }
 
@@ -180,14 +158,16 @@ void copy_block(hipStream_t s, const double* L, int64_t nJ, int64_t n_mo, int64_
 
 void CCSD::set_cholesky(const double* L) {
   require(L != nullptr, "etg_set_cholesky: null pointer");
+  tape_.reset();
   const int64_t o = n_o, v = n_v, J = n_J;
   copy_block(stream_, L, J, n_mo, 0, o, 0, o, Lmo_oo_.as<double>());
   copy_block(stream_, L, J, n_mo, 0, o, o, v, Lmo_ov_.as<double>());
   copy_block(stream_, L, J, n_mo, o, v, 0, o, Lmo_vo_.as<double>());
   copy_block(stream_, L, J, n_mo, o, v, o, v, Lmo_vv_.as<double>());
   // g_ovov = L_ov^T L_ov depends only on the undressed ov block
-  gemm('T', 'N', o * v, o * v, J, 1.0, Lmo_ov_.as<double>(), J, Lmo_ov_.as<double>(), J, 0.0,
-       g_ovov_.as<double>(), o * v, Prec::F64);
+  DirectOps X(*this);
+  X.gemm('T', 'N', o * v, o * v, J, 1.0, Lmo_ov_.as<double>(), J, Lmo_ov_.as<double>(), J, 0.0,
+         g_ovov_.as<double>(), o * v, Prec::F64);
   have_L_ = true;
   if (!have_t_) {
     fill(stream_, t_.as<double>(), n_t1 + n_t2, 0.0);
@@ -199,10 +179,12 @@ void CCSD::set_cholesky(const double* L) {
 
 void CCSD::set_oei(const double* h) {
   require(h != nullptr, "etg_set_oei: null pointer");
+  tape_.reset();
   ETG_HIP(hipMemcpyAsync(h_.as<double>(), h, sizeof(double) * n_mo * n_mo, hipMemcpyHostToDevice,
                          stream_));
   synchronize();
   have_h_ = true;
+  have_F_ = false;
 }
 
 void CCSD::set_orbital_energies(const double* eps) {
@@ -215,6 +197,7 @@ void CCSD::set_orbital_energies(const double* eps) {
 
 void CCSD::set_amplitudes(const double* t) {
   require(t != nullptr, "etg_set_amplitudes: null pointer");
+  tape_.reset();
   ETG_HIP(hipMemcpyAsync(t_.as<double>(), t, sizeof(double) * (n_t1 + n_t2),
                          hipMemcpyHostToDevice, stream_));
   synchronize();
@@ -246,30 +229,32 @@ void CCSD::get_fock(double* out) {
 // T1 transformation of the Cholesky vectors (t1_ccs.F90:208-499)
 // ---------------------------------------------------------------------------
 
-void CCSD::construct_t1_cholesky() {
-  require(have_L_, "construct_t1_cholesky: Cholesky vectors not set");
+void CCSD::program_t1_cholesky(Ops& X) {
   const int64_t o = n_o, v = n_v, J = n_J;
   const double* t1 = t_.as<double>();
-  hipStream_t s = stream_;
-  const size_t b_oo = sizeof(double) * J * o * o, b_vo = sizeof(double) * J * v * o,
-               b_vv = sizeof(double) * J * v * v;
+  double* Loo = Lt_oo_.as<double>();
+  double* Lvo = Lt_vo_.as<double>();
+  double* Lvv = Lt_vv_.as<double>();
   // oo: L_ij = L_ij + sum_a L_ia t_aj          ((J i) x a) (a x j)
-  ETG_HIP(hipMemcpyAsync(Lt_oo_.as<double>(), Lmo_oo_.as<double>(), b_oo, hipMemcpyDeviceToDevice, s));
-  gemm('N', 'N', J * o, o, v, 1.0, Lmo_ov_.as<double>(), J * o, t1, v, 1.0, Lt_oo_.as<double>(),
-       J * o, Prec::F64);
+  X.copy_sub(Lmo_oo_.as<double>(), J * o * o, Loo, J * o * o, J * o * o, 1, 1.0, 0.0);
+  X.gemm('N', 'N', J * o, o, v, 1.0, Lmo_ov_.as<double>(), J * o, t1, v, 1.0, Loo, J * o, Prec::F64);
   // vo: L_ai = L_ai + sum_b L_ab t_bi - sum_j t_aj L'_ji   (L' = dressed oo block)
-  ETG_HIP(hipMemcpyAsync(Lt_vo_.as<double>(), Lmo_vo_.as<double>(), b_vo, hipMemcpyDeviceToDevice, s));
-  gemm('N', 'N', J * v, o, v, 1.0, Lmo_vv_.as<double>(), J * v, t1, v, 1.0, Lt_vo_.as<double>(),
-       J * v, Prec::F64);
-  gemm_batched('N', 'T', J, v, o, -1.0, Lt_oo_.as<double>(), J, J * o, t1, v, 0, 1.0,
-               Lt_vo_.as<double>(), J, J * v, o, Prec::F64);
+  X.copy_sub(Lmo_vo_.as<double>(), J * v * o, Lvo, J * v * o, J * v * o, 1, 1.0, 0.0);
+  X.gemm('N', 'N', J * v, o, v, 1.0, Lmo_vv_.as<double>(), J * v, t1, v, 1.0, Lvo, J * v, Prec::F64);
+  X.gemm_batched('N', 'T', J, v, o, -1.0, Loo, J, J * o, t1, v, 0, 1.0, Lvo, J, J * v, o, Prec::F64);
   // vv: L_ab = L_ab - sum_j t_aj L_jb         batched over b: (J x j)(j x a)
-  ETG_HIP(hipMemcpyAsync(Lt_vv_.as<double>(), Lmo_vv_.as<double>(), b_vv, hipMemcpyDeviceToDevice, s));
-  gemm_batched('N', 'T', J, v, o, -1.0, Lmo_ov_.as<double>(), J, J * o, t1, v, 0, 1.0,
-               Lt_vv_.as<double>(), J, J * v, v, Prec::F64);
+  X.copy_sub(Lmo_vv_.as<double>(), J * v * v, Lvv, J * v * v, J * v * v, 1, 1.0, 0.0);
+  X.gemm_batched('N', 'T', J, v, o, -1.0, Lmo_ov_.as<double>(), J, J * o, t1, v, 0, 1.0, Lvv, J,
+                 J * v, v, Prec::F64);
   // (J, c, a) copy of the vv block: a-batches of the ladder become contiguous columns
   const int64_t d[4] = {J, v, v, 1};
-  sort4(s, Lt_vv_.as<double>(), d, "1324", Lt_vvT_.as<double>());
+  X.sort4(Lvv, d, "1324", Lt_vvT_.as<double>(), 1.0, 0.0);
+}
+
+void CCSD::construct_t1_cholesky() {
+  require(have_L_, "construct_t1_cholesky: Cholesky vectors not set");
+  DirectOps X(*this);
+  program_t1_cholesky(X);
   have_F_ = false;
 }
 
@@ -277,96 +262,50 @@ void CCSD::construct_t1_cholesky() {
 // T1 Fock matrix (fock_ccs.F90:242-1109): F = h^T1 + sum_k (2 g_pqkk - g_pkkq)
 // ---------------------------------------------------------------------------
 
-namespace {
-__global__ void t1_xy_kernel(const double* t1, double* X, double* Y, int64_t n_o, int64_t n_v) {
-  const int64_t n = n_o + n_v;
-  for (int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; k < n * n;
-       k += int64_t(gridDim.x) * blockDim.x) {
-    const int64_t p = k % n, q = k / n;
-    double x = (p == q) ? 1.0 : 0.0, y = x;
-    if (p >= n_o && q < n_o) x = -t1[(p - n_o) + n_v * q];  // X(o+a, i) = -t_ai
-    if (p < n_o && q >= n_o) y = t1[(q - n_o) + n_v * p];   // Y(i, o+a) = t_ai
-    X[k] = x;
-    Y[k] = y;
-  }
+void CCSD::program_fock(Ops& X) {
+  const int64_t o = n_o, v = n_v, J = n_J, n = n_mo;
+  double* F = F_.as<double>();
+  const double* Lov = Lmo_ov_.as<double>();
+  const double* Loo = Lt_oo_.as<double>();
+  // h^T1 = X h Y^T,  X = I - t1, Y = I + t1^T     (t1_ccs.F90:34-95, oei_ccs.F90:35-97)
+  double* Xm = X.buf(SL_SMALL0, "fock_X", n * n);
+  double* Ym = X.buf(SL_SMALL1, "fock_Y", n * n);
+  double* Z = X.buf(SL_SMALL2, "fock_hY", n * n);
+  X.t1_xy(t_.as<double>(), Xm, Ym);
+  X.gemm('N', 'T', n, n, n, 1.0, h_.as<double>(), n, Ym, n, 0.0, Z, n, Prec::F64);
+  X.gemm('N', 'N', n, n, n, 1.0, Xm, n, Z, n, 0.0, F, n, Prec::F64);
+  // Coulomb: F_pq += 2 sum_J L_pq X_J,  X_J = sum_k L_kk
+  double* XJ = X.buf(SL_SMALL2, "fock_XJ", J);
+  X.trace_oo(Loo, XJ);
+  double* c = X.buf(SL_SMALL0, "fock_coul_oo", o * o);
+  X.gemm('T', 'N', o * o, 1, J, 2.0, Loo, J, XJ, J, 0.0, c, o * o, Prec::F64);
+  X.copy_sub(c, o, F, n, o, o, 1.0, 1.0);
+  c = X.buf(SL_SMALL0, "fock_coul_ov", o * v);
+  X.gemm('T', 'N', o * v, 1, J, 2.0, Lov, J, XJ, J, 0.0, c, o * v, Prec::F64);
+  X.copy_sub(c, o, F + n * o, n, o, v, 1.0, 1.0);
+  c = X.buf(SL_SMALL0, "fock_coul_vo", v * o);
+  X.gemm('T', 'N', v * o, 1, J, 2.0, Lt_vo_.as<double>(), J, XJ, J, 0.0, c, v * o, Prec::F64);
+  X.copy_sub(c, v, F + o, n, v, o, 1.0, 1.0);
+  c = X.buf(SL_SMALL0, "fock_coul_vv", v * v);
+  X.gemm('T', 'N', v * v, 1, J, 2.0, Lt_vv_.as<double>(), J, XJ, J, 0.0, c, v * v, Prec::F64);
+  X.copy_sub(c, v, F + o + n * o, n, v, v, 1.0, 1.0);
+  // Exchange: F_pq -= sum_{J,k} L_pk L_kq with left factors reordered to (J, k, p)
+  double* Lleft = X.buf(SL_SMALL3, "fock_Lleft_oo", J * o * o);
+  const int64_t d_oo[4] = {J, o, o, 1};
+  X.sort4(Loo, d_oo, "1324", Lleft, 1.0, 0.0);                        // (J, k, i)
+  X.gemm('T', 'N', o, o, J * o, -1.0, Lleft, J * o, Loo, J * o, 1.0, F, n, Prec::F64);
+  X.gemm('T', 'N', o, v, J * o, -1.0, Lleft, J * o, Lov, J * o, 1.0, F + n * o, n, Prec::F64);
+  Lleft = X.buf(SL_SMALL3, "fock_Lleft_vo", J * o * v);
+  const int64_t d_vo[4] = {J, v, o, 1};
+  X.sort4(Lt_vo_.as<double>(), d_vo, "1324", Lleft, 1.0, 0.0);       // (J, k, a)
+  X.gemm('T', 'N', v, o, J * o, -1.0, Lleft, J * o, Loo, J * o, 1.0, F + o, n, Prec::F64);
+  X.gemm('T', 'N', v, v, J * o, -1.0, Lleft, J * o, Lov, J * o, 1.0, F + o + n * o, n, Prec::F64);
 }
-
-__global__ void trace_oo_kernel(const double* Loo, int64_t nJ, int64_t n_o, double* XJ) {
-  for (int64_t J = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; J < nJ;
-       J += int64_t(gridDim.x) * blockDim.x) {
-    double s = 0.0;
-    for (int64_t k = 0; k < n_o; ++k) s += Loo[J + nJ * (k + n_o * k)];
-    XJ[J] = s;
-  }
-}
-
-// vector f(b + v j) = F(j, o + b)
-__global__ void fock_ov_vector_kernel(const double* F, int64_t n_mo, int64_t n_o, int64_t n_v,
-                                      double* f) {
-  for (int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; k < n_o * n_v;
-       k += int64_t(gridDim.x) * blockDim.x) {
-    const int64_t b = k % n_v, j = k / n_v;
-    f[k] = F[j + n_mo * (n_o + b)];
-  }
-}
-
-// dst(r, c) = alpha * src(r, c) + beta * dst   for an m x n sub-matrix
-__global__ void copy_sub_kernel(const double* src, int64_t lds, double* dst, int64_t ldd, int64_t m,
-                                int64_t n, double alpha, double beta) {
-  for (int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; k < m * n;
-       k += int64_t(gridDim.x) * blockDim.x) {
-    const int64_t r = k % m, c = k / m;
-    const double v = alpha * src[r + lds * c];
-    dst[r + ldd * c] = (beta == 0.0) ? v : v + beta * dst[r + ldd * c];
-  }
-}
-
-void copy_sub(hipStream_t s, const double* src, int64_t lds, double* dst, int64_t ldd, int64_t m,
-              int64_t n, double alpha = 1.0, double beta = 0.0) {
-  const int64_t tot = m * n;
-  if (tot == 0) return;
-  copy_sub_kernel<<<unsigned(std::min<int64_t>(ceil_div(tot, 256), 1 << 20)), 256, 0, s>>>(
-      src, lds, dst, ldd, m, n, alpha, beta);
-  ETG_LAUNCH("copy_sub");
-}
-}  // namespace
 
 void CCSD::construct_fock() {
   require(have_L_ && have_h_, "construct_fock: Cholesky vectors and one-electron integrals required");
-  const int64_t o = n_o, v = n_v, J = n_J, n = n_mo;
-  hipStream_t s = stream_;
-  double* X = tmp_small_[0].as<double>();
-  double* Y = tmp_small_[1].as<double>();
-  double* Z = tmp_small_[2].as<double>();
-  double* F = F_.as<double>();
-  // h^T1 = X h Y^T
-  t1_xy_kernel<<<unsigned(ceil_div(n * n, 256)), 256, 0, s>>>(t_.as<double>(), X, Y, o, v);
-  ETG_LAUNCH("t1_xy");
-  gemm('N', 'N', n, n, n, 1.0, X, n, h_.as<double>(), n, 0.0, Z, n, Prec::F64);
-  gemm('N', 'T', n, n, n, 1.0, Z, n, Y, n, 0.0, F, n, Prec::F64);
-  // Coulomb: F_pq += 2 sum_J L_pq X_J,  X_J = sum_k L_kk
-  double* XJ = tmp_small_[2].as<double>();
-  trace_oo_kernel<<<unsigned(ceil_div(J, 256)), 256, 0, s>>>(Lt_oo_.as<double>(), J, o, XJ);
-  ETG_LAUNCH("trace_oo");
-  const double* Lov = Lmo_ov_.as<double>();
-  gemm('T', 'N', o * o, 1, J, 2.0, Lt_oo_.as<double>(), J, XJ, J, 0.0, X, o * o, Prec::F64);
-  copy_sub(s, X, o, F, n, o, o, 1.0, 1.0);
-  gemm('T', 'N', o * v, 1, J, 2.0, Lov, J, XJ, J, 0.0, X, o * v, Prec::F64);
-  copy_sub(s, X, o, F + n * o, n, o, v, 1.0, 1.0);
-  gemm('T', 'N', v * o, 1, J, 2.0, Lt_vo_.as<double>(), J, XJ, J, 0.0, X, v * o, Prec::F64);
-  copy_sub(s, X, v, F + o, n, v, o, 1.0, 1.0);
-  gemm('T', 'N', v * v, 1, J, 2.0, Lt_vv_.as<double>(), J, XJ, J, 0.0, X, v * v, Prec::F64);
-  copy_sub(s, X, v, F + o + n * o, n, v, v, 1.0, 1.0);
-  // Exchange: F_pq -= sum_{J,k} L_pk L_kq with left factors reordered to (J, k, p)
-  double* Lleft = tmp_small_[3].as<double>();
-  const int64_t d_oo[4] = {J, o, o, 1};
-  sort4(s, Lt_oo_.as<double>(), d_oo, "1324", Lleft);               // (J, k, i)
-  gemm('T', 'N', o, o, J * o, -1.0, Lleft, J * o, Lt_oo_.as<double>(), J * o, 1.0, F, n, Prec::F64);
-  gemm('T', 'N', o, v, J * o, -1.0, Lleft, J * o, Lov, J * o, 1.0, F + n * o, n, Prec::F64);
-  const int64_t d_vo[4] = {J, v, o, 1};
-  sort4(s, Lt_vo_.as<double>(), d_vo, "1324", Lleft);               // (J, k, a)
-  gemm('T', 'N', v, o, J * o, -1.0, Lleft, J * o, Lt_oo_.as<double>(), J * o, 1.0, F + o, n, Prec::F64);
-  gemm('T', 'N', v, v, J * o, -1.0, Lleft, J * o, Lov, J * o, 1.0, F + o + n * o, n, Prec::F64);
+  DirectOps X(*this);
+  program_fock(X);
   have_F_ = true;
 }
 
@@ -396,10 +335,12 @@ double CCSD::omega_norm() {
 
 void CCSD::set_t2_to_mp2_guess() {
   require(have_L_ && have_eps_, "set_t2_to_mp2_guess: Cholesky vectors and orbital energies required");
+  tape_.reset();
   const int64_t N = n_t1;
-  double* g = work(S_A);
-  gemm('T', 'N', N, N, n_J, 1.0, Lt_vo_.as<double>(), n_J, Lt_vo_.as<double>(), n_J, 0.0, g, N,
-       Prec::F64);
+  double* g = scratch(SL_A, N * N, "mp2_g_vovo");
+  DirectOps X(*this);
+  X.gemm('T', 'N', N, N, n_J, 1.0, Lt_vo_.as<double>(), n_J, Lt_vo_.as<double>(), n_J, 0.0, g, N,
+         Prec::F64);
   mp2_guess(stream_, g, eps_.as<double>(), t_.as<double>() + n_t1, n_o, n_v);
   synchronize();
   have_t_ = true;
@@ -419,137 +360,134 @@ void CCSD::get_orbital_differences(double* host, int64_t n) {
 // Omega (omega_ccsd.F90:39-110)
 // ---------------------------------------------------------------------------
 
+void CCSD::program_omega(Ops& X, Prec prec) {
+  const int64_t N = n_t1;
+  // t_aibj (full) and u_aibj = 2 t_aibj - t_ajbi      (omega_ccsd.F90:881-914)
+  double* T = X.buf(SL_T, "t_aibj", N * N);
+  double* U = X.buf(SL_U, "u_aibj", N * N);
+  X.squareup(t_.as<double>() + n_t1, T, N);
+  const int64_t d[4] = {n_v, n_o, n_v, n_o};
+  X.sort4(T, d, "1234", U, 2.0, 0.0);
+  X.sort4(T, d, "1432", U, -1.0, 1.0);
+  // singles: F_ai + doubles A1, B1, C1
+  omega_singles(X, prec, U);
+  // doubles: E2, symmetric sum, B2, 1/2 diagonal, pack, then C2/D2 and A2
+  omega_e2_b2(X, prec, T, U);
+  omega_c2_d2(X, prec, T, U);
+  X.ladder(t_.as<double>() + n_t1, Lt_vvT_.as<double>(), omega_.as<double>() + n_t1, prec);
+}
+
 void CCSD::construct_omega(Prec prec) {
   require(have_L_ && have_t_, "construct_omega: Cholesky vectors and amplitudes required");
+  DirectOps X(*this);
   if (!have_F_) construct_fock();
   last_omega_flops = 0.0;
-  const int64_t N = n_t1;
-  hipStream_t s = stream_;
-  // t_aibj (full) and u_aibj = 2 t_aibj - t_ajbi      (omega_ccsd.F90:881-914)
-  double* T = work(S_T);
-  double* U = work(S_U);
-  squareup(s, t_.as<double>() + n_t1, T, N);
-  const int64_t d[4] = {n_v, n_o, n_v, n_o};
-  sort4(s, T, d, "1234", U, 2.0, 0.0);
-  sort4(s, T, d, "1432", U, -1.0, 1.0);
-  // singles: F_ai + doubles A1, B1, C1
-  omega_singles(prec);
-  // doubles: E2, symmetric sum, B2, 1/2 diagonal, pack, then C2/D2 and A2
-  omega_doubles_e2_b2(prec);
-  omega_c2_d2(prec);
-  omega_a2(prec);
+  program_omega(X, prec);
 }
 
-void CCSD::omega_singles(Prec prec) {
+void CCSD::omega_singles(Ops& X, Prec prec, const double* U) {
   const int64_t o = n_o, v = n_v, J = n_J, N = n_t1, n = n_mo;
-  hipStream_t s = stream_;
   double* om1 = omega_.as<double>();
-  const double* U = work(S_U);
   // Omega_ai = F_ai                                    (omega_ccs.F90:62-86)
-  copy_sub(s, F_.as<double>() + o, n, om1, v, v, o);
+  X.copy_sub(F_.as<double>() + o, n, om1, v, v, o, 1.0, 0.0);
   // A1: X_Jbi = sum_cj L^mo_Jcj u_cjbi ; Omega_ai += sum_Jb L^T1_Jab X_Jbi   (omega_doubles.F90:36-139)
-  double* X = tmp_small_[3].as<double>();
-  gemm('N', 'N', J, N, N, 1.0, Lmo_vo_.as<double>(), J, U, N, 0.0, X, J, prec);
-  gemm('T', 'N', v, o, J * v, 1.0, Lt_vvT_.as<double>(), J * v, X, J * v, 1.0, om1, v, prec);
+  double* Xj = X.buf(SL_SMALL3, "a1_X", J * N);
+  X.gemm('N', 'N', J, N, N, 1.0, Lmo_vo_.as<double>(), J, U, N, 0.0, Xj, J, prec);
+  X.gemm('T', 'N', v, o, J * v, 1.0, Lt_vvT_.as<double>(), J * v, Xj, J * v, 1.0, om1, v, prec);
   // B1: Omega_ai -= sum_jbk u_ajbk g_kbji             (omega_doubles.F90:141-205)
-  double* g_kbji = work(S_A);
-  double* g_jbki = work(S_B);
-  gemm('T', 'N', o * v, o * o, J, 1.0, Lmo_ov_.as<double>(), J, Lt_oo_.as<double>(), J, 0.0,
-       g_kbji, o * v, prec);
+  double* g_kbji = X.buf(SL_A, "b1_g_kbji", o * v * o * o);
+  double* g_jbki = X.buf(SL_B, "b1_g_jbki", o * v * o * o);
+  X.gemm('T', 'N', o * v, o * o, J, 1.0, Lmo_ov_.as<double>(), J, Lt_oo_.as<double>(), J, 0.0,
+         g_kbji, o * v, prec);
   const int64_t d[4] = {o, v, o, o};
-  sort4(s, g_kbji, d, "3214", g_jbki);
-  gemm('N', 'N', v, o, o * o * v, -1.0, U, v, g_jbki, o * o * v, 1.0, om1, v, prec);
+  X.sort4(g_kbji, d, "3214", g_jbki, 1.0, 0.0);
+  X.gemm('N', 'N', v, o, o * o * v, -1.0, U, v, g_jbki, o * o * v, 1.0, om1, v, prec);
   // C1: Omega_ai += sum_bj u_aibj F_jb                (omega_doubles.F90:207-250)
-  double* f = work(S_C);
-  fock_ov_vector_kernel<<<unsigned(ceil_div(N, 256)), 256, 0, s>>>(F_.as<double>(), n, o, v, f);
-  ETG_LAUNCH("fock_ov_vector");
-  gemm('N', 'N', N, 1, N, 1.0, U, N, f, N, 1.0, om1, N, prec);
+  double* f = X.buf(SL_C, "c1_f", N);
+  X.fock_ov_vec(F_.as<double>(), f);
+  X.gemm('N', 'N', N, 1, N, 1.0, U, N, f, N, 1.0, om1, N, prec);
 }
 
-void CCSD::omega_doubles_e2_b2(Prec prec) {
+void CCSD::omega_e2_b2(Ops& X, Prec prec, const double* T, const double* U) {
   const int64_t o = n_o, v = n_v, J = n_J, N = n_t1, n = n_mo;
-  hipStream_t s = stream_;
-  const double* T = work(S_T);
-  const double* U = work(S_U);
   const double* F = F_.as<double>();
   const double* g_ovov = g_ovov_.as<double>();
   const int64_t d[4] = {v, o, v, o};
   // E2 (omega_ccsd.F90:712-878)
-  double* u_e2 = work(S_A);                 // u_kl^bd = 2 t_bkdl - t_bldk = u(b,k,d,l)
-  sort4(s, U, d, "1432", u_e2);
-  double* om = work(S_B);
-  fill(s, om, N * N, 0.0);
-  double* Xbc = tmp_small_[0].as<double>();
-  double* Ykj = tmp_small_[1].as<double>();
-  copy_sub(s, F + o + n * o, n, Xbc, v, v, v);
-  gemm('N', 'N', v, v, o * v * o, -1.0, u_e2, v, g_ovov, o * v * o, 1.0, Xbc, v, prec);
-  gemm('N', 'N', v, o * v * o, v, 1.0, Xbc, v, T, v, 1.0, om, v, prec);
-  copy_sub(s, F, n, Ykj, o, o, o);
-  gemm('N', 'N', o, o, v * o * v, 1.0, g_ovov, o, u_e2, v * o * v, 1.0, Ykj, o, prec);
-  gemm('N', 'N', v * o * v, o, o, -1.0, T, v * o * v, Ykj, o, 1.0, om, v * o * v, prec);
+  double* u_e2 = X.buf(SL_A, "e2_u", N * N);   // u_kl^bd = 2 t_bkdl - t_bldk = u(b,k,d,l)
+  X.sort4(U, d, "1432", u_e2, 1.0, 0.0);
+  double* om = X.buf(SL_B, "e2_omega", N * N);
+  X.fill0(om, N * N);
+  double* Xbc = X.buf(SL_SMALL0, "e2_X", v * v);
+  double* Ykj = X.buf(SL_SMALL1, "e2_Y", o * o);
+  X.copy_sub(F + o + n * o, n, Xbc, v, v, v, 1.0, 0.0);
+  X.gemm('N', 'N', v, v, o * v * o, -1.0, u_e2, v, g_ovov, o * v * o, 1.0, Xbc, v, prec);
+  X.gemm('N', 'N', v, o * v * o, v, 1.0, Xbc, v, T, v, 1.0, om, v, prec);
+  X.copy_sub(F, n, Ykj, o, o, o, 1.0, 0.0);
+  X.gemm('N', 'N', o, o, v * o * v, 1.0, g_ovov, o, u_e2, v * o * v, 1.0, Ykj, o, prec);
+  X.gemm('N', 'N', v * o * v, o, o, -1.0, T, v * o * v, Ykj, o, 1.0, om, v * o * v, prec);
   // symmetric sum
-  double* om2 = work(S_C);
-  symmetric_sum(s, om, om2, N);
+  double* om2 = X.buf(SL_C, "e2_omega_sym", N * N);
+  X.symsum(om, om2, N);
   // B2 in (a,b,i,j) order (omega_ccsd.F90:483-583)
-  double* om_abij = work(S_A);
-  sort4(s, om2, d, "1324", om_abij);
-  double* t_abij = work(S_B);
-  sort4(s, T, d, "1324", t_abij);
-  double* g_vovo = work(S_C);
-  gemm('T', 'N', N, N, J, 1.0, Lt_vo_.as<double>(), J, Lt_vo_.as<double>(), J, 0.0, g_vovo, N, prec);
-  sort4(s, g_vovo, d, "1324", om_abij, 1.0, 1.0);
-  double* g_kilj = tmp_small_[3].as<double>();
-  double* X_klij = tmp_small_[2].as<double>();
-  gemm('T', 'N', o * o, o * o, J, 1.0, Lt_oo_.as<double>(), J, Lt_oo_.as<double>(), J, 0.0, g_kilj,
-       o * o, prec);
+  double* om_abij = X.buf(SL_A, "b2_omega_abij", N * N);
+  X.sort4(om2, d, "1324", om_abij, 1.0, 0.0);
+  double* t_abij = X.buf(SL_B, "b2_t_abij", N * N);
+  X.sort4(T, d, "1324", t_abij, 1.0, 0.0);
+  double* g_vovo = X.buf(SL_C, "b2_g_vovo", N * N);
+  X.gemm('T', 'N', N, N, J, 1.0, Lt_vo_.as<double>(), J, Lt_vo_.as<double>(), J, 0.0, g_vovo, N, prec);
+  X.sort4(g_vovo, d, "1324", om_abij, 1.0, 1.0);
+  double* g_kilj = X.buf(SL_SMALL3, "b2_g_kilj", o * o * o * o);
+  double* X_klij = X.buf(SL_SMALL2, "b2_X", o * o * o * o);
+  X.gemm('T', 'N', o * o, o * o, J, 1.0, Lt_oo_.as<double>(), J, Lt_oo_.as<double>(), J, 0.0, g_kilj,
+         o * o, prec);
   const int64_t d_oooo[4] = {o, o, o, o};
-  sort4(s, g_kilj, d_oooo, "1324", X_klij);
-  double* g_klcd = work(S_C);
+  X.sort4(g_kilj, d_oooo, "1324", X_klij, 1.0, 0.0);
+  double* g_klcd = X.buf(SL_C, "b2_g_klcd", N * N);
   const int64_t d_ovov[4] = {o, v, o, v};
-  sort4(s, g_ovov, d_ovov, "1324", g_klcd);
-  gemm('N', 'N', o * o, o * o, v * v, 1.0, g_klcd, o * o, t_abij, v * v, 1.0, X_klij, o * o, prec);
-  gemm('N', 'N', v * v, o * o, o * o, 1.0, t_abij, v * v, X_klij, o * o, 1.0, om_abij, v * v, prec);
+  X.sort4(g_ovov, d_ovov, "1324", g_klcd, 1.0, 0.0);
+  X.gemm('N', 'N', o * o, o * o, v * v, 1.0, g_klcd, o * o, t_abij, v * v, 1.0, X_klij, o * o, prec);
+  X.gemm('N', 'N', v * v, o * o, o * o, 1.0, t_abij, v * v, X_klij, o * o, 1.0, om_abij, v * v, prec);
   // back to (a,i,b,j), 1/2 on the diagonal and pack into omega2   (omega_ccsd.F90:97-102)
-  double* om_aibj = work(S_C);
+  double* om_aibj = X.buf(SL_C, "b2_omega_aibj", N * N);
   const int64_t d_vvoo[4] = {v, v, o, o};
-  sort4(s, om_abij, d_vvoo, "1324", om_aibj);
-  packin(s, om_aibj, omega_.as<double>() + n_t1, N, 1.0, 0.0, 0.5);
+  X.sort4(om_abij, d_vvoo, "1324", om_aibj, 1.0, 0.0);
+  X.packin(om_aibj, omega_.as<double>() + n_t1, N, 1.0, 0.0, 0.5);
 }
 
-void CCSD::omega_c2_d2(Prec prec) {
+void CCSD::omega_c2_d2(Ops& X, Prec prec, const double* Tfull, const double* U) {
   // omega_ccsd.F90:586-709: four (ov) x (ov) products
   const int64_t o = n_o, v = n_v, J = n_J, N = n_t1;
-  hipStream_t s = stream_;
-  const double* Tfull = work(S_T);
-  const double* U = work(S_U);
   const int64_t d[4] = {v, o, v, o};
-  double* Tm = work(S_A);                    // T[(p,q),(r,s)] = t_psrq
-  sort4(s, Tfull, d, "1432", Tm);
-  double* gtmp = work(S_B);
-  double* C = work(S_C);
-  double* D = work(S_D);
-  gemm('T', 'N', v * v, o * o, J, 1.0, Lt_vv_.as<double>(), J, Lt_oo_.as<double>(), J, 0.0, gtmp,
-       v * v, prec);                         // g_abij
+  double* Tm = X.buf(SL_A, "c2_T", N * N);                // T[(p,q),(r,s)] = t_psrq
+  X.sort4(Tfull, d, "1432", Tm, 1.0, 0.0);
+  double* g_vvoo = X.buf(SL_B, "c2_g_vvoo", N * N);
+  double* C = X.buf(SL_C, "c2_C", N * N);
+  double* D = X.buf(SL_D, "c2_D", N * N);
+  X.gemm('T', 'N', v * v, o * o, J, 1.0, Lt_vv_.as<double>(), J, Lt_oo_.as<double>(), J, 0.0, g_vvoo,
+         v * v, prec);                                       // g_abij
   const int64_t d_vvoo[4] = {v, v, o, o};
-  sort4(s, gtmp, d_vvoo, "2314", C);         // C(b,i,a,j) = g_abij
-  sort4(s, C, d, "1234", D, -1.0, 0.0);      // D = -C
-  gemm('T', 'N', N, N, J, 1.0, Lt_vo_.as<double>(), J, Lmo_ov_.as<double>(), J, 0.0, gtmp, N,
-       prec);                                // g_aijb
+  X.sort4(g_vvoo, d_vvoo, "2314", C, 1.0, 0.0);             // C(b,i,a,j) = g_abij
+  X.sort4(C, d, "1234", D, -1.0, 0.0);                      // D = -C
+  double* g_voov = X.buf(SL_B, "c2_g_voov", N * N);
+  X.gemm('T', 'N', N, N, J, 1.0, Lt_vo_.as<double>(), J, Lmo_ov_.as<double>(), J, 0.0, g_voov, N,
+         prec);                                              // g_aijb
   const int64_t d_voov[4] = {v, o, o, v};
-  sort4(s, gtmp, d_voov, "4312", D, 2.0, 1.0);   // D(c,k,a,i) += 2 g_aikc
-  double* G = work(S_B);
+  X.sort4(g_voov, d_voov, "4312", D, 2.0, 1.0);             // D(c,k,a,i) += 2 g_aikc
+  double* G = X.buf(SL_B, "c2_G", N * N);
   const int64_t d_ovov[4] = {o, v, o, v};
-  sort4(s, g_ovov_.as<double>(), d_ovov, "2341", G);   // G(c,l,d,k) = g_kcld
-  gemm('N', 'N', N, N, N, -0.5, G, N, Tm, N, 1.0, C, N, prec);
-  double* Gc = work(S_E);                    // 2 G(p,s,r,q) - G(p,q,r,s)
-  sort4(s, G, d, "1432", Gc, 2.0, 0.0);
-  sort4(s, G, d, "1234", Gc, -1.0, 1.0);
-  gemm('N', 'N', N, N, N, 0.5, Gc, N, U, N, 1.0, D, N, prec);
-  double* W = work(S_B);
-  gemm('N', 'N', N, N, N, -1.5, Tm, N, C, N, 0.0, W, N, prec);
-  symmetrize_and_pack(s, W, omega_.as<double>() + n_t1, v, o, 1.0, 1.0 / 3.0, 2.0 / 3.0);
-  gemm('N', 'N', N, N, N, 0.5, U, N, D, N, 0.0, W, N, prec);
-  symmetrize_and_pack(s, W, omega_.as<double>() + n_t1, v, o, 1.0, 1.0, 0.0);
+  X.sort4(g_ovov_.as<double>(), d_ovov, "2341", G, 1.0, 0.0);   // G(c,l,d,k) = g_kcld
+  X.gemm('N', 'N', N, N, N, -0.5, G, N, Tm, N, 1.0, C, N, prec);
+  double* Gc = X.buf(SL_E, "c2_Gc", N * N);                 // 2 G(p,s,r,q) - G(p,q,r,s)
+  X.sort4(G, d, "1432", Gc, 2.0, 0.0);
+  X.sort4(G, d, "1234", Gc, -1.0, 1.0);
+  X.gemm('N', 'N', N, N, N, 0.5, Gc, N, U, N, 1.0, D, N, prec);
+  double* W = X.buf(SL_B, "c2_W", N * N);
+  X.gemm('N', 'N', N, N, N, -1.5, Tm, N, C, N, 0.0, W, N, prec);
+  X.sympack(W, omega_.as<double>() + n_t1, v, o, 1.0, 1.0 / 3.0, 2.0 / 3.0);
+  double* V = X.buf(SL_B, "c2_V", N * N);
+  X.gemm('N', 'N', N, N, N, 0.5, U, N, D, N, 0.0, V, N, prec);
+  X.sympack(V, omega_.as<double>() + n_t1, v, o, 1.0, 1.0, 0.0);
 }
 
 // ---------------------------------------------------------------------------
@@ -558,6 +496,7 @@ void CCSD::omega_c2_d2(Prec prec) {
 //   w+ = 1/2 sum_{c<=d} t+_{ij,cd} g+_{cd,ab},  w- = 1/2 sum_{c<d} t-_{ij,cd} g-_{cd,ab}
 // g is generated per (a-batch, b-batch) block as G = L_vvT(:, A)^T L_vvT(:, B)
 // and never stored whole; w+- for all pairs are gathered into packed Omega at the end.
+// The tangent and adjoint recompute G block by block.
 // ---------------------------------------------------------------------------
 
 void CCSD::set_ladder_batch(int64_t nb) {
@@ -570,7 +509,7 @@ void CCSD::set_ladder_batch(int64_t nb) {
 void CCSD::build_ladder_plan() {
   const int64_t v = n_v, nb = ladder_nb_;
   blocks_.clear();
-  std::vector<int32_t> pa, pb, col(size_t(v * v), -1);
+  std::vector<int32_t> pa, pb, col(size_t(v * v), -1), ca, cb;
   int64_t col0 = 0, maxpairs = 0;
   const int64_t nbatch = ceil_div(v, nb);
   for (int64_t A = 0; A < nbatch; ++A) {
@@ -584,6 +523,8 @@ void CCSD::build_ladder_plan() {
           if (a < b) continue;
           pa.push_back(int32_t(aa));
           pb.push_back(int32_t(bb));
+          ca.push_back(int32_t(a));
+          cb.push_back(int32_t(b));
           col[size_t(a + v * b)] = int32_t(col0 + blk.npairs);
           col[size_t(b + v * a)] = int32_t(col0 + blk.npairs);
           ++blk.npairs;
@@ -594,12 +535,15 @@ void CCSD::build_ladder_plan() {
     }
   }
   if (col0 != v * (v + 1) / 2) fail("ladder plan: pair count mismatch");
-  pair_a_.resize(sizeof(int32_t) * pa.size());
-  pair_b_.resize(sizeof(int32_t) * pb.size());
-  pair_col_.resize(sizeof(int32_t) * col.size());
-  ETG_HIP(hipMemcpy(pair_a_.as<int32_t>(), pa.data(), sizeof(int32_t) * pa.size(), hipMemcpyHostToDevice));
-  ETG_HIP(hipMemcpy(pair_b_.as<int32_t>(), pb.data(), sizeof(int32_t) * pb.size(), hipMemcpyHostToDevice));
-  ETG_HIP(hipMemcpy(pair_col_.as<int32_t>(), col.data(), sizeof(int32_t) * col.size(), hipMemcpyHostToDevice));
+  auto upload = [](DeviceBuffer& d, const std::vector<int32_t>& h) {
+    d.resize(sizeof(int32_t) * std::max<size_t>(1, h.size()));
+    ETG_HIP(hipMemcpy(d.as<int32_t>(), h.data(), sizeof(int32_t) * h.size(), hipMemcpyHostToDevice));
+  };
+  upload(pair_a_, pa);
+  upload(pair_b_, pb);
+  upload(pair_col_, col);
+  upload(pcol_a_, ca);
+  upload(pcol_b_, cb);
   const int64_t op = n_o * (n_o + 1) / 2, om = n_o * (n_o - 1) / 2;
   const int64_t vp = v * (v + 1) / 2, vm = v * (v - 1) / 2;
   tp_.resize(sizeof(double) * op * vp);
@@ -611,26 +555,189 @@ void CCSD::build_ladder_plan() {
   gm_.resize(sizeof(double) * std::max<int64_t>(1, vm * maxpairs));
 }
 
-void CCSD::omega_a2(Prec prec) {
+void CCSD::ladder_primal(const double* t2, const double* LT, double* om2, Prec prec) {
   const int64_t o = n_o, v = n_v, J = n_J;
   const int64_t op = o * (o + 1) / 2, om = o * (o - 1) / 2;
   const int64_t vp = v * (v + 1) / 2, vm = v * (v - 1) / 2;
   hipStream_t s = stream_;
-  ladder_t_pm(s, t_.as<double>() + n_t1, tp_.as<double>(), tm_.as<double>(), o, v);
-  const double* LT = Lt_vvT_.as<double>();
+  DirectOps X(*this);
+  ladder_t_pm(s, t2, tp_.as<double>(), tm_.as<double>(), o, v);
   for (const Block& b : blocks_) {
-    gemm('T', 'N', v * b.la, v * b.lb, J, 1.0, LT + J * v * b.a0, J, LT + J * v * b.b0, J, 0.0,
-         gblk_.as<double>(), v * b.la, prec);
+    X.gemm('T', 'N', v * b.la, v * b.lb, J, 1.0, LT + J * v * b.a0, J, LT + J * v * b.b0, J, 0.0,
+           gblk_.as<double>(), v * b.la, prec);
     ladder_g_pm(s, gblk_.as<double>(), v, b.la, b.lb, pair_a_.as<int32_t>() + b.col0,
                 pair_b_.as<int32_t>() + b.col0, b.npairs, gp_.as<double>(), gm_.as<double>());
-    gemm('N', 'N', op, b.npairs, vp, 0.5, tp_.as<double>(), op, gp_.as<double>(), vp, 0.0,
-         wp_.as<double>() + op * b.col0, op, prec);
+    X.gemm('N', 'N', op, b.npairs, vp, 0.5, tp_.as<double>(), op, gp_.as<double>(), vp, 0.0,
+           wp_.as<double>() + op * b.col0, op, prec);
     if (om > 0 && vm > 0)
-      gemm('N', 'N', om, b.npairs, vm, 0.5, tm_.as<double>(), om, gm_.as<double>(), vm, 0.0,
-           wm_.as<double>() + om * b.col0, om, prec);
+      X.gemm('N', 'N', om, b.npairs, vm, 0.5, tm_.as<double>(), om, gm_.as<double>(), vm, 0.0,
+             wm_.as<double>() + om * b.col0, om, prec);
   }
-  ladder_gather(s, wp_.as<double>(), wm_.as<double>(), pair_col_.as<int32_t>(),
-                omega_.as<double>() + n_t1, o, v);
+  ladder_gather(s, wp_.as<double>(), wm_.as<double>(), pair_col_.as<int32_t>(), om2, o, v);
+}
+
+void CCSD::ladder_tangent(const double* t2, const double* dt2, const double* LT, const double* dLT,
+                          double* dom2, Prec prec) {
+  const int64_t o = n_o, v = n_v, J = n_J;
+  const int64_t op = o * (o + 1) / 2, om = o * (o - 1) / 2;
+  const int64_t vp = v * (v + 1) / 2, vm = v * (v - 1) / 2;
+  hipStream_t s = stream_;
+  dtp_.resize(tp_.bytes());
+  dtm_.resize(tm_.bytes());
+  dwp_.resize(wp_.bytes());
+  dwm_.resize(wm_.bytes());
+  dgblk_.resize(gblk_.bytes());
+  dgp_.resize(gp_.bytes());
+  dgm_.resize(gm_.bytes());
+  DirectOps X(*this);
+  ladder_t_pm(s, t2, tp_.as<double>(), tm_.as<double>(), o, v);
+  ladder_t_pm(s, dt2, dtp_.as<double>(), dtm_.as<double>(), o, v);
+  for (const Block& b : blocks_) {
+    const double* LA = LT + J * v * b.a0;
+    const double* LB = LT + J * v * b.b0;
+    X.gemm('T', 'N', v * b.la, v * b.lb, J, 1.0, LA, J, LB, J, 0.0, gblk_.as<double>(), v * b.la, prec);
+    X.gemm('T', 'N', v * b.la, v * b.lb, J, 1.0, dLT + J * v * b.a0, J, LB, J, 0.0,
+           dgblk_.as<double>(), v * b.la, prec);
+    X.gemm('T', 'N', v * b.la, v * b.lb, J, 1.0, LA, J, dLT + J * v * b.b0, J, 1.0,
+           dgblk_.as<double>(), v * b.la, prec);
+    ladder_g_pm(s, gblk_.as<double>(), v, b.la, b.lb, pair_a_.as<int32_t>() + b.col0,
+                pair_b_.as<int32_t>() + b.col0, b.npairs, gp_.as<double>(), gm_.as<double>());
+    ladder_g_pm(s, dgblk_.as<double>(), v, b.la, b.lb, pair_a_.as<int32_t>() + b.col0,
+                pair_b_.as<int32_t>() + b.col0, b.npairs, dgp_.as<double>(), dgm_.as<double>());
+    double* dw = dwp_.as<double>() + op * b.col0;
+    X.gemm('N', 'N', op, b.npairs, vp, 0.5, dtp_.as<double>(), op, gp_.as<double>(), vp, 0.0, dw, op, prec);
+    X.gemm('N', 'N', op, b.npairs, vp, 0.5, tp_.as<double>(), op, dgp_.as<double>(), vp, 1.0, dw, op, prec);
+    if (om > 0 && vm > 0) {
+      dw = dwm_.as<double>() + om * b.col0;
+      X.gemm('N', 'N', om, b.npairs, vm, 0.5, dtm_.as<double>(), om, gm_.as<double>(), vm, 0.0, dw, om, prec);
+      X.gemm('N', 'N', om, b.npairs, vm, 0.5, tm_.as<double>(), om, dgm_.as<double>(), vm, 1.0, dw, om, prec);
+    }
+  }
+  ladder_gather(s, dwp_.as<double>(), dwm_.as<double>(), pair_col_.as<int32_t>(), dom2, o, v);
+}
+
+void CCSD::ladder_adjoint(const double* t2, double* t2b, const double* LT, double* LTb,
+                          const double* om2b, Prec prec) {
+  const int64_t o = n_o, v = n_v, J = n_J;
+  const int64_t op = o * (o + 1) / 2, om = o * (o - 1) / 2;
+  const int64_t vp = v * (v + 1) / 2, vm = v * (v - 1) / 2;
+  hipStream_t s = stream_;
+  dtp_.resize(tp_.bytes());
+  dtm_.resize(tm_.bytes());
+  dwp_.resize(wp_.bytes());
+  dwm_.resize(wm_.bytes());
+  dgblk_.resize(gblk_.bytes());
+  dgp_.resize(gp_.bytes());
+  dgm_.resize(gm_.bytes());
+  DirectOps X(*this);
+  double* tpb = dtp_.as<double>();
+  double* tmb = dtm_.as<double>();
+  double* wpb = dwp_.as<double>();
+  double* wmb = dwm_.as<double>();
+  ladder_t_pm(s, t2, tp_.as<double>(), tm_.as<double>(), o, v);
+  ladder_gather_adjoint(s, om2b, pcol_a_.as<int32_t>(), pcol_b_.as<int32_t>(), wpb, wmb, o, v);
+  fill(s, tpb, op * vp, 0.0);
+  if (om > 0 && vm > 0) fill(s, tmb, om * vm, 0.0);
+  for (const Block& b : blocks_) {
+    const double* LA = LT + J * v * b.a0;
+    const double* LB = LT + J * v * b.b0;
+    X.gemm('T', 'N', v * b.la, v * b.lb, J, 1.0, LA, J, LB, J, 0.0, gblk_.as<double>(), v * b.la, prec);
+    ladder_g_pm(s, gblk_.as<double>(), v, b.la, b.lb, pair_a_.as<int32_t>() + b.col0,
+                pair_b_.as<int32_t>() + b.col0, b.npairs, gp_.as<double>(), gm_.as<double>());
+    // w+ = 1/2 t+ g+  ->  t+_bar += 1/2 w+_bar g+^T,  g+_bar = 1/2 t+^T w+_bar
+    const double* w = wpb + op * b.col0;
+    X.gemm('N', 'T', op, vp, b.npairs, 0.5, w, op, gp_.as<double>(), vp, 1.0, tpb, op, prec);
+    X.gemm('T', 'N', vp, b.npairs, op, 0.5, tp_.as<double>(), op, w, op, 0.0, dgp_.as<double>(), vp, prec);
+    if (om > 0 && vm > 0) {
+      w = wmb + om * b.col0;
+      X.gemm('N', 'T', om, vm, b.npairs, 0.5, w, om, gm_.as<double>(), vm, 1.0, tmb, om, prec);
+      X.gemm('T', 'N', vm, b.npairs, om, 0.5, tm_.as<double>(), om, w, om, 0.0, dgm_.as<double>(), vm, prec);
+    } else {
+      fill(s, dgm_.as<double>(), std::max<int64_t>(1, vm * b.npairs), 0.0);
+    }
+    ladder_g_pm_adjoint(s, dgp_.as<double>(), dgm_.as<double>(), dgblk_.as<double>(), v, b.a0, b.la,
+                        b.b0, b.lb, pair_col_.as<int32_t>(), b.col0);
+    // G = LA^T LB  ->  LA_bar += LB G_bar^T,  LB_bar += LA G_bar
+    X.gemm('N', 'T', J, v * b.la, v * b.lb, 1.0, LB, J, dgblk_.as<double>(), v * b.la, 1.0,
+           LTb + J * v * b.a0, J, prec);
+    X.gemm('N', 'N', J, v * b.lb, v * b.la, 1.0, LA, J, dgblk_.as<double>(), v * b.la, 1.0,
+           LTb + J * v * b.b0, J, prec);
+  }
+  ladder_t_pm_adjoint(s, tpb, tmb, t2b, o, v);
+}
+
+// ---------------------------------------------------------------------------
+// Jacobian transformations (jacobian_ccsd.F90:74-178, jacobian_transpose_ccsd.F90:129-236)
+//   A_mu,nu = <mu~| exp(-T) [H, tau_nu] exp(T) |R>,  tau_aibj = E_ai E_bj,
+// so A = (dOmega/dt) D2 where D2 doubles the doubles diagonal of the trial vector
+// (eT scales c_aiai by two, jacobian_ccsd.F90:123).  rho = A c is the forward
+// (tangent) replay of the recorded residual program along D2 c; sigma = A^T b
+// is its backward (adjoint) replay from b, followed by D2.
+// ---------------------------------------------------------------------------
+
+namespace {
+__global__ void scale_doubles_diagonal_kernel(double* x, int64_t n_t1, double factor) {
+  for (int64_t d = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; d < n_t1;
+       d += int64_t(gridDim.x) * blockDim.x)
+    x[n_t1 + d * (d + 1) / 2 + d] *= factor;
+}
+}  // namespace
+
+void CCSD::prepare_for_jacobian() {
+  require(have_L_ && have_t_ && have_h_, "prepare_for_jacobian: Cholesky vectors, h and amplitudes required");
+  tape_.reset();
+  synchronize();
+  auto tape = std::make_unique<Tape>(*this);
+  const int64_t J = n_J, o = n_o, v = n_v, n = n_mo;
+  tape->register_buffer(Lmo_oo_.as<double>(), J * o * o, true, "L_mo oo");
+  tape->register_buffer(Lmo_ov_.as<double>(), J * o * v, true, "L_mo ov");
+  tape->register_buffer(Lmo_vo_.as<double>(), J * v * o, true, "L_mo vo");
+  tape->register_buffer(Lmo_vv_.as<double>(), J * v * v, true, "L_mo vv");
+  tape->register_buffer(h_.as<double>(), n * n, true, "h");
+  tape->register_buffer(g_ovov_.as<double>(), n_t1 * n_t1, true, "g_ovov");
+  tape->register_buffer(t_.as<double>(), n_t1 + n_t2, false, "t");
+  tape->register_buffer(F_.as<double>(), n * n, false, "F");
+  tape->register_buffer(omega_.as<double>(), n_t1 + n_t2, false, "omega");
+  tape->register_buffer(Lt_oo_.as<double>(), J * o * o, false, "L_t1 oo");
+  tape->register_buffer(Lt_vo_.as<double>(), J * v * o, false, "L_t1 vo");
+  tape->register_buffer(Lt_vv_.as<double>(), J * v * v, false, "L_t1 vv");
+  tape->register_buffer(Lt_vvT_.as<double>(), J * v * v, false, "L_t1 vvT");
+  program_t1_cholesky(*tape);
+  program_fock(*tape);
+  program_omega(*tape, Prec::F64);
+  synchronize();
+  tape_ = std::move(tape);
+  have_F_ = true;
+}
+
+void CCSD::jacobian_transform(const double* c, double* rho) {
+  require(tape_ != nullptr, "jacobian_transformation: prepare_for_jacobian has not been called");
+  require(c != nullptr && rho != nullptr, "jacobian_transformation: null pointer");
+  const int64_t n = n_t1 + n_t2;
+  tape_->zero_derivatives();
+  double* dt = tape_->deriv(t_.as<double>());
+  ETG_HIP(hipMemcpyAsync(dt, c, sizeof(double) * n, hipMemcpyHostToDevice, stream_));
+  scale_doubles_diagonal_kernel<<<unsigned(ceil_div(n_t1, 256)), 256, 0, stream_>>>(dt, n_t1, 2.0);
+  ETG_LAUNCH("scale_doubles_diagonal");
+  tape_->tangent(Prec::F64);
+  ETG_HIP(hipMemcpyAsync(rho, tape_->deriv(omega_.as<double>()), sizeof(double) * n,
+                         hipMemcpyDeviceToHost, stream_));
+  synchronize();
+}
+
+void CCSD::jacobian_transpose_transform(const double* b, double* sigma) {
+  require(tape_ != nullptr, "jacobian_transpose_transformation: prepare_for_jacobian has not been called");
+  require(b != nullptr && sigma != nullptr, "jacobian_transpose_transformation: null pointer");
+  const int64_t n = n_t1 + n_t2;
+  tape_->zero_derivatives();
+  ETG_HIP(hipMemcpyAsync(tape_->deriv(omega_.as<double>()), b, sizeof(double) * n,
+                         hipMemcpyHostToDevice, stream_));
+  tape_->adjoint(Prec::F64);
+  double* tb = tape_->deriv(t_.as<double>());
+  scale_doubles_diagonal_kernel<<<unsigned(ceil_div(n_t1, 256)), 256, 0, stream_>>>(tb, n_t1, 2.0);
+  ETG_LAUNCH("scale_doubles_diagonal");
+  ETG_HIP(hipMemcpyAsync(sigma, tb, sizeof(double) * n, hipMemcpyDeviceToHost, stream_));
+  synchronize();
 }
 
 }  // namespace etg

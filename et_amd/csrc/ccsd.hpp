@@ -1,16 +1,16 @@
 // Device-resident state of one CCSD wavefunction and the operations eT's
-// ground-state solvers call on it (reference: src/wavefunctions/ccsd/ccsd_class.F90).
+// solvers call on it (reference: src/wavefunctions/ccsd/ccsd_class.F90).
 #pragma once
 
+#include <memory>
 #include <string>
 #include <vector>
 
 #include "common.hpp"
 #include "gemm.hpp"
+#include "ops.hpp"
 
 namespace etg {
-
-struct Timers;
 
 class CCSD {
  public:
@@ -38,6 +38,11 @@ class CCSD {
   void get_orbital_differences(double* host, int64_t n);
   double omega_norm();                            // ||omega||_2 (FP64 reduction)
 
+  // Jacobian transformations at the current amplitudes (A = dOmega/dt D2)
+  void prepare_for_jacobian();                    // records the residual program
+  void jacobian_transform(const double* c_host, double* rho_host);            // rho = A c
+  void jacobian_transpose_transform(const double* b_host, double* sigma_host);  // sigma = A^T b
+
   // device pointers (for callers that keep data in HBM)
   double* d_omega() { return omega_.as<double>(); }
   double* d_amplitudes() { return t_.as<double>(); }
@@ -57,21 +62,25 @@ class CCSD {
   void set_ledger(bool on);
   std::vector<LedgerEntry> ledger();
 
+  // used by the ops layer
+  void run_gemm(const Gemm& g, Prec prec);
+  double* scratch(int slot, int64_t elems, const char* name);
+  void ladder_primal(const double* t2, const double* LvvT, double* om2, Prec prec);
+  void ladder_tangent(const double* t2, const double* dt2, const double* LvvT, const double* dLvvT,
+                      double* dom2, Prec prec);
+  void ladder_adjoint(const double* t2, double* t2b, const double* LvvT, double* LvvTb,
+                      const double* om2b, Prec prec);
+
  private:
   void require(bool ok, const char* what) const;
-  void gemm(char ta, char tb, int64_t m, int64_t n, int64_t k, double alpha, const double* A,
-            int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc,
-            Prec prec);
-  void gemm_batched(char ta, char tb, int64_t m, int64_t n, int64_t k, double alpha,
-                    const double* A, int64_t lda, int64_t sA, const double* B, int64_t ldb,
-                    int64_t sB, double beta, double* C, int64_t ldc, int64_t sC, int64_t batch,
-                    Prec prec);
   void build_ladder_plan();
-  void omega_singles(Prec prec);
-  void omega_doubles_e2_b2(Prec prec);
-  void omega_c2_d2(Prec prec);
-  void omega_a2(Prec prec);
-  double* work(int slot);
+  // the residual pipeline, written once against Ops
+  void program_t1_cholesky(Ops& X);
+  void program_fock(Ops& X);
+  void program_omega(Ops& X, Prec prec);
+  void omega_singles(Ops& X, Prec prec, const double* U);
+  void omega_e2_b2(Ops& X, Prec prec, const double* T, const double* U);
+  void omega_c2_d2(Ops& X, Prec prec, const double* T, const double* U);
 
   hipStream_t stream_ = nullptr;
   GemmWorkspace gws_;
@@ -81,15 +90,20 @@ class CCSD {
   DeviceBuffer h_, eps_, F_, t_, omega_, diff_;
   DeviceBuffer g_ovov_;                           // independent of t1 (ov block is not dressed)
   DeviceBuffer red_partial_, red_result_;
-  DeviceBuffer tmp_small_[4];
+  DeviceBuffer small_[4];
   std::vector<DeviceBuffer> work_;                // (ov)^2 scratch slots
 
   // ladder plan
   int64_t ladder_nb_ = 0;
   struct Block { int64_t a0, la, b0, lb, col0, npairs; };
   std::vector<Block> blocks_;
-  DeviceBuffer pair_a_, pair_b_, pair_col_;
+  DeviceBuffer pair_a_, pair_b_, pair_col_, pcol_a_, pcol_b_;
   DeviceBuffer tp_, tm_, wp_, wm_, gblk_, gp_, gm_;
+  DeviceBuffer dtp_, dtm_, dwp_, dwm_, dgblk_, dgp_, dgm_;
+
+  // Jacobian record
+  std::unique_ptr<Tape> tape_;
+  DeviceBuffer jvec_;
 
   struct LedgerRec {
     const char* kernel;
@@ -100,7 +114,6 @@ class CCSD {
   std::vector<LedgerRec> ledger_recs_;
   std::vector<hipEvent_t> event_pool_;
   hipEvent_t take_event();
-  void record_gemm(const GemmLaunch& info, double flops, hipEvent_t a, hipEvent_t b);
 
   bool have_L_ = false, have_h_ = false, have_eps_ = false, have_t_ = false, have_F_ = false;
 };

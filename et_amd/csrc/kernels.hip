@@ -377,3 +377,322 @@ void fill(hipStream_t s, double* x, int64_t n, double value) {
 }
 
 }  // namespace etg
+
+// ===========================================================================
+// Adjoint kernels and T1 helpers
+// ===========================================================================
+namespace etg {
+namespace {
+
+__device__ inline int64_t tri_root2(int64_t k) {
+  int64_t x = int64_t((sqrt(8.0 * double(k) + 1.0) - 1.0) * 0.5);
+  while (x * (x + 1) / 2 > k) --x;
+  while ((x + 1) * (x + 2) / 2 <= k) ++x;
+  return x;
+}
+
+__device__ inline int64_t pidx2(int64_t x, int64_t y) {
+  const int64_t hi = x > y ? x : y, lo = x > y ? y : x;
+  return hi * (hi + 1) / 2 + lo;
+}
+
+#define GRID_LOOP(var, n)                                                      \
+  for (int64_t var = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; var < (n); \
+       var += int64_t(gridDim.x) * blockDim.x)
+
+__global__ void scale_kernel(double* x, int64_t n, double beta) {
+  GRID_LOOP(k, n) x[k] = (beta == 0.0) ? 0.0 : beta * x[k];
+}
+
+__global__ void symmetric_sum_acc_kernel(const double* __restrict__ x, double* __restrict__ y,
+                                         int64_t N, double beta) {
+  __shared__ double tile[32][33];
+  const int64_t bx = blockIdx.x, by = blockIdx.y;
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  for (int k = ty; k < 32; k += 8) {
+    const int64_t r = by * 32 + tx, c = bx * 32 + k;
+    tile[k][tx] = (r < N && c < N) ? x[r + N * c] : 0.0;
+  }
+  __syncthreads();
+  for (int k = ty; k < 32; k += 8) {
+    const int64_t r = bx * 32 + tx, c = by * 32 + k;
+    if (r < N && c < N) {
+      const double v = x[r + N * c] + tile[tx][k];
+      y[r + N * c] = (beta == 0.0) ? v : v + beta * y[r + N * c];
+    }
+  }
+}
+
+__global__ void packsum_kernel(const double* __restrict__ full, double* __restrict__ packed,
+                               int64_t N) {
+  GRID_LOOP(k, N * (N + 1) / 2) {
+    const int64_t X = tri_root2(k), Y = k - X * (X + 1) / 2;
+    double v = full[X + N * Y];
+    if (X != Y) v += full[Y + N * X];
+    packed[k] += v;
+  }
+}
+
+__global__ void unpack_upper_add_kernel(const double* __restrict__ packed, double* __restrict__ full,
+                                        int64_t N, double alpha, double diag_scale) {
+  GRID_LOOP(k, N * (N + 1) / 2) {
+    const int64_t X = tri_root2(k), Y = k - X * (X + 1) / 2;
+    double v = alpha * packed[k];
+    if (X == Y) v *= diag_scale;
+    full[Y + N * X] += v;
+  }
+}
+
+// x(R, C) += alpha * [R == C ? p{R,R} : f1 p{R,C} + f2 p{U,W}],
+// U = (R_a, C_i), W = (C_a, R_i)  (the element the psrq terms of the forward pass read)
+__global__ void sympack_adjoint_kernel(const double* __restrict__ p, double* __restrict__ x,
+                                       int64_t n_v, int64_t n_o, double alpha, double f1, double f2) {
+  const int64_t N = n_v * n_o;
+  GRID_LOOP(idx, N * N) {
+    const int64_t R = idx % N, C = idx / N;
+    double v;
+    if (R == C) {
+      v = p[pidx2(R, R)];
+    } else {
+      v = f1 * p[pidx2(R, C)];
+      if (f2 != 0.0) {
+        const int64_t Ra = R % n_v, Ri = R / n_v, Ca = C % n_v, Ci = C / n_v;
+        const int64_t U = Ra + n_v * Ci, W = Ca + n_v * Ri;
+        if (U != W) v += f2 * p[pidx2(U, W)];
+      }
+    }
+    x[idx] += alpha * v;
+  }
+}
+
+// t2[{(c,i),(d,j)}] += coef * tp[ij, cd] + sgn * tm[ij, cd]
+__global__ void ladder_t_pm_adjoint_kernel(const double* __restrict__ tp, const double* __restrict__ tm,
+                                           double* __restrict__ t2, int64_t n_o, int64_t n_v) {
+  const int64_t Nov = n_o * n_v;
+  const int64_t op = n_o * (n_o + 1) / 2, om = n_o * (n_o - 1) / 2;
+  GRID_LOOP(k, Nov * (Nov + 1) / 2) {
+    const int64_t X = tri_root2(k), Y = k - X * (X + 1) / 2;
+    const int64_t c = X % n_v, i = X / n_v, d = Y % n_v, j = Y / n_v;
+    const int64_t il = i < j ? i : j, ih = i < j ? j : i;
+    const int64_t cl = c < d ? c : d, ch = c < d ? d : c;
+    const double coef = (c == d) ? 1.0 : ((i == j) ? 2.0 : 1.0);
+    double v = coef * tp[(ih * (ih + 1) / 2 + il) + op * (ch * (ch + 1) / 2 + cl)];
+    if (i != j && c != d) {
+      const double sgn = ((c < d) == (i < j)) ? 1.0 : -1.0;
+      v += sgn * tm[(ih * (ih - 1) / 2 + il) + om * (ch * (ch - 1) / 2 + cl)];
+    }
+    t2[k] += v;
+  }
+}
+
+// G(c, a', d, b') = [a >= b] (c == d ? 2 gp[cc,k] : gp[cd,k] +- gm[cd,k]), k = col[a + v b] - col0
+__global__ void ladder_g_pm_adjoint_kernel(const double* __restrict__ gp, const double* __restrict__ gm,
+                                           double* __restrict__ G, int64_t n_v, int64_t a0,
+                                           int64_t lenA, int64_t b0, int64_t lenB,
+                                           const int32_t* __restrict__ col, int64_t col0) {
+  const int64_t vp = n_v * (n_v + 1) / 2, vm = n_v * (n_v - 1) / 2;
+  const int64_t rows = n_v * lenA;
+  GRID_LOOP(idx, rows * n_v * lenB) {
+    const int64_t r = idx % rows, q = idx / rows;
+    const int64_t c = r % n_v, aa = r / n_v, d = q % n_v, bb = q / n_v;
+    const int64_t a = a0 + aa, b = b0 + bb;
+    double v = 0.0;
+    if (a >= b) {
+      const int64_t k = int64_t(col[a + n_v * b]) - col0;
+      if (c == d) {
+        v = 2.0 * gp[(c * (c + 1) / 2 + c) + vp * k];
+      } else {
+        const int64_t lo = c < d ? c : d, hi = c < d ? d : c;
+        v = gp[(hi * (hi + 1) / 2 + lo) + vp * k];
+        const double m = gm[(hi * (hi - 1) / 2 + lo) + vm * k];
+        v += (c < d) ? m : -m;
+      }
+    }
+    G[idx] = v;
+  }
+}
+
+// adjoint of ladder_gather: for stored (i' <= j', a' >= b'):
+// wp += s1 om[P1] + s2 om[P2],  wm += s1 om[P1] - s2 om[P2]
+// P1 = {(a',i'),(b',j')}, P2 = {(a',j'),(b',i')}, s = 1/2 on the packed diagonal
+__global__ void ladder_gather_adjoint_kernel(const double* __restrict__ om,
+                                             const int32_t* __restrict__ pa,
+                                             const int32_t* __restrict__ pb, double* __restrict__ wp,
+                                             double* __restrict__ wm, int64_t n_o, int64_t n_v) {
+  const int64_t op = n_o * (n_o + 1) / 2, omn = n_o * (n_o - 1) / 2;
+  const int64_t vp = n_v * (n_v + 1) / 2;
+  GRID_LOOP(idx, op * vp) {
+    const int64_t r = idx % op, cl = idx / op;
+    const int64_t j = tri_root2(r), i = r - j * (j + 1) / 2;       // i <= j
+    const int64_t a = pa[cl], b = pb[cl];                          // a >= b
+    const int64_t X1 = a + n_v * i, Y1 = b + n_v * j;
+    const int64_t X2 = a + n_v * j, Y2 = b + n_v * i;
+    const int64_t k1 = pidx2(X1, Y1), k2 = pidx2(X2, Y2);
+    const double v1 = (X1 == Y1 ? 0.5 : 1.0) * om[k1];
+    const double v2 = (k2 != k1) ? (X2 == Y2 ? 0.5 : 1.0) * om[k2] : 0.0;
+    wp[idx] = v1 + v2;
+    if (i != j) wm[(j * (j - 1) / 2 + i) + omn * cl] = (a != b) ? v1 - v2 : 0.0;
+  }
+}
+
+__global__ void t1_xy_kernel2(const double* t1, double* X, double* Y, int64_t n_o, int64_t n_v,
+                              double diag) {
+  const int64_t n = n_o + n_v;
+  GRID_LOOP(k, n * n) {
+    const int64_t p = k % n, q = k / n;
+    double x = (p == q) ? diag : 0.0, y = x;
+    if (p >= n_o && q < n_o) x = -t1[(p - n_o) + n_v * q];  // X(o+a, i) = -t_ai
+    if (p < n_o && q >= n_o) y = t1[(q - n_o) + n_v * p];   // Y(i, o+a) = t_ai
+    X[k] = x;
+    Y[k] = y;
+  }
+}
+
+__global__ void t1_xy_adjoint_kernel(const double* Xb, const double* Yb, double* t1b, int64_t n_o,
+                                     int64_t n_v) {
+  const int64_t n = n_o + n_v;
+  GRID_LOOP(k, n_o * n_v) {
+    const int64_t a = k % n_v, i = k / n_v;
+    t1b[k] += -Xb[(n_o + a) + n * i] + Yb[i + n * (n_o + a)];
+  }
+}
+
+__global__ void trace_oo_kernel2(const double* Loo, double* XJ, int64_t nJ, int64_t n_o) {
+  GRID_LOOP(J, nJ) {
+    double s = 0.0;
+    for (int64_t k = 0; k < n_o; ++k) s += Loo[J + nJ * (k + n_o * k)];
+    XJ[J] = s;
+  }
+}
+
+__global__ void trace_oo_adjoint_kernel(const double* XJb, double* Loob, int64_t nJ, int64_t n_o) {
+  GRID_LOOP(idx, nJ * n_o) {
+    const int64_t J = idx % nJ, k = idx / nJ;
+    Loob[J + nJ * (k + n_o * k)] += XJb[J];
+  }
+}
+
+__global__ void fock_ov_vector_kernel2(const double* F, double* f, int64_t n_o, int64_t n_v) {
+  const int64_t n = n_o + n_v;
+  GRID_LOOP(k, n_o * n_v) {
+    const int64_t b = k % n_v, j = k / n_v;
+    f[k] = F[j + n * (n_o + b)];
+  }
+}
+
+__global__ void fock_ov_vector_adjoint_kernel(const double* fb, double* Fb, int64_t n_o, int64_t n_v) {
+  const int64_t n = n_o + n_v;
+  GRID_LOOP(k, n_o * n_v) {
+    const int64_t b = k % n_v, j = k / n_v;
+    Fb[j + n * (n_o + b)] += fb[k];
+  }
+}
+
+__global__ void copy_sub_kernel2(const double* src, int64_t lds, double* dst, int64_t ldd, int64_t m,
+                                 int64_t n, double alpha, double beta) {
+  GRID_LOOP(k, m * n) {
+    const int64_t r = k % m, c = k / m;
+    const double v = alpha * src[r + lds * c];
+    dst[r + ldd * c] = (beta == 0.0) ? v : v + beta * dst[r + ldd * c];
+  }
+}
+
+inline unsigned g2(int64_t n) { return grid_for(n); }
+
+}  // namespace
+
+void scale(hipStream_t s, double* x, int64_t n, double beta) {
+  if (n == 0 || beta == 1.0) return;
+  scale_kernel<<<g2(n), kBlock, 0, s>>>(x, n, beta);
+  ETG_LAUNCH("scale");
+}
+
+void symmetric_sum_acc(hipStream_t s, const double* x, double* y, int64_t N, double beta) {
+  if (x == y) fail("symmetric_sum_acc: out of place only");
+  dim3 grid(unsigned(ceil_div(N, 32)), unsigned(ceil_div(N, 32)));
+  symmetric_sum_acc_kernel<<<grid, dim3(32, 8), 0, s>>>(x, y, N, beta);
+  ETG_LAUNCH("symmetric_sum_acc");
+}
+
+void packsum(hipStream_t s, const double* full, double* packed, int64_t N) {
+  packsum_kernel<<<g2(N * (N + 1) / 2), kBlock, 0, s>>>(full, packed, N);
+  ETG_LAUNCH("packsum");
+}
+
+void unpack_upper_add(hipStream_t s, const double* packed, double* full, int64_t N, double alpha,
+                      double diag_scale) {
+  unpack_upper_add_kernel<<<g2(N * (N + 1) / 2), kBlock, 0, s>>>(packed, full, N, alpha, diag_scale);
+  ETG_LAUNCH("unpack_upper_add");
+}
+
+void symmetrize_and_pack_adjoint(hipStream_t s, const double* packed, double* x, int64_t n_v,
+                                 int64_t n_o, double alpha, double f1, double f2) {
+  const int64_t N = n_v * n_o;
+  sympack_adjoint_kernel<<<g2(N * N), kBlock, 0, s>>>(packed, x, n_v, n_o, alpha, f1, f2);
+  ETG_LAUNCH("sympack_adjoint");
+}
+
+void ladder_t_pm_adjoint(hipStream_t s, const double* tp, const double* tm, double* t2, int64_t n_o,
+                         int64_t n_v) {
+  const int64_t N = n_o * n_v;
+  ladder_t_pm_adjoint_kernel<<<g2(N * (N + 1) / 2), kBlock, 0, s>>>(tp, tm, t2, n_o, n_v);
+  ETG_LAUNCH("ladder_t_pm_adjoint");
+}
+
+void ladder_g_pm_adjoint(hipStream_t s, const double* gp, const double* gm, double* G, int64_t n_v,
+                         int64_t a0, int64_t lenA, int64_t b0, int64_t lenB, const int32_t* col,
+                         int64_t col0) {
+  const int64_t n = n_v * lenA * n_v * lenB;
+  if (n == 0) return;
+  ladder_g_pm_adjoint_kernel<<<g2(n), kBlock, 0, s>>>(gp, gm, G, n_v, a0, lenA, b0, lenB, col, col0);
+  ETG_LAUNCH("ladder_g_pm_adjoint");
+}
+
+void ladder_gather_adjoint(hipStream_t s, const double* omega2, const int32_t* pcol_a,
+                           const int32_t* pcol_b, double* wp, double* wm, int64_t n_o, int64_t n_v) {
+  const int64_t n = (n_o * (n_o + 1) / 2) * (n_v * (n_v + 1) / 2);
+  ladder_gather_adjoint_kernel<<<g2(n), kBlock, 0, s>>>(omega2, pcol_a, pcol_b, wp, wm, n_o, n_v);
+  ETG_LAUNCH("ladder_gather_adjoint");
+}
+
+void t1_xy(hipStream_t s, const double* t1, double* X, double* Y, int64_t n_o, int64_t n_v,
+           double diag) {
+  const int64_t n = n_o + n_v;
+  t1_xy_kernel2<<<g2(n * n), kBlock, 0, s>>>(t1, X, Y, n_o, n_v, diag);
+  ETG_LAUNCH("t1_xy");
+}
+
+void t1_xy_adjoint(hipStream_t s, const double* Xb, const double* Yb, double* t1b, int64_t n_o,
+                   int64_t n_v) {
+  t1_xy_adjoint_kernel<<<g2(n_o * n_v), kBlock, 0, s>>>(Xb, Yb, t1b, n_o, n_v);
+  ETG_LAUNCH("t1_xy_adjoint");
+}
+
+void trace_oo(hipStream_t s, const double* Loo, double* XJ, int64_t nJ, int64_t n_o) {
+  trace_oo_kernel2<<<g2(nJ), kBlock, 0, s>>>(Loo, XJ, nJ, n_o);
+  ETG_LAUNCH("trace_oo");
+}
+
+void trace_oo_adjoint(hipStream_t s, const double* XJb, double* Loob, int64_t nJ, int64_t n_o) {
+  trace_oo_adjoint_kernel<<<g2(nJ * n_o), kBlock, 0, s>>>(XJb, Loob, nJ, n_o);
+  ETG_LAUNCH("trace_oo_adjoint");
+}
+
+void fock_ov_vector(hipStream_t s, const double* F, double* f, int64_t n_o, int64_t n_v) {
+  fock_ov_vector_kernel2<<<g2(n_o * n_v), kBlock, 0, s>>>(F, f, n_o, n_v);
+  ETG_LAUNCH("fock_ov_vector");
+}
+
+void fock_ov_vector_adjoint(hipStream_t s, const double* fb, double* Fb, int64_t n_o, int64_t n_v) {
+  fock_ov_vector_adjoint_kernel<<<g2(n_o * n_v), kBlock, 0, s>>>(fb, Fb, n_o, n_v);
+  ETG_LAUNCH("fock_ov_vector_adjoint");
+}
+
+void copy_sub(hipStream_t s, const double* src, int64_t lds, double* dst, int64_t ldd, int64_t m,
+              int64_t n, double alpha, double beta) {
+  if (m * n == 0) return;
+  copy_sub_kernel2<<<g2(m * n), kBlock, 0, s>>>(src, lds, dst, ldd, m, n, alpha, beta);
+  ETG_LAUNCH("copy_sub");
+}
+
+}  // namespace etg

@@ -73,3 +73,41 @@ void axpy(hipStream_t s, int64_t n, double alpha, const double* x, double* y);
 void fill(hipStream_t s, double* x, int64_t n, double value);
 
 }  // namespace etg
+
+namespace etg {
+
+// ---- adjoints of the reorder / packing kernels (derivatives of the residual) ----
+// x (n elements) *= beta
+void scale(hipStream_t s, double* x, int64_t n, double beta);
+// y = x + x^T + beta * y
+void symmetric_sum_acc(hipStream_t s, const double* x, double* y, int64_t N, double beta);
+// adjoint of squareup: packed[k] += full(X,Y) + full(Y,X)   (full(X,X) on the diagonal)
+void packsum(hipStream_t s, const double* full, double* packed, int64_t N);
+// adjoint of packin: full(Y, X) += alpha * s_k * packed[k], X >= Y
+void unpack_upper_add(hipStream_t s, const double* packed, double* full, int64_t N, double alpha,
+                      double diag_scale);
+// adjoint of symmetrize_and_pack: x(R,C) += alpha * (...)
+void symmetrize_and_pack_adjoint(hipStream_t s, const double* packed, double* x, int64_t n_v,
+                                 int64_t n_o, double alpha, double f1, double f2);
+// adjoints of the ladder helpers
+void ladder_t_pm_adjoint(hipStream_t s, const double* tp, const double* tm, double* t2, int64_t n_o,
+                         int64_t n_v);
+void ladder_g_pm_adjoint(hipStream_t s, const double* gp, const double* gm, double* G, int64_t n_v,
+                         int64_t a0, int64_t lenA, int64_t b0, int64_t lenB, const int32_t* col,
+                         int64_t col0);
+void ladder_gather_adjoint(hipStream_t s, const double* omega2, const int32_t* pcol_a,
+                           const int32_t* pcol_b, double* wp, double* wm, int64_t n_o, int64_t n_v);
+// T1-related small kernels (n = n_o + n_v)
+void t1_xy(hipStream_t s, const double* t1, double* X, double* Y, int64_t n_o, int64_t n_v,
+           double diag);
+void t1_xy_adjoint(hipStream_t s, const double* Xb, const double* Yb, double* t1b, int64_t n_o,
+                   int64_t n_v);
+void trace_oo(hipStream_t s, const double* Loo, double* XJ, int64_t nJ, int64_t n_o);
+void trace_oo_adjoint(hipStream_t s, const double* XJb, double* Loob, int64_t nJ, int64_t n_o);
+void fock_ov_vector(hipStream_t s, const double* F, double* f, int64_t n_o, int64_t n_v);
+void fock_ov_vector_adjoint(hipStream_t s, const double* fb, double* Fb, int64_t n_o, int64_t n_v);
+// dst (m x n) = alpha * src + beta * dst
+void copy_sub(hipStream_t s, const double* src, int64_t lds, double* dst, int64_t ldd, int64_t m,
+              int64_t n, double alpha, double beta);
+
+}  // namespace etg

@@ -110,6 +110,27 @@ module etgpu_interface
          integer(c_int64_t), value :: n
       end function etg_get_orbital_differences
 !
+      integer(c_int) function etg_prepare_for_jacobian(handle) bind(C, name='etg_prepare_for_jacobian')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: handle
+      end function etg_prepare_for_jacobian
+!
+      integer(c_int) function etg_jacobian_transformation(handle, c, rho) &
+                                                   bind(C, name='etg_jacobian_transformation')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: handle
+         real(c_double), dimension(*), intent(in) :: c
+         real(c_double), dimension(*), intent(out) :: rho
+      end function etg_jacobian_transformation
+!
+      integer(c_int) function etg_jacobian_transpose_transformation(handle, b, sigma) &
+                                                   bind(C, name='etg_jacobian_transpose_transformation')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: handle
+         real(c_double), dimension(*), intent(in) :: b
+         real(c_double), dimension(*), intent(out) :: sigma
+      end function etg_jacobian_transpose_transformation
+!
    end interface
 !
 contains

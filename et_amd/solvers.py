@@ -87,3 +87,67 @@ class DiisCCGroundState:
             wf.construct_t1_cholesky()
             previous = wf.energy
         raise _ccsd.ETGError('Did not converge in the max number of iterations.')
+
+
+class EigenDavidson:
+    """Lowest eigenpairs of the CC Jacobian, after eT's eigen Davidson solver.
+
+    Follows run_eigen_davidson_solver (src/solvers/eigen_davidson_solver_class.F90:122-318):
+    start vectors on the lowest orbital differences, each new trial vector
+    transformed by the Jacobian (right) or its transpose (left), reduced
+    matrix X^T (A X) and its non-symmetric eigenproblem
+    (eigen_davidson_tool_class.F90:217-379), residuals preconditioned by the
+    orbital differences and Gram-Schmidt orthonormalised against the space
+    (davidson_tool_class.F90:354).  Reduced-space linear algebra in FP64.
+    """
+
+    def __init__(self, wf, n_states, side='right', residual_threshold=1e-6, max_iterations=100,
+                 log=None):
+        self.wf = wf
+        self.n_states = n_states
+        self.side = side
+        self.residual_threshold = residual_threshold
+        self.max_iterations = max_iterations
+        self.log = log
+
+    def run(self):
+        wf = self.wf
+        wf.prepare_for_jacobian(self.side)
+        transform = (wf.jacobian_transformation if self.side == 'right'
+                     else wf.jacobian_transpose_transformation)
+        diag = wf.get_orbital_differences()
+        n = diag.size
+        V = np.zeros((n, 0))
+        AV = np.zeros((n, 0))
+        new = []
+        for k in np.argsort(diag, kind='stable')[:self.n_states]:
+            e = np.zeros(n)
+            e[k] = 1.0
+            new.append(e)
+        for it in range(1, self.max_iterations + 1):
+            for x in new:
+                for _ in range(2):
+                    x = x - V @ (V.T @ x)
+                nx = np.linalg.norm(x)
+                if nx < 1e-12:
+                    continue
+                x = x / nx
+                V = np.column_stack([V, x])
+                AV = np.column_stack([AV, transform(x)])
+            w, S = np.linalg.eig(V.T @ AV)
+            order = np.argsort(w.real)[:self.n_states]
+            w, S = w[order].real, S[:, order].real
+            new, norms = [], []
+            for r in range(self.n_states):
+                x = V @ S[:, r]
+                res = AV @ S[:, r] - w[r] * x
+                rn = float(np.linalg.norm(res) / np.linalg.norm(x))
+                norms.append(rn)
+                if rn > self.residual_threshold:
+                    new.append(res / (w[r] - diag + 1e-12))
+            if self.log:
+                self.log('%3d  %s  max|res| %.3e' % (it, ' '.join('%.10f' % e for e in w), max(norms)))
+            if not new:
+                self.eigenvectors = V @ S
+                return w
+        raise _ccsd.ETGError('Davidson did not converge in the max number of iterations.')

@@ -104,6 +104,23 @@ int etg_omega_norm(etg_handle* handle, double* norm);
  * Replaces: get_orbital_differences (src/wavefunctions/doubles/doubles_class.F90:421-470). */
 int etg_get_orbital_differences(etg_handle* handle, double* x, int64_t n);
 
+/* Records the residual program (T1 dressing, Fock, Omega) at the current
+ * amplitudes with every intermediate kept in HBM, for the Jacobian
+ * transformations below.  Invalidated by any change of amplitudes or inputs.
+ * Replaces: prepare_for_jacobian / prepare_for_jacobian_transpose
+ * (src/wavefunctions/ccsd/jacobian_ccsd.F90:46-71, jacobian_transpose_ccsd.F90:44-127). */
+int etg_prepare_for_jacobian(etg_handle* handle);
+
+/* rho = A c with A_mu,nu = <mu~| exp(-T) [H, tau_nu] exp(T) |R> in eT's
+ * biorthonormal packed basis (n_t1 + n_t2 entries each).
+ * Replaces: jacobian_transformation (src/wavefunctions/ccsd/jacobian_ccsd.F90:74-178). */
+int etg_jacobian_transformation(etg_handle* handle, const double* c, double* rho);
+
+/* sigma^T = b^T A (same A).
+ * Replaces: jacobian_transpose_transformation
+ * (src/wavefunctions/ccsd/jacobian_transpose_ccsd.F90:129-236). */
+int etg_jacobian_transpose_transformation(etg_handle* handle, const double* b, double* sigma);
+
 /* Virtual batch length of the ladder term's g_acbd blocks (memory/launch
  * trade-off; results do not depend on it beyond rounding). */
 int etg_set_ladder_batch(etg_handle* handle, int64_t batch);

@@ -98,3 +98,22 @@ def test_oracle_jacobian_reproduces_et_excitation_energies():
                           diag, 2, tol=1e-9)
     np.testing.assert_allclose(w, g['right_excitation_energies'], rtol=0, atol=1e-10)
     np.testing.assert_allclose(w, g['left_excitation_energies'], rtol=0, atol=1e-10)
+
+
+@pytest.mark.gpu
+def test_gpu_eom_reproduces_et_excitation_energies():
+    """GPU ground state, then right and left Davidson with the GPU Jacobian transformations."""
+    from et_amd import ccsd, solvers
+    g = fixture()['golden']
+    m = inputs()
+    n_o = m['n_o']
+    n_v = m['L_mo'].shape[1] - n_o
+    with ccsd.CCSD(n_o, n_v, m['n_J']) as wf:
+        wf.set_cholesky(m['L_mo'])
+        wf.set_oei(m['h_mo'])
+        wf.set_orbital_energies(m['eps'])
+        solvers.DiisCCGroundState(wf, residual_threshold=1e-11).run()
+        right = solvers.EigenDavidson(wf, 2, 'right', residual_threshold=1e-8).run()
+        left = solvers.EigenDavidson(wf, 2, 'left', residual_threshold=1e-8).run()
+    np.testing.assert_allclose(right, g['right_excitation_energies'], rtol=0, atol=1e-10)
+    np.testing.assert_allclose(left, g['left_excitation_energies'], rtol=0, atol=1e-10)

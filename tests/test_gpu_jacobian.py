@@ -1,0 +1,106 @@
+"""GPU Jacobian transformations through the C ABI against the oracle.
+
+rho = A c is checked against the oracle's complex-step derivative of the
+residual along D2 c (exact to rounding, pinned by eT's golden excitation
+energies); sigma = A^T b against the transpose of the oracle's full A on
+tiny systems, and on larger ones through <b, A c> = <A^T b, c>.  Tolerance:
+1e-11 relative to the largest element (FP64 throughout).
+"""
+import numpy as np
+import pytest
+
+from et_amd import ccsd
+from et_amd import synthetic as syn
+from oracle import ccsd as O
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    # n_o, n_v, n_J, seed, ladder batch
+    (1, 2, 4, 1, None),
+    (2, 3, 6, 2, None),
+    (3, 4, 9, 3, 2),
+    (4, 3, 7, 4, None),       # n_o > n_v
+    (3, 7, 12, 5, 3),
+]
+
+
+def make(n_o, n_v, n_J, seed):
+    return syn.system(seed, n_o, n_v, n_J, t2_sigma=0.03, t1_sigma=0.03, J_decay=6.0, L_scale=0.3)
+
+
+def load(s, batch=None):
+    wf = ccsd.CCSD(s['n_o'], s['n_v'], s['n_J'])
+    wf.set_cholesky(s['L'])
+    wf.set_oei(s['h'])
+    wf.set_orbital_energies(s['eps'])
+    wf.set_amplitudes(np.concatenate([s['t1'].ravel(order='F'), s['t2']]))
+    wf.construct_t1_cholesky()
+    if batch:
+        wf.set_ladder_batch(batch)
+    return wf
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.mark.parametrize('n_o,n_v,n_J,seed,batch', CASES)
+def test_right_jacobian_matches_complex_step(n_o, n_v, n_J, seed, batch):
+    print('case', n_o, n_v, n_J, seed, batch, flush=True)
+    s = make(n_o, n_v, n_J, seed)
+    n = n_o * n_v + O.n_t2(n_o, n_v)
+    rng = np.random.default_rng(seed)
+    with load(s, batch) as wf:
+        om_direct = wf.construct_omega()
+        wf.prepare_for_jacobian()
+        om_tape = wf.get_omega()
+        assert rel(om_tape, om_direct) < 1e-13          # the recorded program is the residual
+        for _ in range(2):
+            c = rng.normal(size=n)
+            rho = wf.jacobian_transformation(c)
+            ref = O.jacobian_transform(s['L'], s['h'], s['t1'], s['t2'], n_o, c)
+            assert rel(rho, ref) < 1e-11
+
+
+@pytest.mark.parametrize('n_o,n_v,n_J,seed,batch', CASES[:4])
+def test_left_jacobian_is_transpose_of_oracle_matrix(n_o, n_v, n_J, seed, batch):
+    print('case', n_o, n_v, n_J, seed, batch, flush=True)
+    s = make(n_o, n_v, n_J, seed)
+    A = O.jacobian_matrix(s['L'], s['h'], s['t1'], s['t2'], n_o)
+    rng = np.random.default_rng(100 + seed)
+    with load(s, batch) as wf:
+        wf.prepare_for_jacobian()
+        for _ in range(2):
+            b = rng.normal(size=A.shape[0])
+            sigma = wf.jacobian_transpose_transformation(b)
+            assert rel(sigma, A.T @ b) < 1e-11
+
+
+def test_adjoint_identity_larger_system():
+    n_o, n_v, n_J = 6, 17, 30
+    s = make(n_o, n_v, n_J, 9)
+    n = n_o * n_v + O.n_t2(n_o, n_v)
+    rng = np.random.default_rng(7)
+    with load(s, 5) as wf:
+        wf.prepare_for_jacobian()
+        for _ in range(3):
+            b, c = rng.normal(size=n), rng.normal(size=n)
+            Ac = wf.jacobian_transformation(c)
+            Atb = wf.jacobian_transpose_transformation(b)
+            lhs, rhs = float(b @ Ac), float(Atb @ c)
+            assert abs(lhs - rhs) < 1e-11 * (np.abs(b).sum() * np.abs(Ac).max())
+        ref = O.jacobian_transform(s['L'], s['h'], s['t1'], s['t2'], n_o, c)
+        assert rel(Ac, ref) < 1e-11
+
+
+def test_jacobian_requires_preparation():
+    s = make(2, 3, 5, 4)
+    with load(s) as wf:
+        with pytest.raises(ccsd.ETGError):
+            wf.jacobian_transformation(np.zeros(wf.n_gs_amplitudes))
+        wf.prepare_for_jacobian()
+        wf.jacobian_transformation(np.zeros(wf.n_gs_amplitudes))
+        wf.set_amplitudes(wf.get_amplitudes())          # invalidates the record
+        with pytest.raises(ccsd.ETGError):
+            wf.jacobian_transpose_transformation(np.zeros(wf.n_gs_amplitudes))
