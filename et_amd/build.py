@@ -17,9 +17,12 @@ LIB = os.path.join(HERE, 'libetgpu.so')
 OBJDIR = os.path.join(HERE, '_build')
 ARCH = os.environ.get('ETG_OFFLOAD_ARCH', 'gfx950')
 
-# packed-FP32 instructions are disabled for all device code
+# packed-FP32 instructions are disabled for all device code.  Matrix-core
+# accumulators are kept in architectural VGPRs (the VGPR form of MFMA): with
+# AGPR accumulators the compiler copies them out and back on every k step.
 HIPFLAGS = ['-O3', '-std=c++17', '-fPIC', f'--offload-arch={ARCH}',
             '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
+            '-mllvm', '-amdgpu-mfma-vgpr-form=1',
             '-Wall', '-Wno-unused-result']
 
 

@@ -82,6 +82,17 @@ struct Mfma<float> {
   static __device__ inline int row(int lrow, int r) { return 4 * lrow + r; }
 };
 
+// Raw buffer access (gfx9 descriptor word 3).  Offsets at or past kOutOfRange
+// are beyond the record count and read as 0.
+constexpr uint32_t kOutOfRange = 0x80000000u;
+__device__ inline __amdgpu_buffer_rsrc_t make_rsrc(const double* base) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(base), (short)0, int(kOutOfRange - 1),
+                                           0x00020000);
+}
+__device__ inline double load_f64(__amdgpu_buffer_rsrc_t r, uint32_t byte_off) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, byte_off, 0, 0));
+}
+
 // LDS tile orientation follows the operand's contiguous dimension in HBM so
 // that both the global loads and the LDS stores are unit-stride:
 //   contiguous along k  -> tile[mn][k], leading dim BK + 1 (odd stride: the 16
@@ -96,13 +107,11 @@ struct TileLayout {
 };
 
 // T: arithmetic type of the products (double or float); operands are FP64 in memory.
-// Workgroups per CU the register budget is sized for: the FP64 128 x 128 tile
-// keeps 128 accumulator registers per lane and runs one wave per SIMD.
-template <typename T, int BM, int BN>
-constexpr int min_blocks() { return (sizeof(T) == 8 && BM * BN >= 128 * 128) ? 1 : 2; }
-
+// Two workgroups per CU: the FP64 128 x 128 tile fits its 128 accumulator
+// registers and operand staging in 256 VGPRs and 72 KB of LDS, so one
+// workgroup's barrier and LDS refill overlap the other's matrix-core work.
 template <typename T, bool TA, bool TB, int BM, int BN, int BK>
-__global__ __launch_bounds__(kThreads, (min_blocks<T, BM, BN>())) void gemm_mfma16(Params p) {
+__global__ __launch_bounds__(kThreads, 2) void gemm_mfma16(Params p) {
   using M = Mfma<T>;
   using acc_t = typename M::acc_t;
   constexpr int WM = BM / 2, WN = BN / 2;  // wave sub-tile
@@ -131,7 +140,11 @@ __global__ __launch_bounds__(kThreads, (min_blocks<T, BM, BN>())) void gemm_mfma
 
   T ra[A_PER], rb[B_PER];
 
-  // element r of this thread's share of the tile: (mn, k) inside the tile
+  // Element r of this thread's share of a tile sits at (mn, k) inside the tile.
+  // Tiles are read with buffer loads: an element outside the matrix or beyond
+  // the split's k range gets an offset past the buffer's record count and the
+  // hardware returns 0 for it.  Every load is unconditional, so the k loop has
+  // no branches and the accumulators stay in place across iterations.
   auto a_coord = [&](int r, int& mm, int& kk) {
     const int idx = tid + r * kThreads;
     if (TA) { kk = idx % BK; mm = idx / BK; } else { mm = idx % BM; kk = idx / BM; }
@@ -140,24 +153,38 @@ __global__ __launch_bounds__(kThreads, (min_blocks<T, BM, BN>())) void gemm_mfma
     const int idx = tid + r * kThreads;
     if (TB) { nn = idx % BN; kk = idx / BN; } else { kk = idx % BK; nn = idx / BK; }
   };
+  const uint32_t lda = uint32_t(p.lda), ldb = uint32_t(p.ldb);
+  uint32_t a_off[A_PER], b_off[B_PER];   // byte offsets from the tile's (row, k0) origin
+#pragma unroll
+  for (int r = 0; r < A_PER; ++r) {
+    int mm, kk;
+    a_coord(r, mm, kk);
+    a_off[r] = (m0 + mm < p.m) ? 8u * (TA ? uint32_t(mm) * lda + kk : uint32_t(kk) * lda + mm) : kOutOfRange;
+  }
+#pragma unroll
+  for (int r = 0; r < B_PER; ++r) {
+    int nn, kk;
+    b_coord(r, nn, kk);
+    b_off[r] = (n0 + nn < p.n) ? 8u * (TB ? uint32_t(kk) * ldb + nn : uint32_t(nn) * ldb + kk) : kOutOfRange;
+  }
+  const double* A0 = A + (TA ? m0 * p.lda : m0);
+  const double* B0 = B + (TB ? n0 : n0 * p.ldb);
   auto load = [&](int64_t k0) {
+    const __amdgpu_buffer_rsrc_t ra_rsrc = make_rsrc(A0 + (TA ? k0 : k0 * p.lda));
+    const __amdgpu_buffer_rsrc_t rb_rsrc = make_rsrc(B0 + (TB ? k0 * p.ldb : k0));
+    const int64_t left = kend - k0;
+    const int kleft = left < BK ? int(left) : BK;
 #pragma unroll
     for (int r = 0; r < A_PER; ++r) {
       int mm, kk;
       a_coord(r, mm, kk);
-      const int64_t gm = m0 + mm, gk = k0 + kk;
-      double v = 0.0;
-      if (gm < p.m && gk < kend) v = TA ? A[gk + gm * p.lda] : A[gm + gk * p.lda];
-      ra[r] = M::cvt(v);
+      ra[r] = M::cvt(load_f64(ra_rsrc, kk < kleft ? a_off[r] : kOutOfRange));
     }
 #pragma unroll
     for (int r = 0; r < B_PER; ++r) {
       int nn, kk;
       b_coord(r, nn, kk);
-      const int64_t gn = n0 + nn, gk = k0 + kk;
-      double v = 0.0;
-      if (gn < p.n && gk < kend) v = TB ? B[gn + gk * p.ldb] : B[gk + gn * p.ldb];
-      rb[r] = M::cvt(v);
+      rb[r] = M::cvt(load_f64(rb_rsrc, kk < kleft ? b_off[r] : kOutOfRange));
     }
   };
   auto store = [&](int buf) {
@@ -182,6 +209,7 @@ __global__ __launch_bounds__(kThreads, (min_blocks<T, BM, BN>())) void gemm_mfma
     for (int j = 0; j < TNs; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
   const int64_t nsteps = (kend > kbeg) ? (kend - kbeg + BK - 1) / BK : 0;
+  const int64_t klast = kbeg + (nsteps - 1) * BK;
   if (nsteps > 0) {
     load(kbeg);
     store(0);
@@ -190,7 +218,8 @@ __global__ __launch_bounds__(kThreads, (min_blocks<T, BM, BN>())) void gemm_mfma
   const int lrow = lane >> 4, lcol = lane & 15;
   for (int64_t s = 0; s < nsteps; ++s) {
     const int buf = int(s & 1);
-    if (s + 1 < nsteps) load(kbeg + (s + 1) * BK);
+    // The last step reloads its own tile into the idle buffer: no branch.
+    load(min(kbeg + (s + 1) * BK, klast));
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       T a[TMs], b[TNs];
@@ -203,7 +232,7 @@ __global__ __launch_bounds__(kThreads, (min_blocks<T, BM, BN>())) void gemm_mfma
 #pragma unroll
         for (int j = 0; j < TNs; ++j) acc[i][j] = M::run(a[i], b[j], acc[i][j]);
     }
-    if (s + 1 < nsteps) store(buf ^ 1);
+    store(buf ^ 1);
     __syncthreads();
   }
 
@@ -348,6 +377,8 @@ GemmLaunch gemm(hipStream_t stream, const Gemm& g, Prec prec, GemmWorkspace* ws)
   if (g.lda < std::max<int64_t>(1, rowsA)) fail("gemm: lda too small");
   if (g.ldb < std::max<int64_t>(1, rowsB)) fail("gemm: ldb too small");
   if (g.ldc < std::max<int64_t>(1, g.m)) fail("gemm: ldc too small");
+  // byte offsets inside a tile (128 rows or columns of an operand) are 31-bit
+  if (std::max(g.lda, g.ldb) >= (int64_t(1) << 21)) fail("gemm: leading dimension too large");
   if (g.batch > 65535) fail("gemm: batch too large");
   if (g.k == 0) {
     const int64_t mn = g.m * g.n;
