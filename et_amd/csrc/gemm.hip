@@ -110,7 +110,7 @@ struct TileLayout {
 // Two workgroups per CU: the FP64 128 x 128 tile fits its 128 accumulator
 // registers and operand staging in 256 VGPRs and 72 KB of LDS, so one
 // workgroup's barrier and LDS refill overlap the other's matrix-core work.
-template <typename T, bool TA, bool TB, int BM, int BN, int BK>
+template <typename T, bool TA, bool TB, int BM, int BN, int BK, bool WIDE>
 __global__ __launch_bounds__(kThreads, 2) void gemm_mfma16(Params p) {
   using M = Mfma<T>;
   using acc_t = typename M::acc_t;
@@ -145,6 +145,11 @@ __global__ __launch_bounds__(kThreads, 2) void gemm_mfma16(Params p) {
   // the split's k range gets an offset past the buffer's record count and the
   // hardware returns 0 for it.  Every load is unconditional, so the k loop has
   // no branches and the accumulators stay in place across iterations.
+  // Byte offsets are 31-bit.  With WIDE = false each operand has one buffer
+  // base per k step and a per-element offset (leading dimensions < 2^21);
+  // with WIDE = true element r gets its own base, r * SA (SB) operand columns
+  // after element 0, and the per-thread offset spans at most 16 columns
+  // (leading dimensions < 2^24) at the cost of more scalar work per step.
   auto a_coord = [&](int r, int& mm, int& kk) {
     const int idx = tid + r * kThreads;
     if (TA) { kk = idx % BK; mm = idx / BK; } else { mm = idx % BM; kk = idx / BM; }
@@ -153,38 +158,48 @@ __global__ __launch_bounds__(kThreads, 2) void gemm_mfma16(Params p) {
     const int idx = tid + r * kThreads;
     if (TB) { nn = idx % BN; kk = idx / BN; } else { kk = idx % BK; nn = idx / BK; }
   };
+  constexpr int SA = TA ? kThreads / BK : kThreads / BM;    // operand columns between elements
+  constexpr int SB = TB ? kThreads / BN : kThreads / BK;
+  static_assert(kThreads % BK == 0 && kThreads % BM == 0 && kThreads % BN == 0, "tile/threads");
   const uint32_t lda = uint32_t(p.lda), ldb = uint32_t(p.ldb);
-  uint32_t a_off[A_PER], b_off[B_PER];   // byte offsets from the tile's (row, k0) origin
+  uint32_t a_off[A_PER], b_off[B_PER];   // in-range byte offsets (kOutOfRange outside the matrix)
 #pragma unroll
   for (int r = 0; r < A_PER; ++r) {
     int mm, kk;
     a_coord(r, mm, kk);
-    a_off[r] = (m0 + mm < p.m) ? 8u * (TA ? uint32_t(mm) * lda + kk : uint32_t(kk) * lda + mm) : kOutOfRange;
+    const int mo = WIDE ? mm - r * (TA ? SA : 0) : mm, ko = WIDE ? kk - r * (TA ? 0 : SA) : kk;
+    a_off[r] = (m0 + mm < p.m) ? 8u * (TA ? uint32_t(mo) * lda + ko : uint32_t(ko) * lda + mo)
+                               : kOutOfRange;
   }
 #pragma unroll
   for (int r = 0; r < B_PER; ++r) {
     int nn, kk;
     b_coord(r, nn, kk);
-    b_off[r] = (n0 + nn < p.n) ? 8u * (TB ? uint32_t(kk) * ldb + nn : uint32_t(nn) * ldb + kk) : kOutOfRange;
+    const int no = WIDE ? nn - r * (TB ? 0 : SB) : nn, ko = WIDE ? kk - r * (TB ? SB : 0) : kk;
+    b_off[r] = (n0 + nn < p.n) ? 8u * (TB ? uint32_t(ko) * ldb + no : uint32_t(no) * ldb + ko)
+                               : kOutOfRange;
   }
   const double* A0 = A + (TA ? m0 * p.lda : m0);
   const double* B0 = B + (TB ? n0 : n0 * p.ldb);
   auto load = [&](int64_t k0) {
-    const __amdgpu_buffer_rsrc_t ra_rsrc = make_rsrc(A0 + (TA ? k0 : k0 * p.lda));
-    const __amdgpu_buffer_rsrc_t rb_rsrc = make_rsrc(B0 + (TB ? k0 * p.ldb : k0));
+    const double* Ak = A0 + (TA ? k0 : k0 * p.lda);
+    const double* Bk = B0 + (TB ? k0 * p.ldb : k0);
+    const __amdgpu_buffer_rsrc_t a_rsrc = make_rsrc(Ak), b_rsrc = make_rsrc(Bk);
     const int64_t left = kend - k0;
     const int kleft = left < BK ? int(left) : BK;
 #pragma unroll
     for (int r = 0; r < A_PER; ++r) {
       int mm, kk;
       a_coord(r, mm, kk);
-      ra[r] = M::cvt(load_f64(ra_rsrc, kk < kleft ? a_off[r] : kOutOfRange));
+      const uint32_t off = kk < kleft ? a_off[r] : kOutOfRange;
+      ra[r] = M::cvt(load_f64(WIDE ? make_rsrc(Ak + int64_t(r) * SA * p.lda) : a_rsrc, off));
     }
 #pragma unroll
     for (int r = 0; r < B_PER; ++r) {
       int nn, kk;
       b_coord(r, nn, kk);
-      rb[r] = M::cvt(load_f64(rb_rsrc, kk < kleft ? b_off[r] : kOutOfRange));
+      const uint32_t off = kk < kleft ? b_off[r] : kOutOfRange;
+      rb[r] = M::cvt(load_f64(WIDE ? make_rsrc(Bk + int64_t(r) * SB * p.ldb) : b_rsrc, off));
     }
   };
   auto store = [&](int buf) {
@@ -292,35 +307,77 @@ __global__ void splitk_reduce(const double* __restrict__ W, int splits, int64_t 
   *c = (beta == 0.0) ? v : v + beta * *c;
 }
 
-template <typename T, bool TA, bool TB, int BM, int BN>
+template <typename T, bool TA, bool TB, int BM, int BN, bool WIDE>
 const char* kernel_name() {
   static const std::string name = std::string("gemm_mfma16<") + (sizeof(T) == 8 ? "f64," : "f32,") +
                                   (TA ? "T" : "N") + (TB ? "T" : "N") + "," + std::to_string(BM) +
-                                  "x" + std::to_string(BN) + ">";
+                                  "x" + std::to_string(BN) + (WIDE ? ",wide" : "") + ">";
   return name.c_str();
 }
 
-template <typename T, int BM, int BN, int BK>
+template <typename T, int BM, int BN, int BK, bool WIDE>
 const char* launch_cfg(hipStream_t stream, const Gemm& g, Params p, int splits) {
   p.tiles_m = int(ceil_div(g.m, BM));
   p.tiles_n = int(ceil_div(g.n, BN));
   dim3 grid(p.tiles_m * p.tiles_n, splits, unsigned(g.batch));
   const char* name;
   if (!g.ta && !g.tb) {
-    gemm_mfma16<T, false, false, BM, BN, BK><<<grid, kThreads, 0, stream>>>(p);
-    name = kernel_name<T, false, false, BM, BN>();
+    gemm_mfma16<T, false, false, BM, BN, BK, WIDE><<<grid, kThreads, 0, stream>>>(p);
+    name = kernel_name<T, false, false, BM, BN, WIDE>();
   } else if (!g.ta && g.tb) {
-    gemm_mfma16<T, false, true, BM, BN, BK><<<grid, kThreads, 0, stream>>>(p);
-    name = kernel_name<T, false, true, BM, BN>();
+    gemm_mfma16<T, false, true, BM, BN, BK, WIDE><<<grid, kThreads, 0, stream>>>(p);
+    name = kernel_name<T, false, true, BM, BN, WIDE>();
   } else if (g.ta && !g.tb) {
-    gemm_mfma16<T, true, false, BM, BN, BK><<<grid, kThreads, 0, stream>>>(p);
-    name = kernel_name<T, true, false, BM, BN>();
+    gemm_mfma16<T, true, false, BM, BN, BK, WIDE><<<grid, kThreads, 0, stream>>>(p);
+    name = kernel_name<T, true, false, BM, BN, WIDE>();
   } else {
-    gemm_mfma16<T, true, true, BM, BN, BK><<<grid, kThreads, 0, stream>>>(p);
-    name = kernel_name<T, true, true, BM, BN>();
+    gemm_mfma16<T, true, true, BM, BN, BK, WIDE><<<grid, kThreads, 0, stream>>>(p);
+    name = kernel_name<T, true, true, BM, BN, WIDE>();
   }
   ETG_LAUNCH("gemm_mfma16");
   return name;
+}
+
+template <typename T, int BM, int BN, int BK>
+const char* launch_tile(hipStream_t stream, const Gemm& g, const Params& p, int splits) {
+  // one buffer base per operand covers 128 columns when the leading dimensions are below 2^21
+  const bool wide = std::max(g.lda, g.ldb) >= (int64_t(1) << 21);
+  return wide ? launch_cfg<T, BM, BN, BK, true>(stream, g, p, splits)
+              : launch_cfg<T, BM, BN, BK, false>(stream, g, p, splits);
+}
+
+struct Choice {
+  int bm = 128, splits = 1;
+};
+
+Choice choose_tiling(const Gemm& g, double peak, bool can_split) {
+  struct Cfg { int bm; int64_t slots; double eff; };
+  const Cfg cfgs[2] = {{128, 512, 0.83}, {64, 1024, 0.62}};
+  const double hbm = 5.0e12;                       // sustained bytes/s for the split-K round trip
+  const double max_ws = 2.0e9;                     // bytes of split-K partials
+  Choice best;
+  double best_t = 1e300;
+  for (const Cfg& c : cfgs) {
+    const int64_t tiles = ceil_div(g.m, c.bm) * ceil_div(g.n, c.bm) * g.batch;
+    static const int kSplits[] = {1,  2,  3,  4,  5,  6,  7,  8,  9,   10,  11,  12,  13,  14,  16,
+                                  18, 20, 24, 28, 32, 40, 48, 64, 96, 128, 192, 256, 384, 512, 1024};
+    for (int s : kSplits) {
+      if (s > 1 && (!can_split || g.k < int64_t(s) * 256)) break;
+      const double mn = double(g.m) * double(g.n) * double(g.batch);
+      if (s > 1 && s * mn * 8.0 > max_ws) break;
+      const int64_t kps = ceil_div(ceil_div(g.k, s), 16) * 16;
+      const int64_t rounds = ceil_div(tiles * s, c.slots);
+      const double t_round = double(c.slots) * 2.0 * c.bm * c.bm * double(kps) / (c.eff * peak);
+      double t = double(rounds) * t_round;
+      if (s > 1) t += (2.0 * s + 2.0) * mn * 8.0 / hbm + 4e-6;
+      if (t < best_t * 0.98) {
+        best_t = t;
+        best.bm = c.bm;
+        best.splits = s;
+      }
+    }
+  }
+  return best;
 }
 
 template <typename T>
@@ -334,17 +391,13 @@ GemmLaunch launch_typed(hipStream_t stream, const Gemm& g, GemmWorkspace* ws) {
   p.W = nullptr;
   p.k_per_split = g.k;
 
-  // Tile size: large tiles when the output is large enough to fill the chip.
-  const bool big = (ceil_div(g.m, 128) * ceil_div(g.n, 128) * g.batch) >= 512;
-  const int64_t BMc = big ? 128 : 64, BNc = big ? 128 : 64;
-  const int64_t tiles = ceil_div(g.m, BMc) * ceil_div(g.n, BNc) * g.batch;
-  // Split K when there are too few output tiles and K is long.
-  int splits = 1;
-  const int64_t target = 1024;
-  if (tiles < target && g.k >= 2048 && ws != nullptr) {
-    splits = int(std::min<int64_t>(std::max<int64_t>(1, target / tiles), g.k / 512));
-    splits = std::max(1, std::min(splits, 256));
-  }
+  // Tile size and split-K count from a cost model: a launch runs in rounds of
+  // `slots` concurrent tiles (two 128 x 128 or four 64 x 64 workgroups per CU
+  // on 256 CUs), each round at the tile's sustained matrix-core efficiency;
+  // a split adds its partial sums' round trip through HBM.
+  const Choice ch = choose_tiling(g, sizeof(T) == 8 ? 78.6e12 : 157.3e12, ws != nullptr);
+  const bool big = ch.bm == 128;
+  int splits = ch.splits;
   if (splits > 1) {
     const int64_t kps = ceil_div(ceil_div(g.k, splits), 16) * 16;
     splits = int(ceil_div(g.k, kps));
@@ -355,8 +408,8 @@ GemmLaunch launch_typed(hipStream_t stream, const Gemm& g, GemmWorkspace* ws) {
   }
   GemmLaunch info;
   info.splits = splits;
-  if (big) info.kernel = launch_cfg<T, 128, 128, 16>(stream, g, p, splits);
-  else info.kernel = launch_cfg<T, 64, 64, 16>(stream, g, p, splits);
+  if (big) info.kernel = launch_tile<T, 128, 128, 16>(stream, g, p, splits);
+  else info.kernel = launch_tile<T, 64, 64, 16>(stream, g, p, splits);
   if (splits > 1) {
     const int64_t mn = g.m * g.n;
     dim3 grid(unsigned(ceil_div(mn, 256)), 1, unsigned(g.batch));
@@ -377,8 +430,8 @@ GemmLaunch gemm(hipStream_t stream, const Gemm& g, Prec prec, GemmWorkspace* ws)
   if (g.lda < std::max<int64_t>(1, rowsA)) fail("gemm: lda too small");
   if (g.ldb < std::max<int64_t>(1, rowsB)) fail("gemm: ldb too small");
   if (g.ldc < std::max<int64_t>(1, g.m)) fail("gemm: ldc too small");
-  // byte offsets inside a tile (128 rows or columns of an operand) are 31-bit
-  if (std::max(g.lda, g.ldb) >= (int64_t(1) << 21)) fail("gemm: leading dimension too large");
+  // per-thread byte offsets inside a tile span 16 operand columns and are 31-bit
+  if (std::max(g.lda, g.ldb) >= (int64_t(1) << 24)) fail("gemm: leading dimension too large");
   if (g.batch > 65535) fail("gemm: batch too large");
   if (g.k == 0) {
     const int64_t mn = g.m * g.n;

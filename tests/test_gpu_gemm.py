@@ -45,3 +45,22 @@ def test_zero_k_scales_c():
     C = np.arange(12.0).reshape(3, 4)
     out = ccsd.device_gemm('N', 'N', 1.0, np.zeros((3, 0)), np.zeros((0, 4)), 2.0, C)
     np.testing.assert_array_equal(out, 2.0 * C)
+
+
+BIG = (1 << 21) + 3      # leading dimension past one buffer base per tile (per-element bases)
+
+
+@pytest.mark.parametrize('ta,tb,m,n,k', [
+    ('N', 'N', BIG, 3, 8),       # A: lda = BIG, contiguous along m
+    ('T', 'N', 5, 6, BIG),       # A and B contiguous along k, lda = ldb = BIG (split-K)
+    ('N', 'T', 7, BIG, 5),       # B stored n x k, ldb = BIG
+    ('T', 'T', 4, BIG, 6),
+])
+def test_fp64_gemm_large_leading_dimension(ta, tb, m, n, k):
+    rng = np.random.default_rng(11)
+    A = rng.normal(size=(k, m) if ta == 'T' else (m, k))
+    B = rng.normal(size=(n, k) if tb == 'T' else (k, n))
+    C = rng.normal(size=(m, n))
+    out = ccsd.device_gemm(ta, tb, 1.1, A, B, 0.5, C)
+    ref = 1.1 * (A.T if ta == 'T' else A) @ (B.T if tb == 'T' else B) + 0.5 * C
+    assert np.abs(out - ref).max() < 1e-14 * (np.sqrt(k) + 1.0) * 8
