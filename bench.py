@@ -26,7 +26,8 @@ from et_amd import flops as flopcount  # noqa: E402
 from et_amd import synthetic as syn  # noqa: E402
 
 METRIC = 'CCSD omega evals/s & time-to-solution, mixed vs FP64 GPU at |ΔE|≤1e-8 Eh'
-PEAKS = {'fp64': 78.6e12, 'fp32': 157.3e12}
+# dense matrix-core peaks; split-bf16 counts its three bf16 products per term
+PEAKS = {'fp64': 78.6e12, 'fp32': 157.3e12, 'bf16x3': 2.5e15 / 3}
 WORKLOAD = dict(n_o=48, n_v=400, n_J=1600, seed=0)
 
 
@@ -82,7 +83,7 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=3)
     ap.add_argument('--warmup', type=int, default=1)
-    ap.add_argument('--precision', choices=['fp64', 'fp32'], default='fp64')
+    ap.add_argument('--precision', choices=['fp64', 'fp32', 'bf16x3'], default='fp64')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-mixed', action='store_true')
     args = ap.parse_args()
@@ -144,18 +145,28 @@ def main():
 
     extra = {}
     if not args.no_mixed and args.precision == 'fp64':
+        # the same residual in the reduced-precision forms, against the FP64 one
         ref = wf.get_omega()
-        t0 = time.perf_counter()
-        wf.construct_omega(ccsd.FP32, copy_out=False)
-        wf.synchronize()
-        dt32 = time.perf_counter() - t0
-        om32 = wf.get_omega()
-        extra['mixed'] = {
-            'precision': 'fp32-mfma',
-            'omega_evals_per_s': 1.0 / dt32,
-            'max_abs_err_vs_fp64': float(np.abs(om32 - ref).max()),
-            'rel_err_vs_fp64': float(np.abs(om32 - ref).max() / np.abs(ref).max()),
-        }
+        forms = []
+        for name, p in (('fp32-mfma', ccsd.FP32), ('split-bf16x3', ccsd.BF16X3)):
+            wf.construct_omega(p, copy_out=False)           # first call sizes the workspaces
+            wf.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(2):
+                wf.construct_omega(p, copy_out=False)
+            wf.synchronize()
+            dtp = (time.perf_counter() - t0) / 2
+            om = wf.get_omega()
+            err = np.abs(om - ref)
+            forms.append({
+                'precision': name,
+                'omega_evals_per_s': 1.0 / dtp,
+                'speedup_vs_fp64': dt / dtp,
+                'max_abs_err_vs_fp64': float(err.max()),
+                'rel_err_vs_fp64': float(err.max() / np.abs(ref).max()),
+            })
+            log('%s: %.3f s per residual, rel err %.2e' % (name, dtp, forms[-1]['rel_err_vs_fp64']))
+        extra['mixed'] = forms
     wf.close()
 
     cpu = None
@@ -173,7 +184,7 @@ def main():
         'higher_is_better': True,
         'scaling': 'weak',
         'vs_baseline': None,
-        'dtype': 'f64' if args.precision == 'fp64' else 'f32',
+        'dtype': {'fp64': 'f64', 'fp32': 'f32', 'bf16x3': 'bf16x3-f32acc'}[args.precision],
         'data': 'synthetic',
         'config': {'workload': 'omega_o48_v400_J1600_seed0', 'n_o': n_o, 'n_v': n_v, 'n_J': n_J,
                    'seed': w['seed'], 'precision': args.precision, 'F_omega': f_omega},

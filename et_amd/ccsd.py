@@ -17,7 +17,8 @@ LIB_PATH = os.path.join(_HERE, 'libetgpu.so')
 
 FP64 = 0
 FP32 = 1
-PRECISIONS = {'fp64': FP64, 'fp32': FP32}
+BF16X3 = 2
+PRECISIONS = {'fp64': FP64, 'fp32': FP32, 'bf16x3': BF16X3}
 
 
 class ETGError(RuntimeError):
@@ -53,6 +54,9 @@ _SIGNATURES = {
     'etg_prepare_for_jacobian': [ctypes.c_void_p],
     'etg_jacobian_transformation': [ctypes.c_void_p, _c_dbl_p, _c_dbl_p],
     'etg_jacobian_transpose_transformation': [ctypes.c_void_p, _c_dbl_p, _c_dbl_p],
+    'etg_diis_initialize': [ctypes.c_void_p, _i64],
+    'etg_quasi_newton_update': [ctypes.c_void_p],
+    'etg_diis_update': [ctypes.c_void_p],
     'etg_set_kernel_ledger': [ctypes.c_void_p, ctypes.c_int],
     'etg_kernel_ledger': [ctypes.c_void_p, _i64, ctypes.c_char_p, _i64, ctypes.POINTER(_i64),
                           _c_dbl_p, _c_dbl_p, ctypes.POINTER(_i64)],
@@ -257,6 +261,18 @@ class CCSD:
         sigma = np.empty(self.n_gs_amplitudes)
         _check(self._lib.etg_jacobian_transpose_transformation(self._h, _ptr(b), _ptr(sigma)))
         return sigma
+
+    def diis_initialize(self, dimension=8):
+        """HBM records for DIIS (diis_tool_class.F90:249-277)."""
+        _check(self._lib.etg_diis_initialize(self._h, int(dimension)))
+
+    def quasi_newton_update(self):
+        """Device quasi-Newton step t += -omega/eps (quasi_newton_updater_class.F90:81-118)."""
+        _check(self._lib.etg_quasi_newton_update(self._h))
+
+    def diis_update(self):
+        """Device DIIS extrapolation of the amplitudes (diis_tool_class.F90:302-347)."""
+        _check(self._lib.etg_diis_update(self._h))
 
     def set_kernel_ledger(self, on):
         _check(self._lib.etg_set_kernel_ledger(self._h, 1 if on else 0))

@@ -40,6 +40,7 @@ etg::Prec prec_of(int p) {
   switch (p) {
     case ETG_FP64: return etg::Prec::F64;
     case ETG_FP32: return etg::Prec::F32;
+    case ETG_BF16X3: return etg::Prec::BF16X3;
     default: etg::fail("unknown precision " + std::to_string(p));
   }
 }
@@ -165,6 +166,18 @@ int etg_jacobian_transformation(etg_handle* h, const double* c, double* rho) {
 
 int etg_jacobian_transpose_transformation(etg_handle* h, const double* b, double* sigma) {
   return guarded([&] { wf_of(h).jacobian_transpose_transform(b, sigma); });
+}
+
+int etg_diis_initialize(etg_handle* h, int64_t dimension) {
+  return guarded([&] { wf_of(h).diis_initialize(dimension); });
+}
+
+int etg_quasi_newton_update(etg_handle* h) {
+  return guarded([&] { wf_of(h).quasi_newton_update(); });
+}
+
+int etg_diis_update(etg_handle* h) {
+  return guarded([&] { wf_of(h).diis_update(); });
 }
 
 int etg_set_kernel_ledger(etg_handle* h, int on) {

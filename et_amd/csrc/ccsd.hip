@@ -202,6 +202,7 @@ void CCSD::set_amplitudes(const double* t) {
                          hipMemcpyHostToDevice, stream_));
   synchronize();
   have_t_ = true;
+  have_omega_ = have_update_ = false;
 }
 
 void CCSD::get_amplitudes(double* t) {
@@ -344,6 +345,7 @@ void CCSD::set_t2_to_mp2_guess() {
   mp2_guess(stream_, g, eps_.as<double>(), t_.as<double>() + n_t1, n_o, n_v);
   synchronize();
   have_t_ = true;
+  have_omega_ = have_update_ = false;
 }
 
 void CCSD::get_orbital_differences(double* host, int64_t n) {
@@ -383,6 +385,7 @@ void CCSD::construct_omega(Prec prec) {
   if (!have_F_) construct_fock();
   last_omega_flops = 0.0;
   program_omega(X, prec);
+  have_omega_ = true;
 }
 
 void CCSD::omega_singles(Ops& X, Prec prec, const double* U) {
@@ -708,6 +711,7 @@ void CCSD::prepare_for_jacobian() {
   synchronize();
   tape_ = std::move(tape);
   have_F_ = true;
+  have_omega_ = true;
 }
 
 void CCSD::jacobian_transform(const double* c, double* rho) {

@@ -38,6 +38,13 @@ class CCSD {
   void get_orbital_differences(double* host, int64_t n);
   double omega_norm();                            // ||omega||_2 (FP64 reduction)
 
+  // ground-state amplitude update on the device (FP64): quasi-Newton step and
+  // DIIS extrapolation over records kept in HBM
+  static constexpr int kMaxDiis = 32;
+  void diis_initialize(int64_t dimension);
+  void quasi_newton_update();                     // e = -omega/eps (doubles diagonal x2); t += e
+  void diis_update();                             // records (e, t); t = DIIS-extrapolated t
+
   // Jacobian transformations at the current amplitudes (A = dOmega/dt D2)
   void prepare_for_jacobian();                    // records the residual program
   void jacobian_transform(const double* c_host, double* rho_host);            // rho = A c
@@ -100,6 +107,12 @@ class CCSD {
   DeviceBuffer pair_a_, pair_b_, pair_col_, pcol_a_, pcol_b_;
   DeviceBuffer tp_, tm_, wp_, wm_, gblk_, gp_, gm_;
   DeviceBuffer dtp_, dtm_, dwp_, dwm_, dgblk_, dgp_, dgm_;
+
+  // DIIS records: errors and parameters, dim x (n_t1 + n_t2) each
+  int diis_dim_ = 0;
+  int64_t diis_count_ = 0;
+  DeviceBuffer diis_e_, diis_x_, diis_G_, diis_w_, diis_info_, update_;
+  bool have_update_ = false, have_omega_ = false;
 
   // Jacobian record
   std::unique_ptr<Tape> tape_;

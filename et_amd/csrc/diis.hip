@@ -1,0 +1,187 @@
+// Ground-state amplitude update and DIIS extrapolation on the device, in FP64.
+//
+// eT's solver loop (run_diis_cc_gs, diis_cc_gs_class.F90:253-293) calls
+//   t_updater%update(wf, amplitudes, omega)   -> quasi-Newton step
+//   diis%update(omega, amplitudes)            -> DIIS extrapolation
+// Both work here on the handle's HBM-resident vectors: the update vector, the
+// DIIS records (errors and parameters), the DIIS matrix G and its weights
+// never leave the device; only the solver's status flag is read back.
+#include <algorithm>
+#include <cmath>
+
+#include "ccsd.hpp"
+#include "kernels.hpp"
+
+namespace etg {
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ inline int64_t tri_root_u(int64_t k) {
+  int64_t x = int64_t((sqrt(8.0 * double(k) + 1.0) - 1.0) * 0.5);
+  while (x * (x + 1) / 2 > k) --x;
+  while ((x + 1) * (x + 2) / 2 <= k) ++x;
+  return x;
+}
+
+// e = -omega / (eps_a - eps_i [+ eps_b - eps_j]), doubled on the packed doubles
+// diagonal; t += e.  quasi_newton_updater_class.F90:81-118 (denominators from
+// get_orbital_differences, doubles_class.F90:421-470) followed by
+// scale_amplitudes_ccsd (ccsd_class.F90:648-671) on the update.
+__global__ void quasi_newton_kernel(const double* __restrict__ eps, const double* __restrict__ omega,
+                                    double* __restrict__ t, double* __restrict__ e, int64_t n_o,
+                                    int64_t n_v, int64_t n) {
+  const int64_t N = n_o * n_v;
+  for (int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; k < n;
+       k += int64_t(gridDim.x) * blockDim.x) {
+    double d, f = 1.0;
+    if (k < N) {
+      d = eps[n_o + k % n_v] - eps[k / n_v];
+    } else {
+      const int64_t q = k - N;
+      const int64_t X = tri_root_u(q), Y = q - X * (X + 1) / 2;
+      d = eps[n_o + X % n_v] - eps[X / n_v] + eps[n_o + Y % n_v] - eps[Y / n_v];
+      if (X == Y) f = 2.0;
+    }
+    const double u = -f * omega[k] / d;
+    e[k] = u;
+    t[k] += u;
+  }
+}
+
+// G(s, i) = G(i, s) for the row s just computed.
+__global__ void mirror_row_kernel(double* G, int ld, int s, int dim) {
+  const int i = threadIdx.x;
+  if (i < dim) G[i + s * ld] = G[s + i * ld];
+}
+
+// Solves the padded DIIS equation (diis_tool_class.F90:592-640)
+//   [ G  -1 ] [w]   [ 0 ]
+//   [ -1  0 ] [l] = [-1 ]
+// by Gaussian elimination with partial pivoting (the elimination dgesv does),
+// one thread, FP64.  info = 0 on success, k > 0 if pivot k is exactly zero.
+__global__ void diis_solve_kernel(const double* __restrict__ G, int ld, int dim,
+                                  double* __restrict__ w, int* __restrict__ info) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  constexpr int kMax = CCSD::kMaxDiis + 1;
+  double A[kMax][kMax + 1];
+  const int n = dim + 1;
+  for (int i = 0; i < n; ++i) {
+    for (int j = 0; j < n; ++j)
+      A[i][j] = (i < dim && j < dim) ? G[i + j * ld] : ((i == dim && j == dim) ? 0.0 : -1.0);
+    A[i][n] = (i == dim) ? -1.0 : 0.0;
+  }
+  int status = 0;
+  for (int c = 0; c < n; ++c) {
+    int p = c;
+    for (int r = c + 1; r < n; ++r)
+      if (fabs(A[r][c]) > fabs(A[p][c])) p = r;
+    if (A[p][c] == 0.0) {
+      status = c + 1;
+      break;
+    }
+    if (p != c)
+      for (int j = c; j <= n; ++j) {
+        const double tmp = A[c][j];
+        A[c][j] = A[p][j];
+        A[p][j] = tmp;
+      }
+    for (int r = c + 1; r < n; ++r) {
+      const double f = A[r][c] / A[c][c];
+      for (int j = c; j <= n; ++j) A[r][j] -= f * A[c][j];
+    }
+  }
+  if (status == 0) {
+    for (int i = n - 1; i >= 0; --i) {
+      double s = A[i][n];
+      for (int j = i + 1; j < n; ++j) s -= A[i][j] * A[j][n];
+      A[i][n] = s / A[i][i];
+    }
+    for (int i = 0; i < dim; ++i) w[i] = A[i][n];
+  }
+  *info = status;
+}
+
+// x = sum_s w_s X_s over the dim records (diis_tool construct_extrapolated_vector).
+// Leaves x unchanged when the solve failed.
+__global__ void extrapolate_kernel(const double* __restrict__ X, int64_t n, int dim,
+                                   const double* __restrict__ w, const int* __restrict__ info,
+                                   double* __restrict__ x) {
+  if (*info != 0) return;
+  for (int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; k < n;
+       k += int64_t(gridDim.x) * blockDim.x) {
+    double s = 0.0;
+    for (int r = 0; r < dim; ++r) s += w[r] * X[k + int64_t(r) * n];
+    x[k] = s;
+  }
+}
+
+unsigned grid_for_n(int64_t n) {
+  return unsigned(std::min<int64_t>((n + kBlock - 1) / kBlock, 1 << 16));
+}
+
+}  // namespace
+
+void CCSD::diis_initialize(int64_t dimension) {
+  if (dimension < 1 || dimension > kMaxDiis) fail("diis_initialize: dimension must be in [1, 32]");
+  const int64_t n = n_t1 + n_t2;
+  diis_dim_ = int(dimension);
+  diis_count_ = 0;
+  diis_e_.resize(sizeof(double) * size_t(n) * size_t(dimension));
+  diis_x_.resize(sizeof(double) * size_t(n) * size_t(dimension));
+  diis_G_.resize(sizeof(double) * kMaxDiis * kMaxDiis);
+  diis_w_.resize(sizeof(double) * kMaxDiis);
+  diis_info_.resize(sizeof(int));
+  fill(stream_, diis_G_.as<double>(), kMaxDiis * kMaxDiis, 0.0);
+}
+
+void CCSD::quasi_newton_update() {
+  require(have_eps_ && have_t_, "quasi_newton_update: orbital energies and amplitudes required");
+  require(have_omega_, "quasi_newton_update: the residual has not been constructed");
+  const int64_t n = n_t1 + n_t2;
+  update_.resize(sizeof(double) * n);
+  quasi_newton_kernel<<<grid_for_n(n), kBlock, 0, stream_>>>(eps_.as<double>(), omega_.as<double>(),
+                                                             t_.as<double>(), update_.as<double>(),
+                                                             n_o, n_v, n);
+  ETG_LAUNCH("quasi_newton");
+  have_update_ = true;
+  have_omega_ = false;        // the residual belongs to the amplitudes before the step
+  tape_.reset();
+}
+
+void CCSD::diis_update() {
+  require(diis_dim_ > 0, "diis_update: DIIS has not been initialised");
+  require(have_update_, "diis_update: no update vector (call quasi_newton_update first)");
+  const int64_t n = n_t1 + n_t2;
+  // record slot: the oldest record is replaced once the space is full
+  // (diis_tool_class.F90 save_errors_and_parameters / modify_mappings_to_remove_oldest_record)
+  const int slot = int(diis_count_ % diis_dim_);
+  const int dim = int(std::min<int64_t>(diis_count_ + 1, diis_dim_));
+  double* E = diis_e_.as<double>();
+  double* X = diis_x_.as<double>();
+  double* G = diis_G_.as<double>();
+  ETG_HIP(hipMemcpyAsync(E + int64_t(slot) * n, update_.as<double>(), sizeof(double) * n,
+                         hipMemcpyDeviceToDevice, stream_));
+  ETG_HIP(hipMemcpyAsync(X + int64_t(slot) * n, t_.as<double>(), sizeof(double) * n,
+                         hipMemcpyDeviceToDevice, stream_));
+  // new row of G (construct_new_G_elements, diis_tool_class.F90:503-546)
+  for (int i = 0; i < dim; ++i)
+    dot(stream_, E + int64_t(slot) * n, E + int64_t(i) * n, n, red_partial_.as<double>(),
+        G + slot + int64_t(i) * kMaxDiis);
+  mirror_row_kernel<<<1, kMaxDiis, 0, stream_>>>(G, kMaxDiis, slot, dim);
+  ETG_LAUNCH("diis_mirror_row");
+  diis_solve_kernel<<<1, 1, 0, stream_>>>(G, kMaxDiis, dim, diis_w_.as<double>(), diis_info_.as<int>());
+  ETG_LAUNCH("diis_solve");
+  extrapolate_kernel<<<grid_for_n(n), kBlock, 0, stream_>>>(X, n, dim, diis_w_.as<double>(),
+                                                            diis_info_.as<int>(), t_.as<double>());
+  ETG_LAUNCH("diis_extrapolate");
+  int info = 0;
+  ETG_HIP(hipMemcpyAsync(&info, diis_info_.as<int>(), sizeof(int), hipMemcpyDeviceToHost, stream_));
+  synchronize();
+  if (info != 0) fail("diis_update: could not solve the DIIS matrix equation (zero pivot)");
+  ++diis_count_;
+  have_update_ = false;
+  tape_.reset();
+}
+
+}  // namespace etg

@@ -430,8 +430,6 @@ GemmLaunch gemm(hipStream_t stream, const Gemm& g, Prec prec, GemmWorkspace* ws)
   if (g.lda < std::max<int64_t>(1, rowsA)) fail("gemm: lda too small");
   if (g.ldb < std::max<int64_t>(1, rowsB)) fail("gemm: ldb too small");
   if (g.ldc < std::max<int64_t>(1, g.m)) fail("gemm: ldc too small");
-  // per-thread byte offsets inside a tile span 16 operand columns and are 31-bit
-  if (std::max(g.lda, g.ldb) >= (int64_t(1) << 24)) fail("gemm: leading dimension too large");
   if (g.batch > 65535) fail("gemm: batch too large");
   if (g.k == 0) {
     const int64_t mn = g.m * g.n;
@@ -440,12 +438,11 @@ GemmLaunch gemm(hipStream_t stream, const Gemm& g, Prec prec, GemmWorkspace* ws)
     ETG_LAUNCH("scale_matrix");
     return GemmLaunch{"scale_matrix", 1};
   }
-  switch (prec) {
-    case Prec::F64: return launch_typed<double>(stream, g, ws);
-    case Prec::F32: return launch_typed<float>(stream, g, ws);
-    case Prec::BF16X3: break;
-  }
-  fail("gemm: split-bf16 products are not available in this build");
+  if (prec == Prec::BF16X3) return gemm_bf16x3(stream, g, ws);
+  // per-thread byte offsets inside a tile span 16 operand columns and are 31-bit
+  if (std::max(g.lda, g.ldb) >= (int64_t(1) << 24)) fail("gemm: leading dimension too large");
+  if (prec == Prec::F32) return launch_typed<float>(stream, g, ws);
+  return launch_typed<double>(stream, g, ws);
 }
 
 }  // namespace etg

@@ -13,9 +13,9 @@ namespace etg {
 // Arithmetic used for the products.
 //  F64  : FP64 operands, v_mfma_f64_16x16x4 (exact FP64 arithmetic).
 //  F32  : FP64 operands rounded to FP32 on load, v_mfma_f32_16x16x4, FP32 accumulation.
-//  BF16X3: FP64 operands split on load into bf16 hi + lo (x ~ hi + lo); three
-//         products hi*hi + hi*lo + lo*hi per term with v_mfma_f32_16x16x32_bf16,
-//         FP32 accumulation.
+//  BF16X3: FP64 operands split into bf16 hi + lo (x ~ hi + lo) by a pack pass;
+//         three products hi*hi + hi*lo + lo*hi per term with
+//         v_mfma_f32_16x16x32_bf16, FP32 accumulation.
 enum class Prec : int { F64 = 0, F32 = 1, BF16X3 = 2 };
 
 struct Gemm {
@@ -31,9 +31,9 @@ struct Gemm {
   int64_t batch = 1;
 };
 
-// Scratch used for split-K partial sums; grown on demand.
+// Scratch for split-K partial sums and the split-bf16 operand packs; grown on demand.
 struct GemmWorkspace {
-  DeviceBuffer buf;
+  DeviceBuffer buf, packA, packB;
 };
 
 // Which kernel instantiation a product ran on (for the per-kernel ledger).
@@ -45,6 +45,9 @@ struct GemmLaunch {
 // Launches the product on `stream`.  Validates shapes and leading dimensions
 // on the host (throws etg::Error on inconsistent arguments).
 GemmLaunch gemm(hipStream_t stream, const Gemm& g, Prec prec, GemmWorkspace* ws);
+
+// The split-bf16 path (packs both operands into ws, then the bf16 matrix-core kernel).
+GemmLaunch gemm_bf16x3(hipStream_t stream, const Gemm& g, GemmWorkspace* ws);
 
 // Convenience: C = alpha op(A) op(B) + beta C, no batching.
 inline GemmLaunch gemm(hipStream_t s, char ta, char tb, int64_t m, int64_t n, int64_t k,

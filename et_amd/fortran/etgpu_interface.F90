@@ -16,6 +16,7 @@ module etgpu_interface
 !
    integer(c_int), parameter :: etg_fp64 = 0
    integer(c_int), parameter :: etg_fp32 = 1
+   integer(c_int), parameter :: etg_bf16x3 = 2
 !
    interface
 !
@@ -130,6 +131,25 @@ module etgpu_interface
          real(c_double), dimension(*), intent(in) :: b
          real(c_double), dimension(*), intent(out) :: sigma
       end function etg_jacobian_transpose_transformation
+!
+!     diis%initialize (diis_tool_class.F90:249-277)
+      integer(c_int) function etg_diis_initialize(handle, dimension) bind(C, name='etg_diis_initialize')
+         import :: c_int, c_ptr, c_int64_t
+         type(c_ptr), value :: handle
+         integer(c_int64_t), value :: dimension
+      end function etg_diis_initialize
+!
+!     t_updater%update (quasi_newton_updater_class.F90:81-118), on the device
+      integer(c_int) function etg_quasi_newton_update(handle) bind(C, name='etg_quasi_newton_update')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: handle
+      end function etg_quasi_newton_update
+!
+!     diis%update(omega, amplitudes) (diis_tool_class.F90:302-347), on the device
+      integer(c_int) function etg_diis_update(handle) bind(C, name='etg_diis_update')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: handle
+      end function etg_diis_update
 !
    end interface
 !

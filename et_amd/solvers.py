@@ -49,14 +49,20 @@ class DIIS:
 
 
 class DiisCCGroundState:
-    """run_diis_cc_gs over a et_amd.ccsd.CCSD wavefunction."""
+    """run_diis_cc_gs over a et_amd.ccsd.CCSD wavefunction, every vector in HBM.
+
+    Per iteration (diis_cc_gs_class.F90:253-293): T1 Fock, energy, residual
+    (in `precision`), its FP64 norm, then the device quasi-Newton step and DIIS
+    extrapolation and the T1 dressing of the Cholesky vectors.  Only scalars
+    (energy, norm) reach the host.
+    """
 
     def __init__(self, wf, residual_threshold=1e-5, max_iterations=100, diis_dimension=8,
                  precision=_ccsd.FP64, log=None):
         self.wf = wf
         self.residual_threshold = residual_threshold
         self.max_iterations = max_iterations
-        self.diis = DIIS(diis_dimension)
+        self.diis_dimension = diis_dimension
         self.precision = precision
         self.log = log
         self.history = []
@@ -65,25 +71,20 @@ class DiisCCGroundState:
         wf = self.wf
         if not restart:
             wf.set_initial_amplitudes_guess()
-        denom = wf.get_orbital_differences()
-        diag = wf.n_t1 + np.arange(wf.n_t1) * (np.arange(wf.n_t1) + 1) // 2 + np.arange(wf.n_t1)
+        wf.diis_initialize(self.diis_dimension)
         previous = 0.0
         for it in range(1, self.max_iterations + 1):
             wf.construct_fock('gs')
             wf.calculate_energy()
-            omega = wf.construct_omega(self.precision)
-            norm = float(np.linalg.norm(omega))
+            wf.construct_omega(self.precision, copy_out=False)
+            norm = wf.omega_norm()
             self.history.append((wf.energy, norm, abs(wf.energy - previous)))
             if self.log:
                 self.log('%3d %25.12f %11.4e %11.4e' % (it, wf.energy, norm, abs(wf.energy - previous)))
             if norm < self.residual_threshold:
                 return wf.energy
-            amplitudes = wf.get_amplitudes()
-            update = -omega / denom
-            update[diag] *= 2.0            # scale_amplitudes_ccsd on the update and the residual
-            amplitudes += update
-            amplitudes = self.diis.update(update, amplitudes)
-            wf.set_amplitudes(amplitudes)
+            wf.quasi_newton_update()
+            wf.diis_update()
             wf.construct_t1_cholesky()
             previous = wf.energy
         raise _ccsd.ETGError('Did not converge in the max number of iterations.')

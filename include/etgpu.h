@@ -35,7 +35,9 @@ typedef struct etg_handle etg_handle;
 /* Arithmetic of the bulk contractions in etg_construct_omega. */
 enum etg_precision {
   ETG_FP64 = 0,   /* FP64 matrix cores throughout */
-  ETG_FP32 = 1    /* FP32 matrix-core products, FP32 accumulation */
+  ETG_FP32 = 1,   /* FP32 matrix-core products, FP32 accumulation */
+  ETG_BF16X3 = 2  /* operands split into bf16 hi + lo, three bf16 matrix-core
+                     products per term (hi*hi + hi*lo + lo*hi), FP32 accumulation */
 };
 
 /* Creates the device state for a CCSD wavefunction with n_o occupied and
@@ -120,6 +122,28 @@ int etg_jacobian_transformation(etg_handle* handle, const double* c, double* rho
  * Replaces: jacobian_transpose_transformation
  * (src/wavefunctions/ccsd/jacobian_transpose_ccsd.F90:129-236). */
 int etg_jacobian_transpose_transformation(etg_handle* handle, const double* b, double* sigma);
+
+/* Allocates HBM records for DIIS of the given dimension (1..32) and clears
+ * the history.
+ * Replaces: diis%initialize (src/solver_tools/diis_tool_class.F90:249-277)
+ * as run_diis_cc_gs calls it (src/solvers/cc/diis_cc_gs_class.F90:230). */
+int etg_diis_initialize(etg_handle* handle, int64_t dimension);
+
+/* Quasi-Newton step on the device, FP64: e = -Omega / eps (orbital
+ * differences; doubles diagonal doubled), amplitudes += e.  Needs a residual
+ * constructed at the current amplitudes; e is kept for etg_diis_update.
+ * Replaces: t_updater%update(wf, amplitudes, omega)
+ * (src/solver_tools/amplitude_updaters/amplitude_updater_class.F90:86-121,
+ * quasi_newton_updater_class.F90:81-118) at diis_cc_gs_class.F90:284-285. */
+int etg_quasi_newton_update(etg_handle* handle);
+
+/* DIIS extrapolation on the device, FP64: stores (e, amplitudes) as the newest
+ * record (the oldest is dropped once the space is full), adds the new row of
+ * G_ij = e_i . e_j, solves the padded DIIS equation and sets the amplitudes to
+ * the extrapolated vector.  Fails (amplitudes unchanged) on a singular system.
+ * Replaces: diis%update(omega, amplitudes)
+ * (src/solver_tools/diis_tool_class.F90:302-347) at diis_cc_gs_class.F90:287. */
+int etg_diis_update(etg_handle* handle);
 
 /* Virtual batch length of the ladder term's g_acbd blocks (memory/launch
  * trade-off; results do not depend on it beyond rounding). */

@@ -64,3 +64,27 @@ def test_fp64_gemm_large_leading_dimension(ta, tb, m, n, k):
     out = ccsd.device_gemm(ta, tb, 1.1, A, B, 0.5, C)
     ref = 1.1 * (A.T if ta == 'T' else A) @ (B.T if tb == 'T' else B) + 0.5 * C
     assert np.abs(out - ref).max() < 1e-14 * (np.sqrt(k) + 1.0) * 8
+
+
+@pytest.mark.parametrize('ta', ['N', 'T'])
+@pytest.mark.parametrize('tb', ['N', 'T'])
+@pytest.mark.parametrize('m,n,k', [(1, 1, 1), (17, 33, 5), (100, 70, 37), (129, 257, 31),
+                                   (300, 200, 500), (48, 48, 4000), (640, 520, 130)])
+def test_bf16x3_gemm(ta, tb, m, n, k):
+    """Split-bf16 products against an elementwise bound.
+
+    x = hi + lo + d with |d| <= 2^-16 |x| and |lo| <= 2^-8 |x|, so a product term
+    loses at most 3 * 2^-16 |a b| (lo*lo dropped, two splitting residuals); FP32
+    accumulation adds at most k 2^-24 sum |a b|.  A missing hi*lo or lo*hi term,
+    or a wrong lane layout, gives errors of order 2^-8 |a b| and fails.
+    """
+    rng = np.random.default_rng(m * 17 + n * 3 + k)
+    A = rng.normal(size=(k, m) if ta == 'T' else (m, k))
+    B = rng.normal(size=(n, k) if tb == 'T' else (k, n))
+    C = rng.normal(size=(m, n))
+    opA = A.T if ta == 'T' else A
+    opB = B.T if tb == 'T' else B
+    out = ccsd.device_gemm(ta, tb, 0.7, A, B, -0.3, C, precision=ccsd.BF16X3)
+    ref = 0.7 * opA @ opB - 0.3 * C
+    bound = 0.7 * (4 * 2.0 ** -16 + k * 2.0 ** -24) * (np.abs(opA) @ np.abs(opB)) + 1e-15 * np.abs(C)
+    assert np.all(np.abs(out - ref) <= bound)

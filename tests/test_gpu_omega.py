@@ -80,6 +80,20 @@ def test_omega_fp32_within_tolerance(n_o, n_v, n_J, seed, batch):
     assert rel_err(om, ref) < 1e-5
 
 
+@pytest.mark.parametrize('n_o,n_v,n_J,seed,batch', CASES[2:5])
+def test_omega_bf16x3_within_tolerance(n_o, n_v, n_J, seed, batch):
+    """Split-bf16 products: ~2^-16 relative per product term, FP32 accumulation."""
+    s = make(n_o, n_v, n_J, seed)
+    Lt, F = oracle_state(s)
+    ref = O.omega(s['L'], Lt, F, s['t1'], s['t2'], n_o)
+    with load(s, batch) as wf:
+        om = wf.construct_omega(ccsd.BF16X3)
+        om64 = wf.construct_omega(ccsd.FP64)
+    err = rel_err(om, ref)
+    assert err < 2e-4
+    assert rel_err(om64, ref) < 1e-12 < err      # the split-bf16 residual is not the FP64 one
+
+
 def test_ladder_batching_does_not_change_omega():
     s = make(5, 17, 23, 8)
     with load(s) as wf:
