@@ -8,7 +8,15 @@ seed 0 (Cholesky vectors 0.5 exp(-J/400) N(0,1), symmetric in p,q; packed
 t2 ~ 1e-3 N(0,1); t1 = 0; canonical orbital energies).  A step is one full
 residual evaluation (T1 Fock, singles and doubles terms, the O^2V^4 ladder
 with g_acbd generated on the fly) with all inputs resident in HBM and the
-result left in HBM.  Prints one JSON line.
+result left in HBM.  The same residual is also timed with FP32 and split-bf16
+matrix-core products and compared with the FP64 one.
+
+Time to solution (skip with --no-tts): eight non-interacting H2O-He / cc-pVDZ
+molecules (o = 48, v = 184, n_J = 2832, from the committed inputs), ground state
+from the MP2 guess to ||Omega_FP64|| <= 1e-10 with FP64 residuals and with the
+mixed-precision schedule (split-bf16, FP32, then FP64 residuals).
+
+Prints one JSON line.
 """
 import argparse
 import json
@@ -78,6 +86,56 @@ def cpu_baseline(n_o, n_v, n_J, cols=4800):
     }
 
 
+def time_to_solution(copies=8, threshold=1e-10):
+    """Ground state from the MP2 guess to ||Omega_FP64|| <= threshold, FP64 vs mixed precision.
+
+    System: `copies` non-interacting H2O-He / cc-pVDZ (eT's tests/ccsd_gs_energy
+    molecule) from the committed inputs; its exact CCSD correlation energy is
+    `copies` times eT's printed one.
+    """
+    from et_amd import solvers, workloads
+    golden = json.load(open(os.path.join(ROOT, 'tests', 'golden', 'h2o_he_ccpvdz.json')))['golden']
+    r = workloads.replicate(workloads.h2o_he_inputs(), copies)
+    n_o = r['n_o']
+    n_v = r['L_mo'].shape[1] - n_o
+    e_ref = copies * golden['ccsd_correlation_energy']
+    out = {'workload': 'h2o_he_ccpvdz_x%d' % copies, 'n_o': n_o, 'n_v': n_v, 'n_J': r['n_J'],
+           'residual_threshold': threshold, 'et_correlation_energy_x%d' % copies: e_ref}
+    with ccsd.CCSD(n_o, n_v, r['n_J']) as wf:
+        wf.set_cholesky(r['L_mo'])
+        wf.set_oei(r['h_mo'])
+        wf.set_orbital_energies(r['eps'])
+        del r
+        wf.set_initial_amplitudes_guess()
+        wf.construct_omega(ccsd.BF16X3, copy_out=False)      # size the workspaces once
+        results = {}
+        for name in ('fp64', 'mixed'):
+            wf.synchronize()
+            t0 = time.perf_counter()
+            if name == 'fp64':
+                sv = solvers.DiisCCGroundState(wf, residual_threshold=threshold)
+                sv.run()
+                iters = {'fp64': len(sv.history)}
+            else:
+                sv = solvers.MixedPrecisionGroundState(wf, residual_threshold=threshold)
+                sv.run()
+                iters = {}
+                for p, _, _ in sv.history:
+                    k = {ccsd.FP64: 'fp64', ccsd.FP32: 'fp32', ccsd.BF16X3: 'bf16x3'}[p]
+                    iters[k] = iters.get(k, 0) + 1
+            wf.synchronize()
+            dt = time.perf_counter() - t0
+            results[name] = {'seconds': dt, 'iterations': iters,
+                             'correlation_energy': wf.correlation_energy,
+                             'abs_err_vs_et_x%d' % copies: abs(wf.correlation_energy - e_ref)}
+            log('time to solution %s: %.2f s, %s, E_corr %.12f' % (name, dt, iters, wf.correlation_energy))
+    results['mixed']['abs_err_vs_fp64_gpu'] = abs(results['mixed']['correlation_energy']
+                                                  - results['fp64']['correlation_energy'])
+    out.update(results)
+    out['speedup_mixed_vs_fp64'] = results['fp64']['seconds'] / results['mixed']['seconds']
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -86,6 +144,7 @@ def main():
     ap.add_argument('--precision', choices=['fp64', 'fp32', 'bf16x3'], default='fp64')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-mixed', action='store_true')
+    ap.add_argument('--no-tts', action='store_true', help='skip the time-to-solution comparison')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one GPU: every box in this setup has a single MI355X, so '
@@ -168,6 +227,9 @@ def main():
             log('%s: %.3f s per residual, rel err %.2e' % (name, dtp, forms[-1]['rel_err_vs_fp64']))
         extra['mixed'] = forms
     wf.close()
+
+    if not args.no_tts and args.precision == 'fp64':
+        extra['time_to_solution'] = time_to_solution()
 
     cpu = None
     if not args.no_cpu:

@@ -90,6 +90,83 @@ class DiisCCGroundState:
         raise _ccsd.ETGError('Did not converge in the max number of iterations.')
 
 
+class MixedPrecisionGroundState:
+    """Amplitude iterations on reduced-precision residuals, finished on FP64 ones.
+
+    The residual is evaluated with the bulk contractions in the precisions of
+    `schedule` (e.g. split-bf16, then FP32) and finally in FP64.  Each reduced
+    form has a noise floor, ||Omega_p(t) - Omega_FP64(t)||, measured once at the
+    starting amplitudes; a phase ends when its residual norm falls below
+    `margin` times its floor (below that, its steps follow rounding noise) or
+    stops decreasing.  The step is taken with the last residual of a phase and
+    the DIIS space carries over (or restarts, `restart_diis`).  The last phase
+    is run_diis_cc_gs unchanged, so convergence is decided on the FP64 residual
+    norm against `residual_threshold`.  Quasi-Newton steps and DIIS
+    extrapolation are FP64 on the device throughout.
+    """
+
+    def __init__(self, wf, residual_threshold=1e-10, max_iterations=100, diis_dimension=8,
+                 schedule=(_ccsd.BF16X3, _ccsd.FP32), margin=30.0, restart_diis=False, log=None):
+        self.wf = wf
+        self.residual_threshold = residual_threshold
+        self.max_iterations = max_iterations
+        self.diis_dimension = diis_dimension
+        self.schedule = tuple(schedule)
+        self.margin = margin
+        self.restart_diis = restart_diis
+        self.log = log
+        self.history = []          # (precision, energy, norm)
+        self.noise = {}
+
+    def _measure_noise(self):
+        wf = self.wf
+        wf.construct_fock('gs')
+        ref = wf.construct_omega(_ccsd.FP64)
+        for p in self.schedule:
+            self.noise[p] = float(np.linalg.norm(wf.construct_omega(p) - ref))
+
+    def run(self):
+        wf = self.wf
+        wf.set_initial_amplitudes_guess()
+        self._measure_noise()
+        phases = [(p, self.margin * self.noise[p]) for p in self.schedule]
+        phases.append((_ccsd.FP64, self.residual_threshold))
+        it, phase = 0, 0
+        best, since_best = np.inf, 0
+        wf.diis_initialize(self.diis_dimension)
+        while it < self.max_iterations:
+            it += 1
+            precision, threshold = phases[phase]
+            wf.construct_fock('gs')
+            wf.calculate_energy()
+            wf.construct_omega(precision, copy_out=False)
+            norm = wf.omega_norm()
+            self.history.append((precision, wf.energy, norm))
+            if self.log:
+                self.log('%3d  %-6s %25.12f %11.4e' % (it, _PREC_NAMES[precision], wf.energy, norm))
+            if norm < threshold and precision == _ccsd.FP64:
+                return wf.energy
+            if norm < 0.5 * best:
+                best, since_best = norm, 0
+            else:
+                since_best += 1
+            # the step is taken with this residual; the next one is evaluated in the
+            # next phase's precision once this one is at its noise margin or stalls
+            advance = precision != _ccsd.FP64 and (norm < threshold or since_best >= 4)
+            wf.quasi_newton_update()
+            wf.diis_update()
+            wf.construct_t1_cholesky()
+            if advance:
+                phase += 1
+                best, since_best = np.inf, 0
+                if self.restart_diis:
+                    wf.diis_initialize(self.diis_dimension)
+        raise _ccsd.ETGError('Did not converge in the max number of iterations.')
+
+
+_PREC_NAMES = {_ccsd.FP64: 'fp64', _ccsd.FP32: 'fp32', _ccsd.BF16X3: 'bf16x3'}
+
+
 class EigenDavidson:
     """Lowest eigenpairs of the CC Jacobian, after eT's eigen Davidson solver.
 

@@ -43,6 +43,30 @@ def test_reference_energies_and_cholesky_count():
     assert np.abs(F0 - np.diag(m['eps'])).max() < 1e-10      # canonical RHF through the Cholesky vectors
 
 
+def test_committed_inputs_match_the_front_end():
+    """tests/golden/h2o_he_ccpvdz_inputs.npz (read by et_amd.workloads) is the front end's output."""
+    from et_amd import workloads
+    m = inputs()
+    w = workloads.h2o_he_inputs()
+    assert w['n_o'] == m['n_o'] and w['L_mo'].shape == m['L_mo'].shape
+    assert np.abs(w['L_mo'] - m['L_mo']).max() < 1e-14
+    assert np.abs(w['h_mo'] - m['h_mo']).max() < 1e-14
+    assert np.abs(w['eps'] - m['eps']).max() < 1e-14
+    assert abs(w['e_hf'] - m['e_hf']) < 1e-12
+
+
+def test_replicated_system_is_block_diagonal():
+    """Two copies: the oracle's energy of the MP2 guess is twice the single one."""
+    from et_amd import workloads
+    w = workloads.h2o_he_inputs()
+    r = workloads.replicate(w, 2)
+    e1 = O.correlation_energy(w['L_mo'], np.zeros((w['L_mo'].shape[1] - w['n_o'], w['n_o'])),
+                              O.mp2_guess(w['L_mo'], w['eps'], w['n_o']), w['n_o'])
+    e2 = O.correlation_energy(r['L_mo'], np.zeros((r['L_mo'].shape[1] - r['n_o'], r['n_o'])),
+                              O.mp2_guess(r['L_mo'], r['eps'], r['n_o']), r['n_o'])
+    assert abs(e2 - 2 * e1) < 1e-12
+
+
 def test_oracle_reproduces_et_diis_trajectory():
     g = fixture()['golden']
     m = inputs()
@@ -76,6 +100,45 @@ def test_gpu_ground_state_reproduces_et():
     assert len(solver.history) == len(table)
     for (it, e_ref, norm_ref, _), (e_it, norm, _) in zip(table, solver.history):
         assert abs(e_it - e_ref) < 1e-10, it
+    assert abs(e - g['ccsd_energy']) < 1e-10
+
+
+@pytest.mark.gpu
+def test_gpu_replicated_ground_state_is_size_extensive():
+    """Three non-interacting copies: E_corr = 3 x eT's, along the single system's 19 iterations."""
+    from et_amd import ccsd, solvers, workloads
+    g = fixture()['golden']
+    r = workloads.replicate(workloads.h2o_he_inputs(), 3)
+    n_v = r['L_mo'].shape[1] - r['n_o']
+    with ccsd.CCSD(r['n_o'], n_v, r['n_J']) as wf:
+        wf.set_cholesky(r['L_mo'])
+        wf.set_oei(r['h_mo'])
+        wf.set_orbital_energies(r['eps'])
+        solver = solvers.DiisCCGroundState(wf, residual_threshold=1e-10 * np.sqrt(3))
+        solver.run()
+        e_corr = wf.correlation_energy
+    assert len(solver.history) == len(g['ccsd_diis_iterations'])
+    assert abs(e_corr - 3 * g['ccsd_correlation_energy']) < 3e-10
+
+
+@pytest.mark.gpu
+def test_gpu_mixed_precision_ground_state_reproduces_et():
+    """Split-bf16 and FP32 residual phases, finished on FP64 residuals: eT's E_CCSD to 1e-10."""
+    from et_amd import ccsd, solvers
+    g = fixture()['golden']
+    m = inputs()
+    n_o = m['n_o']
+    n_v = m['L_mo'].shape[1] - n_o
+    with ccsd.CCSD(n_o, n_v, m['n_J']) as wf:
+        wf.set_cholesky(m['L_mo'])
+        wf.set_oei(m['h_mo'])
+        wf.set_orbital_energies(m['eps'])
+        wf.hf_energy = m['e_hf']
+        solver = solvers.MixedPrecisionGroundState(wf, residual_threshold=1e-10)
+        e = solver.run()
+    phases = [p for p, _, _ in solver.history]
+    assert phases[0] == ccsd.BF16X3 and phases[-1] == ccsd.FP64 and ccsd.FP32 in phases
+    assert solver.noise[ccsd.BF16X3] > solver.noise[ccsd.FP32] > 0
     assert abs(e - g['ccsd_energy']) < 1e-10
 
 
