@@ -57,6 +57,15 @@ _SIGNATURES = {
     'etg_diis_initialize': [ctypes.c_void_p, _i64],
     'etg_quasi_newton_update': [ctypes.c_void_p],
     'etg_diis_update': [ctypes.c_void_p],
+    'etg_prepare_for_multiplier_equation': [ctypes.c_void_p],
+    'etg_set_initial_multipliers_guess': [ctypes.c_void_p],
+    'etg_set_multipliers': [ctypes.c_void_p, _c_dbl_p],
+    'etg_get_multipliers': [ctypes.c_void_p, _c_dbl_p],
+    'etg_construct_eta': [ctypes.c_void_p, _c_dbl_p],
+    'etg_construct_multiplier_equation': [ctypes.c_void_p, _c_dbl_p],
+    'etg_multiplier_residual_norm': [ctypes.c_void_p, _c_dbl_p],
+    'etg_multiplier_quasi_newton_update': [ctypes.c_void_p],
+    'etg_multiplier_diis_update': [ctypes.c_void_p],
     'etg_set_kernel_ledger': [ctypes.c_void_p, ctypes.c_int],
     'etg_kernel_ledger': [ctypes.c_void_p, _i64, ctypes.c_char_p, _i64, ctypes.POINTER(_i64),
                           _c_dbl_p, _c_dbl_p, ctypes.POINTER(_i64)],
@@ -273,6 +282,45 @@ class CCSD:
     def diis_update(self):
         """Device DIIS extrapolation of the amplitudes (diis_tool_class.F90:302-347)."""
         _check(self._lib.etg_diis_update(self._h))
+
+    # ---- multiplier equations (multiplier_equation_ccs.F90, multiplier_equation_ccsd.F90) ----
+    def prepare_for_multiplier_equation(self):
+        _check(self._lib.etg_prepare_for_multiplier_equation(self._h))
+
+    def set_initial_multipliers_guess(self):
+        _check(self._lib.etg_set_initial_multipliers_guess(self._h))
+
+    def set_multipliers(self, tbar):
+        tbar = np.ascontiguousarray(tbar, dtype=np.float64)
+        if tbar.shape != (self.n_gs_amplitudes,):
+            raise ETGError('multipliers must have length n_t1 + n_t2')
+        _check(self._lib.etg_set_multipliers(self._h, _ptr(tbar)))
+
+    def get_multipliers(self):
+        out = np.empty(self.n_gs_amplitudes)
+        _check(self._lib.etg_get_multipliers(self._h, _ptr(out)))
+        return out
+
+    def construct_eta(self, copy_out=True):
+        out = np.empty(self.n_gs_amplitudes) if copy_out else None
+        _check(self._lib.etg_construct_eta(self._h, _ptr(out) if copy_out else None))
+        return out
+
+    def construct_multiplier_equation(self, copy_out=True):
+        out = np.empty(self.n_gs_amplitudes) if copy_out else None
+        _check(self._lib.etg_construct_multiplier_equation(self._h, _ptr(out) if copy_out else None))
+        return out
+
+    def multiplier_residual_norm(self):
+        x = ctypes.c_double()
+        _check(self._lib.etg_multiplier_residual_norm(self._h, ctypes.byref(x)))
+        return x.value
+
+    def multiplier_quasi_newton_update(self):
+        _check(self._lib.etg_multiplier_quasi_newton_update(self._h))
+
+    def multiplier_diis_update(self):
+        _check(self._lib.etg_multiplier_diis_update(self._h))
 
     def set_kernel_ledger(self, on):
         _check(self._lib.etg_set_kernel_ledger(self._h, 1 if on else 0))

@@ -180,6 +180,45 @@ int etg_diis_update(etg_handle* h) {
   return guarded([&] { wf_of(h).diis_update(); });
 }
 
+int etg_prepare_for_multiplier_equation(etg_handle* h) {
+  return guarded([&] { wf_of(h).prepare_for_jacobian(); });
+}
+
+int etg_set_initial_multipliers_guess(etg_handle* h) {
+  return guarded([&] { wf_of(h).set_initial_multipliers_guess(); });
+}
+
+int etg_set_multipliers(etg_handle* h, const double* tbar) {
+  return guarded([&] { wf_of(h).set_multipliers(tbar); });
+}
+
+int etg_get_multipliers(etg_handle* h, double* tbar) {
+  return guarded([&] { wf_of(h).get_multipliers(tbar); });
+}
+
+int etg_construct_eta(etg_handle* h, double* eta) {
+  return guarded([&] { wf_of(h).construct_eta(eta); });
+}
+
+int etg_construct_multiplier_equation(etg_handle* h, double* residual) {
+  return guarded([&] { wf_of(h).construct_multiplier_equation(residual); });
+}
+
+int etg_multiplier_residual_norm(etg_handle* h, double* norm) {
+  return guarded([&] {
+    if (norm == nullptr) etg::fail("etg_multiplier_residual_norm: null output");
+    *norm = wf_of(h).multiplier_residual_norm();
+  });
+}
+
+int etg_multiplier_quasi_newton_update(etg_handle* h) {
+  return guarded([&] { wf_of(h).multiplier_quasi_newton_update(); });
+}
+
+int etg_multiplier_diis_update(etg_handle* h) {
+  return guarded([&] { wf_of(h).multiplier_diis_update(); });
+}
+
 int etg_set_kernel_ledger(etg_handle* h, int on) {
   return guarded([&] { wf_of(h).set_ledger(on != 0); });
 }

@@ -733,15 +733,26 @@ void CCSD::jacobian_transpose_transform(const double* b, double* sigma) {
   require(tape_ != nullptr, "jacobian_transpose_transformation: prepare_for_jacobian has not been called");
   require(b != nullptr && sigma != nullptr, "jacobian_transpose_transformation: null pointer");
   const int64_t n = n_t1 + n_t2;
+  jvec_.resize(sizeof(double) * n);
+  ETG_HIP(hipMemcpyAsync(jvec_.as<double>(), b, sizeof(double) * n, hipMemcpyHostToDevice, stream_));
+  transpose_transform_device(jvec_.as<double>(), jvec_.as<double>());
+  ETG_HIP(hipMemcpyAsync(sigma, jvec_.as<double>(), sizeof(double) * n, hipMemcpyDeviceToHost,
+                         stream_));
+  synchronize();
+}
+
+// sigma = A^T b on device vectors (b is read before sigma is written; they may alias).
+void CCSD::transpose_transform_device(const double* b, double* sigma) {
+  require(tape_ != nullptr, "jacobian_transpose_transformation: prepare_for_jacobian has not been called");
+  const int64_t n = n_t1 + n_t2;
   tape_->zero_derivatives();
   ETG_HIP(hipMemcpyAsync(tape_->deriv(omega_.as<double>()), b, sizeof(double) * n,
-                         hipMemcpyHostToDevice, stream_));
+                         hipMemcpyDeviceToDevice, stream_));
   tape_->adjoint(Prec::F64);
   double* tb = tape_->deriv(t_.as<double>());
   scale_doubles_diagonal_kernel<<<unsigned(ceil_div(n_t1, 256)), 256, 0, stream_>>>(tb, n_t1, 2.0);
   ETG_LAUNCH("scale_doubles_diagonal");
-  ETG_HIP(hipMemcpyAsync(sigma, tb, sizeof(double) * n, hipMemcpyDeviceToHost, stream_));
-  synchronize();
+  ETG_HIP(hipMemcpyAsync(sigma, tb, sizeof(double) * n, hipMemcpyDeviceToDevice, stream_));
 }
 
 }  // namespace etg

@@ -45,6 +45,17 @@ class CCSD {
   void quasi_newton_update();                     // e = -omega/eps (doubles diagonal x2); t += e
   void diis_update();                             // records (e, t); t = DIIS-extrapolated t
 
+  // multiplier (Lambda) equations t-bar^T A + eta = 0 on the device
+  // (multiplier_equation_ccs.F90:58-94, multiplier_equation_ccsd.F90:41-110)
+  void set_initial_multipliers_guess();           // t-bar1 = t1, t-bar2 = 2 t_aibj - t_ajbi
+  void set_multipliers(const double* host);
+  void get_multipliers(double* host);
+  void construct_eta(double* host);               // eta (host may be null: kept in HBM)
+  void construct_multiplier_equation(double* host);   // R = A^T t-bar + eta (host may be null)
+  double multiplier_residual_norm();
+  void multiplier_quasi_newton_update();          // e = -R/eps; t-bar += e
+  void multiplier_diis_update();                  // records (e, t-bar); t-bar extrapolated
+
   // Jacobian transformations at the current amplitudes (A = dOmega/dt D2)
   void prepare_for_jacobian();                    // records the residual program
   void jacobian_transform(const double* c_host, double* rho_host);            // rho = A c
@@ -81,6 +92,8 @@ class CCSD {
  private:
   void require(bool ok, const char* what) const;
   void build_ladder_plan();
+  void diis_step(const double* e, double* x);
+  void transpose_transform_device(const double* b, double* sigma);  // device pointers
   // the residual pipeline, written once against Ops
   void program_t1_cholesky(Ops& X);
   void program_fock(Ops& X);
@@ -113,6 +126,10 @@ class CCSD {
   int64_t diis_count_ = 0;
   DeviceBuffer diis_e_, diis_x_, diis_G_, diis_w_, diis_info_, update_;
   bool have_update_ = false, have_omega_ = false;
+
+  // multipliers: t-bar, eta, the multiplier residual and its update vector
+  DeviceBuffer tbar_, eta_, mres_, mupdate_;
+  bool have_tbar_ = false, have_mres_ = false, have_mupdate_ = false;
 
   // Jacobian record
   std::unique_ptr<Tape> tape_;

@@ -24,13 +24,15 @@ __device__ inline int64_t tri_root_u(int64_t k) {
   return x;
 }
 
-// e = -omega / (eps_a - eps_i [+ eps_b - eps_j]), doubled on the packed doubles
-// diagonal; t += e.  quasi_newton_updater_class.F90:81-118 (denominators from
-// get_orbital_differences, doubles_class.F90:421-470) followed by
-// scale_amplitudes_ccsd (ccsd_class.F90:648-671) on the update.
+// e = -omega / (eps_a - eps_i [+ eps_b - eps_j]), times diag_factor on the packed
+// doubles diagonal; t += e.  quasi_newton_updater_class.F90:81-118 (denominators
+// from get_orbital_differences, doubles_class.F90:421-470); diag_factor = 2 is
+// scale_amplitudes_ccsd (ccsd_class.F90:648-671) applied to the update (ground
+// state), 1 the multiplier updater without scaling
+// (cc_multipliers_solver_factory_class.F90:249-251).
 __global__ void quasi_newton_kernel(const double* __restrict__ eps, const double* __restrict__ omega,
                                     double* __restrict__ t, double* __restrict__ e, int64_t n_o,
-                                    int64_t n_v, int64_t n) {
+                                    int64_t n_v, int64_t n, double diag_factor) {
   const int64_t N = n_o * n_v;
   for (int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; k < n;
        k += int64_t(gridDim.x) * blockDim.x) {
@@ -41,7 +43,7 @@ __global__ void quasi_newton_kernel(const double* __restrict__ eps, const double
       const int64_t q = k - N;
       const int64_t X = tri_root_u(q), Y = q - X * (X + 1) / 2;
       d = eps[n_o + X % n_v] - eps[X / n_v] + eps[n_o + Y % n_v] - eps[Y / n_v];
-      if (X == Y) f = 2.0;
+      if (X == Y) f = diag_factor;
     }
     const double u = -f * omega[k] / d;
     e[k] = u;
@@ -122,6 +124,12 @@ unsigned grid_for_n(int64_t n) {
 
 }  // namespace
 
+void quasi_newton_step(hipStream_t s, const double* eps, const double* residual, double* x, double* e,
+                       int64_t n_o, int64_t n_v, int64_t n, double diag_factor) {
+  quasi_newton_kernel<<<grid_for_n(n), kBlock, 0, s>>>(eps, residual, x, e, n_o, n_v, n, diag_factor);
+  ETG_LAUNCH("quasi_newton");
+}
+
 void CCSD::diis_initialize(int64_t dimension) {
   if (dimension < 1 || dimension > kMaxDiis) fail("diis_initialize: dimension must be in [1, 32]");
   const int64_t n = n_t1 + n_t2;
@@ -142,7 +150,7 @@ void CCSD::quasi_newton_update() {
   update_.resize(sizeof(double) * n);
   quasi_newton_kernel<<<grid_for_n(n), kBlock, 0, stream_>>>(eps_.as<double>(), omega_.as<double>(),
                                                              t_.as<double>(), update_.as<double>(),
-                                                             n_o, n_v, n);
+                                                             n_o, n_v, n, 2.0);
   ETG_LAUNCH("quasi_newton");
   have_update_ = true;
   have_omega_ = false;        // the residual belongs to the amplitudes before the step
@@ -150,8 +158,14 @@ void CCSD::quasi_newton_update() {
 }
 
 void CCSD::diis_update() {
-  require(diis_dim_ > 0, "diis_update: DIIS has not been initialised");
   require(have_update_, "diis_update: no update vector (call quasi_newton_update first)");
+  diis_step(update_.as<double>(), t_.as<double>());
+  have_update_ = false;
+  tape_.reset();
+}
+
+void CCSD::diis_step(const double* e, double* x) {
+  require(diis_dim_ > 0, "diis_update: DIIS has not been initialised");
   const int64_t n = n_t1 + n_t2;
   // record slot: the oldest record is replaced once the space is full
   // (diis_tool_class.F90 save_errors_and_parameters / modify_mappings_to_remove_oldest_record)
@@ -160,10 +174,10 @@ void CCSD::diis_update() {
   double* E = diis_e_.as<double>();
   double* X = diis_x_.as<double>();
   double* G = diis_G_.as<double>();
-  ETG_HIP(hipMemcpyAsync(E + int64_t(slot) * n, update_.as<double>(), sizeof(double) * n,
-                         hipMemcpyDeviceToDevice, stream_));
-  ETG_HIP(hipMemcpyAsync(X + int64_t(slot) * n, t_.as<double>(), sizeof(double) * n,
-                         hipMemcpyDeviceToDevice, stream_));
+  ETG_HIP(hipMemcpyAsync(E + int64_t(slot) * n, e, sizeof(double) * n, hipMemcpyDeviceToDevice,
+                         stream_));
+  ETG_HIP(hipMemcpyAsync(X + int64_t(slot) * n, x, sizeof(double) * n, hipMemcpyDeviceToDevice,
+                         stream_));
   // new row of G (construct_new_G_elements, diis_tool_class.F90:503-546)
   for (int i = 0; i < dim; ++i)
     dot(stream_, E + int64_t(slot) * n, E + int64_t(i) * n, n, red_partial_.as<double>(),
@@ -173,15 +187,13 @@ void CCSD::diis_update() {
   diis_solve_kernel<<<1, 1, 0, stream_>>>(G, kMaxDiis, dim, diis_w_.as<double>(), diis_info_.as<int>());
   ETG_LAUNCH("diis_solve");
   extrapolate_kernel<<<grid_for_n(n), kBlock, 0, stream_>>>(X, n, dim, diis_w_.as<double>(),
-                                                            diis_info_.as<int>(), t_.as<double>());
+                                                            diis_info_.as<int>(), x);
   ETG_LAUNCH("diis_extrapolate");
   int info = 0;
   ETG_HIP(hipMemcpyAsync(&info, diis_info_.as<int>(), sizeof(int), hipMemcpyDeviceToHost, stream_));
   synchronize();
   if (info != 0) fail("diis_update: could not solve the DIIS matrix equation (zero pivot)");
   ++diis_count_;
-  have_update_ = false;
-  tape_.reset();
 }
 
 }  // namespace etg

@@ -106,6 +106,9 @@ void trace_oo(hipStream_t s, const double* Loo, double* XJ, int64_t nJ, int64_t 
 void trace_oo_adjoint(hipStream_t s, const double* XJb, double* Loob, int64_t nJ, int64_t n_o);
 void fock_ov_vector(hipStream_t s, const double* F, double* f, int64_t n_o, int64_t n_v);
 void fock_ov_vector_adjoint(hipStream_t s, const double* fb, double* Fb, int64_t n_o, int64_t n_v);
+// e = -residual / eps (x diag_factor on the packed doubles diagonal); x += e
+void quasi_newton_step(hipStream_t s, const double* eps, const double* residual, double* x, double* e,
+                       int64_t n_o, int64_t n_v, int64_t n, double diag_factor);
 // dst (m x n) = alpha * src + beta * dst
 void copy_sub(hipStream_t s, const double* src, int64_t lds, double* dst, int64_t ldd, int64_t m,
               int64_t n, double alpha, double beta);

@@ -151,6 +151,66 @@ module etgpu_interface
          type(c_ptr), value :: handle
       end function etg_diis_update
 !
+!     prepare_for_multiplier_equation (multiplier_equation_ccsd.F90:41-53)
+      integer(c_int) function etg_prepare_for_multiplier_equation(handle) bind(C, name='etg_prepare_for_multiplier_equation')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: handle
+      end function etg_prepare_for_multiplier_equation
+!
+!     set_initial_multipliers_guess (ccsd_class.F90:423-475)
+      integer(c_int) function etg_set_initial_multipliers_guess(handle) bind(C, name='etg_set_initial_multipliers_guess')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: handle
+      end function etg_set_initial_multipliers_guess
+!
+!     tbar_updater%update (diis_cc_multipliers_class.F90:269)
+      integer(c_int) function etg_multiplier_quasi_newton_update(handle) bind(C, name='etg_multiplier_quasi_newton_update')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: handle
+      end function etg_multiplier_quasi_newton_update
+!
+!     diis%update(residual, multipliers) (diis_cc_multipliers_class.F90:271)
+      integer(c_int) function etg_multiplier_diis_update(handle) bind(C, name='etg_multiplier_diis_update')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: handle
+      end function etg_multiplier_diis_update
+!
+!     set_multipliers (set_get_ccsd.F90:68-82)
+      integer(c_int) function etg_set_multipliers(handle, x) bind(C, name='etg_set_multipliers')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: handle
+         real(c_double), dimension(*), intent(in) :: x
+      end function etg_set_multipliers
+!
+!     get_multipliers (set_get_ccsd.F90:85-99)
+      integer(c_int) function etg_get_multipliers(handle, x) bind(C, name='etg_get_multipliers')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: handle
+         real(c_double), dimension(*), intent(out) :: x
+      end function etg_get_multipliers
+!
+!     construct_eta (multiplier_equation_ccsd.F90:56-110)
+      integer(c_int) function etg_construct_eta(handle, x) bind(C, name='etg_construct_eta')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: handle
+         real(c_double), dimension(*), intent(out) :: x
+      end function etg_construct_eta
+!
+!     construct_multiplier_equation (multiplier_equation_ccs.F90:58-94)
+      integer(c_int) function etg_construct_multiplier_equation(handle, x) bind(C, name='etg_construct_multiplier_equation')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: handle
+         real(c_double), dimension(*), intent(out) :: x
+      end function etg_construct_multiplier_equation
+!
+!     get_l2_norm of the multiplier residual (diis_cc_multipliers_class.F90:249)
+      integer(c_int) function etg_multiplier_residual_norm(handle, norm) &
+                                                   bind(C, name='etg_multiplier_residual_norm')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: handle
+         real(c_double), intent(out) :: norm
+      end function etg_multiplier_residual_norm
+!
    end interface
 !
 contains

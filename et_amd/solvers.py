@@ -164,6 +164,42 @@ class MixedPrecisionGroundState:
         raise _ccsd.ETGError('Did not converge in the max number of iterations.')
 
 
+class DiisCCMultipliers:
+    """run_diis_cc_multipliers (diis_cc_multipliers_class.F90:195-299), vectors in HBM.
+
+    t-bar starts at (t1, 2 t_aibj - t_ajbi); each iteration forms R = A^T t-bar + eta
+    with the left transformation, tests ||R||, then takes the quasi-Newton step
+    t-bar += -R/eps (no diagonal scaling) and the DIIS extrapolation, all FP64 on
+    the device.
+    """
+
+    def __init__(self, wf, residual_threshold=1e-5, max_iterations=100, diis_dimension=8, log=None):
+        self.wf = wf
+        self.residual_threshold = residual_threshold
+        self.max_iterations = max_iterations
+        self.diis_dimension = diis_dimension
+        self.log = log
+        self.history = []
+
+    def run(self):
+        wf = self.wf
+        wf.prepare_for_multiplier_equation()
+        wf.diis_initialize(self.diis_dimension)
+        wf.set_initial_multipliers_guess()
+        for it in range(1, self.max_iterations + 1):
+            wf.construct_multiplier_equation(copy_out=False)
+            norm = wf.multiplier_residual_norm()
+            self.history.append(norm)
+            if self.log:
+                self.log('%3d %11.4e' % (it, norm))
+            if norm < self.residual_threshold:
+                return wf.get_multipliers()
+            wf.multiplier_quasi_newton_update()
+            wf.multiplier_diis_update()
+        raise _ccsd.ETGError('was not able to converge the equations in the given number of '
+                             'maximum iterations.')
+
+
 _PREC_NAMES = {_ccsd.FP64: 'fp64', _ccsd.FP32: 'fp32', _ccsd.BF16X3: 'bf16x3'}
 
 

@@ -145,6 +145,46 @@ int etg_quasi_newton_update(etg_handle* handle);
  * (src/solver_tools/diis_tool_class.F90:302-347) at diis_cc_gs_class.F90:287. */
 int etg_diis_update(etg_handle* handle);
 
+/* ---- multiplier (Lambda) equations  t-bar^T A + eta = 0 ---------------- */
+
+/* Records the residual program at the current amplitudes for the left
+ * transformation (same record as etg_prepare_for_jacobian).
+ * Replaces: prepare_for_multiplier_equation
+ * (src/wavefunctions/ccsd/multiplier_equation_ccsd.F90:41-53). */
+int etg_prepare_for_multiplier_equation(etg_handle* handle);
+
+/* t-bar1 = t1, t-bar2 = 2 t_aibj - t_ajbi (packed), on the device.
+ * Replaces: set_initial_multipliers_guess (ccsd_class.F90:423-475, no restart). */
+int etg_set_initial_multipliers_guess(etg_handle* handle);
+
+/* Multipliers t-bar (n_t1 + n_t2, amplitude layout).
+ * Replaces: set_multipliers / get_multipliers (set_get_ccsd.F90:68-99). */
+int etg_set_multipliers(etg_handle* handle, const double* tbar);
+int etg_get_multipliers(etg_handle* handle, double* tbar);
+
+/* eta_ai = 2 F_ia, eta_aibj = 4 g_iajb - 2 g_ibja (packed), from the T1 Fock
+ * matrix of the last Fock construction or record.  eta may be NULL (kept in HBM).
+ * Replaces: construct_eta (multiplier_equation_ccsd.F90:56-110). */
+int etg_construct_eta(etg_handle* handle, double* eta);
+
+/* R = A^T t-bar + eta on the device (needs etg_prepare_for_multiplier_equation
+ * at the current amplitudes).  residual may be NULL (kept in HBM).
+ * Replaces: construct_multiplier_equation (multiplier_equation_ccs.F90:58-94). */
+int etg_construct_multiplier_equation(etg_handle* handle, double* residual);
+
+/* ||R||_2 (FP64 reduction on the device) of the last multiplier residual.
+ * Replaces: get_l2_norm(residual) (diis_cc_multipliers_class.F90:249). */
+int etg_multiplier_residual_norm(etg_handle* handle, double* norm);
+
+/* e = -R / eps (no diagonal scaling), t-bar += e, on the device.
+ * Replaces: tbar_updater%update (diis_cc_multipliers_class.F90:269) with the
+ * quasi-Newton updater eT builds for it (cc_multipliers_solver_factory_class.F90:249-251). */
+int etg_multiplier_quasi_newton_update(etg_handle* handle);
+
+/* DIIS extrapolation of the multipliers over the records of etg_diis_initialize.
+ * Replaces: diis%update(residual, multipliers) (diis_cc_multipliers_class.F90:271). */
+int etg_multiplier_diis_update(etg_handle* handle);
+
 /* Virtual batch length of the ladder term's g_acbd blocks (memory/launch
  * trade-off; results do not depend on it beyond rounding). */
 int etg_set_ladder_batch(etg_handle* handle, int64_t batch);

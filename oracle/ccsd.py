@@ -150,7 +150,8 @@ def correlation_energy(L_t1, t1, t2, n_o):
     g = eri(L_t1, o, v, o, v)                                 # g_iajb
     tau = unpack_t2(t2, n_v, n_o) + np.einsum('ai,bj->aibj', t1, t1)
     Lg = 2.0 * g - g.transpose(0, 3, 2, 1)                    # 2 g_iajb - g_ibja
-    return float(np.einsum('aibj,iajb->', tau, Lg))
+    e = np.einsum('aibj,iajb->', tau, Lg)
+    return e if np.iscomplexobj(e) else float(e)          # complex-safe for complex steps
 
 
 def mp2_guess(L_t1, eps, n_o):
@@ -497,6 +498,42 @@ def jacobian_transform(L_mo, h_mo, t1, t2, n_o, c, step=1e-30):
     t1c = t1 + 1j * step * c[:n_t1].reshape(n_v, n_o, order='F')
     t2c = t2 + 1j * step * c[n_t1:]
     return residual(L_mo, h_mo, t1c, t2c, n_o).imag / step
+
+
+# ----------------------------------------------------------------------------
+# Multiplier (Lambda) equations: t-bar^T A + eta = 0
+# (multiplier_equation_ccs.F90:58-94, multiplier_equation_ccsd.F90:41-110)
+# ----------------------------------------------------------------------------
+
+def construct_eta(L_mo, h_mo, t1, n_o):
+    """construct_eta_ccsd (multiplier_equation_ccsd.F90:56-110).
+
+    eta_ai = 2 F_ia (T1 Fock), eta_aibj = 4 g_iajb - 2 g_ibja packed; g_iajb from
+    the ov Cholesky block, which the T1 transformation leaves unchanged.
+    """
+    n_v = L_mo.shape[1] - n_o
+    Lt = t1_cholesky(L_mo, t1, n_o)
+    F = fock_t1(h_mo, Lt, t1, n_o)
+    eta1 = 2.0 * F[:n_o, n_o:].T                       # (v, o): 2 F_ia
+    Lov = L_mo[:, :n_o, n_o:]                          # (J, i, a)
+    g = np.einsum('Jia,Jjb->iajb', Lov, Lov)
+    e2 = 4.0 * g - 2.0 * g.transpose(0, 3, 2, 1)      # (i, a, j, b)
+    eta2 = pack_aibj(e2.transpose(1, 0, 3, 2))         # (a, i, b, j) packed
+    return np.concatenate([eta1.ravel(order='F'), eta2])
+
+
+def initial_multipliers(t1, t2, n_o):
+    """set_initial_multipliers_guess_ccsd (ccsd_class.F90:423-475): t-bar1 = t1,
+    t-bar2 = 2 t_aibj - t_ajbi (construct_packed_contravariant, reordering.F90:4542)."""
+    n_v = t1.shape[0]
+    T = unpack_t2(t2, n_v, n_o)                       # (a, i, b, j)
+    return np.concatenate([t1.ravel(order='F'), pack_aibj(2.0 * T - T.transpose(0, 3, 2, 1))])
+
+
+def multipliers_dense(L_mo, h_mo, t1, t2, n_o):
+    """Solution of A^T t-bar = -eta with the full A (tiny systems only)."""
+    A = jacobian_matrix(L_mo, h_mo, t1, t2, n_o)
+    return np.linalg.solve(A.T, -construct_eta(L_mo, h_mo, t1, n_o))
 
 
 def jacobian_matrix(L_mo, h_mo, t1, t2, n_o):
