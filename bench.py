@@ -86,6 +86,29 @@ def cpu_baseline(n_o, n_v, n_J, cols=4800):
     }
 
 
+def jacobian_throughput(wf):
+    """Right and left Jacobian transformations of one trial vector on the same workload.
+
+    The residual program is recorded once (etg_prepare_for_jacobian); a right
+    transformation replays it forward, a left one backward with the g_acbd
+    blocks recomputed.  The trial vector and result are host arrays here, so
+    each rate includes two PCIe copies of n_t1 + n_t2 doubles.
+    """
+    t0 = time.perf_counter()
+    wf.prepare_for_jacobian()
+    prep = time.perf_counter() - t0
+    X = np.random.default_rng(5).standard_normal((wf.n_gs_amplitudes, 1))
+    out = {'prepare_s': prep}
+    for side in ('right', 'left'):
+        for prec, pname in ((ccsd.FP64, 'fp64'), (ccsd.BF16X3, 'split-bf16x3')):
+            t0 = time.perf_counter()
+            wf.jacobian_transformations(X, side, precision=prec)
+            dt = time.perf_counter() - t0
+            out['%s_%s_transforms_per_s' % (side, pname)] = 1.0 / dt
+            log('%s transformation (%s): %.3f s' % (side, pname, dt))
+    return out
+
+
 def time_to_solution(copies=8, threshold=1e-10):
     """Ground state from the MP2 guess to ||Omega_FP64|| <= threshold, FP64 vs mixed precision.
 
@@ -145,6 +168,7 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-mixed', action='store_true')
     ap.add_argument('--no-tts', action='store_true', help='skip the time-to-solution comparison')
+    ap.add_argument('--no-jacobian', action='store_true', help='skip the Jacobian transformation rates')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one GPU: every box in this setup has a single MI355X, so '
@@ -203,6 +227,8 @@ def main():
     log('ledger:', json.dumps([(e[0], e[1], round(e[2], 4), '%.3e' % e[3]) for e in ledger]))
 
     extra = {}
+    if not args.no_jacobian and args.precision == 'fp64':
+        extra['jacobian'] = jacobian_throughput(wf)
     if not args.no_mixed and args.precision == 'fp64':
         # the same residual in the reduced-precision forms, against the FP64 one
         ref = wf.get_omega()

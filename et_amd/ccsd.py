@@ -54,6 +54,8 @@ _SIGNATURES = {
     'etg_prepare_for_jacobian': [ctypes.c_void_p],
     'etg_jacobian_transformation': [ctypes.c_void_p, _c_dbl_p, _c_dbl_p],
     'etg_jacobian_transpose_transformation': [ctypes.c_void_p, _c_dbl_p, _c_dbl_p],
+    'etg_jacobian_transformations': [ctypes.c_void_p, ctypes.c_int, _i64, _c_dbl_p, _c_dbl_p,
+                                     ctypes.c_int],
     'etg_diis_initialize': [ctypes.c_void_p, _i64],
     'etg_quasi_newton_update': [ctypes.c_void_p],
     'etg_diis_update': [ctypes.c_void_p],
@@ -270,6 +272,16 @@ class CCSD:
         sigma = np.empty(self.n_gs_amplitudes)
         _check(self._lib.etg_jacobian_transpose_transformation(self._h, _ptr(b), _ptr(sigma)))
         return sigma
+
+    def jacobian_transformations(self, X, side='right', precision=FP64):
+        """Columns of X transformed by A (right) or A^T (left) in one call."""
+        X = np.asfortranarray(X, dtype=np.float64)
+        if X.ndim != 2 or X.shape[0] != self.n_gs_amplitudes:
+            raise ETGError('trial vectors must be an (n_t1 + n_t2) x n_vec array')
+        R = np.empty_like(X, order='F')
+        _check(self._lib.etg_jacobian_transformations(self._h, 0 if side == 'right' else 1, X.shape[1],
+                                                      _ptr(X), _ptr(R), int(precision)))
+        return R
 
     def diis_initialize(self, dimension=8):
         """HBM records for DIIS (diis_tool_class.F90:249-277)."""

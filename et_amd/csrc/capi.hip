@@ -168,6 +168,11 @@ int etg_jacobian_transpose_transformation(etg_handle* h, const double* b, double
   return guarded([&] { wf_of(h).jacobian_transpose_transform(b, sigma); });
 }
 
+int etg_jacobian_transformations(etg_handle* h, int side, int64_t n_vec, const double* X, double* R,
+                                 int precision) {
+  return guarded([&] { wf_of(h).transform_batch(side, n_vec, X, R, prec_of(precision)); });
+}
+
 int etg_diis_initialize(etg_handle* h, int64_t dimension) {
   return guarded([&] { wf_of(h).diis_initialize(dimension); });
 }

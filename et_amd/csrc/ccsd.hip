@@ -715,44 +715,54 @@ void CCSD::prepare_for_jacobian() {
 }
 
 void CCSD::jacobian_transform(const double* c, double* rho) {
-  require(tape_ != nullptr, "jacobian_transformation: prepare_for_jacobian has not been called");
   require(c != nullptr && rho != nullptr, "jacobian_transformation: null pointer");
-  const int64_t n = n_t1 + n_t2;
-  tape_->zero_derivatives();
-  double* dt = tape_->deriv(t_.as<double>());
-  ETG_HIP(hipMemcpyAsync(dt, c, sizeof(double) * n, hipMemcpyHostToDevice, stream_));
-  scale_doubles_diagonal_kernel<<<unsigned(ceil_div(n_t1, 256)), 256, 0, stream_>>>(dt, n_t1, 2.0);
-  ETG_LAUNCH("scale_doubles_diagonal");
-  tape_->tangent(Prec::F64);
-  ETG_HIP(hipMemcpyAsync(rho, tape_->deriv(omega_.as<double>()), sizeof(double) * n,
-                         hipMemcpyDeviceToHost, stream_));
-  synchronize();
+  transform_batch(0, 1, c, rho, Prec::F64);
 }
 
 void CCSD::jacobian_transpose_transform(const double* b, double* sigma) {
-  require(tape_ != nullptr, "jacobian_transpose_transformation: prepare_for_jacobian has not been called");
   require(b != nullptr && sigma != nullptr, "jacobian_transpose_transformation: null pointer");
-  const int64_t n = n_t1 + n_t2;
-  jvec_.resize(sizeof(double) * n);
-  ETG_HIP(hipMemcpyAsync(jvec_.as<double>(), b, sizeof(double) * n, hipMemcpyHostToDevice, stream_));
-  transpose_transform_device(jvec_.as<double>(), jvec_.as<double>());
-  ETG_HIP(hipMemcpyAsync(sigma, jvec_.as<double>(), sizeof(double) * n, hipMemcpyDeviceToHost,
-                         stream_));
-  synchronize();
+  transform_batch(1, 1, b, sigma, Prec::F64);
 }
 
 // sigma = A^T b on device vectors (b is read before sigma is written; they may alias).
 void CCSD::transpose_transform_device(const double* b, double* sigma) {
-  require(tape_ != nullptr, "jacobian_transpose_transformation: prepare_for_jacobian has not been called");
+  transform_batch(1, 1, b, sigma, Prec::F64);
+}
+
+// R_k = A X_k (side 0) or A^T X_k (side 1), k < n_vec, columns of length
+// n_t1 + n_t2; X and R are host or device pointers (the copies use unified
+// addressing).  Column k of X is read before column k of R is written.
+void CCSD::transform_batch(int side, int64_t n_vec, const double* X, double* R, Prec prec) {
+  require(tape_ != nullptr, side == 0 ? "jacobian_transformation: prepare_for_jacobian has not been called"
+                                      : "jacobian_transpose_transformation: prepare_for_jacobian has not been called");
+  require(side == 0 || side == 1, "jacobian_transformations: side must be 0 (right) or 1 (left)");
+  require(n_vec >= 0, "jacobian_transformations: negative vector count");
+  require(n_vec == 0 || (X != nullptr && R != nullptr), "jacobian_transformations: null pointer");
   const int64_t n = n_t1 + n_t2;
-  tape_->zero_derivatives();
-  ETG_HIP(hipMemcpyAsync(tape_->deriv(omega_.as<double>()), b, sizeof(double) * n,
-                         hipMemcpyDeviceToDevice, stream_));
-  tape_->adjoint(Prec::F64);
-  double* tb = tape_->deriv(t_.as<double>());
-  scale_doubles_diagonal_kernel<<<unsigned(ceil_div(n_t1, 256)), 256, 0, stream_>>>(tb, n_t1, 2.0);
-  ETG_LAUNCH("scale_doubles_diagonal");
-  ETG_HIP(hipMemcpyAsync(sigma, tb, sizeof(double) * n, hipMemcpyDeviceToDevice, stream_));
+  const unsigned g = unsigned(ceil_div(n_t1, 256));
+  for (int64_t k = 0; k < n_vec; ++k) {
+    const double* x = X + k * n;
+    double* r = R + k * n;
+    tape_->zero_derivatives();
+    if (side == 0) {
+      double* dt = tape_->deriv(t_.as<double>());
+      ETG_HIP(hipMemcpyAsync(dt, x, sizeof(double) * n, hipMemcpyDefault, stream_));
+      scale_doubles_diagonal_kernel<<<g, 256, 0, stream_>>>(dt, n_t1, 2.0);
+      ETG_LAUNCH("scale_doubles_diagonal");
+      tape_->tangent(prec);
+      ETG_HIP(hipMemcpyAsync(r, tape_->deriv(omega_.as<double>()), sizeof(double) * n,
+                             hipMemcpyDefault, stream_));
+    } else {
+      ETG_HIP(hipMemcpyAsync(tape_->deriv(omega_.as<double>()), x, sizeof(double) * n,
+                             hipMemcpyDefault, stream_));
+      tape_->adjoint(prec);
+      double* tb = tape_->deriv(t_.as<double>());
+      scale_doubles_diagonal_kernel<<<g, 256, 0, stream_>>>(tb, n_t1, 2.0);
+      ETG_LAUNCH("scale_doubles_diagonal");
+      ETG_HIP(hipMemcpyAsync(r, tb, sizeof(double) * n, hipMemcpyDefault, stream_));
+    }
+  }
+  synchronize();
 }
 
 }  // namespace etg

@@ -60,6 +60,8 @@ class CCSD {
   void prepare_for_jacobian();                    // records the residual program
   void jacobian_transform(const double* c_host, double* rho_host);            // rho = A c
   void jacobian_transpose_transform(const double* b_host, double* sigma_host);  // sigma = A^T b
+  // n_vec transformations, side 0 (A x) or 1 (A^T x); host or device pointers
+  void transform_batch(int side, int64_t n_vec, const double* X, double* R, Prec prec);
 
   // device pointers (for callers that keep data in HBM)
   double* d_omega() { return omega_.as<double>(); }

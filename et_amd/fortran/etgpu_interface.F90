@@ -211,6 +211,18 @@ module etgpu_interface
          real(c_double), intent(out) :: norm
       end function etg_multiplier_residual_norm
 !
+!     construct_Jacobian_transform over n_vec trial vectors (ccs_class.F90:925-978)
+      integer(c_int) function etg_jacobian_transformations(handle, side, n_vec, X, R, precision) &
+                                                   bind(C, name='etg_jacobian_transformations')
+         import :: c_int, c_ptr, c_double, c_int64_t
+         type(c_ptr), value :: handle
+         integer(c_int), value :: side
+         integer(c_int64_t), value :: n_vec
+         real(c_double), dimension(*), intent(in) :: X
+         real(c_double), dimension(*), intent(out) :: R
+         integer(c_int), value :: precision
+      end function etg_jacobian_transformations
+!
    end interface
 !
 contains

@@ -185,6 +185,16 @@ int etg_multiplier_quasi_newton_update(etg_handle* handle);
  * Replaces: diis%update(residual, multipliers) (diis_cc_multipliers_class.F90:271). */
 int etg_multiplier_diis_update(etg_handle* handle);
 
+/* n_vec transformations at once: R_k = A X_k (side 0) or A^T X_k (side 1)
+ * with the bulk products in `precision` (etg_precision), X and R column-major
+ * (n_t1 + n_t2) x n_vec, host or device (HIP) pointers.  Same preparation and
+ * conventions as etg_jacobian_transformation.
+ * Replaces: the per-vector loop of construct_Jacobian_transform(r_or_l, X, R)
+ * (src/wavefunctions/ccs/ccs_class.F90:925-978) over a Davidson iteration's
+ * new trial vectors. */
+int etg_jacobian_transformations(etg_handle* handle, int side, int64_t n_vec, const double* X,
+                                 double* R, int precision);
+
 /* Virtual batch length of the ladder term's g_acbd blocks (memory/launch
  * trade-off; results do not depend on it beyond rounding). */
 int etg_set_ladder_batch(etg_handle* handle, int64_t batch);

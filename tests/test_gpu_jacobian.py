@@ -104,3 +104,23 @@ def test_jacobian_requires_preparation():
         wf.set_amplitudes(wf.get_amplitudes())          # invalidates the record
         with pytest.raises(ccsd.ETGError):
             wf.jacobian_transpose_transformation(np.zeros(wf.n_gs_amplitudes))
+
+
+def test_batched_transformations_match_single_calls():
+    """etg_jacobian_transformations: FP64 columns equal the single-vector calls; the
+    reduced-precision forms stay within their rounding of the FP64 ones."""
+    s = make(3, 7, 12, 5)
+    n = 3 * 7 + O.n_t2(3, 7)
+    rng = np.random.default_rng(21)
+    X = rng.normal(size=(n, 3))
+    with load(s, 3) as wf:
+        wf.prepare_for_jacobian()
+        for side, single in (('right', wf.jacobian_transformation),
+                             ('left', wf.jacobian_transpose_transformation)):
+            R = wf.jacobian_transformations(X, side)
+            for k in range(3):
+                assert rel(R[:, k], single(X[:, k])) < 1e-14
+            for prec, tol in ((ccsd.FP32, 1e-5), (ccsd.BF16X3, 2e-4)):
+                Rp = wf.jacobian_transformations(X, side, precision=prec)
+                assert 0 < rel(Rp, R) < tol
+        assert wf.jacobian_transformations(X[:, :0], 'right').shape == (n, 0)
