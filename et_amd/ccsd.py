@@ -56,6 +56,9 @@ _SIGNATURES = {
     'etg_jacobian_transpose_transformation': [ctypes.c_void_p, _c_dbl_p, _c_dbl_p],
     'etg_jacobian_transformations': [ctypes.c_void_p, ctypes.c_int, _i64, _c_dbl_p, _c_dbl_p,
                                      ctypes.c_int],
+    'etg_davidson': [ctypes.c_void_p, ctypes.c_int, _i64, ctypes.c_double, _i64, _i64, ctypes.c_int,
+                     _c_dbl_p, _c_dbl_p, _c_dbl_p, ctypes.POINTER(_i64)],
+    'etg_test_reduced_eigen': [_i64, _c_dbl_p, _i64, _c_dbl_p, _c_dbl_p, _c_dbl_p],
     'etg_diis_initialize': [ctypes.c_void_p, _i64],
     'etg_quasi_newton_update': [ctypes.c_void_p],
     'etg_diis_update': [ctypes.c_void_p],
@@ -115,6 +118,17 @@ def device_gemm(ta, tb, alpha, A, B, beta, C, precision=FP64):
     _check(lib.etg_test_gemm(ta.encode(), tb.encode(), m, n, k, alpha, _ptr(A), max(1, A.shape[0]),
                              _ptr(B), max(1, B.shape[0]), beta, _ptr(C), max(1, m), precision))
     return C
+
+
+def reduced_eigen(G, n_want):
+    """Lowest n_want eigenpairs of a small real matrix with the device reduced-space solver."""
+    lib = load_library()
+    G = np.asfortranarray(G, dtype=np.float64)
+    k = G.shape[0]
+    wr, wi = np.empty(k), np.empty(k)
+    X = np.empty((k, max(n_want, 1)), order='F')
+    _check(lib.etg_test_reduced_eigen(k, _ptr(G), n_want, _ptr(wr), _ptr(wi), _ptr(X)))
+    return wr, wi, X[:, :n_want]
 
 
 def bench_gemm(m, n, k, precision=FP64, reps=5, ta='N', tb='N'):
@@ -282,6 +296,21 @@ class CCSD:
         _check(self._lib.etg_jacobian_transformations(self._h, 0 if side == 'right' else 1, X.shape[1],
                                                       _ptr(X), _ptr(R), int(precision)))
         return R
+
+    def davidson(self, n_states, side='right', residual_threshold=1e-6, max_iterations=100,
+                 max_dim=0, precision=FP64, vectors=False):
+        """Device Davidson (eigen_davidson_solver_class.F90:122-318): lowest roots of A or A^T.
+
+        Returns (energies, residual-norm history per iteration[, eigenvectors])."""
+        e = np.empty(n_states)
+        hist = np.full((max_iterations, n_states), np.nan)
+        X = np.empty((self.n_gs_amplitudes, n_states), order='F') if vectors else None
+        its = _i64()
+        _check(self._lib.etg_davidson(self._h, 0 if side == 'right' else 1, n_states, residual_threshold,
+                                      max_iterations, max_dim, int(precision), _ptr(e),
+                                      _ptr(X) if vectors else None, _ptr(hist), ctypes.byref(its)))
+        hist = hist[:its.value]
+        return (e, hist, X) if vectors else (e, hist)
 
     def diis_initialize(self, dimension=8):
         """HBM records for DIIS (diis_tool_class.F90:249-277)."""

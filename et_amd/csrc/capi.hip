@@ -173,6 +173,17 @@ int etg_jacobian_transformations(etg_handle* h, int side, int64_t n_vec, const d
   return guarded([&] { wf_of(h).transform_batch(side, n_vec, X, R, prec_of(precision)); });
 }
 
+int etg_davidson(etg_handle* h, int side, int64_t n_states, double residual_threshold,
+                 int64_t max_iterations, int64_t max_dim, int precision, double* energies,
+                 double* vectors, double* residual_norms, int64_t* iterations) {
+  return guarded([&] {
+    if (energies == nullptr) etg::fail("etg_davidson: null energies");
+    const int64_t it = wf_of(h).davidson(side, n_states, residual_threshold, max_iterations, max_dim,
+                                         prec_of(precision), energies, vectors, residual_norms);
+    if (iterations) *iterations = it;
+  });
+}
+
 int etg_diis_initialize(etg_handle* h, int64_t dimension) {
   return guarded([&] { wf_of(h).diis_initialize(dimension); });
 }
@@ -280,6 +291,22 @@ int etg_test_gemm(char ta, char tb, int64_t m, int64_t n, int64_t k, double alph
               d.c.as<double>(), ldc, prec_of(precision), &ws);
     ETG_HIP(hipDeviceSynchronize());
     ETG_HIP(hipMemcpy(C, d.c.as<double>(), sizeof(double) * ldc * n, hipMemcpyDeviceToHost));
+  });
+}
+
+int etg_test_reduced_eigen(int64_t k, const double* G, int64_t n_want, double* wr, double* wi,
+                           double* X) {
+  return guarded([&] {
+    if (G == nullptr || wr == nullptr || wi == nullptr || (n_want > 0 && X == nullptr))
+      etg::fail("etg_test_reduced_eigen: null pointer");
+    if (k < 1 || k > 128) etg::fail("etg_test_reduced_eigen: dimension must be in [1, 128]");
+    DeviceArrays d;
+    d.a.resize(sizeof(double) * k * k);
+    d.b.resize(sizeof(double) * k * std::max<int64_t>(1, n_want));
+    ETG_HIP(hipMemcpy(d.a.as<double>(), G, sizeof(double) * k * k, hipMemcpyHostToDevice));
+    etg::reduced_eigen_solve(nullptr, d.c, d.a.as<double>(), k, k, n_want, wr, wi, d.b.as<double>(), k);
+    if (n_want > 0)
+      ETG_HIP(hipMemcpy(X, d.b.as<double>(), sizeof(double) * k * n_want, hipMemcpyDeviceToHost));
   });
 }
 

@@ -12,6 +12,14 @@
 
 namespace etg {
 
+// Lowest n_want eigenpairs of a small non-symmetric matrix on the device (one
+// wavefront; dimension <= 128): G (device, ld ldg) -> wr, wi (host, k values by
+// ascending real part), X (device, k x n_want, ld ldx).  Throws on complex
+// wanted roots or QR non-convergence.
+void reduced_eigen_solve(hipStream_t stream, DeviceBuffer& scratch, const double* G_dev, int64_t k,
+                         int64_t ldg, int64_t n_want, double* wr, double* wi, double* X_dev,
+                         int64_t ldx);
+
 class CCSD {
  public:
   CCSD(int64_t n_o, int64_t n_v, int64_t n_J);
@@ -62,6 +70,19 @@ class CCSD {
   void jacobian_transpose_transform(const double* b_host, double* sigma_host);  // sigma = A^T b
   // n_vec transformations, side 0 (A x) or 1 (A^T x); host or device pointers
   void transform_batch(int side, int64_t n_vec, const double* X, double* R, Prec prec);
+
+  // Davidson eigensolver on the device (eigen_davidson_solver_class.F90:122-318):
+  // n_states lowest roots of A (side 0) or A^T (side 1), transforms in `prec`,
+  // reduced-space algebra in FP64.  energies (n_states), vectors (n x n_states,
+  // host or device, may be null), residual_norms (max_iterations x n_states, may
+  // be null).  Returns the number of iterations.
+  int64_t davidson(int side, int64_t n_states, double threshold, int64_t max_iterations,
+                   int64_t max_dim, Prec prec, double* energies, double* vectors,
+                   double* residual_norms);
+  // lowest n_want eigenpairs of the k x k matrix G (device, ld ldg): wr, wi (host,
+  // k, ascending real part), X (device, k x n_want, ld ldx)
+  void reduced_eigen(const double* G_dev, int64_t k, int64_t ldg, int64_t n_want, double* wr,
+                     double* wi, double* X_dev, int64_t ldx);
 
   // device pointers (for callers that keep data in HBM)
   double* d_omega() { return omega_.as<double>(); }
@@ -128,6 +149,9 @@ class CCSD {
   int64_t diis_count_ = 0;
   DeviceBuffer diis_e_, diis_x_, diis_G_, diis_w_, diis_info_, update_;
   bool have_update_ = false, have_omega_ = false;
+
+  // Davidson space (V, AV), Ritz block, reduced matrices
+  DeviceBuffer dav_V_, dav_AV_, dav_Y_, dav_small_, red_eig_;
 
   // multipliers: t-bar, eta, the multiplier residual and its update vector
   DeviceBuffer tbar_, eta_, mres_, mupdate_;

@@ -1,0 +1,526 @@
+// Davidson eigensolver for the CC Jacobian with every vector and all reduced-
+// space linear algebra on the device, in FP64.
+//
+// Follows eT's eigen Davidson solver (run_eigen_davidson_solver,
+// src/solvers/eigen_davidson_solver_class.F90:122-318, and the tools in
+// src/solver_tools/davidson/): trial vectors V and their transforms AV in HBM,
+// reduced matrix G = V^T A V, its lowest eigenpairs, residuals
+// R = (A X - w X) / ||X|| (construct_residual_eigen_davidson_tool,
+// eigen_davidson_tool_class.F90:382-461), diagonal preconditioning by the
+// orbital differences, Gram-Schmidt orthonormalisation against the space
+// (davidson_tool_class.F90:354), restart on the current Ritz vectors when the
+// reduced dimension would exceed its maximum (default max(100, 10 n_states),
+// cc_es_amplitudes_solver_factory_class.F90:307), linear-dependence threshold
+// min(1e-11, 0.1 threshold) (:315).
+//
+// The reduced eigenproblem (non-symmetric, dimension <= 128) is solved by one
+// wavefront: Householder reduction to upper Hessenberg form and the Francis
+// double-shift QR iteration for the eigenvalues (the published EISPACK
+// orthes / hqr algorithms), then inverse iteration on G - w I with a partially
+// pivoted LU for the wanted eigenvectors.  Complex wanted roots are reported as
+// an error.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "ccsd.hpp"
+#include "kernels.hpp"
+
+namespace etg {
+namespace {
+
+constexpr int kMaxRed = 128;
+constexpr int kLanes = 64;
+
+__device__ inline double sgn(double a, double b) { return b >= 0.0 ? fabs(a) : -fabs(a); }
+
+// H(i, j) of the k x k working matrix in LDS, column-major with ld kMaxRed.
+#define HH(i, j) H[(i) + (j) * kMaxRed]
+
+__global__ __launch_bounds__(kLanes) void reduced_eigen_kernel(const double* __restrict__ G, int k,
+                                                               int ldg, int n_want,
+                                                               double* __restrict__ wr_out,
+                                                               double* __restrict__ wi_out,
+                                                               double* __restrict__ X, int ldx,
+                                                               int* __restrict__ info) {
+  __shared__ double H[kMaxRed * kMaxRed];
+  __shared__ double wr[kMaxRed], wi[kMaxRed], ort[kMaxRed], y[kMaxRed];
+  __shared__ int order[kMaxRed], piv[kMaxRed];
+  __shared__ int status;
+  const int lane = threadIdx.x;
+  if (lane == 0) status = 0;
+  for (int q = lane; q < k * k; q += kLanes) HH(q % k, q / k) = G[q % k + (q / k) * int64_t(ldg)];
+  __syncthreads();
+
+  // ---- Householder reduction to upper Hessenberg form (orthes) ----
+  for (int m = 1; m < k - 1; ++m) {
+    double scale = 0.0;
+    for (int i = m; i < k; ++i) scale += fabs(HH(i, m - 1));
+    if (scale == 0.0) continue;
+    double h = 0.0;
+    for (int i = m; i < k; ++i) {
+      const double u = HH(i, m - 1) / scale;
+      h += u * u;
+    }
+    const double g = -sgn(sqrt(h), HH(m, m - 1));
+    h -= HH(m, m - 1) / scale * g;
+    __syncthreads();
+    for (int i = m + lane; i < k; i += kLanes) ort[i] = HH(i, m - 1) / scale - (i == m ? g : 0.0);
+    __syncthreads();
+    for (int j = m + lane; j < k; j += kLanes) {           // (I - u u^T / h) H
+      double f = 0.0;
+      for (int i = m; i < k; ++i) f += ort[i] * HH(i, j);
+      f /= h;
+      for (int i = m; i < k; ++i) HH(i, j) -= f * ort[i];
+    }
+    __syncthreads();
+    for (int i = lane; i < k; i += kLanes) {               // H (I - u u^T / h)
+      double f = 0.0;
+      for (int j = m; j < k; ++j) f += ort[j] * HH(i, j);
+      f /= h;
+      for (int j = m; j < k; ++j) HH(i, j) -= f * ort[j];
+    }
+    __syncthreads();
+    for (int i = m + lane; i < k; i += kLanes) HH(i, m - 1) = (i == m) ? scale * g : 0.0;
+    __syncthreads();
+  }
+
+  // ---- eigenvalues of the Hessenberg matrix (Francis double-shift QR, hqr) ----
+  // Every lane runs the scalar control on the same LDS values; the row and
+  // column sweeps of a double-shift step are split across lanes.
+  double anorm = 0.0;
+  for (int i = 0; i < k; ++i)
+    for (int j = (i > 0 ? i - 1 : 0); j < k; ++j) anorm += fabs(HH(i, j));
+  int nn = k - 1;
+  double t = 0.0;
+  while (nn >= 0 && status == 0) {
+    int its = 0, l;
+    do {
+      for (l = nn; l >= 1; --l) {
+        double s = fabs(HH(l - 1, l - 1)) + fabs(HH(l, l));
+        if (s == 0.0) s = anorm;
+        if (fabs(HH(l, l - 1)) + s == s) break;
+      }
+      __syncthreads();
+      if (l >= 1 && lane == 0) HH(l, l - 1) = 0.0;
+      __syncthreads();
+      double x = HH(nn, nn);
+      if (l == nn) {
+        if (lane == 0) { wr[nn] = x + t; wi[nn] = 0.0; }
+        --nn;
+      } else {
+        double yv = HH(nn - 1, nn - 1);
+        double w = HH(nn, nn - 1) * HH(nn - 1, nn);
+        if (l == nn - 1) {
+          const double p = 0.5 * (yv - x);
+          const double q = p * p + w;
+          double z = sqrt(fabs(q));
+          x += t;
+          if (lane == 0) {
+            if (q >= 0.0) {
+              z = p + sgn(z, p);
+              wr[nn - 1] = wr[nn] = x + z;
+              if (z != 0.0) wr[nn] = x - w / z;
+              wi[nn - 1] = wi[nn] = 0.0;
+            } else {
+              wr[nn - 1] = wr[nn] = x + p;
+              wi[nn - 1] = -z;
+              wi[nn] = z;
+            }
+          }
+          nn -= 2;
+        } else {
+          if (its == 60) {
+            if (lane == 0) status = 1;
+            __syncthreads();
+            break;
+          }
+          if (its == 10 || its == 20 || its == 40) {        // exceptional shift
+            t += x;
+            __syncthreads();
+            for (int i = lane; i <= nn; i += kLanes) HH(i, i) -= x;
+            __syncthreads();
+            const double s = fabs(HH(nn, nn - 1)) + fabs(HH(nn - 1, nn - 2));
+            yv = x = 0.75 * s;
+            w = -0.4375 * s * s;
+          }
+          ++its;
+          int m;
+          double p = 0.0, q = 0.0, r = 0.0, z;
+          for (m = nn - 2; m >= l; --m) {
+            z = HH(m, m);
+            r = x - z;
+            const double s0 = yv - z;
+            p = (r * s0 - w) / HH(m + 1, m) + HH(m, m + 1);
+            q = HH(m + 1, m + 1) - z - r - s0;
+            r = HH(m + 2, m + 1);
+            const double s = fabs(p) + fabs(q) + fabs(r);
+            p /= s;
+            q /= s;
+            r /= s;
+            if (m == l) break;
+            const double u = fabs(HH(m, m - 1)) * (fabs(q) + fabs(r));
+            const double v = fabs(p) * (fabs(HH(m - 1, m - 1)) + fabs(z) + fabs(HH(m + 1, m + 1)));
+            if (u + v == v) break;
+          }
+          __syncthreads();
+          for (int i = m + 2 + lane; i <= nn; i += kLanes) {
+            HH(i, i - 2) = 0.0;
+            if (i != m + 2) HH(i, i - 3) = 0.0;
+          }
+          __syncthreads();
+          for (int kk = m; kk <= nn - 1; ++kk) {
+            if (kk != m) {
+              p = HH(kk, kk - 1);
+              q = HH(kk + 1, kk - 1);
+              r = (kk != nn - 1) ? HH(kk + 2, kk - 1) : 0.0;
+              x = fabs(p) + fabs(q) + fabs(r);
+              if (x != 0.0) {
+                p /= x;
+                q /= x;
+                r /= x;
+              }
+            }
+            const double s = sgn(sqrt(p * p + q * q + r * r), p);
+            __syncthreads();
+            if (s != 0.0) {
+              if (lane == 0) {
+                if (kk == m) {
+                  if (l != m) HH(kk, kk - 1) = -HH(kk, kk - 1);
+                } else {
+                  HH(kk, kk - 1) = -s * x;
+                }
+              }
+              p += s;
+              x = p / s;
+              yv = q / s;
+              z = r / s;
+              q /= p;
+              r /= p;
+              __syncthreads();
+              for (int j = kk + lane; j <= nn; j += kLanes) {      // row modification
+                double pp = HH(kk, j) + q * HH(kk + 1, j);
+                if (kk != nn - 1) {
+                  pp += r * HH(kk + 2, j);
+                  HH(kk + 2, j) -= pp * z;
+                }
+                HH(kk + 1, j) -= pp * yv;
+                HH(kk, j) -= pp * x;
+              }
+              __syncthreads();
+              const int mmin = nn < kk + 3 ? nn : kk + 3;
+              for (int i = l + lane; i <= mmin; i += kLanes) {    // column modification
+                double pp = x * HH(i, kk) + yv * HH(i, kk + 1);
+                if (kk != nn - 1) {
+                  pp += z * HH(i, kk + 2);
+                  HH(i, kk + 2) -= pp * r;
+                }
+                HH(i, kk + 1) -= pp * q;
+                HH(i, kk) -= pp;
+              }
+              __syncthreads();
+            }
+          }
+        }
+      }
+      __syncthreads();
+    } while (l < nn - 1 && status == 0);
+  }
+  __syncthreads();
+
+  // ---- order by real part ----
+  if (lane == 0) {
+    for (int i = 0; i < k; ++i) order[i] = i;
+    for (int i = 1; i < k; ++i) {
+      const int o = order[i];
+      int j = i - 1;
+      while (j >= 0 && (wr[order[j]] > wr[o] || (wr[order[j]] == wr[o] && wi[order[j]] > wi[o]))) {
+        order[j + 1] = order[j];
+        --j;
+      }
+      order[j + 1] = o;
+    }
+    for (int i = 0; i < k; ++i) {
+      wr_out[i] = wr[order[i]];
+      wi_out[i] = wi[order[i]];
+    }
+    for (int w = 0; w < n_want && status == 0; ++w)
+      if (fabs(wi[order[w]]) > 1e-12 * fmax(1.0, fabs(wr[order[w]]))) status = 2;
+  }
+  __syncthreads();
+
+  // ---- eigenvectors by inverse iteration on G - w I ----
+  for (int w = 0; w < n_want && status == 0; ++w) {
+    const double lam = wr[order[w]];
+    const double shift = lam + 1e-13 * fmax(anorm / k, fabs(lam));
+    for (int q = lane; q < k * k; q += kLanes) {
+      const int i = q % k, j = q / k;
+      HH(i, j) = G[i + j * int64_t(ldg)] - (i == j ? shift : 0.0);
+    }
+    __syncthreads();
+    for (int c = 0; c < k; ++c) {                             // LU with partial pivoting
+      int p = c;
+      for (int r = c + 1; r < k; ++r)
+        if (fabs(HH(r, c)) > fabs(HH(p, c))) p = r;
+      __syncthreads();
+      if (lane == 0) {
+        piv[c] = p;
+        if (HH(p, c) == 0.0) HH(p, c) = 1e-300 + 1e-16 * anorm;
+      }
+      __syncthreads();
+      if (p != c)
+        for (int j = lane; j < k; j += kLanes) {
+          const double tmp = HH(c, j);
+          HH(c, j) = HH(p, j);
+          HH(p, j) = tmp;
+        }
+      __syncthreads();
+      const double d = HH(c, c);
+      for (int r = c + 1 + lane; r < k; r += kLanes) {
+        const double f = HH(r, c) / d;
+        HH(r, c) = f;
+        for (int j = c + 1; j < k; ++j) HH(r, j) -= f * HH(c, j);
+      }
+      __syncthreads();
+    }
+    for (int i = lane; i < k; i += kLanes) y[i] = 1.0 + 0.01 * ((i * 7 + w * 3) % 11);
+    __syncthreads();
+    for (int iter = 0; iter < 3; ++iter) {
+      if (lane == 0) {
+        for (int c = 0; c < k; ++c) {                          // P, then L (unit lower)
+          const int p = piv[c];
+          const double tmp = y[c];
+          y[c] = y[p];
+          y[p] = tmp;
+        }
+        for (int c = 0; c < k; ++c) {
+          double s = y[c];
+          for (int j = 0; j < c; ++j) s -= HH(c, j) * y[j];
+          y[c] = s;
+        }
+        for (int c = k - 1; c >= 0; --c) {                     // U
+          double s = y[c];
+          for (int j = c + 1; j < k; ++j) s -= HH(c, j) * y[j];
+          y[c] = s / HH(c, c);
+        }
+        // orthogonal to earlier vectors of (numerically) the same eigenvalue
+        for (int u = 0; u < w; ++u) {
+          if (fabs(wr[order[u]] - lam) > 1e-8 * fmax(1.0, fabs(lam))) continue;
+          double dp = 0.0;
+          for (int i = 0; i < k; ++i) dp += X[i + u * int64_t(ldx)] * y[i];
+          for (int i = 0; i < k; ++i) y[i] -= dp * X[i + u * int64_t(ldx)];
+        }
+        double nrm = 0.0, big = 0.0;
+        int ib = 0;
+        for (int i = 0; i < k; ++i) {
+          nrm += y[i] * y[i];
+          if (fabs(y[i]) > big) { big = fabs(y[i]); ib = i; }
+        }
+        nrm = sqrt(nrm);
+        const double sg = y[ib] < 0.0 ? -1.0 : 1.0;
+        for (int i = 0; i < k; ++i) y[i] = sg * y[i] / nrm;
+      }
+      __syncthreads();
+    }
+    for (int i = lane; i < k; i += kLanes) X[i + w * int64_t(ldx)] = y[i];
+    __syncthreads();
+  }
+  if (lane == 0) *info = status;
+}
+#undef HH
+
+constexpr int kBlock = 256;
+unsigned grid_for_d(int64_t n) { return unsigned(std::min<int64_t>((n + kBlock - 1) / kBlock, 1 << 16)); }
+
+// R_j = AY_j - w_j Y_j  (n x s, column-major, ld n)
+__global__ void ritz_residual_kernel(const double* __restrict__ AY, const double* __restrict__ Y,
+                                     const double* __restrict__ w, int64_t n, int s,
+                                     double* __restrict__ R) {
+  for (int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; q < n * s;
+       q += int64_t(gridDim.x) * blockDim.x) {
+    const int j = int(q / n);
+    R[q] = AY[q] - w[j] * Y[q];
+  }
+}
+
+// x_i = r_i / (w - d_i), guarded where the denominator vanishes (diagonal preconditioner)
+__global__ void precondition_kernel(const double* __restrict__ r, const double* __restrict__ d,
+                                    double w, int64_t n, double* __restrict__ x) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    double den = w - d[i];
+    if (fabs(den) < 1e-8) den = den < 0.0 ? -1e-8 : 1e-8;
+    x[i] = r[i] / den;
+  }
+}
+
+}  // namespace
+
+void reduced_eigen_solve(hipStream_t stream, DeviceBuffer& scratch, const double* G_dev, int64_t k,
+                         int64_t ldg, int64_t n_want, double* wr, double* wi, double* X_dev,
+                         int64_t ldx) {
+  if (k < 1 || k > kMaxRed) fail("reduced eigenproblem: dimension must be in [1, 128]");
+  if (n_want < 0 || n_want > k) fail("reduced eigenproblem: invalid number of wanted roots");
+  if (ldg < k || ldx < k) fail("reduced eigenproblem: leading dimension below the dimension");
+  scratch.resize(sizeof(double) * 2 * kMaxRed + sizeof(int));
+  double* dwr = scratch.as<double>();
+  double* dwi = dwr + kMaxRed;
+  int* dinfo = reinterpret_cast<int*>(dwi + kMaxRed);
+  reduced_eigen_kernel<<<1, kLanes, 0, stream>>>(G_dev, int(k), int(ldg), int(n_want), dwr, dwi, X_dev,
+                                                 int(ldx), dinfo);
+  ETG_LAUNCH("reduced_eigen");
+  int info = 0;
+  ETG_HIP(hipMemcpyAsync(wr, dwr, sizeof(double) * k, hipMemcpyDeviceToHost, stream));
+  ETG_HIP(hipMemcpyAsync(wi, dwi, sizeof(double) * k, hipMemcpyDeviceToHost, stream));
+  ETG_HIP(hipMemcpyAsync(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost, stream));
+  ETG_HIP(hipStreamSynchronize(stream));
+  if (info == 1) fail("reduced eigenproblem: QR iteration did not converge");
+  if (info == 2) fail("reduced eigenproblem: a wanted root is complex");
+}
+
+void CCSD::reduced_eigen(const double* G_dev, int64_t k, int64_t ldg, int64_t n_want, double* wr,
+                         double* wi, double* X_dev, int64_t ldx) {
+  reduced_eigen_solve(stream_, red_eig_, G_dev, k, ldg, n_want, wr, wi, X_dev, ldx);
+}
+
+int64_t CCSD::davidson(int side, int64_t n_states, double threshold, int64_t max_iterations,
+                       int64_t max_dim, Prec prec, double* energies, double* vectors,
+                       double* residual_norms) {
+  require(tape_ != nullptr, "davidson: prepare_for_jacobian has not been called");
+  require(have_eps_, "davidson: orbital energies required");
+  require(side == 0 || side == 1, "davidson: side must be 0 (right) or 1 (left)");
+  const int64_t n = n_t1 + n_t2;
+  require(n_states >= 1 && n_states <= std::min<int64_t>(n, 64), "davidson: n_states must be in [1, 64]");
+  if (max_dim <= 0) max_dim = std::max<int64_t>(100, 10 * n_states);
+  max_dim = std::min<int64_t>(std::min<int64_t>(max_dim, kMaxRed), n);
+  require(max_dim >= 2 * n_states, "davidson: maximum reduced dimension below twice the number of states");
+  const double lindep = std::min(1e-11, 0.1 * threshold);
+  const int64_t s = n_states;
+
+  // HBM: V, AV (n x max_dim), Y, AY, R (n x s), G (max_dim^2), Xr (max_dim x s)
+  dav_V_.resize(sizeof(double) * size_t(n) * size_t(max_dim));
+  dav_AV_.resize(sizeof(double) * size_t(n) * size_t(max_dim));
+  dav_Y_.resize(sizeof(double) * size_t(n) * size_t(3 * s));
+  dav_small_.resize(sizeof(double) * size_t(max_dim * max_dim + max_dim * s + 2 * max_dim));
+  double* V = dav_V_.as<double>();
+  double* AV = dav_AV_.as<double>();
+  double* Y = dav_Y_.as<double>();
+  double* AY = Y + n * s;
+  double* Rb = Y + 2 * n * s;
+  double* G = dav_small_.as<double>();
+  double* Xr = G + max_dim * max_dim;
+  double* cvec = Xr + max_dim * s;
+  diff_.resize(sizeof(double) * n);
+  double* d = diff_.as<double>();
+  orbital_differences(stream_, eps_.as<double>(), d, n_o, n_v, n);
+
+  // start vectors: unit vectors on the lowest orbital differences
+  std::vector<double> hd(n);
+  ETG_HIP(hipMemcpyAsync(hd.data(), d, sizeof(double) * n, hipMemcpyDeviceToHost, stream_));
+  synchronize();
+  std::vector<int64_t> idx(n);
+  for (int64_t i = 0; i < n; ++i) idx[i] = i;
+  std::partial_sort(idx.begin(), idx.begin() + s, idx.end(),
+                    [&](int64_t a, int64_t b) { return hd[a] < hd[b] || (hd[a] == hd[b] && a < b); });
+  fill(stream_, V, n * s, 0.0);
+  const double one = 1.0;
+  for (int64_t j = 0; j < s; ++j)
+    ETG_HIP(hipMemcpyAsync(V + j * n + idx[j], &one, sizeof(double), hipMemcpyHostToDevice, stream_));
+  synchronize();
+  int64_t k = 0, m = s;                   // space dimension, new vectors at V[:, k..k+m)
+
+  auto dot_host = [&](const double* a, const double* b) {
+    double* res = red_result_.as<double>() + 3;
+    dot(stream_, a, b, n, red_partial_.as<double>(), res);
+    double v = 0.0;
+    ETG_HIP(hipMemcpyAsync(&v, res, sizeof(double), hipMemcpyDeviceToHost, stream_));
+    synchronize();
+    return v;
+  };
+  std::vector<double> wr(max_dim), wi(max_dim), norms(s);
+  for (int64_t it = 1; it <= max_iterations; ++it) {
+    // transforms of the new trial vectors
+    transform_batch(side, m, V + k * n, AV + k * n, prec);
+    // G = V^T AV: new columns for all rows, new rows for the old columns
+    const int64_t kn = k + m;
+    run_gemm(make_gemm('T', 'N', kn, m, n, 1.0, V, n, AV + k * n, n, 0.0, G + k * max_dim, max_dim),
+             Prec::F64);
+    if (k > 0)
+      run_gemm(make_gemm('T', 'N', m, k, n, 1.0, V + k * n, n, AV, n, 0.0, G + k, max_dim), Prec::F64);
+    k = kn;
+    reduced_eigen(G, k, max_dim, s, wr.data(), wi.data(), Xr, max_dim);
+    // Ritz vectors and residuals
+    run_gemm(make_gemm('N', 'N', n, s, k, 1.0, V, n, Xr, max_dim, 0.0, Y, n), Prec::F64);
+    run_gemm(make_gemm('N', 'N', n, s, k, 1.0, AV, n, Xr, max_dim, 0.0, AY, n), Prec::F64);
+    ETG_HIP(hipMemcpyAsync(cvec, wr.data(), sizeof(double) * s, hipMemcpyHostToDevice, stream_));
+    ritz_residual_kernel<<<grid_for_d(n * s), kBlock, 0, stream_>>>(AY, Y, cvec, n, int(s), Rb);
+    ETG_LAUNCH("ritz_residual");
+    bool converged = true;
+    for (int64_t j = 0; j < s; ++j) {
+      const double ry = std::sqrt(dot_host(Rb + j * n, Rb + j * n));
+      const double yy = std::sqrt(dot_host(Y + j * n, Y + j * n));
+      norms[j] = ry / yy;
+      if (norms[j] >= threshold) converged = false;
+    }
+    for (int64_t j = 0; j < s; ++j) {
+      energies[j] = wr[j];
+      if (residual_norms) residual_norms[(it - 1) * s + j] = norms[j];
+    }
+    if (converged) {
+      if (vectors) {
+        for (int64_t j = 0; j < s; ++j) {
+          const double yy = std::sqrt(dot_host(Y + j * n, Y + j * n));
+          scale(stream_, Y + j * n, n, 1.0 / yy);
+        }
+        ETG_HIP(hipMemcpyAsync(vectors, Y, sizeof(double) * n * s, hipMemcpyDefault, stream_));
+        synchronize();
+      }
+      return it;
+    }
+    // restart on the Ritz vectors when the space would overflow
+    int64_t n_new = 0;
+    for (int64_t j = 0; j < s; ++j) n_new += norms[j] >= threshold;
+    if (k + n_new > max_dim) {
+      ETG_HIP(hipMemcpyAsync(V, Y, sizeof(double) * n * s, hipMemcpyDeviceToDevice, stream_));
+      ETG_HIP(hipMemcpyAsync(AV, AY, sizeof(double) * n * s, hipMemcpyDeviceToDevice, stream_));
+      // orthonormalise the Ritz vectors (and their transforms alike)
+      for (int64_t j = 0; j < s; ++j) {
+        for (int pass = 0; pass < 2; ++pass)
+          for (int64_t i = 0; i < j; ++i) {
+            const double c = dot_host(V + i * n, V + j * n);
+            axpy(stream_, n, -c, V + i * n, V + j * n);
+            axpy(stream_, n, -c, AV + i * n, AV + j * n);
+          }
+        const double nv = std::sqrt(dot_host(V + j * n, V + j * n));
+        scale(stream_, V + j * n, n, 1.0 / nv);
+        scale(stream_, AV + j * n, n, 1.0 / nv);
+      }
+      run_gemm(make_gemm('T', 'N', s, s, n, 1.0, V, n, AV, n, 0.0, G, max_dim), Prec::F64);
+      k = s;
+    }
+    // preconditioned residuals, orthonormalised against the space (two passes)
+    m = 0;
+    for (int64_t j = 0; j < s; ++j) {
+      if (norms[j] < threshold) continue;
+      double* x = V + (k + m) * n;
+      precondition_kernel<<<grid_for_d(n), kBlock, 0, stream_>>>(Rb + j * n, d, wr[j], n, x);
+      ETG_LAUNCH("precondition");
+      double nx = std::sqrt(dot_host(x, x));
+      if (nx == 0.0) continue;
+      scale(stream_, x, n, 1.0 / nx);
+      const int64_t kk = k + m;
+      for (int pass = 0; pass < 2; ++pass) {
+        run_gemm(make_gemm('T', 'N', kk, 1, n, 1.0, V, n, x, n, 0.0, cvec + s, max_dim), Prec::F64);
+        run_gemm(make_gemm('N', 'N', n, 1, kk, -1.0, V, n, cvec + s, max_dim, 1.0, x, n), Prec::F64);
+      }
+      nx = std::sqrt(dot_host(x, x));
+      if (nx < lindep) continue;
+      scale(stream_, x, n, 1.0 / nx);
+      ++m;
+    }
+    if (m == 0) fail("davidson: no new trial vectors (linear dependence) before convergence");
+  }
+  fail("davidson: did not converge in the maximum number of iterations");
+}
+
+}  // namespace etg

@@ -62,6 +62,18 @@ inline GemmLaunch gemm(hipStream_t s, char ta, char tb, int64_t m, int64_t n, in
   return gemm(s, g, prec, ws);
 }
 
+// Gemm descriptor for C = alpha op(A) op(B) + beta C without batching.
+inline Gemm make_gemm(char ta, char tb, int64_t m, int64_t n, int64_t k, double alpha, const double* A,
+                      int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc) {
+  Gemm g;
+  g.ta = (ta == 'T' || ta == 't');
+  g.tb = (tb == 'T' || tb == 't');
+  g.m = m; g.n = n; g.k = k;
+  g.alpha = alpha; g.beta = beta;
+  g.A = A; g.lda = lda; g.B = B; g.ldb = ldb; g.C = C; g.ldc = ldc;
+  return g;
+}
+
 // Floating-point operations the product performs (2 m n k per batch entry).
 inline double gemm_flops(const Gemm& g) {
   return 2.0 * double(g.m) * double(g.n) * double(g.k) * double(g.batch);

@@ -1,51 +1,17 @@
-"""Ground-state solver driving the GPU wavefunction, mirroring eT's DIIS CC solver.
+"""Solvers mirroring eT's CC solvers over the device-resident CCSD wavefunction.
 
-Follows run_diis_cc_gs (src/solvers/cc/diis_cc_gs_class.F90:209-315): each
-iteration builds the T1 Fock matrix, the energy and the residual on the GPU;
-the quasi-Newton step with orbital-energy denominators
-(quasi_newton_updater_class.F90:81-118, scaled amplitudes and residual as
-amplitude_updater_class.F90:86-120 does) and the DIIS extrapolation
-(diis_tool_class.F90:302-760, dimension 8, no CROP) follow, then the
-Cholesky vectors are T1-dressed again.  Convergence on ||Omega|| like
-convergence_tool%has_converged with energy convergence off
-(convergence_tool_class.F90:79-120).
+DiisCCGroundState: run_diis_cc_gs (src/solvers/cc/diis_cc_gs_class.F90:209-315);
+quasi-Newton step and DIIS extrapolation on the device in FP64.
+MixedPrecisionGroundState: the same iteration on split-bf16, FP32, then FP64
+residuals.
+DiisCCMultipliers: run_diis_cc_multipliers (diis_cc_multipliers_class.F90:195-299).
+EigenDavidson: run_eigen_davidson_solver (eigen_davidson_solver_class.F90:122-318),
+with the whole Davidson space on the device.
+Only scalars (energies, norms) reach the host during the iterations.
 """
 import numpy as np
 
 from . import ccsd as _ccsd
-
-
-class DIIS:
-    """Pulay DIIS with records in host memory (eT's diis_tool with records_in_memory)."""
-
-    def __init__(self, dimension=8):
-        self.dimension = dimension
-        self.e, self.x = [], []
-        self.G = np.zeros((0, 0))
-
-    def update(self, e, x):
-        if len(self.e) == self.dimension:
-            self.e.pop(0)
-            self.x.pop(0)
-            self.G = self.G[1:, 1:]
-        self.e.append(e.copy())
-        self.x.append(x.copy())
-        n = len(self.e)
-        G = np.zeros((n, n))
-        G[:n - 1, :n - 1] = self.G
-        for k in range(n):
-            G[n - 1, k] = G[k, n - 1] = float(self.e[k] @ e)
-        self.G = G
-        A = -np.ones((n + 1, n + 1))
-        A[:n, :n] = G
-        A[n, n] = 0.0
-        rhs = np.zeros(n + 1)
-        rhs[n] = -1.0
-        w = np.linalg.solve(A, rhs)[:n]
-        out = np.zeros_like(x)
-        for wk, xk in zip(w, self.x):
-            out += wk * xk
-        return out
 
 
 class DiisCCGroundState:
@@ -204,64 +170,36 @@ _PREC_NAMES = {_ccsd.FP64: 'fp64', _ccsd.FP32: 'fp32', _ccsd.BF16X3: 'bf16x3'}
 
 
 class EigenDavidson:
-    """Lowest eigenpairs of the CC Jacobian, after eT's eigen Davidson solver.
+    """Lowest eigenpairs of the CC Jacobian, eT's eigen Davidson solver on the device.
 
-    Follows run_eigen_davidson_solver (src/solvers/eigen_davidson_solver_class.F90:122-318):
+    run_eigen_davidson_solver (src/solvers/eigen_davidson_solver_class.F90:122-318):
     start vectors on the lowest orbital differences, each new trial vector
-    transformed by the Jacobian (right) or its transpose (left), reduced
-    matrix X^T (A X) and its non-symmetric eigenproblem
-    (eigen_davidson_tool_class.F90:217-379), residuals preconditioned by the
-    orbital differences and Gram-Schmidt orthonormalised against the space
-    (davidson_tool_class.F90:354).  Reduced-space linear algebra in FP64.
+    transformed by the Jacobian (right) or its transpose (left), the reduced
+    matrix X^T (A X) and its non-symmetric eigenproblem, residuals preconditioned
+    by the orbital differences and orthonormalised against the space.  Trial
+    vectors, transforms and the reduced-space algebra stay on the device (FP64);
+    the transformations run in `precision`.
     """
 
     def __init__(self, wf, n_states, side='right', residual_threshold=1e-6, max_iterations=100,
-                 log=None):
+                 max_dim=0, precision=_ccsd.FP64, log=None):
         self.wf = wf
         self.n_states = n_states
         self.side = side
         self.residual_threshold = residual_threshold
         self.max_iterations = max_iterations
+        self.max_dim = max_dim
+        self.precision = precision
         self.log = log
 
     def run(self):
         wf = self.wf
         wf.prepare_for_jacobian(self.side)
-        transform = (wf.jacobian_transformation if self.side == 'right'
-                     else wf.jacobian_transpose_transformation)
-        diag = wf.get_orbital_differences()
-        n = diag.size
-        V = np.zeros((n, 0))
-        AV = np.zeros((n, 0))
-        new = []
-        for k in np.argsort(diag, kind='stable')[:self.n_states]:
-            e = np.zeros(n)
-            e[k] = 1.0
-            new.append(e)
-        for it in range(1, self.max_iterations + 1):
-            for x in new:
-                for _ in range(2):
-                    x = x - V @ (V.T @ x)
-                nx = np.linalg.norm(x)
-                if nx < 1e-12:
-                    continue
-                x = x / nx
-                V = np.column_stack([V, x])
-                AV = np.column_stack([AV, transform(x)])
-            w, S = np.linalg.eig(V.T @ AV)
-            order = np.argsort(w.real)[:self.n_states]
-            w, S = w[order].real, S[:, order].real
-            new, norms = [], []
-            for r in range(self.n_states):
-                x = V @ S[:, r]
-                res = AV @ S[:, r] - w[r] * x
-                rn = float(np.linalg.norm(res) / np.linalg.norm(x))
-                norms.append(rn)
-                if rn > self.residual_threshold:
-                    new.append(res / (w[r] - diag + 1e-12))
-            if self.log:
-                self.log('%3d  %s  max|res| %.3e' % (it, ' '.join('%.10f' % e for e in w), max(norms)))
-            if not new:
-                self.eigenvectors = V @ S
-                return w
-        raise _ccsd.ETGError('Davidson did not converge in the max number of iterations.')
+        w, hist, X = wf.davidson(self.n_states, self.side, self.residual_threshold, self.max_iterations,
+                                 self.max_dim, self.precision, vectors=True)
+        self.history = hist
+        self.eigenvectors = X
+        if self.log:
+            for it, row in enumerate(hist, 1):
+                self.log('%3d  max|res| %.3e' % (it, row.max()))
+        return w

@@ -145,6 +145,23 @@ int etg_quasi_newton_update(etg_handle* handle);
  * (src/solver_tools/diis_tool_class.F90:302-347) at diis_cc_gs_class.F90:287. */
 int etg_diis_update(etg_handle* handle);
 
+/* Davidson solution of the n_states lowest roots of A (side 0, right) or A^T
+ * (side 1, left) with trial vectors, transforms and all reduced-space algebra
+ * (Gram-Schmidt, G = V^T A V, its eigenproblem, residuals) on the device in
+ * FP64; the transformations themselves in `precision`.  Start vectors on the
+ * lowest orbital differences, diagonal preconditioner, convergence when every
+ * ||A x - w x|| / ||x|| < residual_threshold, restart on the Ritz vectors at
+ * max_dim (<= 0: max(100, 10 n_states); at most 128).  Needs
+ * etg_prepare_for_jacobian.  energies: n_states; vectors: (n_t1 + n_t2) x
+ * n_states, host or device, may be NULL; residual_norms: max_iterations x
+ * n_states per-iteration norms, may be NULL; iterations: may be NULL.
+ * Replaces: run_eigen_davidson_solver
+ * (src/solvers/eigen_davidson_solver_class.F90:122-318) with eT's eigen
+ * Davidson tool (src/solver_tools/davidson/eigen_davidson_tool_class.F90). */
+int etg_davidson(etg_handle* handle, int side, int64_t n_states, double residual_threshold,
+                 int64_t max_iterations, int64_t max_dim, int precision, double* energies,
+                 double* vectors, double* residual_norms, int64_t* iterations);
+
 /* ---- multiplier (Lambda) equations  t-bar^T A + eta = 0 ---------------- */
 
 /* Records the residual program at the current amplitudes for the left
@@ -228,6 +245,12 @@ int etg_test_gemm(char ta, char tb, int64_t m, int64_t n, int64_t k, double alph
 
 /* Seconds per product for `reps` back-to-back m x n x k products on
  * device-resident operands. */
+/* Diagnostic: lowest n_want eigenpairs of a k x k real matrix (host, column-
+ * major) with the device reduced-space solver: wr, wi (k, ascending real
+ * part), X (k x n_want, unit vectors).  Fails on complex wanted roots. */
+int etg_test_reduced_eigen(int64_t k, const double* G, int64_t n_want, double* wr, double* wi,
+                           double* X);
+
 int etg_bench_gemm(char ta, char tb, int64_t m, int64_t n, int64_t k, int precision, int reps,
                    double* seconds);
 
