@@ -159,6 +159,22 @@ def time_to_solution(copies=8, threshold=1e-10):
     return out
 
 
+def copy_calibration(nbytes=1 << 30, reps=3):
+    """Plain copies of known size: each reads and writes `nbytes` of HBM once.
+
+    Run under rocprofv3 --pmc FETCH_SIZE (then WRITE_SIZE) to scale the HBM
+    counters of the residual's kernels.
+    """
+    import torch
+    a = torch.ones(nbytes // 8, dtype=torch.float64, device='cuda')
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({'copy_bytes': nbytes, 'copies': reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -169,7 +185,12 @@ def main():
     ap.add_argument('--no-mixed', action='store_true')
     ap.add_argument('--no-tts', action='store_true', help='skip the time-to-solution comparison')
     ap.add_argument('--no-jacobian', action='store_true', help='skip the Jacobian transformation rates')
+    ap.add_argument('--copy-calibration', action='store_true',
+                    help='only run three 1 GiB device-to-device copies (HBM counter calibration)')
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print('bench.py measures one GPU: every box in this setup has a single MI355X, so '
               '--gpus %d is not supported (the multi-GPU sharding is described in DESIGN.md).'

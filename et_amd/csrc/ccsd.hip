@@ -126,6 +126,13 @@ std::vector<CCSD::LedgerEntry> CCSD::ledger() {
   return out;
 }
 
+namespace {
+Gemm tagged(Gemm g, uint64_t tag) {
+  g.a_tag = tag;
+  return g;
+}
+}  // namespace
+
 void CCSD::run_gemm(const Gemm& g, Prec prec) {
   hipEvent_t a = nullptr, b = nullptr;
   if (ledger_on_) {
@@ -563,18 +570,19 @@ void CCSD::ladder_primal(const double* t2, const double* LT, double* om2, Prec p
   const int64_t op = o * (o + 1) / 2, om = o * (o - 1) / 2;
   const int64_t vp = v * (v + 1) / 2, vm = v * (v - 1) / 2;
   hipStream_t s = stream_;
-  DirectOps X(*this);
+  const uint64_t gen = ++ladder_gen_ << 3;     // t+/- are rebuilt below: new pack tags
   ladder_t_pm(s, t2, tp_.as<double>(), tm_.as<double>(), o, v);
   for (const Block& b : blocks_) {
-    X.gemm('T', 'N', v * b.la, v * b.lb, J, 1.0, LT + J * v * b.a0, J, LT + J * v * b.b0, J, 0.0,
-           gblk_.as<double>(), v * b.la, prec);
+    run_gemm(tagged(make_gemm('T', 'N', v * b.la, v * b.lb, J, 1.0, LT + J * v * b.a0, J,
+                              LT + J * v * b.b0, J, 0.0, gblk_.as<double>(), v * b.la), gen | 7),
+             prec);
     ladder_g_pm(s, gblk_.as<double>(), v, b.la, b.lb, pair_a_.as<int32_t>() + b.col0,
                 pair_b_.as<int32_t>() + b.col0, b.npairs, gp_.as<double>(), gm_.as<double>());
-    X.gemm('N', 'N', op, b.npairs, vp, 0.5, tp_.as<double>(), op, gp_.as<double>(), vp, 0.0,
-           wp_.as<double>() + op * b.col0, op, prec);
+    run_gemm(tagged(make_gemm('N', 'N', op, b.npairs, vp, 0.5, tp_.as<double>(), op, gp_.as<double>(),
+                              vp, 0.0, wp_.as<double>() + op * b.col0, op), gen | 1), prec);
     if (om > 0 && vm > 0)
-      X.gemm('N', 'N', om, b.npairs, vm, 0.5, tm_.as<double>(), om, gm_.as<double>(), vm, 0.0,
-             wm_.as<double>() + om * b.col0, om, prec);
+      run_gemm(tagged(make_gemm('N', 'N', om, b.npairs, vm, 0.5, tm_.as<double>(), om, gm_.as<double>(),
+                                vm, 0.0, wm_.as<double>() + om * b.col0, om), gen | 2), prec);
   }
   ladder_gather(s, wp_.as<double>(), wm_.as<double>(), pair_col_.as<int32_t>(), om2, o, v);
 }
@@ -593,6 +601,7 @@ void CCSD::ladder_tangent(const double* t2, const double* dt2, const double* LT,
   dgp_.resize(gp_.bytes());
   dgm_.resize(gm_.bytes());
   DirectOps X(*this);
+  const uint64_t gen = ++ladder_gen_ << 3;
   ladder_t_pm(s, t2, tp_.as<double>(), tm_.as<double>(), o, v);
   ladder_t_pm(s, dt2, dtp_.as<double>(), dtm_.as<double>(), o, v);
   for (const Block& b : blocks_) {
@@ -608,12 +617,16 @@ void CCSD::ladder_tangent(const double* t2, const double* dt2, const double* LT,
     ladder_g_pm(s, dgblk_.as<double>(), v, b.la, b.lb, pair_a_.as<int32_t>() + b.col0,
                 pair_b_.as<int32_t>() + b.col0, b.npairs, dgp_.as<double>(), dgm_.as<double>());
     double* dw = dwp_.as<double>() + op * b.col0;
-    X.gemm('N', 'N', op, b.npairs, vp, 0.5, dtp_.as<double>(), op, gp_.as<double>(), vp, 0.0, dw, op, prec);
-    X.gemm('N', 'N', op, b.npairs, vp, 0.5, tp_.as<double>(), op, dgp_.as<double>(), vp, 1.0, dw, op, prec);
+    run_gemm(tagged(make_gemm('N', 'N', op, b.npairs, vp, 0.5, dtp_.as<double>(), op, gp_.as<double>(),
+                              vp, 0.0, dw, op), gen | 3), prec);
+    run_gemm(tagged(make_gemm('N', 'N', op, b.npairs, vp, 0.5, tp_.as<double>(), op, dgp_.as<double>(),
+                              vp, 1.0, dw, op), gen | 1), prec);
     if (om > 0 && vm > 0) {
       dw = dwm_.as<double>() + om * b.col0;
-      X.gemm('N', 'N', om, b.npairs, vm, 0.5, dtm_.as<double>(), om, gm_.as<double>(), vm, 0.0, dw, om, prec);
-      X.gemm('N', 'N', om, b.npairs, vm, 0.5, tm_.as<double>(), om, dgm_.as<double>(), vm, 1.0, dw, om, prec);
+      run_gemm(tagged(make_gemm('N', 'N', om, b.npairs, vm, 0.5, dtm_.as<double>(), om,
+                                gm_.as<double>(), vm, 0.0, dw, om), gen | 4), prec);
+      run_gemm(tagged(make_gemm('N', 'N', om, b.npairs, vm, 0.5, tm_.as<double>(), om,
+                                dgm_.as<double>(), vm, 1.0, dw, om), gen | 2), prec);
     }
   }
   ladder_gather(s, dwp_.as<double>(), dwm_.as<double>(), pair_col_.as<int32_t>(), dom2, o, v);
@@ -637,6 +650,7 @@ void CCSD::ladder_adjoint(const double* t2, double* t2b, const double* LT, doubl
   double* tmb = dtm_.as<double>();
   double* wpb = dwp_.as<double>();
   double* wmb = dwm_.as<double>();
+  const uint64_t gen = ++ladder_gen_ << 3;
   ladder_t_pm(s, t2, tp_.as<double>(), tm_.as<double>(), o, v);
   ladder_gather_adjoint(s, om2b, pcol_a_.as<int32_t>(), pcol_b_.as<int32_t>(), wpb, wmb, o, v);
   fill(s, tpb, op * vp, 0.0);
@@ -650,11 +664,13 @@ void CCSD::ladder_adjoint(const double* t2, double* t2b, const double* LT, doubl
     // w+ = 1/2 t+ g+  ->  t+_bar += 1/2 w+_bar g+^T,  g+_bar = 1/2 t+^T w+_bar
     const double* w = wpb + op * b.col0;
     X.gemm('N', 'T', op, vp, b.npairs, 0.5, w, op, gp_.as<double>(), vp, 1.0, tpb, op, prec);
-    X.gemm('T', 'N', vp, b.npairs, op, 0.5, tp_.as<double>(), op, w, op, 0.0, dgp_.as<double>(), vp, prec);
+    run_gemm(tagged(make_gemm('T', 'N', vp, b.npairs, op, 0.5, tp_.as<double>(), op, w, op, 0.0,
+                              dgp_.as<double>(), vp), gen | 5), prec);
     if (om > 0 && vm > 0) {
       w = wmb + om * b.col0;
       X.gemm('N', 'T', om, vm, b.npairs, 0.5, w, om, gm_.as<double>(), vm, 1.0, tmb, om, prec);
-      X.gemm('T', 'N', vm, b.npairs, om, 0.5, tm_.as<double>(), om, w, om, 0.0, dgm_.as<double>(), vm, prec);
+      run_gemm(tagged(make_gemm('T', 'N', vm, b.npairs, om, 0.5, tm_.as<double>(), om, w, om, 0.0,
+                                dgm_.as<double>(), vm), gen | 6), prec);
     } else {
       fill(s, dgm_.as<double>(), std::max<int64_t>(1, vm * b.npairs), 0.0);
     }

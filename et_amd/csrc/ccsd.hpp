@@ -138,6 +138,7 @@ class CCSD {
 
   // ladder plan
   int64_t ladder_nb_ = 0;
+  uint64_t ladder_gen_ = 0;                       // pack-cache tag generation of the ladder operands
   struct Block { int64_t a0, la, b0, lb, col0, npairs; };
   std::vector<Block> blocks_;
   DeviceBuffer pair_a_, pair_b_, pair_col_, pcol_a_, pcol_b_;

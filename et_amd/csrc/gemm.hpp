@@ -29,11 +29,27 @@ struct Gemm {
   double* C = nullptr;
   int64_t ldc = 0, strideC = 0;
   int64_t batch = 1;
+  // Nonzero: the caller guarantees that A holds the same data whenever it passes
+  // this tag, so the split-bf16 path may reuse the packed A of an earlier call.
+  uint64_t a_tag = 0;
+};
+
+// A packed split-bf16 operand kept for reuse under a caller tag.
+struct PackedOperand {
+  uint64_t tag = 0;
+  bool ta = false;
+  int64_t m = 0, k = 0, lda = 0, strideA = 0, batch = 0;
+  const double* A = nullptr;
+  uint64_t used = 0;                 // last use (least recently used entry is replaced)
+  DeviceBuffer buf;
 };
 
 // Scratch for split-K partial sums and the split-bf16 operand packs; grown on demand.
 struct GemmWorkspace {
   DeviceBuffer buf, packA, packB;
+  static constexpr int kCached = 4;
+  PackedOperand cached[kCached];
+  uint64_t clock = 0;
 };
 
 // Which kernel instantiation a product ran on (for the per-kernel ledger).

@@ -250,13 +250,37 @@ GemmLaunch gemm_bf16x3(hipStream_t stream, const Gemm& g, GemmWorkspace* ws) {
   const int64_t Kp = ceil_div(g.k, kDepth) * kDepth;
   const size_t a_elems = size_t(g.batch) * size_t(m_p) * size_t(Kp);
   const size_t b_elems = size_t(g.batch) * size_t(n_p) * size_t(Kp);
-  ws->packA.resize(2 * a_elems * sizeof(__bf16));
-  ws->packB.resize(2 * b_elems * sizeof(__bf16));
-  __bf16* Ahi = ws->packA.as<__bf16>();
+  // A: reuse a pack made under the same caller tag for the same operand
+  PackedOperand* hit = nullptr;
+  if (g.a_tag != 0)
+    for (PackedOperand& e : ws->cached)
+      if (e.tag == g.a_tag && e.A == g.A && e.ta == g.ta && e.m == g.m && e.k == g.k &&
+          e.lda == g.lda && e.strideA == g.strideA && e.batch == g.batch)
+        hit = &e;
+  __bf16* Ahi;
+  if (hit) {
+    hit->used = ++ws->clock;
+    Ahi = hit->buf.as<__bf16>();
+  } else {
+    DeviceBuffer* dst = &ws->packA;
+    if (g.a_tag != 0) {
+      PackedOperand* lru = &ws->cached[0];
+      for (PackedOperand& e : ws->cached)
+        if (e.used < lru->used) lru = &e;
+      PackedOperand& e = *lru;
+      e.used = ++ws->clock;
+      e.tag = g.a_tag; e.A = g.A; e.ta = g.ta; e.m = g.m; e.k = g.k;
+      e.lda = g.lda; e.strideA = g.strideA; e.batch = g.batch;
+      dst = &e.buf;
+    }
+    dst->resize(2 * a_elems * sizeof(__bf16));
+    Ahi = dst->as<__bf16>();
+    pack(stream, g.A, g.lda, g.strideA, g.ta, g.m, g.k, m_p, Kp, g.batch, Ahi, Ahi + a_elems);
+  }
   __bf16* Alo = Ahi + a_elems;
+  ws->packB.resize(2 * b_elems * sizeof(__bf16));
   __bf16* Bhi = ws->packB.as<__bf16>();
   __bf16* Blo = Bhi + b_elems;
-  pack(stream, g.A, g.lda, g.strideA, g.ta, g.m, g.k, m_p, Kp, g.batch, Ahi, Alo);
   pack(stream, g.B, g.ldb, g.strideB, !g.tb, g.n, g.k, n_p, Kp, g.batch, Bhi, Blo);
 
   Params p{};
