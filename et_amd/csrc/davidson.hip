@@ -17,8 +17,9 @@
 // wavefront: Householder reduction to upper Hessenberg form and the Francis
 // double-shift QR iteration for the eigenvalues (the published EISPACK
 // orthes / hqr algorithms), then inverse iteration on G - w I with a partially
-// pivoted LU for the wanted eigenvectors.  Complex wanted roots are reported as
-// an error.
+// pivoted LU for the wanted eigenvectors (for a complex pair a +- ib, the real
+// 2k system of u + i v).  Complex pairs are carried as (u, v) column pairs with
+// real and imaginary residuals, as eT's Davidson tool does.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -37,15 +38,84 @@ __device__ inline double sgn(double a, double b) { return b >= 0.0 ? fabs(a) : -
 // H(i, j) of the k x k working matrix in LDS, column-major with ld kMaxRed.
 #define HH(i, j) H[(i) + (j) * kMaxRed]
 
+// Inverse iteration on the nm x nm matrix M (ld ldm; LDS or global): LU with
+// partial pivoting in place, then three solves M y = y starting from a fixed
+// pattern, each normalised to unit 2-norm (largest component positive).  For
+// real roots M = G - (w + d) I; for a complex pair a +- ib the real 2k system
+// [[G - a I, b I], [-b I, G - a I]] acting on [u; v] (eigenvector u + i v).
+__device__ void inverse_iteration(double* M, int ldm, int nm, double anorm, int* piv, double* y,
+                                  int seed, int lane) {
+#define MM(i, j) M[(i) + (j) * ldm]
+  for (int c = 0; c < nm; ++c) {
+    int p = c;
+    for (int r = c + 1; r < nm; ++r)
+      if (fabs(MM(r, c)) > fabs(MM(p, c))) p = r;
+    __syncthreads();
+    if (lane == 0) {
+      piv[c] = p;
+      if (MM(p, c) == 0.0) MM(p, c) = 1e-300 + 1e-16 * anorm;
+    }
+    __syncthreads();
+    if (p != c)
+      for (int j = lane; j < nm; j += kLanes) {
+        const double tmp = MM(c, j);
+        MM(c, j) = MM(p, j);
+        MM(p, j) = tmp;
+      }
+    __syncthreads();
+    const double d = MM(c, c);
+    for (int r = c + 1 + lane; r < nm; r += kLanes) {
+      const double f = MM(r, c) / d;
+      MM(r, c) = f;
+      for (int j = c + 1; j < nm; ++j) MM(r, j) -= f * MM(c, j);
+    }
+    __syncthreads();
+  }
+  for (int i = lane; i < nm; i += kLanes) y[i] = 1.0 + 0.01 * ((i * 7 + seed * 3) % 11);
+  __syncthreads();
+  for (int iter = 0; iter < 3; ++iter) {
+    if (lane == 0) {
+      for (int c = 0; c < nm; ++c) {
+        const int p = piv[c];
+        const double tmp = y[c];
+        y[c] = y[p];
+        y[p] = tmp;
+      }
+      for (int c = 0; c < nm; ++c) {
+        double s = y[c];
+        for (int j = 0; j < c; ++j) s -= MM(c, j) * y[j];
+        y[c] = s;
+      }
+      for (int c = nm - 1; c >= 0; --c) {
+        double s = y[c];
+        for (int j = c + 1; j < nm; ++j) s -= MM(c, j) * y[j];
+        y[c] = s / MM(c, c);
+      }
+      double nrm = 0.0, big = 0.0;
+      int ib = 0;
+      for (int i = 0; i < nm; ++i) {
+        nrm += y[i] * y[i];
+        if (fabs(y[i]) > big) { big = fabs(y[i]); ib = i; }
+      }
+      nrm = sqrt(nrm);
+      const double sg = y[ib] < 0.0 ? -1.0 : 1.0;
+      for (int i = 0; i < nm; ++i) y[i] = sg * y[i] / nrm;
+    }
+    __syncthreads();
+  }
+#undef MM
+}
+
 __global__ __launch_bounds__(kLanes) void reduced_eigen_kernel(const double* __restrict__ G, int k,
                                                                int ldg, int n_want,
                                                                double* __restrict__ wr_out,
                                                                double* __restrict__ wi_out,
                                                                double* __restrict__ X, int ldx,
+                                                               double* __restrict__ work,
                                                                int* __restrict__ info) {
   __shared__ double H[kMaxRed * kMaxRed];
-  __shared__ double wr[kMaxRed], wi[kMaxRed], ort[kMaxRed], y[kMaxRed];
-  __shared__ int order[kMaxRed], piv[kMaxRed];
+  __shared__ double wr[kMaxRed], wi[kMaxRed], ort[kMaxRed], y[2 * kMaxRed];
+  __shared__ int order[kMaxRed], piv[2 * kMaxRed];
   __shared__ int status;
   const int lane = threadIdx.x;
   if (lane == 0) status = 0;
@@ -244,86 +314,58 @@ __global__ __launch_bounds__(kLanes) void reduced_eigen_kernel(const double* __r
       wr_out[i] = wr[order[i]];
       wi_out[i] = wi[order[i]];
     }
-    for (int w = 0; w < n_want && status == 0; ++w)
-      if (fabs(wi[order[w]]) > 1e-12 * fmax(1.0, fabs(wr[order[w]]))) status = 2;
   }
   __syncthreads();
 
-  // ---- eigenvectors by inverse iteration on G - w I ----
+  // ---- eigenvectors by inverse iteration ----
+  // Real root: column w.  Complex pair a -+ ib (ordered a - ib first): columns
+  // w and w + 1 hold u and v of the eigenvector u + i v of a + ib.
   for (int w = 0; w < n_want && status == 0; ++w) {
-    const double lam = wr[order[w]];
-    const double shift = lam + 1e-13 * fmax(anorm / k, fabs(lam));
-    for (int q = lane; q < k * k; q += kLanes) {
-      const int i = q % k, j = q / k;
-      HH(i, j) = G[i + j * int64_t(ldg)] - (i == j ? shift : 0.0);
-    }
-    __syncthreads();
-    for (int c = 0; c < k; ++c) {                             // LU with partial pivoting
-      int p = c;
-      for (int r = c + 1; r < k; ++r)
-        if (fabs(HH(r, c)) > fabs(HH(p, c))) p = r;
-      __syncthreads();
-      if (lane == 0) {
-        piv[c] = p;
-        if (HH(p, c) == 0.0) HH(p, c) = 1e-300 + 1e-16 * anorm;
+    const double lam = wr[order[w]], mu = wi[order[w]];
+    if (mu == 0.0) {
+      const double shift = lam + 1e-13 * fmax(anorm / k, fabs(lam));
+      for (int q = lane; q < k * k; q += kLanes) {
+        const int i = q % k, j = q / k;
+        HH(i, j) = G[i + j * int64_t(ldg)] - (i == j ? shift : 0.0);
       }
       __syncthreads();
-      if (p != c)
-        for (int j = lane; j < k; j += kLanes) {
-          const double tmp = HH(c, j);
-          HH(c, j) = HH(p, j);
-          HH(p, j) = tmp;
-        }
-      __syncthreads();
-      const double d = HH(c, c);
-      for (int r = c + 1 + lane; r < k; r += kLanes) {
-        const double f = HH(r, c) / d;
-        HH(r, c) = f;
-        for (int j = c + 1; j < k; ++j) HH(r, j) -= f * HH(c, j);
-      }
-      __syncthreads();
-    }
-    for (int i = lane; i < k; i += kLanes) y[i] = 1.0 + 0.01 * ((i * 7 + w * 3) % 11);
-    __syncthreads();
-    for (int iter = 0; iter < 3; ++iter) {
+      inverse_iteration(H, kMaxRed, k, anorm, piv, y, w, lane);
       if (lane == 0) {
-        for (int c = 0; c < k; ++c) {                          // P, then L (unit lower)
-          const int p = piv[c];
-          const double tmp = y[c];
-          y[c] = y[p];
-          y[p] = tmp;
-        }
-        for (int c = 0; c < k; ++c) {
-          double s = y[c];
-          for (int j = 0; j < c; ++j) s -= HH(c, j) * y[j];
-          y[c] = s;
-        }
-        for (int c = k - 1; c >= 0; --c) {                     // U
-          double s = y[c];
-          for (int j = c + 1; j < k; ++j) s -= HH(c, j) * y[j];
-          y[c] = s / HH(c, c);
-        }
         // orthogonal to earlier vectors of (numerically) the same eigenvalue
         for (int u = 0; u < w; ++u) {
-          if (fabs(wr[order[u]] - lam) > 1e-8 * fmax(1.0, fabs(lam))) continue;
+          if (wi[order[u]] != 0.0 || fabs(wr[order[u]] - lam) > 1e-8 * fmax(1.0, fabs(lam))) continue;
           double dp = 0.0;
           for (int i = 0; i < k; ++i) dp += X[i + u * int64_t(ldx)] * y[i];
           for (int i = 0; i < k; ++i) y[i] -= dp * X[i + u * int64_t(ldx)];
         }
-        double nrm = 0.0, big = 0.0;
-        int ib = 0;
-        for (int i = 0; i < k; ++i) {
-          nrm += y[i] * y[i];
-          if (fabs(y[i]) > big) { big = fabs(y[i]); ib = i; }
-        }
+        double nrm = 0.0;
+        for (int i = 0; i < k; ++i) nrm += y[i] * y[i];
         nrm = sqrt(nrm);
-        const double sg = y[ib] < 0.0 ? -1.0 : 1.0;
-        for (int i = 0; i < k; ++i) y[i] = sg * y[i] / nrm;
+        for (int i = 0; i < k; ++i) y[i] /= nrm;
       }
       __syncthreads();
+      for (int i = lane; i < k; i += kLanes) X[i + w * int64_t(ldx)] = y[i];
+      __syncthreads();
+    } else if (mu < 0.0) {
+      const double a = lam + 1e-13 * fmax(anorm / k, fabs(lam)), b = -mu;
+      const int n2 = 2 * k;
+      for (int q = lane; q < n2 * n2; q += kLanes) {
+        const int i = q % n2, j = q / n2;
+        const int bi = i / k, bj = j / k, ii = i % k, jj = j % k;
+        double v;
+        if (bi == bj) v = G[ii + jj * int64_t(ldg)] - (ii == jj ? a : 0.0);
+        else v = (ii == jj) ? (bi == 0 ? b : -b) : 0.0;
+        work[i + int64_t(j) * n2] = v;
+      }
+      __syncthreads();
+      inverse_iteration(work, n2, n2, anorm, piv, y, w, lane);
+      for (int i = lane; i < k; i += kLanes) {
+        X[i + w * int64_t(ldx)] = y[i];
+        if (w + 1 < n_want) X[i + (w + 1) * int64_t(ldx)] = y[k + i];
+      }
+      __syncthreads();
+      ++w;                                  // the partner a + ib is done
     }
-    for (int i = lane; i < k; i += kLanes) X[i + w * int64_t(ldx)] = y[i];
-    __syncthreads();
   }
   if (lane == 0) *info = status;
 }
@@ -332,14 +374,19 @@ __global__ __launch_bounds__(kLanes) void reduced_eigen_kernel(const double* __r
 constexpr int kBlock = 256;
 unsigned grid_for_d(int64_t n) { return unsigned(std::min<int64_t>((n + kBlock - 1) / kBlock, 1 << 16)); }
 
-// R_j = AY_j - w_j Y_j  (n x s, column-major, ld n)
+// R_j = AY_j - w_j Y_j - c_j Y_{p_j}  (n x s, column-major, ld n).  Real root:
+// c_j = 0.  Complex pair a -+ ib in columns (j, j+1) = (u, v): R_j = Au - a u + b v,
+// R_{j+1} = Av - a v - b u (eigen_davidson_tool construct_re/im_residual).
+// coef = [w (s) | c (s) | p (s)] on the device.
 __global__ void ritz_residual_kernel(const double* __restrict__ AY, const double* __restrict__ Y,
-                                     const double* __restrict__ w, int64_t n, int s,
+                                     const double* __restrict__ coef, int64_t n, int s,
                                      double* __restrict__ R) {
   for (int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; q < n * s;
        q += int64_t(gridDim.x) * blockDim.x) {
     const int j = int(q / n);
-    R[q] = AY[q] - w[j] * Y[q];
+    const int64_t i = q - int64_t(j) * n;
+    const int p = int(coef[2 * s + j]);
+    R[q] = AY[q] - coef[j] * Y[q] - coef[s + j] * Y[i + int64_t(p) * n];
   }
 }
 
@@ -362,12 +409,13 @@ void reduced_eigen_solve(hipStream_t stream, DeviceBuffer& scratch, const double
   if (k < 1 || k > kMaxRed) fail("reduced eigenproblem: dimension must be in [1, 128]");
   if (n_want < 0 || n_want > k) fail("reduced eigenproblem: invalid number of wanted roots");
   if (ldg < k || ldx < k) fail("reduced eigenproblem: leading dimension below the dimension");
-  scratch.resize(sizeof(double) * 2 * kMaxRed + sizeof(int));
+  scratch.resize(sizeof(double) * (2 * kMaxRed + 4 * kMaxRed * kMaxRed) + sizeof(int));
   double* dwr = scratch.as<double>();
   double* dwi = dwr + kMaxRed;
-  int* dinfo = reinterpret_cast<int*>(dwi + kMaxRed);
+  double* work = dwi + kMaxRed;             // (2k)^2: complex-pair systems
+  int* dinfo = reinterpret_cast<int*>(work + 4 * kMaxRed * kMaxRed);
   reduced_eigen_kernel<<<1, kLanes, 0, stream>>>(G_dev, int(k), int(ldg), int(n_want), dwr, dwi, X_dev,
-                                                 int(ldx), dinfo);
+                                                 int(ldx), work, dinfo);
   ETG_LAUNCH("reduced_eigen");
   int info = 0;
   ETG_HIP(hipMemcpyAsync(wr, dwr, sizeof(double) * k, hipMemcpyDeviceToHost, stream));
@@ -375,7 +423,6 @@ void reduced_eigen_solve(hipStream_t stream, DeviceBuffer& scratch, const double
   ETG_HIP(hipMemcpyAsync(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost, stream));
   ETG_HIP(hipStreamSynchronize(stream));
   if (info == 1) fail("reduced eigenproblem: QR iteration did not converge");
-  if (info == 2) fail("reduced eigenproblem: a wanted root is complex");
 }
 
 void CCSD::reduced_eigen(const double* G_dev, int64_t k, int64_t ldg, int64_t n_want, double* wr,
@@ -390,26 +437,28 @@ int64_t CCSD::davidson(int side, int64_t n_states, double threshold, int64_t max
   require(have_eps_, "davidson: orbital energies required");
   require(side == 0 || side == 1, "davidson: side must be 0 (right) or 1 (left)");
   const int64_t n = n_t1 + n_t2;
-  require(n_states >= 1 && n_states <= std::min<int64_t>(n, 64), "davidson: n_states must be in [1, 64]");
+  require(n_states >= 1 && n_states <= std::min<int64_t>(n - 1, 62), "davidson: n_states must be in [1, 62]");
   if (max_dim <= 0) max_dim = std::max<int64_t>(100, 10 * n_states);
   max_dim = std::min<int64_t>(std::min<int64_t>(max_dim, kMaxRed), n);
-  require(max_dim >= 2 * n_states, "davidson: maximum reduced dimension below twice the number of states");
+  require(max_dim >= 2 * (n_states + 1),
+          "davidson: maximum reduced dimension below twice the number of states plus two");
   const double lindep = std::min(1e-11, 0.1 * threshold);
-  const int64_t s = n_states;
+  const int64_t s = n_states, s1 = n_states + 1;    // a complex pair may straddle the last root
 
-  // HBM: V, AV (n x max_dim), Y, AY, R (n x s), G (max_dim^2), Xr (max_dim x s)
+  // HBM: V, AV (n x max_dim), Y, AY, R (n x s1), G (max_dim^2), Xr (max_dim x s1), coefficients
   dav_V_.resize(sizeof(double) * size_t(n) * size_t(max_dim));
   dav_AV_.resize(sizeof(double) * size_t(n) * size_t(max_dim));
-  dav_Y_.resize(sizeof(double) * size_t(n) * size_t(3 * s));
-  dav_small_.resize(sizeof(double) * size_t(max_dim * max_dim + max_dim * s + 2 * max_dim));
+  dav_Y_.resize(sizeof(double) * size_t(n) * size_t(3 * s1));
+  dav_small_.resize(sizeof(double) * size_t(max_dim * max_dim + max_dim * s1 + 3 * s1 + max_dim));
   double* V = dav_V_.as<double>();
   double* AV = dav_AV_.as<double>();
   double* Y = dav_Y_.as<double>();
-  double* AY = Y + n * s;
-  double* Rb = Y + 2 * n * s;
+  double* AY = Y + n * s1;
+  double* Rb = Y + 2 * n * s1;
   double* G = dav_small_.as<double>();
   double* Xr = G + max_dim * max_dim;
-  double* cvec = Xr + max_dim * s;
+  double* coef = Xr + max_dim * s1;          // [w | c | partner] (3 s1)
+  double* cvec = coef + 3 * s1;              // Gram-Schmidt coefficients (max_dim)
   diff_.resize(sizeof(double) * n);
   double* d = diff_.as<double>();
   orbital_differences(stream_, eps_.as<double>(), d, n_o, n_v, n);
@@ -437,7 +486,7 @@ int64_t CCSD::davidson(int side, int64_t n_states, double threshold, int64_t max
     synchronize();
     return v;
   };
-  std::vector<double> wr(max_dim), wi(max_dim), norms(s);
+  std::vector<double> wr(max_dim), wi(max_dim), norms(s1), hcoef(3 * s1);
   for (int64_t it = 1; it <= max_iterations; ++it) {
     // transforms of the new trial vectors
     transform_batch(side, m, V + k * n, AV + k * n, prec);
@@ -448,30 +497,55 @@ int64_t CCSD::davidson(int side, int64_t n_states, double threshold, int64_t max
     if (k > 0)
       run_gemm(make_gemm('T', 'N', m, k, n, 1.0, V + k * n, n, AV, n, 0.0, G + k, max_dim), Prec::F64);
     k = kn;
-    reduced_eigen(G, k, max_dim, s, wr.data(), wi.data(), Xr, max_dim);
+    const int64_t nw = std::min(k, s1);
+    reduced_eigen(G, k, max_dim, nw, wr.data(), wi.data(), Xr, max_dim);
+    // tracked columns: real roots, or complex pairs (u, v) in two columns
+    int64_t se = 0;
+    while (se < s && se < nw) {
+      if (wi[se] < 0.0 && se + 1 < nw) {
+        const double b = -wi[se];
+        hcoef[se] = hcoef[se + 1] = wr[se];
+        hcoef[s1 + se] = -b;
+        hcoef[s1 + se + 1] = b;
+        hcoef[2 * s1 + se] = double(se + 1);
+        hcoef[2 * s1 + se + 1] = double(se);
+        se += 2;
+      } else {                            // real root (or an unpaired half: its real part)
+        hcoef[se] = wr[se];
+        hcoef[s1 + se] = 0.0;
+        hcoef[2 * s1 + se] = double(se);
+        se += 1;
+      }
+    }
+    std::vector<double> packed(3 * se);
+    for (int64_t j = 0; j < se; ++j) {
+      packed[j] = hcoef[j];
+      packed[se + j] = hcoef[s1 + j];
+      packed[2 * se + j] = hcoef[2 * s1 + j];
+    }
     // Ritz vectors and residuals
-    run_gemm(make_gemm('N', 'N', n, s, k, 1.0, V, n, Xr, max_dim, 0.0, Y, n), Prec::F64);
-    run_gemm(make_gemm('N', 'N', n, s, k, 1.0, AV, n, Xr, max_dim, 0.0, AY, n), Prec::F64);
-    ETG_HIP(hipMemcpyAsync(cvec, wr.data(), sizeof(double) * s, hipMemcpyHostToDevice, stream_));
-    ritz_residual_kernel<<<grid_for_d(n * s), kBlock, 0, stream_>>>(AY, Y, cvec, n, int(s), Rb);
+    run_gemm(make_gemm('N', 'N', n, se, k, 1.0, V, n, Xr, max_dim, 0.0, Y, n), Prec::F64);
+    run_gemm(make_gemm('N', 'N', n, se, k, 1.0, AV, n, Xr, max_dim, 0.0, AY, n), Prec::F64);
+    ETG_HIP(hipMemcpyAsync(coef, packed.data(), sizeof(double) * 3 * se, hipMemcpyHostToDevice, stream_));
+    ritz_residual_kernel<<<grid_for_d(n * se), kBlock, 0, stream_>>>(AY, Y, coef, n, int(se), Rb);
     ETG_LAUNCH("ritz_residual");
+    synchronize();                        // packed (host) is read by the copy above
     bool converged = true;
-    for (int64_t j = 0; j < s; ++j) {
-      const double ry = std::sqrt(dot_host(Rb + j * n, Rb + j * n));
-      const double yy = std::sqrt(dot_host(Y + j * n, Y + j * n));
-      norms[j] = ry / yy;
+    std::vector<double> ynorm(se);
+    for (int64_t j = 0; j < se; ++j) ynorm[j] = dot_host(Y + j * n, Y + j * n);
+    for (int64_t j = 0; j < se; ++j) {
+      const int64_t p = int64_t(hcoef[2 * s1 + j]);
+      const double yy = std::sqrt(ynorm[j] + (p != j ? ynorm[p] : 0.0));
+      norms[j] = std::sqrt(dot_host(Rb + j * n, Rb + j * n)) / yy;
       if (norms[j] >= threshold) converged = false;
     }
     for (int64_t j = 0; j < s; ++j) {
-      energies[j] = wr[j];
+      energies[j] = hcoef[j];
       if (residual_norms) residual_norms[(it - 1) * s + j] = norms[j];
     }
     if (converged) {
       if (vectors) {
-        for (int64_t j = 0; j < s; ++j) {
-          const double yy = std::sqrt(dot_host(Y + j * n, Y + j * n));
-          scale(stream_, Y + j * n, n, 1.0 / yy);
-        }
+        for (int64_t j = 0; j < s; ++j) scale(stream_, Y + j * n, n, 1.0 / std::sqrt(ynorm[j]));
         ETG_HIP(hipMemcpyAsync(vectors, Y, sizeof(double) * n * s, hipMemcpyDefault, stream_));
         synchronize();
       }
@@ -479,12 +553,12 @@ int64_t CCSD::davidson(int side, int64_t n_states, double threshold, int64_t max
     }
     // restart on the Ritz vectors when the space would overflow
     int64_t n_new = 0;
-    for (int64_t j = 0; j < s; ++j) n_new += norms[j] >= threshold;
+    for (int64_t j = 0; j < se; ++j) n_new += norms[j] >= threshold;
     if (k + n_new > max_dim) {
-      ETG_HIP(hipMemcpyAsync(V, Y, sizeof(double) * n * s, hipMemcpyDeviceToDevice, stream_));
-      ETG_HIP(hipMemcpyAsync(AV, AY, sizeof(double) * n * s, hipMemcpyDeviceToDevice, stream_));
+      ETG_HIP(hipMemcpyAsync(V, Y, sizeof(double) * n * se, hipMemcpyDeviceToDevice, stream_));
+      ETG_HIP(hipMemcpyAsync(AV, AY, sizeof(double) * n * se, hipMemcpyDeviceToDevice, stream_));
       // orthonormalise the Ritz vectors (and their transforms alike)
-      for (int64_t j = 0; j < s; ++j) {
+      for (int64_t j = 0; j < se; ++j) {
         for (int pass = 0; pass < 2; ++pass)
           for (int64_t i = 0; i < j; ++i) {
             const double c = dot_host(V + i * n, V + j * n);
@@ -495,23 +569,23 @@ int64_t CCSD::davidson(int side, int64_t n_states, double threshold, int64_t max
         scale(stream_, V + j * n, n, 1.0 / nv);
         scale(stream_, AV + j * n, n, 1.0 / nv);
       }
-      run_gemm(make_gemm('T', 'N', s, s, n, 1.0, V, n, AV, n, 0.0, G, max_dim), Prec::F64);
-      k = s;
+      run_gemm(make_gemm('T', 'N', se, se, n, 1.0, V, n, AV, n, 0.0, G, max_dim), Prec::F64);
+      k = se;
     }
     // preconditioned residuals, orthonormalised against the space (two passes)
     m = 0;
-    for (int64_t j = 0; j < s; ++j) {
+    for (int64_t j = 0; j < se; ++j) {
       if (norms[j] < threshold) continue;
       double* x = V + (k + m) * n;
-      precondition_kernel<<<grid_for_d(n), kBlock, 0, stream_>>>(Rb + j * n, d, wr[j], n, x);
+      precondition_kernel<<<grid_for_d(n), kBlock, 0, stream_>>>(Rb + j * n, d, hcoef[j], n, x);
       ETG_LAUNCH("precondition");
       double nx = std::sqrt(dot_host(x, x));
       if (nx == 0.0) continue;
       scale(stream_, x, n, 1.0 / nx);
       const int64_t kk = k + m;
       for (int pass = 0; pass < 2; ++pass) {
-        run_gemm(make_gemm('T', 'N', kk, 1, n, 1.0, V, n, x, n, 0.0, cvec + s, max_dim), Prec::F64);
-        run_gemm(make_gemm('N', 'N', n, 1, kk, -1.0, V, n, cvec + s, max_dim, 1.0, x, n), Prec::F64);
+        run_gemm(make_gemm('T', 'N', kk, 1, n, 1.0, V, n, x, n, 0.0, cvec, max_dim), Prec::F64);
+        run_gemm(make_gemm('N', 'N', n, 1, kk, -1.0, V, n, cvec, max_dim, 1.0, x, n), Prec::F64);
       }
       nx = std::sqrt(dot_host(x, x));
       if (nx < lindep) continue;

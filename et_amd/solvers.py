@@ -72,7 +72,7 @@ class MixedPrecisionGroundState:
     """
 
     def __init__(self, wf, residual_threshold=1e-10, max_iterations=100, diis_dimension=8,
-                 schedule=(_ccsd.BF16X3, _ccsd.FP32), margin=30.0, restart_diis=False, log=None):
+                 schedule=(_ccsd.BF16X3, _ccsd.FP32), margin=10.0, restart_diis=False, log=None):
         self.wf = wf
         self.residual_threshold = residual_threshold
         self.max_iterations = max_iterations
