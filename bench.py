@@ -37,6 +37,13 @@ METRIC = 'CCSD omega evals/s & time-to-solution, mixed vs FP64 GPU at |ΔE|≤1e
 # dense matrix-core peaks; split-bf16 counts its three bf16 products per term
 PEAKS = {'fp64': 78.6e12, 'fp32': 157.3e12, 'bf16x3': 2.5e15 / 3}
 WORKLOAD = dict(n_o=48, n_v=400, n_J=1600, seed=0)
+WORKLOAD_NAME = 'omega_o48_v400_J1600_seed0'
+# HBM bytes per launch of the residual's dominant kernel on this workload: rocprofv3
+# --pmc FETCH_SIZE and --pmc WRITE_SIZE (KiB per dispatch, averaged over the kernel's
+# launches in two residual evaluations), each scaled by what the counter reads for a
+# plain 1 GiB device copy (bench.py --copy-calibration: FETCH_SIZE x 2.000,
+# WRITE_SIZE x 1.000).  Summary and commands: profiles/omega_fp64_o48_v400_J1600_hbm_traffic.md
+MEASURED_TRAFFIC = {(WORKLOAD_NAME, 'gemm_mfma16<f64,NN,128x128>'): 1.0175e11}
 
 
 def log(*a):
@@ -230,6 +237,7 @@ def main():
     ledger = wf.kernel_ledger()
     wf.set_kernel_ledger(False)
     dom = max(ledger, key=lambda e: e[2])
+    traffic = MEASURED_TRAFFIC.get((WORKLOAD_NAME, dom[0])) if args.precision == 'fp64' else None
     peak = PEAKS[args.precision]
     f_omega = flopcount.omega_flops(n_o, n_v, n_J)
     roofline = {
@@ -240,12 +248,13 @@ def main():
         'peak': peak / 1e12,
         'unit': 'TFLOP/s',
         'frac': dom[3] / dom[2] / peak,
-        'traffic': None,
+        'traffic': traffic,
+        'algorithmic_bytes': dom[4] / dom[1],
         'avg_launch_ms': dom[2] / dom[1] * 1e3,
         'omega_achieved': f_omega / dt / 1e12,
         'omega_frac': f_omega / dt / peak,
     }
-    log('ledger:', json.dumps([(e[0], e[1], round(e[2], 4), '%.3e' % e[3]) for e in ledger]))
+    log('ledger:', json.dumps([(e[0], e[1], round(e[2], 4), '%.3e' % e[3], '%.3e' % e[4]) for e in ledger]))
 
     extra = {}
     if not args.no_jacobian and args.precision == 'fp64':
@@ -295,7 +304,7 @@ def main():
         'vs_baseline': None,
         'dtype': {'fp64': 'f64', 'fp32': 'f32', 'bf16x3': 'bf16x3-f32acc'}[args.precision],
         'data': 'synthetic',
-        'config': {'workload': 'omega_o48_v400_J1600_seed0', 'n_o': n_o, 'n_v': n_v, 'n_J': n_J,
+        'config': {'workload': WORKLOAD_NAME, 'n_o': n_o, 'n_v': n_v, 'n_J': n_J,
                    'seed': w['seed'], 'precision': args.precision, 'F_omega': f_omega},
         'roofline': roofline,
         'cpu_baseline': cpu,

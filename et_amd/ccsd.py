@@ -73,7 +73,7 @@ _SIGNATURES = {
     'etg_multiplier_diis_update': [ctypes.c_void_p],
     'etg_set_kernel_ledger': [ctypes.c_void_p, ctypes.c_int],
     'etg_kernel_ledger': [ctypes.c_void_p, _i64, ctypes.c_char_p, _i64, ctypes.POINTER(_i64),
-                          _c_dbl_p, _c_dbl_p, ctypes.POINTER(_i64)],
+                          _c_dbl_p, _c_dbl_p, _c_dbl_p, ctypes.POINTER(_i64)],
     'etg_test_gemm': [ctypes.c_char, ctypes.c_char, _i64, _i64, _i64, ctypes.c_double, _c_dbl_p,
                       _i64, _c_dbl_p, _i64, ctypes.c_double, _c_dbl_p, _i64, ctypes.c_int],
     'etg_bench_gemm': [ctypes.c_char, ctypes.c_char, _i64, _i64, _i64, ctypes.c_int, ctypes.c_int,
@@ -367,16 +367,17 @@ class CCSD:
         _check(self._lib.etg_set_kernel_ledger(self._h, 1 if on else 0))
 
     def kernel_ledger(self):
-        """[(kernel, launches, seconds, flops)] aggregated per kernel instantiation."""
+        """[(kernel, launches, seconds, flops, algorithmic bytes)] per kernel instantiation."""
         count = _i64()
-        _check(self._lib.etg_kernel_ledger(self._h, -1, None, 0, None, None, None, ctypes.byref(count)))
+        _check(self._lib.etg_kernel_ledger(self._h, -1, None, 0, None, None, None, None,
+                                           ctypes.byref(count)))
         out = []
         for i in range(count.value):
             name = ctypes.create_string_buffer(128)
-            n, sec, fl = _i64(), ctypes.c_double(), ctypes.c_double()
+            n, sec, fl, by = _i64(), ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
             _check(self._lib.etg_kernel_ledger(self._h, i, name, 128, ctypes.byref(n), ctypes.byref(sec),
-                                               ctypes.byref(fl), None))
-            out.append((name.value.decode(), n.value, sec.value, fl.value))
+                                               ctypes.byref(fl), ctypes.byref(by), None))
+            out.append((name.value.decode(), n.value, sec.value, fl.value, by.value))
         return out
 
     def last_omega_flops(self):

@@ -240,7 +240,8 @@ int etg_set_kernel_ledger(etg_handle* h, int on) {
 }
 
 int etg_kernel_ledger(etg_handle* h, int64_t index, char* name, int64_t name_len,
-                      int64_t* launches, double* seconds, double* flops, int64_t* count) {
+                      int64_t* launches, double* seconds, double* flops, double* bytes,
+                      int64_t* count) {
   return guarded([&] {
     const auto entries = wf_of(h).ledger();
     if (count) *count = int64_t(entries.size());
@@ -254,6 +255,7 @@ int etg_kernel_ledger(etg_handle* h, int64_t index, char* name, int64_t name_len
     if (launches) *launches = e.launches;
     if (seconds) *seconds = e.seconds;
     if (flops) *flops = e.flops;
+    if (bytes) *bytes = e.bytes;
   });
 }
 

@@ -116,12 +116,13 @@ std::vector<CCSD::LedgerEntry> CCSD::ledger() {
     auto it = std::find_if(out.begin(), out.end(),
                            [&](const LedgerEntry& e) { return e.kernel == r.kernel; });
     if (it == out.end()) {
-      out.push_back(LedgerEntry{r.kernel, 0, 0.0, 0.0});
+      out.push_back(LedgerEntry{r.kernel, 0, 0.0, 0.0, 0.0});
       it = out.end() - 1;
     }
     it->launches += 1;
     it->seconds += double(ms) * 1e-3;
     it->flops += r.flops;
+    it->bytes += r.bytes;
   }
   return out;
 }
@@ -143,7 +144,11 @@ void CCSD::run_gemm(const Gemm& g, Prec prec) {
   const GemmLaunch info = etg::gemm(stream_, g, prec, &gws_);
   if (ledger_on_) {
     ETG_HIP(hipEventRecord(b, stream_));
-    ledger_recs_.push_back(LedgerRec{info.kernel, gemm_flops(g), a, b});
+    // algorithmic HBM bytes: read A and B once, write C once (and read it if beta != 0)
+    const double mn = double(g.m) * double(g.n);
+    const double bytes = 8.0 * double(g.batch) *
+                         (double(g.m) * double(g.k) + double(g.k) * double(g.n) + (g.beta != 0.0 ? 2.0 : 1.0) * mn);
+    ledger_recs_.push_back(LedgerRec{info.kernel, gemm_flops(g), bytes, a, b});
   }
   last_omega_flops += gemm_flops(g);
 }

@@ -98,7 +98,7 @@ class CCSD {
   struct LedgerEntry {
     std::string kernel;
     int64_t launches = 0;
-    double seconds = 0.0, flops = 0.0;
+    double seconds = 0.0, flops = 0.0, bytes = 0.0;   // bytes: algorithmic operand traffic
   };
   void set_ledger(bool on);
   std::vector<LedgerEntry> ledger();
@@ -164,7 +164,7 @@ class CCSD {
 
   struct LedgerRec {
     const char* kernel;
-    double flops;
+    double flops, bytes;
     hipEvent_t start, stop;
   };
   bool ledger_on_ = false;

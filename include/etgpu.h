@@ -232,10 +232,14 @@ int etg_last_omega_flops(etg_handle* handle, double* flops);
  * instantiation. */
 int etg_set_kernel_ledger(etg_handle* handle, int on);
 
-/* Entry `index` of the ledger (name copied into `name`, NUL-terminated);
- * `count` receives the number of entries.  index < 0 only returns count. */
+/* Entry `index` of the ledger (name copied into `name`, NUL-terminated):
+ * launches, seconds, 2mnk flops and algorithmic bytes (each operand read once,
+ * C written once and read when beta != 0; FP64 operands) summed over the
+ * kernel's launches; `count` receives the number of entries.  index < 0 only
+ * returns count. */
 int etg_kernel_ledger(etg_handle* handle, int64_t index, char* name, int64_t name_len,
-                      int64_t* launches, double* seconds, double* flops, int64_t* count);
+                      int64_t* launches, double* seconds, double* flops, double* bytes,
+                      int64_t* count);
 
 /* C = alpha op(A) op(B) + beta C on host arrays (copied to the device and
  * back), column-major, op = 'N' or 'T', with the given precision. */
