@@ -54,6 +54,7 @@ _SIGNATURES = {
     'etg_prepare_for_jacobian': [ctypes.c_void_p],
     'etg_jacobian_transformation': [ctypes.c_void_p, _c_dbl_p, _c_dbl_p],
     'etg_jacobian_transpose_transformation': [ctypes.c_void_p, _c_dbl_p, _c_dbl_p],
+    'etg_construct_omega_resident': [ctypes.c_void_p, ctypes.c_int],
     'etg_jacobian_transformations': [ctypes.c_void_p, ctypes.c_int, _i64, _c_dbl_p, _c_dbl_p,
                                      ctypes.c_int],
     'etg_davidson': [ctypes.c_void_p, ctypes.c_int, _i64, ctypes.c_double, _i64, _i64, ctypes.c_int,

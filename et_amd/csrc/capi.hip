@@ -168,6 +168,10 @@ int etg_jacobian_transpose_transformation(etg_handle* h, const double* b, double
   return guarded([&] { wf_of(h).jacobian_transpose_transform(b, sigma); });
 }
 
+int etg_construct_omega_resident(etg_handle* h, int precision) {
+  return guarded([&] { wf_of(h).construct_omega(prec_of(precision)); });
+}
+
 int etg_jacobian_transformations(etg_handle* h, int side, int64_t n_vec, const double* X, double* R,
                                  int precision) {
   return guarded([&] { wf_of(h).transform_batch(side, n_vec, X, R, prec_of(precision)); });

@@ -223,6 +223,14 @@ module etgpu_interface
          integer(c_int), value :: precision
       end function etg_jacobian_transformations
 !
+!     construct_omega with the residual left in HBM (omega_ccsd.F90:39-110)
+      integer(c_int) function etg_construct_omega_resident(handle, precision) &
+                                                   bind(C, name='etg_construct_omega_resident')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: handle
+         integer(c_int), value :: precision
+      end function etg_construct_omega_resident
+!
    end interface
 !
 contains

@@ -123,6 +123,11 @@ int etg_jacobian_transformation(etg_handle* handle, const double* c, double* rho
  * (src/wavefunctions/ccsd/jacobian_transpose_ccsd.F90:129-236). */
 int etg_jacobian_transpose_transformation(etg_handle* handle, const double* b, double* sigma);
 
+/* etg_construct_omega with the residual kept in HBM (for the device solvers;
+ * same as passing omega = NULL, for callers that cannot pass a null array).
+ * Replaces: construct_omega (omega_ccsd.F90:39-110) inside run_diis_cc_gs. */
+int etg_construct_omega_resident(etg_handle* handle, int precision);
+
 /* Allocates HBM records for DIIS of the given dimension (1..32) and clears
  * the history.
  * Replaces: diis%initialize (src/solver_tools/diis_tool_class.F90:249-277)
