@@ -45,6 +45,39 @@ __global__ void permute4_kernel(const double* __restrict__ x, double* __restrict
   }
 }
 
+// Permutation whose output-fastest index is not the input-fastest one: 32 x 32
+// tiles of the (output dim 0, output dim p) plane go through LDS, where p is the
+// output position of the input-fastest index, so reads (along dim p) and writes
+// (along dim 0) are both unit-stride.  q, r: the other two output dims.
+__global__ void permute4_tiled_kernel(const double* __restrict__ x, double* __restrict__ y,
+                                      int64_t e0, int64_t ep, int64_t eq, int64_t er, int64_t s0,
+                                      int64_t sq, int64_t sr, int64_t yp, int64_t yq, int64_t yr,
+                                      double alpha, double beta) {
+  __shared__ double tile[32][33];
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int64_t nt0 = (e0 + 31) / 32;
+  const int64_t t0 = blockIdx.x % nt0, tp = blockIdx.x / nt0;
+  for (int64_t o = blockIdx.y; o < eq * er; o += gridDim.y) {
+    const int64_t iq = o % eq, ir = o / eq;
+    const double* xb = x + iq * sq + ir * sr;
+    double* yb = y + iq * yq + ir * yr;
+    for (int k = ty; k < 32; k += 8) {
+      const int64_t i0 = t0 * 32 + k, ip = tp * 32 + tx;
+      tile[k][tx] = (i0 < e0 && ip < ep) ? xb[i0 * s0 + ip] : 0.0;
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) {
+      const int64_t ip = tp * 32 + k, i0 = t0 * 32 + tx;
+      if (i0 < e0 && ip < ep) {
+        double* dst = yb + i0 + ip * yp;
+        const double v = alpha * tile[tx][k];
+        *dst = (beta == 0.0) ? v : v + beta * *dst;
+      }
+    }
+    __syncthreads();
+  }
+}
+
 __global__ void squareup_kernel(const double* __restrict__ packed, double* __restrict__ full,
                                 int64_t N) {
   const int64_t n = N * N;
@@ -125,23 +158,38 @@ __global__ void ladder_t_pm_kernel(const double* __restrict__ t2, double* __rest
   (void)vm;
 }
 
-__global__ void ladder_g_pm_kernel(const double* __restrict__ G, int64_t n_v, int64_t lenA,
-                                   int64_t lenB, const int32_t* __restrict__ pa,
-                                   const int32_t* __restrict__ pb, int64_t npairs,
-                                   double* __restrict__ gp, double* __restrict__ gm) {
+// g+ (v(v+1)/2 x npairs) and g- (v(v-1)/2 x npairs) of a G block.  For pair k = (a, b) the v x v block
+// M(c, d) = G(c + v a, d + v b) gives g+_cd = M(c,d) + M(d,c) (c <= d) and
+// g-_cd = M(c,d) - M(d,c) (c < d); one workgroup per tile pair (ti <= tj) reads
+// the tiles M(ti, tj) and M(tj, ti) unit-stride into LDS.
+__global__ void ladder_g_pm_tiled_kernel(const double* __restrict__ G, int64_t n_v, int64_t lenA,
+                                         const int32_t* __restrict__ pa, const int32_t* __restrict__ pb,
+                                         int64_t npairs, double* __restrict__ gp, double* __restrict__ gm) {
+  __shared__ double A[32][33], B[32][33];
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
   const int64_t vp = n_v * (n_v + 1) / 2, vm = n_v * (n_v - 1) / 2;
   const int64_t ldG = n_v * lenA;
-  const int64_t n = vp * npairs;
-  for (int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; idx < n;
-       idx += int64_t(gridDim.x) * blockDim.x) {
-    const int64_t rcd = idx % vp, k = idx / vp;
-    const int64_t d = tri_root(rcd), c = rcd - d * (d + 1) / 2;
+  const int64_t tj = tri_root(blockIdx.x), ti = int64_t(blockIdx.x) - tj * (tj + 1) / 2;
+  const int64_t c0 = ti * 32, d0 = tj * 32;
+  for (int64_t k = blockIdx.y; k < npairs; k += gridDim.y) {
     const int64_t a = pa[k], b = pb[k];
-    // G(c, a, d, b) and G(d, a, c, b)
-    const double x1 = G[(c + n_v * a) + ldG * (d + n_v * b)];
-    const double x2 = G[(d + n_v * a) + ldG * (c + n_v * b)];
-    gp[idx] = x1 + x2;
-    if (c < d) gm[(d * (d - 1) / 2 + c) + vm * k] = x1 - x2;
+    const double* M = G + n_v * a + ldG * n_v * b;           // M(c, d) = M[c + ldG d]
+    for (int q = ty; q < 32; q += 8) {
+      const int64_t c = c0 + tx, d = d0 + q;                 // A[tx][q] = M(c0+tx, d0+q)
+      A[tx][q] = (c < n_v && d < n_v) ? M[c + ldG * d] : 0.0;
+      const int64_t c2 = d0 + tx, d2 = c0 + q;               // B[tx][q] = M(d0+tx, c0+q)
+      B[tx][q] = (c2 < n_v && d2 < n_v) ? M[c2 + ldG * d2] : 0.0;
+    }
+    __syncthreads();
+    for (int q = ty; q < 32; q += 8) {
+      const int64_t c = c0 + tx, d = d0 + q;
+      if (c <= d && d < n_v) {
+        const double x1 = A[tx][q], x2 = B[q][tx];           // M(c, d), M(d, c)
+        gp[(d * (d + 1) / 2 + c) + vp * k] = x1 + x2;
+        if (c < d) gm[(d * (d - 1) / 2 + c) + vm * k] = x1 - x2;
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -261,6 +309,23 @@ void permute4(hipStream_t s, const double* x, double* y, const int64_t e[4], con
               double alpha, double beta) {
   const int64_t n = e[0] * e[1] * e[2] * e[3];
   if (n == 0) return;
+  int p = -1;                                   // output position of the input-fastest index
+  for (int k = 0; k < 4; ++k)
+    if (st[k] == 1 && e[k] > 1) p = k;
+  if (p > 0 && e[0] >= 8 && e[p] >= 8) {
+    int oth[2], no = 0;
+    for (int k = 1; k < 4; ++k)
+      if (k != p) oth[no++] = k;
+    const int64_t ostride[4] = {1, e[0], e[0] * e[1], e[0] * e[1] * e[2]};
+    const int64_t tiles = ceil_div(e[0], 32) * ceil_div(e[p], 32);
+    const int64_t others = e[oth[0]] * e[oth[1]];
+    dim3 grid(unsigned(tiles), unsigned(std::min<int64_t>(others, 65535)));
+    permute4_tiled_kernel<<<grid, kBlock, 0, s>>>(x, y, e[0], e[p], e[oth[0]], e[oth[1]], st[0],
+                                                  st[oth[0]], st[oth[1]], ostride[p], ostride[oth[0]],
+                                                  ostride[oth[1]], alpha, beta);
+    ETG_LAUNCH("permute4_tiled");
+    return;
+  }
   permute4_kernel<<<grid_for(n), kBlock, 0, s>>>(x, y, e[0], e[1], e[2], e[3], st[0], st[1], st[2],
                                                  st[3], alpha, beta);
   ETG_LAUNCH("permute4");
@@ -318,7 +383,10 @@ void ladder_g_pm(hipStream_t s, const double* G, int64_t n_v, int64_t lenA, int6
                  const int32_t* pair_a, const int32_t* pair_b, int64_t npairs, double* gp, double* gm) {
   const int64_t n = (n_v * (n_v + 1) / 2) * npairs;
   if (n == 0) return;
-  ladder_g_pm_kernel<<<grid_for(n), kBlock, 0, s>>>(G, n_v, lenA, lenB, pair_a, pair_b, npairs, gp, gm);
+  (void)lenB;
+  const int64_t nt = (n_v + 31) / 32;
+  dim3 grid(unsigned(nt * (nt + 1) / 2), unsigned(std::min<int64_t>(npairs, 65535)));
+  ladder_g_pm_tiled_kernel<<<grid, kBlock, 0, s>>>(G, n_v, lenA, pair_a, pair_b, npairs, gp, gm);
   ETG_LAUNCH("ladder_g_pm");
 }
 
