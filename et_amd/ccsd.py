@@ -72,6 +72,7 @@ _SIGNATURES = {
     'etg_multiplier_residual_norm': [ctypes.c_void_p, _c_dbl_p],
     'etg_multiplier_quasi_newton_update': [ctypes.c_void_p],
     'etg_multiplier_diis_update': [ctypes.c_void_p],
+    'etg_set_transform_graphs': [ctypes.c_void_p, ctypes.c_int],
     'etg_set_kernel_ledger': [ctypes.c_void_p, ctypes.c_int],
     'etg_kernel_ledger': [ctypes.c_void_p, _i64, ctypes.c_char_p, _i64, ctypes.POINTER(_i64),
                           _c_dbl_p, _c_dbl_p, _c_dbl_p, ctypes.POINTER(_i64)],
@@ -363,6 +364,10 @@ class CCSD:
 
     def multiplier_diis_update(self):
         _check(self._lib.etg_multiplier_diis_update(self._h))
+
+    def set_transform_graphs(self, on):
+        """Replay captured HIP graphs of the transformation passes (default) or run them eagerly."""
+        _check(self._lib.etg_set_transform_graphs(self._h, 1 if on else 0))
 
     def set_kernel_ledger(self, on):
         _check(self._lib.etg_set_kernel_ledger(self._h, 1 if on else 0))

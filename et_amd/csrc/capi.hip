@@ -239,6 +239,10 @@ int etg_multiplier_diis_update(etg_handle* h) {
   return guarded([&] { wf_of(h).multiplier_diis_update(); });
 }
 
+int etg_set_transform_graphs(etg_handle* h, int on) {
+  return guarded([&] { wf_of(h).set_transform_graphs(on != 0); });
+}
+
 int etg_set_kernel_ledger(etg_handle* h, int on) {
   return guarded([&] { wf_of(h).set_ledger(on != 0); });
 }

@@ -52,6 +52,7 @@ CCSD::CCSD(int64_t no, int64_t nv, int64_t nJ) : n_o(no), n_v(nv), n_J(nJ) {
 }
 
 CCSD::~CCSD() {
+  drop_transform_graphs();
   if (stream_) {
     (void)hipStreamSynchronize(stream_);
     (void)hipStreamDestroy(stream_);
@@ -128,8 +129,8 @@ std::vector<CCSD::LedgerEntry> CCSD::ledger() {
 }
 
 namespace {
-Gemm tagged(Gemm g, uint64_t tag) {
-  g.a_tag = tag;
+Gemm tagged(Gemm g, uint64_t tag, bool on) {
+  g.a_tag = on ? tag : 0;
   return g;
 }
 }  // namespace
@@ -141,7 +142,7 @@ void CCSD::run_gemm(const Gemm& g, Prec prec) {
     b = take_event();
     ETG_HIP(hipEventRecord(a, stream_));
   }
-  const GemmLaunch info = etg::gemm(stream_, g, prec, &gws_);
+  const GemmLaunch info = etg::gemm(stream_, g, prec, gws_cur_);
   if (ledger_on_) {
     ETG_HIP(hipEventRecord(b, stream_));
     // algorithmic HBM bytes: read A and B once, write C once (and read it if beta != 0)
@@ -170,7 +171,7 @@ void copy_block(hipStream_t s, const double* L, int64_t nJ, int64_t n_mo, int64_
 
 void CCSD::set_cholesky(const double* L) {
   require(L != nullptr, "etg_set_cholesky: null pointer");
-  tape_.reset();
+  reset_record();
   const int64_t o = n_o, v = n_v, J = n_J;
   copy_block(stream_, L, J, n_mo, 0, o, 0, o, Lmo_oo_.as<double>());
   copy_block(stream_, L, J, n_mo, 0, o, o, v, Lmo_ov_.as<double>());
@@ -191,7 +192,7 @@ void CCSD::set_cholesky(const double* L) {
 
 void CCSD::set_oei(const double* h) {
   require(h != nullptr, "etg_set_oei: null pointer");
-  tape_.reset();
+  reset_record();
   ETG_HIP(hipMemcpyAsync(h_.as<double>(), h, sizeof(double) * n_mo * n_mo, hipMemcpyHostToDevice,
                          stream_));
   synchronize();
@@ -209,7 +210,7 @@ void CCSD::set_orbital_energies(const double* eps) {
 
 void CCSD::set_amplitudes(const double* t) {
   require(t != nullptr, "etg_set_amplitudes: null pointer");
-  tape_.reset();
+  reset_record();
   ETG_HIP(hipMemcpyAsync(t_.as<double>(), t, sizeof(double) * (n_t1 + n_t2),
                          hipMemcpyHostToDevice, stream_));
   synchronize();
@@ -348,7 +349,7 @@ double CCSD::omega_norm() {
 
 void CCSD::set_t2_to_mp2_guess() {
   require(have_L_ && have_eps_, "set_t2_to_mp2_guess: Cholesky vectors and orbital energies required");
-  tape_.reset();
+  reset_record();
   const int64_t N = n_t1;
   double* g = scratch(SL_A, N * N, "mp2_g_vovo");
   DirectOps X(*this);
@@ -517,6 +518,7 @@ void CCSD::omega_c2_d2(Ops& X, Prec prec, const double* Tfull, const double* U) 
 void CCSD::set_ladder_batch(int64_t nb) {
   if (nb < 1) fail("ladder batch must be positive");
   synchronize();
+  drop_transform_graphs();              // captured passes hold the old plan's buffers
   ladder_nb_ = std::min(nb, n_v);
   build_ladder_plan();
 }
@@ -579,15 +581,15 @@ void CCSD::ladder_primal(const double* t2, const double* LT, double* om2, Prec p
   ladder_t_pm(s, t2, tp_.as<double>(), tm_.as<double>(), o, v);
   for (const Block& b : blocks_) {
     run_gemm(tagged(make_gemm('T', 'N', v * b.la, v * b.lb, J, 1.0, LT + J * v * b.a0, J,
-                              LT + J * v * b.b0, J, 0.0, gblk_.as<double>(), v * b.la), gen | 7),
+                              LT + J * v * b.b0, J, 0.0, gblk_.as<double>(), v * b.la), gen | 7, pack_cache_),
              prec);
     ladder_g_pm(s, gblk_.as<double>(), v, b.la, b.lb, pair_a_.as<int32_t>() + b.col0,
                 pair_b_.as<int32_t>() + b.col0, b.npairs, gp_.as<double>(), gm_.as<double>());
     run_gemm(tagged(make_gemm('N', 'N', op, b.npairs, vp, 0.5, tp_.as<double>(), op, gp_.as<double>(),
-                              vp, 0.0, wp_.as<double>() + op * b.col0, op), gen | 1), prec);
+                              vp, 0.0, wp_.as<double>() + op * b.col0, op), gen | 1, pack_cache_), prec);
     if (om > 0 && vm > 0)
       run_gemm(tagged(make_gemm('N', 'N', om, b.npairs, vm, 0.5, tm_.as<double>(), om, gm_.as<double>(),
-                                vm, 0.0, wm_.as<double>() + om * b.col0, om), gen | 2), prec);
+                                vm, 0.0, wm_.as<double>() + om * b.col0, om), gen | 2, pack_cache_), prec);
   }
   ladder_gather(s, wp_.as<double>(), wm_.as<double>(), pair_col_.as<int32_t>(), om2, o, v);
 }
@@ -623,15 +625,15 @@ void CCSD::ladder_tangent(const double* t2, const double* dt2, const double* LT,
                 pair_b_.as<int32_t>() + b.col0, b.npairs, dgp_.as<double>(), dgm_.as<double>());
     double* dw = dwp_.as<double>() + op * b.col0;
     run_gemm(tagged(make_gemm('N', 'N', op, b.npairs, vp, 0.5, dtp_.as<double>(), op, gp_.as<double>(),
-                              vp, 0.0, dw, op), gen | 3), prec);
+                              vp, 0.0, dw, op), gen | 3, pack_cache_), prec);
     run_gemm(tagged(make_gemm('N', 'N', op, b.npairs, vp, 0.5, tp_.as<double>(), op, dgp_.as<double>(),
-                              vp, 1.0, dw, op), gen | 1), prec);
+                              vp, 1.0, dw, op), gen | 1, pack_cache_), prec);
     if (om > 0 && vm > 0) {
       dw = dwm_.as<double>() + om * b.col0;
       run_gemm(tagged(make_gemm('N', 'N', om, b.npairs, vm, 0.5, dtm_.as<double>(), om,
-                                gm_.as<double>(), vm, 0.0, dw, om), gen | 4), prec);
+                                gm_.as<double>(), vm, 0.0, dw, om), gen | 4, pack_cache_), prec);
       run_gemm(tagged(make_gemm('N', 'N', om, b.npairs, vm, 0.5, tm_.as<double>(), om,
-                                dgm_.as<double>(), vm, 1.0, dw, om), gen | 2), prec);
+                                dgm_.as<double>(), vm, 1.0, dw, om), gen | 2, pack_cache_), prec);
     }
   }
   ladder_gather(s, dwp_.as<double>(), dwm_.as<double>(), pair_col_.as<int32_t>(), dom2, o, v);
@@ -670,12 +672,12 @@ void CCSD::ladder_adjoint(const double* t2, double* t2b, const double* LT, doubl
     const double* w = wpb + op * b.col0;
     X.gemm('N', 'T', op, vp, b.npairs, 0.5, w, op, gp_.as<double>(), vp, 1.0, tpb, op, prec);
     run_gemm(tagged(make_gemm('T', 'N', vp, b.npairs, op, 0.5, tp_.as<double>(), op, w, op, 0.0,
-                              dgp_.as<double>(), vp), gen | 5), prec);
+                              dgp_.as<double>(), vp), gen | 5, pack_cache_), prec);
     if (om > 0 && vm > 0) {
       w = wmb + om * b.col0;
       X.gemm('N', 'T', om, vm, b.npairs, 0.5, w, om, gm_.as<double>(), vm, 1.0, tmb, om, prec);
       run_gemm(tagged(make_gemm('T', 'N', vm, b.npairs, om, 0.5, tm_.as<double>(), om, w, om, 0.0,
-                                dgm_.as<double>(), vm), gen | 6), prec);
+                                dgm_.as<double>(), vm), gen | 6, pack_cache_), prec);
     } else {
       fill(s, dgm_.as<double>(), std::max<int64_t>(1, vm * b.npairs), 0.0);
     }
@@ -709,7 +711,7 @@ __global__ void scale_doubles_diagonal_kernel(double* x, int64_t n_t1, double fa
 
 void CCSD::prepare_for_jacobian() {
   require(have_L_ && have_t_ && have_h_, "prepare_for_jacobian: Cholesky vectors, h and amplitudes required");
-  tape_.reset();
+  reset_record();
   synchronize();
   auto tape = std::make_unique<Tape>(*this);
   const int64_t J = n_J, o = n_o, v = n_v, n = n_mo;
@@ -730,6 +732,7 @@ void CCSD::prepare_for_jacobian() {
   program_fock(*tape);
   program_omega(*tape, Prec::F64);
   synchronize();
+  drop_transform_graphs();
   tape_ = std::move(tape);
   have_F_ = true;
   have_omega_ = true;
@@ -750,9 +753,56 @@ void CCSD::transpose_transform_device(const double* b, double* sigma) {
   transform_batch(1, 1, b, sigma, Prec::F64);
 }
 
+// One transformation of the staged vector jin_ into jout_ (device buffers of
+// fixed address, so the sequence can be captured once and replayed).
+void CCSD::transform_body(int side, Prec prec) {
+  // the passes use their own product workspace: its blocks must not move
+  // between a capture and the replays, whatever other products need
+  struct Use {
+    CCSD& w;
+    explicit Use(CCSD& x) : w(x) { w.gws_cur_ = &w.gws_pass_; }
+    ~Use() { w.gws_cur_ = &w.gws_; }
+  } use(*this);
+  const int64_t n = n_t1 + n_t2;
+  const unsigned g = unsigned(ceil_div(n_t1, 256));
+  tape_->zero_derivatives();
+  if (side == 0) {
+    double* dt = tape_->deriv(t_.as<double>());
+    ETG_HIP(hipMemcpyAsync(dt, jin_.as<double>(), sizeof(double) * n, hipMemcpyDeviceToDevice, stream_));
+    scale_doubles_diagonal_kernel<<<g, 256, 0, stream_>>>(dt, n_t1, 2.0);
+    ETG_LAUNCH("scale_doubles_diagonal");
+    tape_->tangent(prec);
+    ETG_HIP(hipMemcpyAsync(jout_.as<double>(), tape_->deriv(omega_.as<double>()), sizeof(double) * n,
+                           hipMemcpyDeviceToDevice, stream_));
+  } else {
+    ETG_HIP(hipMemcpyAsync(tape_->deriv(omega_.as<double>()), jin_.as<double>(), sizeof(double) * n,
+                           hipMemcpyDeviceToDevice, stream_));
+    tape_->adjoint(prec);
+    double* tb = tape_->deriv(t_.as<double>());
+    scale_doubles_diagonal_kernel<<<g, 256, 0, stream_>>>(tb, n_t1, 2.0);
+    ETG_LAUNCH("scale_doubles_diagonal");
+    ETG_HIP(hipMemcpyAsync(jout_.as<double>(), tb, sizeof(double) * n, hipMemcpyDeviceToDevice, stream_));
+  }
+}
+
+// The record and everything captured from it are dropped together.
+void CCSD::reset_record() {
+  drop_transform_graphs();
+  tape_.reset();
+}
+
+void CCSD::drop_transform_graphs() {
+  for (auto& e : tgraphs_)
+    if (e.exec) (void)hipGraphExecDestroy(e.exec);
+  tgraphs_.clear();
+}
+
 // R_k = A X_k (side 0) or A^T X_k (side 1), k < n_vec, columns of length
 // n_t1 + n_t2; X and R are host or device pointers (the copies use unified
-// addressing).  Column k of X is read before column k of R is written.
+// addressing).  Column k of X is read before column k of R is written.  The
+// first transformation of a (side, precision) runs eagerly and sizes every
+// buffer; the same sequence is then captured into a HIP graph and replayed for
+// the following vectors (and later calls on the same record).
 void CCSD::transform_batch(int side, int64_t n_vec, const double* X, double* R, Prec prec) {
   require(tape_ != nullptr, side == 0 ? "jacobian_transformation: prepare_for_jacobian has not been called"
                                       : "jacobian_transpose_transformation: prepare_for_jacobian has not been called");
@@ -760,29 +810,47 @@ void CCSD::transform_batch(int side, int64_t n_vec, const double* X, double* R, 
   require(n_vec >= 0, "jacobian_transformations: negative vector count");
   require(n_vec == 0 || (X != nullptr && R != nullptr), "jacobian_transformations: null pointer");
   const int64_t n = n_t1 + n_t2;
-  const unsigned g = unsigned(ceil_div(n_t1, 256));
-  for (int64_t k = 0; k < n_vec; ++k) {
-    const double* x = X + k * n;
-    double* r = R + k * n;
-    tape_->zero_derivatives();
-    if (side == 0) {
-      double* dt = tape_->deriv(t_.as<double>());
-      ETG_HIP(hipMemcpyAsync(dt, x, sizeof(double) * n, hipMemcpyDefault, stream_));
-      scale_doubles_diagonal_kernel<<<g, 256, 0, stream_>>>(dt, n_t1, 2.0);
-      ETG_LAUNCH("scale_doubles_diagonal");
-      tape_->tangent(prec);
-      ETG_HIP(hipMemcpyAsync(r, tape_->deriv(omega_.as<double>()), sizeof(double) * n,
-                             hipMemcpyDefault, stream_));
-    } else {
-      ETG_HIP(hipMemcpyAsync(tape_->deriv(omega_.as<double>()), x, sizeof(double) * n,
-                             hipMemcpyDefault, stream_));
-      tape_->adjoint(prec);
-      double* tb = tape_->deriv(t_.as<double>());
-      scale_doubles_diagonal_kernel<<<g, 256, 0, stream_>>>(tb, n_t1, 2.0);
-      ETG_LAUNCH("scale_doubles_diagonal");
-      ETG_HIP(hipMemcpyAsync(r, tb, sizeof(double) * n, hipMemcpyDefault, stream_));
-    }
+  jin_.resize(sizeof(double) * n);
+  jout_.resize(sizeof(double) * n);
+  if (graph_tape_ != tape_.get()) {
+    drop_transform_graphs();
+    graph_tape_ = tape_.get();
   }
+  const bool graphs = use_graphs_ && !ledger_on_;
+  pack_cache_ = false;                  // packs go to fixed workspaces inside the passes
+  for (int64_t k = 0; k < n_vec; ++k) {
+    ETG_HIP(hipMemcpyAsync(jin_.as<double>(), X + k * n, sizeof(double) * n, hipMemcpyDefault, stream_));
+    TransformGraph* tg = nullptr;
+    for (auto& e : tgraphs_)
+      if (e.side == side && e.prec == prec) tg = &e;
+    if (graphs && tg != nullptr) {
+      ETG_HIP(hipGraphLaunch(tg->exec, stream_));
+    } else {
+      transform_body(side, prec);
+      if (graphs) {
+        synchronize();                  // buffers are sized; capture the same sequence
+        hipGraph_t graph = nullptr;
+        ETG_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+        try {
+          transform_body(side, prec);
+        } catch (...) {
+          (void)hipStreamEndCapture(stream_, &graph);
+          if (graph) (void)hipGraphDestroy(graph);
+          pack_cache_ = true;
+          throw;
+        }
+        ETG_HIP(hipStreamEndCapture(stream_, &graph));
+        TransformGraph e;
+        e.side = side;
+        e.prec = prec;
+        ETG_HIP(hipGraphInstantiate(&e.exec, graph, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(graph);
+        tgraphs_.push_back(e);
+      }
+    }
+    ETG_HIP(hipMemcpyAsync(R + k * n, jout_.as<double>(), sizeof(double) * n, hipMemcpyDefault, stream_));
+  }
+  pack_cache_ = true;
   synchronize();
 }
 

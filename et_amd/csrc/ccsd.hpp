@@ -70,6 +70,8 @@ class CCSD {
   void jacobian_transpose_transform(const double* b_host, double* sigma_host);  // sigma = A^T b
   // n_vec transformations, side 0 (A x) or 1 (A^T x); host or device pointers
   void transform_batch(int side, int64_t n_vec, const double* X, double* R, Prec prec);
+  // replay captured HIP graphs of the transformation passes (on by default)
+  void set_transform_graphs(bool on) { use_graphs_ = on; }
 
   // Davidson eigensolver on the device (eigen_davidson_solver_class.F90:122-318):
   // n_states lowest roots of A (side 0) or A^T (side 1), transforms in `prec`,
@@ -160,6 +162,21 @@ class CCSD {
 
   // Jacobian record
   std::unique_ptr<Tape> tape_;
+  // captured transformation passes (valid for the record graph_tape_)
+  struct TransformGraph {
+    int side = 0;
+    Prec prec = Prec::F64;
+    hipGraphExec_t exec = nullptr;
+  };
+  std::vector<TransformGraph> tgraphs_;
+  const Tape* graph_tape_ = nullptr;
+  bool use_graphs_ = true, pack_cache_ = true;
+  DeviceBuffer jin_, jout_;
+  void transform_body(int side, Prec prec);
+  void drop_transform_graphs();
+  void reset_record();
+  GemmWorkspace gws_pass_;                        // products inside transformation passes
+  GemmWorkspace* gws_cur_ = &gws_;
   DeviceBuffer jvec_;
 
   struct LedgerRec {

@@ -154,14 +154,14 @@ void CCSD::quasi_newton_update() {
   ETG_LAUNCH("quasi_newton");
   have_update_ = true;
   have_omega_ = false;        // the residual belongs to the amplitudes before the step
-  tape_.reset();
+  reset_record();
 }
 
 void CCSD::diis_update() {
   require(have_update_, "diis_update: no update vector (call quasi_newton_update first)");
   diis_step(update_.as<double>(), t_.as<double>());
   have_update_ = false;
-  tape_.reset();
+  reset_record();
 }
 
 void CCSD::diis_step(const double* e, double* x) {

@@ -246,6 +246,10 @@ int etg_kernel_ledger(etg_handle* handle, int64_t index, char* name, int64_t nam
                       int64_t* launches, double* seconds, double* flops, double* bytes,
                       int64_t* count);
 
+/* Diagnostic: replay captured HIP graphs of the transformation passes (on, the
+ * default) or run every pass eagerly (off).  Results are identical. */
+int etg_set_transform_graphs(etg_handle* handle, int on);
+
 /* C = alpha op(A) op(B) + beta C on host arrays (copied to the device and
  * back), column-major, op = 'N' or 'T', with the given precision. */
 int etg_test_gemm(char ta, char tb, int64_t m, int64_t n, int64_t k, double alpha,

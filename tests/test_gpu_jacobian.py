@@ -124,3 +124,42 @@ def test_batched_transformations_match_single_calls():
                 Rp = wf.jacobian_transformations(X, side, precision=prec)
                 assert 0 < rel(Rp, R) < tol
         assert wf.jacobian_transformations(X[:, :0], 'right').shape == (n, 0)
+
+
+def test_graph_replays_equal_eager_transformations():
+    """Captured transformation passes: every replay equals the eager pass exactly, with eager
+    work and synchronisation between replays, after a new record and a new ladder batching."""
+    s = make(3, 7, 12, 5)
+    n = 3 * 7 + O.n_t2(3, 7)
+    X = np.random.default_rng(33).normal(size=(n, 3))
+    with load(s, 3) as wf:
+        wf.prepare_for_jacobian()
+        ref = {}
+        wf.set_transform_graphs(False)
+        for side in ('right', 'left'):
+            for prec in (ccsd.FP64, ccsd.BF16X3):
+                ref[side, prec] = wf.jacobian_transformations(X, side, precision=prec)
+        wf.set_transform_graphs(True)
+        for rep in range(3):
+            for side in ('right', 'left'):
+                for prec in (ccsd.FP64, ccsd.BF16X3):
+                    R = wf.jacobian_transformations(X, side, precision=prec)
+                    assert np.array_equal(R, ref[side, prec]), (rep, side, prec)
+            wf.construct_omega()                      # eager work between the replays
+            wf.synchronize()
+        # a new record (amplitudes changed) and a new ladder plan drop the captured passes
+        t = wf.get_amplitudes()
+        t[5] += 1e-3
+        wf.set_amplitudes(t)
+        wf.construct_t1_cholesky()
+        wf.set_ladder_batch(2)
+        wf.prepare_for_jacobian()
+        for side, single in (('right', wf.jacobian_transformation),
+                             ('left', wf.jacobian_transpose_transformation)):
+            R = wf.jacobian_transformations(X, side)
+            for k in range(3):
+                assert rel(R[:, k], single(X[:, k])) < 1e-14
+            wf.set_transform_graphs(False)
+            E = wf.jacobian_transformations(X, side)
+            wf.set_transform_graphs(True)
+            assert np.array_equal(R, E)
