@@ -14,7 +14,10 @@ matrix-core products and compared with the FP64 one.
 Time to solution (skip with --no-tts): eight non-interacting H2O-He / cc-pVDZ
 molecules (o = 48, v = 184, n_J = 2832, from the committed inputs), ground state
 from the MP2 guess to ||Omega_FP64|| <= 1e-10 with FP64 residuals and with the
-mixed-precision schedule (split-bf16, FP32, then FP64 residuals).
+mixed-precision schedule (split-bf16, FP32, then FP64 residuals).  Then 12
+EOM roots of four copies (o = 24, v = 92), right and left Davidson to residuals
+<= 1e-6, with FP64 transformations and with split-bf16 transformations
+finished by FP64 Davidson from their vectors; errors against eT in eV.
 
 Prints one JSON line.
 """
@@ -37,6 +40,7 @@ METRIC = 'CCSD omega evals/s & time-to-solution, mixed vs FP64 GPU at |ΔE|≤1e
 # dense matrix-core peaks; split-bf16 counts its three bf16 products per term
 PEAKS = {'fp64': 78.6e12, 'fp32': 157.3e12, 'bf16x3': 2.5e15 / 3}
 WORKLOAD = dict(n_o=48, n_v=400, n_J=1600, seed=0)
+HARTREE_TO_EV = 27.21138602          # CODATA 2014, the factor eT prints
 WORKLOAD_NAME = 'omega_o48_v400_J1600_seed0'
 # HBM bytes per launch of the residual's dominant kernel on this workload: rocprofv3
 # --pmc FETCH_SIZE and --pmc WRITE_SIZE (KiB per dispatch, averaged over the kernel's
@@ -166,6 +170,83 @@ def time_to_solution(copies=8, threshold=1e-10):
     return out
 
 
+def eom_time_to_solution(copies=4, n_states=12, threshold=1e-6):
+    """12 EOM roots, right and left Davidson, FP64 vs split-bf16 then FP64.
+
+    System: `copies` non-interacting H2O-He / cc-pVDZ after an FP64 ground state;
+    with 4 copies the 12 lowest roots are eT's three lowest excitation energies
+    (tests/restart_ccsd_v1.0), each 4-fold.  Errors in eV (CODATA 2014 factor).
+    """
+    from et_amd import solvers, workloads
+    w_et = np.load(os.path.join(ROOT, 'tests', 'golden', 'h2o_he_v1_0_vectors.npz'))['excitation_energies']
+    r = workloads.replicate(workloads.h2o_he_inputs(), copies)
+    n_o = r['n_o']
+    n_v = r['L_mo'].shape[1] - n_o
+    w_ref = np.sort(np.repeat(w_et, copies))[:n_states]
+    # Start vectors: unit vectors on the three lowest orbital differences within
+    # each copy.  The replicated singles space also holds charge-transfer
+    # excitations between copies with exactly the same orbital differences; A
+    # never couples them to a copy's own excitations, so starts taken from the
+    # lowest differences overall can leave a copy's roots out of reach.
+    m1 = workloads.h2o_he_inputs()
+    o1 = m1['n_o']
+    v1 = len(m1['eps']) - o1
+    d1 = (m1['eps'][o1:][None, :] - m1['eps'][:o1][:, None]).ravel()     # (i, a), a fastest
+    low = np.argsort(d1, kind='stable')[:n_states // copies]
+    out = {'workload': 'h2o_he_ccpvdz_x%d' % copies, 'n_o': n_o, 'n_v': n_v, 'n_J': r['n_J'],
+           'n_states': n_states, 'residual_threshold': threshold}
+    with ccsd.CCSD(n_o, n_v, r['n_J']) as wf:
+        wf.set_cholesky(r['L_mo'])
+        wf.set_oei(r['h_mo'])
+        wf.set_orbital_energies(r['eps'])
+        del r
+        start = np.zeros((wf.n_gs_amplitudes, n_states), order='F')
+        col = 0
+        for c in range(copies):
+            for q in low:
+                i, a = divmod(int(q), v1)
+                start[(c * v1 + a) + n_v * (c * o1 + i), col] = 1.0
+                col += 1
+        assert col == n_states
+        solvers.DiisCCGroundState(wf, residual_threshold=1e-10).run()
+        for side in ('right', 'left'):
+            wf.prepare_for_jacobian(side)
+            wf.jacobian_transformations(np.ones((wf.n_gs_amplitudes, 1)), side, ccsd.BF16X3)
+            res = {}
+            for name in ('fp64', 'mixed'):
+                wf.synchronize()
+                t0 = time.perf_counter()
+                if name == 'fp64':
+                    sv = solvers.EigenDavidson(wf, n_states, side, residual_threshold=threshold,
+                                               start=start)
+                    w = sv.run()
+                    iters = {'fp64': len(sv.history)}
+                else:
+                    sv = solvers.MixedPrecisionDavidson(wf, n_states, side, residual_threshold=threshold,
+                                                        start=start)
+                    w = sv.run()
+                    names = {ccsd.FP64: 'fp64', ccsd.BF16X3: 'bf16x3', ccsd.FP32: 'fp32'}
+                    iters = {names[p]: k for p, k, _, _ in sv.phases}
+                    phases = [{'precision': names[p], 'iterations': k, 'stalled': st, 'max_residual': r}
+                              for p, k, st, r in sv.phases]
+                wf.synchronize()
+                dt = time.perf_counter() - t0
+                res[name] = {'seconds': dt, 'iterations': iters,
+                             'max_abs_err_vs_et_ev': float(np.abs(w - w_ref).max() * HARTREE_TO_EV)}
+                res[name]['energies'] = [float(x) for x in w]
+                if name == 'mixed':
+                    res[name]['phases'] = phases
+                    res[name]['noise'] = {names[p]: v for p, v in sv.noise.items()}
+                log('EOM %s %s: %.2f s, %s, max|dw| vs eT %.2e eV'
+                    % (side, name, dt, iters, res[name]['max_abs_err_vs_et_ev']))
+            res['mixed']['max_abs_err_vs_fp64_gpu_ev'] = float(
+                np.abs(np.array(res['mixed']['energies']) - np.array(res['fp64']['energies'])).max()
+                * HARTREE_TO_EV)
+            res['speedup_mixed_vs_fp64'] = res['fp64']['seconds'] / res['mixed']['seconds']
+            out[side] = res
+    return out
+
+
 def copy_calibration(nbytes=1 << 30, reps=3):
     """Plain copies of known size: each reads and writes `nbytes` of HBM once.
 
@@ -286,6 +367,7 @@ def main():
 
     if not args.no_tts and args.precision == 'fp64':
         extra['time_to_solution'] = time_to_solution()
+        extra['eom_time_to_solution'] = eom_time_to_solution()
 
     cpu = None
     if not args.no_cpu:

@@ -58,7 +58,7 @@ _SIGNATURES = {
     'etg_jacobian_transformations': [ctypes.c_void_p, ctypes.c_int, _i64, _c_dbl_p, _c_dbl_p,
                                      ctypes.c_int],
     'etg_davidson': [ctypes.c_void_p, ctypes.c_int, _i64, ctypes.c_double, _i64, _i64, ctypes.c_int,
-                     _c_dbl_p, _c_dbl_p, _c_dbl_p, ctypes.POINTER(_i64)],
+                     _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, ctypes.POINTER(_i64)],
     'etg_test_reduced_eigen': [_i64, _c_dbl_p, _i64, _c_dbl_p, _c_dbl_p, _c_dbl_p],
     'etg_diis_initialize': [ctypes.c_void_p, _i64],
     'etg_quasi_newton_update': [ctypes.c_void_p],
@@ -160,6 +160,7 @@ class CCSD:
         self.hf_energy = 0.0
         self.correlation_energy = 0.0
         self.energy = 0.0
+        self.davidson_stalled = False
 
     def close(self):
         if getattr(self, '_h', None) is not None and self._h.value:
@@ -300,18 +301,26 @@ class CCSD:
         return R
 
     def davidson(self, n_states, side='right', residual_threshold=1e-6, max_iterations=100,
-                 max_dim=0, precision=FP64, vectors=False):
+                 max_dim=0, precision=FP64, vectors=False, start=None):
         """Device Davidson (eigen_davidson_solver_class.F90:122-318): lowest roots of A or A^T.
 
-        Returns (energies, residual-norm history per iteration[, eigenvectors])."""
+        start: optional (n_t1 + n_t2) x n_states start vectors.
+        Returns (energies, residual-norm history per iteration[, eigenvectors]).  With
+        FP32/BF16X3 transformations the solve may end at their rounding floor before
+        the threshold; davidson_stalled then reads True."""
         e = np.empty(n_states)
         hist = np.full((max_iterations, n_states), np.nan)
         X = np.empty((self.n_gs_amplitudes, n_states), order='F') if vectors else None
+        S = None if start is None else np.asfortranarray(start, dtype=np.float64)
+        if S is not None and S.shape != (self.n_gs_amplitudes, n_states):
+            raise ETGError('start vectors must be (n_t1 + n_t2) x n_states')
         its = _i64()
         _check(self._lib.etg_davidson(self._h, 0 if side == 'right' else 1, n_states, residual_threshold,
-                                      max_iterations, max_dim, int(precision), _ptr(e),
+                                      max_iterations, max_dim, int(precision),
+                                      _ptr(S) if S is not None else None, _ptr(e),
                                       _ptr(X) if vectors else None, _ptr(hist), ctypes.byref(its)))
-        hist = hist[:its.value]
+        self.davidson_stalled = its.value < 0
+        hist = hist[:abs(its.value)]
         return (e, hist, X) if vectors else (e, hist)
 
     def diis_initialize(self, dimension=8):

@@ -178,12 +178,14 @@ int etg_jacobian_transformations(etg_handle* h, int side, int64_t n_vec, const d
 }
 
 int etg_davidson(etg_handle* h, int side, int64_t n_states, double residual_threshold,
-                 int64_t max_iterations, int64_t max_dim, int precision, double* energies,
-                 double* vectors, double* residual_norms, int64_t* iterations) {
+                 int64_t max_iterations, int64_t max_dim, int precision,
+                 const double* start_vectors, double* energies, double* vectors,
+                 double* residual_norms, int64_t* iterations) {
   return guarded([&] {
     if (energies == nullptr) etg::fail("etg_davidson: null energies");
     const int64_t it = wf_of(h).davidson(side, n_states, residual_threshold, max_iterations, max_dim,
-                                         prec_of(precision), energies, vectors, residual_norms);
+                                         prec_of(precision), start_vectors, energies, vectors,
+                                         residual_norms);
     if (iterations) *iterations = it;
   });
 }

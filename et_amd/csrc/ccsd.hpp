@@ -75,12 +75,13 @@ class CCSD {
 
   // Davidson eigensolver on the device (eigen_davidson_solver_class.F90:122-318):
   // n_states lowest roots of A (side 0) or A^T (side 1), transforms in `prec`,
-  // reduced-space algebra in FP64.  energies (n_states), vectors (n x n_states,
+  // reduced-space algebra in FP64.  start (n x n_states, host or device) or null
+  // for unit vectors on the lowest orbital differences; energies (n_states), vectors (n x n_states,
   // host or device, may be null), residual_norms (max_iterations x n_states, may
   // be null).  Returns the number of iterations.
   int64_t davidson(int side, int64_t n_states, double threshold, int64_t max_iterations,
-                   int64_t max_dim, Prec prec, double* energies, double* vectors,
-                   double* residual_norms);
+                   int64_t max_dim, Prec prec, const double* start, double* energies,
+                   double* vectors, double* residual_norms);
   // lowest n_want eigenpairs of the k x k matrix G (device, ld ldg): wr, wi (host,
   // k, ascending real part), X (device, k x n_want, ld ldx)
   void reduced_eigen(const double* G_dev, int64_t k, int64_t ldg, int64_t n_want, double* wr,

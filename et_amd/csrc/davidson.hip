@@ -431,8 +431,8 @@ void CCSD::reduced_eigen(const double* G_dev, int64_t k, int64_t ldg, int64_t n_
 }
 
 int64_t CCSD::davidson(int side, int64_t n_states, double threshold, int64_t max_iterations,
-                       int64_t max_dim, Prec prec, double* energies, double* vectors,
-                       double* residual_norms) {
+                       int64_t max_dim, Prec prec, const double* start, double* energies,
+                       double* vectors, double* residual_norms) {
   require(tape_ != nullptr, "davidson: prepare_for_jacobian has not been called");
   require(have_eps_, "davidson: orbital energies required");
   require(side == 0 || side == 1, "davidson: side must be 0 (right) or 1 (left)");
@@ -463,21 +463,6 @@ int64_t CCSD::davidson(int side, int64_t n_states, double threshold, int64_t max
   double* d = diff_.as<double>();
   orbital_differences(stream_, eps_.as<double>(), d, n_o, n_v, n);
 
-  // start vectors: unit vectors on the lowest orbital differences
-  std::vector<double> hd(n);
-  ETG_HIP(hipMemcpyAsync(hd.data(), d, sizeof(double) * n, hipMemcpyDeviceToHost, stream_));
-  synchronize();
-  std::vector<int64_t> idx(n);
-  for (int64_t i = 0; i < n; ++i) idx[i] = i;
-  std::partial_sort(idx.begin(), idx.begin() + s, idx.end(),
-                    [&](int64_t a, int64_t b) { return hd[a] < hd[b] || (hd[a] == hd[b] && a < b); });
-  fill(stream_, V, n * s, 0.0);
-  const double one = 1.0;
-  for (int64_t j = 0; j < s; ++j)
-    ETG_HIP(hipMemcpyAsync(V + j * n + idx[j], &one, sizeof(double), hipMemcpyHostToDevice, stream_));
-  synchronize();
-  int64_t k = 0, m = s;                   // space dimension, new vectors at V[:, k..k+m)
-
   auto dot_host = [&](const double* a, const double* b) {
     double* res = red_result_.as<double>() + 3;
     dot(stream_, a, b, n, red_partial_.as<double>(), res);
@@ -486,7 +471,35 @@ int64_t CCSD::davidson(int side, int64_t n_states, double threshold, int64_t max
     synchronize();
     return v;
   };
+  if (start) {
+    // caller's start vectors (e.g. a reduced-precision solution), orthonormalised
+    ETG_HIP(hipMemcpyAsync(V, start, sizeof(double) * n * s, hipMemcpyDefault, stream_));
+    for (int64_t j = 0; j < s; ++j) {
+      for (int pass = 0; pass < 2; ++pass)
+        for (int64_t i = 0; i < j; ++i) axpy(stream_, n, -dot_host(V + i * n, V + j * n), V + i * n, V + j * n);
+      const double nv = std::sqrt(dot_host(V + j * n, V + j * n));
+      if (!(nv > 1e-10)) fail("davidson: start vectors are linearly dependent");
+      scale(stream_, V + j * n, n, 1.0 / nv);
+    }
+  } else {
+    // unit vectors on the lowest orbital differences
+    std::vector<double> hd(n);
+    ETG_HIP(hipMemcpyAsync(hd.data(), d, sizeof(double) * n, hipMemcpyDeviceToHost, stream_));
+    synchronize();
+    std::vector<int64_t> idx(n);
+    for (int64_t i = 0; i < n; ++i) idx[i] = i;
+    std::partial_sort(idx.begin(), idx.begin() + s, idx.end(),
+                      [&](int64_t a, int64_t b) { return hd[a] < hd[b] || (hd[a] == hd[b] && a < b); });
+    fill(stream_, V, n * s, 0.0);
+    const double one = 1.0;
+    for (int64_t j = 0; j < s; ++j)
+      ETG_HIP(hipMemcpyAsync(V + j * n + idx[j], &one, sizeof(double), hipMemcpyHostToDevice, stream_));
+  }
+  synchronize();
+  int64_t k = 0, m = s;                   // space dimension, new vectors at V[:, k..k+m)
   std::vector<double> wr(max_dim), wi(max_dim), norms(s1), hcoef(3 * s1);
+  double best = HUGE_VAL;                 // reduced precision: smallest largest-residual so far
+  int64_t since_best = 0;
   for (int64_t it = 1; it <= max_iterations; ++it) {
     // transforms of the new trial vectors
     transform_batch(side, m, V + k * n, AV + k * n, prec);
@@ -543,13 +556,27 @@ int64_t CCSD::davidson(int side, int64_t n_states, double threshold, int64_t max
       energies[j] = hcoef[j];
       if (residual_norms) residual_norms[(it - 1) * s + j] = norms[j];
     }
-    if (converged) {
+    // With reduced-precision transformations the residuals bottom out at the
+    // transforms' rounding level: the solve also ends once the largest residual
+    // has not halved in 4 iterations (returned as -iterations).
+    bool stalled = false;
+    if (prec != Prec::F64) {
+      double worst = 0.0;
+      for (int64_t j = 0; j < s; ++j) worst = std::max(worst, norms[j]);
+      if (worst < 0.5 * best) {
+        best = worst;
+        since_best = 0;
+      } else {
+        stalled = ++since_best >= 4;
+      }
+    }
+    if (converged || stalled) {
       if (vectors) {
         for (int64_t j = 0; j < s; ++j) scale(stream_, Y + j * n, n, 1.0 / std::sqrt(ynorm[j]));
         ETG_HIP(hipMemcpyAsync(vectors, Y, sizeof(double) * n * s, hipMemcpyDefault, stream_));
         synchronize();
       }
-      return it;
+      return converged ? it : -it;
     }
     // restart on the Ritz vectors when the space would overflow
     int64_t n_new = 0;

@@ -223,6 +223,23 @@ module etgpu_interface
          integer(c_int), value :: precision
       end function etg_jacobian_transformations
 !
+!     run_eigen_davidson_solver as one device call (eigen_davidson_solver_class.F90:122-318);
+!     start_vectors may be c_null_ptr (unit vectors on the lowest orbital differences)
+      integer(c_int) function etg_davidson(handle, side, n_states, residual_threshold, max_iterations, &
+                                           max_dim, precision, start_vectors, energies, vectors, &
+                                           residual_norms, iterations) bind(C, name='etg_davidson')
+         import :: c_int, c_ptr, c_double, c_int64_t
+         type(c_ptr), value :: handle
+         integer(c_int), value :: side
+         integer(c_int64_t), value :: n_states
+         real(c_double), value :: residual_threshold
+         integer(c_int64_t), value :: max_iterations, max_dim
+         integer(c_int), value :: precision
+         type(c_ptr), value :: start_vectors, vectors, residual_norms
+         real(c_double), dimension(*), intent(out) :: energies
+         integer(c_int64_t), intent(out) :: iterations
+      end function etg_davidson
+!
 !     construct_omega with the residual left in HBM (omega_ccsd.F90:39-110)
       integer(c_int) function etg_construct_omega_resident(handle, precision) &
                                                    bind(C, name='etg_construct_omega_resident')

@@ -7,6 +7,8 @@ residuals.
 DiisCCMultipliers: run_diis_cc_multipliers (diis_cc_multipliers_class.F90:195-299).
 EigenDavidson: run_eigen_davidson_solver (eigen_davidson_solver_class.F90:122-318),
 with the whole Davidson space on the device.
+MixedPrecisionDavidson: the same eigensolver on split-bf16 transformations,
+finished by an FP64 solve started from the reduced-precision eigenvectors.
 Only scalars (energies, norms) reach the host during the iterations.
 """
 import numpy as np
@@ -182,7 +184,7 @@ class EigenDavidson:
     """
 
     def __init__(self, wf, n_states, side='right', residual_threshold=1e-6, max_iterations=100,
-                 max_dim=0, precision=_ccsd.FP64, log=None):
+                 max_dim=0, precision=_ccsd.FP64, log=None, start=None):
         self.wf = wf
         self.n_states = n_states
         self.side = side
@@ -191,15 +193,72 @@ class EigenDavidson:
         self.max_dim = max_dim
         self.precision = precision
         self.log = log
+        self.start = start
 
     def run(self):
         wf = self.wf
         wf.prepare_for_jacobian(self.side)
         w, hist, X = wf.davidson(self.n_states, self.side, self.residual_threshold, self.max_iterations,
-                                 self.max_dim, self.precision, vectors=True)
+                                 self.max_dim, self.precision, vectors=True, start=self.start)
         self.history = hist
         self.eigenvectors = X
         if self.log:
             for it, row in enumerate(hist, 1):
                 self.log('%3d  max|res| %.3e' % (it, row.max()))
+        return w
+
+
+class MixedPrecisionDavidson:
+    """EOM roots from reduced-precision transformations, finished in FP64.
+
+    Each precision of `schedule` (e.g. split-bf16) runs the device Davidson with
+    the transformations in that precision down to `margin` times its noise floor
+    ||A_p x - A_FP64 x|| (measured once on a normalised random probe), or until
+    its residuals stop decreasing; its Ritz vectors start the next phase.  The
+    last phase is the FP64 eigen Davidson solver, so convergence is decided on
+    FP64 transformations against `residual_threshold`.  The reduced-space
+    algebra is FP64 throughout.
+    """
+
+    def __init__(self, wf, n_states, side='right', residual_threshold=1e-6, max_iterations=100,
+                 max_dim=0, schedule=(_ccsd.BF16X3,), margin=10.0, log=None, seed=7, start=None):
+        self.wf = wf
+        self.n_states = n_states
+        self.side = side
+        self.residual_threshold = residual_threshold
+        self.max_iterations = max_iterations
+        self.max_dim = max_dim
+        self.schedule = tuple(schedule)
+        self.margin = margin
+        self.log = log
+        self.seed = seed
+        self.start = start
+        self.noise = {}
+        self.phases = []          # (precision, iterations, stalled, final max residual)
+
+    def _measure_noise(self):
+        wf = self.wf
+        x = np.random.default_rng(self.seed).standard_normal((wf.n_gs_amplitudes, 1))
+        x /= np.linalg.norm(x)
+        ref = wf.jacobian_transformations(x, self.side, _ccsd.FP64)
+        for p in self.schedule:
+            self.noise[p] = float(np.linalg.norm(wf.jacobian_transformations(x, self.side, p) - ref))
+
+    def run(self):
+        wf = self.wf
+        wf.prepare_for_jacobian(self.side)
+        self._measure_noise()
+        start = self.start
+        phases = [(p, max(self.residual_threshold, self.margin * self.noise[p])) for p in self.schedule]
+        phases.append((_ccsd.FP64, self.residual_threshold))
+        for p, threshold in phases:
+            w, hist, start = wf.davidson(self.n_states, self.side, threshold, self.max_iterations,
+                                         self.max_dim, p, vectors=True, start=start)
+            stalled = p != _ccsd.FP64 and wf.davidson_stalled
+            self.phases.append((p, len(hist), stalled, float(hist[-1].max())))
+            if self.log:
+                self.log('%-6s %3d iterations%s, max|res| %.3e' % (_PREC_NAMES[p], len(hist),
+                                                                  ' (stalled)' if stalled else '',
+                                                                  hist[-1].max()))
+        self.eigenvectors = start
         return w

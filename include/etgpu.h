@@ -153,19 +153,26 @@ int etg_diis_update(etg_handle* handle);
 /* Davidson solution of the n_states lowest roots of A (side 0, right) or A^T
  * (side 1, left) with trial vectors, transforms and all reduced-space algebra
  * (Gram-Schmidt, G = V^T A V, its eigenproblem, residuals) on the device in
- * FP64; the transformations themselves in `precision`.  Start vectors on the
- * lowest orbital differences, diagonal preconditioner, convergence when every
+ * FP64; the transformations themselves in `precision`.  Start vectors: the
+ * columns of start_vectors ((n_t1 + n_t2) x n_states, host or device,
+ * orthonormalised), or with NULL unit vectors on the lowest orbital
+ * differences (eT's default); diagonal preconditioner, convergence when every
  * ||A x - w x|| / ||x|| < residual_threshold, restart on the Ritz vectors at
  * max_dim (<= 0: max(100, 10 n_states); at most 128).  Needs
  * etg_prepare_for_jacobian.  energies: n_states; vectors: (n_t1 + n_t2) x
  * n_states, host or device, may be NULL; residual_norms: max_iterations x
- * n_states per-iteration norms, may be NULL; iterations: may be NULL.
+ * n_states per-iteration norms, may be NULL; iterations: may be NULL.  With
+ * precision FP32 or BF16X3 the solve also ends, with *iterations = -(count),
+ * when the largest residual has not halved in 4 iterations (the transforms'
+ * rounding floor); vectors then hold the current Ritz vectors, the start of an
+ * FP64 solve.
  * Replaces: run_eigen_davidson_solver
  * (src/solvers/eigen_davidson_solver_class.F90:122-318) with eT's eigen
  * Davidson tool (src/solver_tools/davidson/eigen_davidson_tool_class.F90). */
 int etg_davidson(etg_handle* handle, int side, int64_t n_states, double residual_threshold,
-                 int64_t max_iterations, int64_t max_dim, int precision, double* energies,
-                 double* vectors, double* residual_norms, int64_t* iterations);
+                 int64_t max_iterations, int64_t max_dim, int precision,
+                 const double* start_vectors, double* energies, double* vectors,
+                 double* residual_norms, int64_t* iterations);
 
 /* ---- multiplier (Lambda) equations  t-bar^T A + eta = 0 ---------------- */
 

@@ -180,3 +180,34 @@ def test_gpu_eom_reproduces_et_excitation_energies():
         left = solvers.EigenDavidson(wf, 2, 'left', residual_threshold=1e-8).run()
     np.testing.assert_allclose(right, g['right_excitation_energies'], rtol=0, atol=1e-10)
     np.testing.assert_allclose(left, g['left_excitation_energies'], rtol=0, atol=1e-10)
+
+
+def v1_0_excitation_energies():
+    path = os.path.join(os.path.dirname(FIXTURE), 'h2o_he_v1_0_vectors.npz')
+    return np.load(path)['excitation_energies']
+
+
+@pytest.mark.gpu
+def test_gpu_mixed_precision_eom_reproduces_et():
+    """Split-bf16 Davidson phase, finished by FP64 Davidson from its vectors: eT's three
+    lowest excitation energies (tests/restart_ccsd_v1.0) to 1e-10 Eh, right and left.
+    A second FP64 solve started from the converged vectors converges at once."""
+    from et_amd import ccsd, solvers
+    w_ref = v1_0_excitation_energies()
+    m = inputs()
+    n_o = m['n_o']
+    n_v = m['L_mo'].shape[1] - n_o
+    with ccsd.CCSD(n_o, n_v, m['n_J']) as wf:
+        wf.set_cholesky(m['L_mo'])
+        wf.set_oei(m['h_mo'])
+        wf.set_orbital_energies(m['eps'])
+        solvers.DiisCCGroundState(wf, residual_threshold=1e-11).run()
+        for side in ('right', 'left'):
+            sv = solvers.MixedPrecisionDavidson(wf, 3, side, residual_threshold=1e-8)
+            w = sv.run()
+            np.testing.assert_allclose(w, w_ref, rtol=0, atol=1e-10)
+            assert [p for p, *_ in sv.phases] == [ccsd.BF16X3, ccsd.FP64]
+            assert sv.noise[ccsd.BF16X3] > 0
+            again, hist = wf.davidson(3, side, 1e-8, 10, 0, ccsd.FP64, start=sv.eigenvectors)
+            assert len(hist) == 1 and not wf.davidson_stalled
+            np.testing.assert_allclose(again, w_ref, rtol=0, atol=1e-10)
