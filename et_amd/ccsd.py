@@ -73,6 +73,7 @@ _SIGNATURES = {
     'etg_multiplier_quasi_newton_update': [ctypes.c_void_p],
     'etg_multiplier_diis_update': [ctypes.c_void_p],
     'etg_set_transform_graphs': [ctypes.c_void_p, ctypes.c_int],
+    'etg_set_transform_batch': [ctypes.c_void_p, _i64],
     'etg_set_kernel_ledger': [ctypes.c_void_p, ctypes.c_int],
     'etg_kernel_ledger': [ctypes.c_void_p, _i64, ctypes.c_char_p, _i64, ctypes.POINTER(_i64),
                           _c_dbl_p, _c_dbl_p, _c_dbl_p, ctypes.POINTER(_i64)],
@@ -378,6 +379,10 @@ class CCSD:
         """Replay captured HIP graphs of the transformation passes (default) or run them eagerly."""
         _check(self._lib.etg_set_transform_graphs(self._h, 1 if on else 0))
 
+
+    def set_transform_batch(self, max_vectors):
+        """At most max_vectors trial vectors per transformation pass (0: automatic)."""
+        _check(self._lib.etg_set_transform_batch(self._h, int(max_vectors)))
     def set_kernel_ledger(self, on):
         _check(self._lib.etg_set_kernel_ledger(self._h, 1 if on else 0))
 

@@ -245,6 +245,10 @@ int etg_set_transform_graphs(etg_handle* h, int on) {
   return guarded([&] { wf_of(h).set_transform_graphs(on != 0); });
 }
 
+int etg_set_transform_batch(etg_handle* h, int64_t max_vectors) {
+  return guarded([&] { wf_of(h).set_transform_batch(max_vectors); });
+}
+
 int etg_set_kernel_ledger(etg_handle* h, int on) {
   return guarded([&] { wf_of(h).set_ledger(on != 0); });
 }

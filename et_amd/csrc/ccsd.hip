@@ -142,13 +142,22 @@ void CCSD::run_gemm(const Gemm& g, Prec prec) {
     b = take_event();
     ETG_HIP(hipEventRecord(a, stream_));
   }
-  const GemmLaunch info = etg::gemm(stream_, g, prec, gws_cur_);
+  GemmLaunch info;
+  if (!pack_cache_ && g.a_tag != 0) {
+    Gemm h = g;
+    h.a_tag = 0;
+    info = etg::gemm(stream_, h, prec, gws_cur_);
+  } else {
+    info = etg::gemm(stream_, g, prec, gws_cur_);
+  }
   if (ledger_on_) {
     ETG_HIP(hipEventRecord(b, stream_));
-    // algorithmic HBM bytes: read A and B once, write C once (and read it if beta != 0)
-    const double mn = double(g.m) * double(g.n);
-    const double bytes = 8.0 * double(g.batch) *
-                         (double(g.m) * double(g.k) + double(g.k) * double(g.n) + (g.beta != 0.0 ? 2.0 : 1.0) * mn);
+    // algorithmic HBM bytes: read A and B once (once in all for an operand shared
+    // by the batch), write C once (and read it if beta != 0)
+    const double mn = double(g.m) * double(g.n), nbat = double(g.batch);
+    const double bytes = 8.0 * (double(g.m) * double(g.k) * (g.strideA == 0 ? 1.0 : nbat) +
+                                double(g.k) * double(g.n) * (g.strideB == 0 ? 1.0 : nbat) +
+                                (g.beta != 0.0 ? 2.0 : 1.0) * mn * nbat);
     ledger_recs_.push_back(LedgerRec{info.kernel, gemm_flops(g), bytes, a, b});
   }
   last_omega_flops += gemm_flops(g);
@@ -563,13 +572,22 @@ void CCSD::build_ladder_plan() {
   upload(pcol_b_, cb);
   const int64_t op = n_o * (n_o + 1) / 2, om = n_o * (n_o - 1) / 2;
   const int64_t vp = v * (v + 1) / 2, vm = v * (v - 1) / 2;
-  tp_.resize(sizeof(double) * op * vp);
-  tm_.resize(sizeof(double) * std::max<int64_t>(1, om * vm));
-  wp_.resize(sizeof(double) * op * vp);
-  wm_.resize(sizeof(double) * std::max<int64_t>(1, om * vp));
-  gblk_.resize(sizeof(double) * v * v * std::min(nb, v) * std::min(nb, v));
-  gp_.resize(sizeof(double) * vp * maxpairs);
-  gm_.resize(sizeof(double) * std::max<int64_t>(1, vm * maxpairs));
+  lsz_tp_ = op * vp;
+  lsz_tm_ = std::max<int64_t>(1, om * vm);
+  lsz_wp_ = op * vp;
+  lsz_wm_ = std::max<int64_t>(1, om * vp);
+  lsz_gblk_ = v * v * std::min(nb, v) * std::min(nb, v);
+  lsz_gp_ = vp * maxpairs;
+  lsz_gm_ = std::max<int64_t>(1, vm * maxpairs);
+  tp_.resize(sizeof(double) * lsz_tp_);
+  tm_.resize(sizeof(double) * lsz_tm_);
+  wp_.resize(sizeof(double) * lsz_wp_);
+  wm_.resize(sizeof(double) * lsz_wm_);
+  gblk_.resize(sizeof(double) * lsz_gblk_);
+  gp_.resize(sizeof(double) * lsz_gp_);
+  gm_.resize(sizeof(double) * lsz_gm_);
+  reserved_nb_ = 0;                     // per-vector sizes changed: size the batches again
+  nb_cap_ = 0;
 }
 
 void CCSD::ladder_primal(const double* t2, const double* LT, double* om2, Prec prec) {
@@ -594,74 +612,117 @@ void CCSD::ladder_primal(const double* t2, const double* LT, double* om2, Prec p
   ladder_gather(s, wp_.as<double>(), wm_.as<double>(), pair_col_.as<int32_t>(), om2, o, v);
 }
 
-void CCSD::ladder_tangent(const double* t2, const double* dt2, const double* LT, const double* dLT,
-                          double* dom2, Prec prec) {
+// Per-vector work of the ladder derivatives: t+/- and w+/- of a derivative
+// vector, its g block and g+/- (elements).
+int64_t CCSD::ladder_vector_elems() const {
+  return lsz_tp_ + lsz_tm_ + lsz_wp_ + lsz_wm_ + lsz_gblk_ + lsz_gp_ + lsz_gm_;
+}
+
+bool CCSD::reserve_ladder_vectors(int64_t nb) {
+  bool moved = false;
+  auto grow = [&](DeviceBuffer& b, int64_t e) { moved |= b.resize(sizeof(double) * size_t(nb * e)); };
+  grow(dtp_, lsz_tp_);
+  grow(dtm_, lsz_tm_);
+  grow(dwp_, lsz_wp_);
+  grow(dwm_, lsz_wm_);
+  grow(dgblk_, lsz_gblk_);
+  grow(dgp_, lsz_gp_);
+  grow(dgm_, lsz_gm_);
+  return moved;
+}
+
+// Tangent of the ladder term for nb vectors (derivative k of t2 at dt2 + k s_dt2,
+// of LT at dLT + k s_dL, into dom2 + k s_dom2):
+//   d(om2) = ladder(dt2, g) + ladder(t2, dg),  dg_acbd = dL_ac . L_bd + L_ac . dL_bd.
+// g+/- of each (a, b) block is built once and applied to every vector; the
+// vector dimension is the batch of the products.
+void CCSD::ladder_tangent(const double* t2, const double* dt2, int64_t s_dt2, const double* LT,
+                          const double* dLT, int64_t s_dL, double* dom2, int64_t s_dom2, int64_t nb,
+                          Prec prec) {
+  if (dt2 == nullptr || dLT == nullptr || dom2 == nullptr) fail("ladder tangent: missing derivative");
   const int64_t o = n_o, v = n_v, J = n_J;
   const int64_t op = o * (o + 1) / 2, om = o * (o - 1) / 2;
   const int64_t vp = v * (v + 1) / 2, vm = v * (v - 1) / 2;
   hipStream_t s = stream_;
-  dtp_.resize(tp_.bytes());
-  dtm_.resize(tm_.bytes());
-  dwp_.resize(wp_.bytes());
-  dwm_.resize(wm_.bytes());
-  dgblk_.resize(gblk_.bytes());
-  dgp_.resize(gp_.bytes());
-  dgm_.resize(gm_.bytes());
+  reserve_ladder_vectors(nb);
   DirectOps X(*this);
-  const uint64_t gen = ++ladder_gen_ << 3;
+  double* dtp = dtp_.as<double>();
+  double* dtm = dtm_.as<double>();
+  double* dwp = dwp_.as<double>();
+  double* dwm = dwm_.as<double>();
+  double* dgb = dgblk_.as<double>();
+  double* dgp = dgp_.as<double>();
+  double* dgm = dgm_.as<double>();
+  auto batched = [&](Gemm g, int64_t sA, int64_t sB, int64_t sC) {
+    g.batch = nb;
+    g.strideA = sA;
+    g.strideB = sB;
+    g.strideC = sC;
+    run_gemm(g, prec);
+  };
   ladder_t_pm(s, t2, tp_.as<double>(), tm_.as<double>(), o, v);
-  ladder_t_pm(s, dt2, dtp_.as<double>(), dtm_.as<double>(), o, v);
+  for (int64_t k = 0; k < nb; ++k)
+    ladder_t_pm(s, dt2 + k * s_dt2, dtp + k * lsz_tp_, dtm + k * lsz_tm_, o, v);
   for (const Block& b : blocks_) {
     const double* LA = LT + J * v * b.a0;
     const double* LB = LT + J * v * b.b0;
     X.gemm('T', 'N', v * b.la, v * b.lb, J, 1.0, LA, J, LB, J, 0.0, gblk_.as<double>(), v * b.la, prec);
-    X.gemm('T', 'N', v * b.la, v * b.lb, J, 1.0, dLT + J * v * b.a0, J, LB, J, 0.0,
-           dgblk_.as<double>(), v * b.la, prec);
-    X.gemm('T', 'N', v * b.la, v * b.lb, J, 1.0, LA, J, dLT + J * v * b.b0, J, 1.0,
-           dgblk_.as<double>(), v * b.la, prec);
+    batched(make_gemm('T', 'N', v * b.la, v * b.lb, J, 1.0, dLT + J * v * b.a0, J, LB, J, 0.0, dgb,
+                      v * b.la), s_dL, 0, lsz_gblk_);
+    batched(make_gemm('T', 'N', v * b.la, v * b.lb, J, 1.0, LA, J, dLT + J * v * b.b0, J, 1.0, dgb,
+                      v * b.la), 0, s_dL, lsz_gblk_);
     ladder_g_pm(s, gblk_.as<double>(), v, b.la, b.lb, pair_a_.as<int32_t>() + b.col0,
                 pair_b_.as<int32_t>() + b.col0, b.npairs, gp_.as<double>(), gm_.as<double>());
-    ladder_g_pm(s, dgblk_.as<double>(), v, b.la, b.lb, pair_a_.as<int32_t>() + b.col0,
-                pair_b_.as<int32_t>() + b.col0, b.npairs, dgp_.as<double>(), dgm_.as<double>());
-    double* dw = dwp_.as<double>() + op * b.col0;
-    run_gemm(tagged(make_gemm('N', 'N', op, b.npairs, vp, 0.5, dtp_.as<double>(), op, gp_.as<double>(),
-                              vp, 0.0, dw, op), gen | 3, pack_cache_), prec);
-    run_gemm(tagged(make_gemm('N', 'N', op, b.npairs, vp, 0.5, tp_.as<double>(), op, dgp_.as<double>(),
-                              vp, 1.0, dw, op), gen | 1, pack_cache_), prec);
+    for (int64_t k = 0; k < nb; ++k)
+      ladder_g_pm(s, dgb + k * lsz_gblk_, v, b.la, b.lb, pair_a_.as<int32_t>() + b.col0,
+                  pair_b_.as<int32_t>() + b.col0, b.npairs, dgp + k * lsz_gp_, dgm + k * lsz_gm_);
+    batched(make_gemm('N', 'N', op, b.npairs, vp, 0.5, dtp, op, gp_.as<double>(), vp, 0.0,
+                      dwp + op * b.col0, op), lsz_tp_, 0, lsz_wp_);
+    batched(make_gemm('N', 'N', op, b.npairs, vp, 0.5, tp_.as<double>(), op, dgp, vp, 1.0,
+                      dwp + op * b.col0, op), 0, lsz_gp_, lsz_wp_);
     if (om > 0 && vm > 0) {
-      dw = dwm_.as<double>() + om * b.col0;
-      run_gemm(tagged(make_gemm('N', 'N', om, b.npairs, vm, 0.5, dtm_.as<double>(), om,
-                                gm_.as<double>(), vm, 0.0, dw, om), gen | 4, pack_cache_), prec);
-      run_gemm(tagged(make_gemm('N', 'N', om, b.npairs, vm, 0.5, tm_.as<double>(), om,
-                                dgm_.as<double>(), vm, 1.0, dw, om), gen | 2, pack_cache_), prec);
+      batched(make_gemm('N', 'N', om, b.npairs, vm, 0.5, dtm, om, gm_.as<double>(), vm, 0.0,
+                        dwm + om * b.col0, om), lsz_tm_, 0, lsz_wm_);
+      batched(make_gemm('N', 'N', om, b.npairs, vm, 0.5, tm_.as<double>(), om, dgm, vm, 1.0,
+                        dwm + om * b.col0, om), 0, lsz_gm_, lsz_wm_);
     }
   }
-  ladder_gather(s, dwp_.as<double>(), dwm_.as<double>(), pair_col_.as<int32_t>(), dom2, o, v);
+  for (int64_t k = 0; k < nb; ++k)
+    ladder_gather(s, dwp + k * lsz_wp_, dwm + k * lsz_wm_, pair_col_.as<int32_t>(), dom2 + k * s_dom2,
+                  o, v);
 }
 
-void CCSD::ladder_adjoint(const double* t2, double* t2b, const double* LT, double* LTb,
-                          const double* om2b, Prec prec) {
+// Adjoint of the ladder term for nb vectors: from om2b + k s_om2b, accumulates
+// into t2b + k s_t2b and LTb + k s_Lb; g+/- recomputed once per block.
+void CCSD::ladder_adjoint(const double* t2, double* t2b, int64_t s_t2b, const double* LT, double* LTb,
+                          int64_t s_Lb, const double* om2b, int64_t s_om2b, int64_t nb, Prec prec) {
+  if (t2b == nullptr || LTb == nullptr || om2b == nullptr) fail("ladder adjoint: missing derivative");
   const int64_t o = n_o, v = n_v, J = n_J;
   const int64_t op = o * (o + 1) / 2, om = o * (o - 1) / 2;
   const int64_t vp = v * (v + 1) / 2, vm = v * (v - 1) / 2;
   hipStream_t s = stream_;
-  dtp_.resize(tp_.bytes());
-  dtm_.resize(tm_.bytes());
-  dwp_.resize(wp_.bytes());
-  dwm_.resize(wm_.bytes());
-  dgblk_.resize(gblk_.bytes());
-  dgp_.resize(gp_.bytes());
-  dgm_.resize(gm_.bytes());
+  reserve_ladder_vectors(nb);
   DirectOps X(*this);
   double* tpb = dtp_.as<double>();
   double* tmb = dtm_.as<double>();
   double* wpb = dwp_.as<double>();
   double* wmb = dwm_.as<double>();
-  const uint64_t gen = ++ladder_gen_ << 3;
+  double* dgb = dgblk_.as<double>();
+  double* dgp = dgp_.as<double>();
+  double* dgm = dgm_.as<double>();
+  auto batched = [&](Gemm g, int64_t sA, int64_t sB, int64_t sC) {
+    g.batch = nb;
+    g.strideA = sA;
+    g.strideB = sB;
+    g.strideC = sC;
+    run_gemm(g, prec);
+  };
   ladder_t_pm(s, t2, tp_.as<double>(), tm_.as<double>(), o, v);
-  ladder_gather_adjoint(s, om2b, pcol_a_.as<int32_t>(), pcol_b_.as<int32_t>(), wpb, wmb, o, v);
-  fill(s, tpb, op * vp, 0.0);
-  if (om > 0 && vm > 0) fill(s, tmb, om * vm, 0.0);
+  for (int64_t k = 0; k < nb; ++k)
+    ladder_gather_adjoint(s, om2b + k * s_om2b, pcol_a_.as<int32_t>(), pcol_b_.as<int32_t>(),
+                          wpb + k * lsz_wp_, wmb + k * lsz_wm_, o, v);
+  fill(s, tpb, nb * lsz_tp_, 0.0);
+  if (om > 0 && vm > 0) fill(s, tmb, nb * lsz_tm_, 0.0);
   for (const Block& b : blocks_) {
     const double* LA = LT + J * v * b.a0;
     const double* LB = LT + J * v * b.b0;
@@ -669,27 +730,29 @@ void CCSD::ladder_adjoint(const double* t2, double* t2b, const double* LT, doubl
     ladder_g_pm(s, gblk_.as<double>(), v, b.la, b.lb, pair_a_.as<int32_t>() + b.col0,
                 pair_b_.as<int32_t>() + b.col0, b.npairs, gp_.as<double>(), gm_.as<double>());
     // w+ = 1/2 t+ g+  ->  t+_bar += 1/2 w+_bar g+^T,  g+_bar = 1/2 t+^T w+_bar
-    const double* w = wpb + op * b.col0;
-    X.gemm('N', 'T', op, vp, b.npairs, 0.5, w, op, gp_.as<double>(), vp, 1.0, tpb, op, prec);
-    run_gemm(tagged(make_gemm('T', 'N', vp, b.npairs, op, 0.5, tp_.as<double>(), op, w, op, 0.0,
-                              dgp_.as<double>(), vp), gen | 5, pack_cache_), prec);
+    batched(make_gemm('N', 'T', op, vp, b.npairs, 0.5, wpb + op * b.col0, op, gp_.as<double>(), vp,
+                      1.0, tpb, op), lsz_wp_, 0, lsz_tp_);
+    batched(make_gemm('T', 'N', vp, b.npairs, op, 0.5, tp_.as<double>(), op, wpb + op * b.col0, op,
+                      0.0, dgp, vp), 0, lsz_wp_, lsz_gp_);
     if (om > 0 && vm > 0) {
-      w = wmb + om * b.col0;
-      X.gemm('N', 'T', om, vm, b.npairs, 0.5, w, om, gm_.as<double>(), vm, 1.0, tmb, om, prec);
-      run_gemm(tagged(make_gemm('T', 'N', vm, b.npairs, om, 0.5, tm_.as<double>(), om, w, om, 0.0,
-                                dgm_.as<double>(), vm), gen | 6, pack_cache_), prec);
+      batched(make_gemm('N', 'T', om, vm, b.npairs, 0.5, wmb + om * b.col0, om, gm_.as<double>(), vm,
+                        1.0, tmb, om), lsz_wm_, 0, lsz_tm_);
+      batched(make_gemm('T', 'N', vm, b.npairs, om, 0.5, tm_.as<double>(), om, wmb + om * b.col0, om,
+                        0.0, dgm, vm), 0, lsz_wm_, lsz_gm_);
     } else {
-      fill(s, dgm_.as<double>(), std::max<int64_t>(1, vm * b.npairs), 0.0);
+      fill(s, dgm, nb * lsz_gm_, 0.0);
     }
-    ladder_g_pm_adjoint(s, dgp_.as<double>(), dgm_.as<double>(), dgblk_.as<double>(), v, b.a0, b.la,
-                        b.b0, b.lb, pair_col_.as<int32_t>(), b.col0);
+    for (int64_t k = 0; k < nb; ++k)
+      ladder_g_pm_adjoint(s, dgp + k * lsz_gp_, dgm + k * lsz_gm_, dgb + k * lsz_gblk_, v, b.a0, b.la,
+                          b.b0, b.lb, pair_col_.as<int32_t>(), b.col0);
     // G = LA^T LB  ->  LA_bar += LB G_bar^T,  LB_bar += LA G_bar
-    X.gemm('N', 'T', J, v * b.la, v * b.lb, 1.0, LB, J, dgblk_.as<double>(), v * b.la, 1.0,
-           LTb + J * v * b.a0, J, prec);
-    X.gemm('N', 'N', J, v * b.lb, v * b.la, 1.0, LA, J, dgblk_.as<double>(), v * b.la, 1.0,
-           LTb + J * v * b.b0, J, prec);
+    batched(make_gemm('N', 'T', J, v * b.la, v * b.lb, 1.0, LB, J, dgb, v * b.la, 1.0,
+                      LTb + J * v * b.a0, J), 0, lsz_gblk_, s_Lb);
+    batched(make_gemm('N', 'N', J, v * b.lb, v * b.la, 1.0, LA, J, dgb, v * b.la, 1.0,
+                      LTb + J * v * b.b0, J), 0, lsz_gblk_, s_Lb);
   }
-  ladder_t_pm_adjoint(s, tpb, tmb, t2b, o, v);
+  for (int64_t k = 0; k < nb; ++k)
+    ladder_t_pm_adjoint(s, tpb + k * lsz_tp_, tmb + k * lsz_tm_, t2b + k * s_t2b, o, v);
 }
 
 // ---------------------------------------------------------------------------
@@ -702,10 +765,12 @@ void CCSD::ladder_adjoint(const double* t2, double* t2b, const double* LT, doubl
 // ---------------------------------------------------------------------------
 
 namespace {
-__global__ void scale_doubles_diagonal_kernel(double* x, int64_t n_t1, double factor) {
+// vector blockIdx.y of a batch at stride n_t1 + n_t2
+__global__ void scale_doubles_diagonal_kernel(double* x, int64_t n_t1, int64_t n, double factor) {
+  double* xv = x + int64_t(blockIdx.y) * n;
   for (int64_t d = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; d < n_t1;
        d += int64_t(gridDim.x) * blockDim.x)
-    x[n_t1 + d * (d + 1) / 2 + d] *= factor;
+    xv[n_t1 + d * (d + 1) / 2 + d] *= factor;
 }
 }  // namespace
 
@@ -753,9 +818,29 @@ void CCSD::transpose_transform_device(const double* b, double* sigma) {
   transform_batch(1, 1, b, sigma, Prec::F64);
 }
 
-// One transformation of the staged vector jin_ into jout_ (device buffers of
-// fixed address, so the sequence can be captured once and replayed).
-void CCSD::transform_body(int side, Prec prec) {
+// Vectors per transformation pass for the current record: as many as fit in
+// 35 % of the free HBM (the record's derivative buffers and the ladder's
+// per-vector work, per vector), at most kMaxTransformBatch or the caller's limit.
+int64_t CCSD::transform_batch_capacity() {
+  size_t free_b = 0, total_b = 0;
+  ETG_HIP(hipMemGetInfo(&free_b, &total_b));
+  const double per_vec = 8.0 * double(tape_->derivative_elems() + ladder_vector_elems() + 2 * (n_t1 + n_t2));
+  int64_t nb = int64_t(0.35 * double(free_b) / per_vec);
+  nb = std::max<int64_t>(1, std::min<int64_t>(nb, kMaxTransformBatch));
+  if (transform_batch_limit_ > 0) nb = std::min(nb, transform_batch_limit_);
+  return nb;
+}
+
+void CCSD::set_transform_batch(int64_t nb) {
+  if (nb < 0) fail("set_transform_batch: negative vector count");
+  transform_batch_limit_ = nb;
+  nb_cap_ = 0;                          // sized again at the next transformation
+}
+
+// One transformation pass over the nb vectors staged in jin_ (stride n) into
+// jout_ (device buffers of fixed address, so the sequence can be captured once
+// and replayed).
+void CCSD::transform_body(int side, Prec prec, int64_t nb) {
   // the passes use their own product workspace: its blocks must not move
   // between a capture and the replays, whatever other products need
   struct Use {
@@ -764,24 +849,26 @@ void CCSD::transform_body(int side, Prec prec) {
     ~Use() { w.gws_cur_ = &w.gws_; }
   } use(*this);
   const int64_t n = n_t1 + n_t2;
-  const unsigned g = unsigned(ceil_div(n_t1, 256));
-  tape_->zero_derivatives();
+  const dim3 g(unsigned(ceil_div(n_t1, 256)), unsigned(nb));
+  tape_->zero_derivatives(nb);
+  // derivative vectors of t and omega are contiguous: their buffers hold n elements
   if (side == 0) {
     double* dt = tape_->deriv(t_.as<double>());
-    ETG_HIP(hipMemcpyAsync(dt, jin_.as<double>(), sizeof(double) * n, hipMemcpyDeviceToDevice, stream_));
-    scale_doubles_diagonal_kernel<<<g, 256, 0, stream_>>>(dt, n_t1, 2.0);
+    ETG_HIP(hipMemcpyAsync(dt, jin_.as<double>(), sizeof(double) * n * nb, hipMemcpyDeviceToDevice,
+                           stream_));
+    scale_doubles_diagonal_kernel<<<g, 256, 0, stream_>>>(dt, n_t1, n, 2.0);
     ETG_LAUNCH("scale_doubles_diagonal");
-    tape_->tangent(prec);
-    ETG_HIP(hipMemcpyAsync(jout_.as<double>(), tape_->deriv(omega_.as<double>()), sizeof(double) * n,
+    tape_->tangent(prec, nb);
+    ETG_HIP(hipMemcpyAsync(jout_.as<double>(), tape_->deriv(omega_.as<double>()), sizeof(double) * n * nb,
                            hipMemcpyDeviceToDevice, stream_));
   } else {
-    ETG_HIP(hipMemcpyAsync(tape_->deriv(omega_.as<double>()), jin_.as<double>(), sizeof(double) * n,
+    ETG_HIP(hipMemcpyAsync(tape_->deriv(omega_.as<double>()), jin_.as<double>(), sizeof(double) * n * nb,
                            hipMemcpyDeviceToDevice, stream_));
-    tape_->adjoint(prec);
+    tape_->adjoint(prec, nb);
     double* tb = tape_->deriv(t_.as<double>());
-    scale_doubles_diagonal_kernel<<<g, 256, 0, stream_>>>(tb, n_t1, 2.0);
+    scale_doubles_diagonal_kernel<<<g, 256, 0, stream_>>>(tb, n_t1, n, 2.0);
     ETG_LAUNCH("scale_doubles_diagonal");
-    ETG_HIP(hipMemcpyAsync(jout_.as<double>(), tb, sizeof(double) * n, hipMemcpyDeviceToDevice, stream_));
+    ETG_HIP(hipMemcpyAsync(jout_.as<double>(), tb, sizeof(double) * n * nb, hipMemcpyDeviceToDevice, stream_));
   }
 }
 
@@ -789,6 +876,8 @@ void CCSD::transform_body(int side, Prec prec) {
 void CCSD::reset_record() {
   drop_transform_graphs();
   tape_.reset();
+  nb_cap_ = 0;
+  reserved_nb_ = 0;
 }
 
 void CCSD::drop_transform_graphs() {
@@ -799,40 +888,62 @@ void CCSD::drop_transform_graphs() {
 
 // R_k = A X_k (side 0) or A^T X_k (side 1), k < n_vec, columns of length
 // n_t1 + n_t2; X and R are host or device pointers (the copies use unified
-// addressing).  Column k of X is read before column k of R is written.  The
-// first transformation of a (side, precision) runs eagerly and sizes every
-// buffer; the same sequence is then captured into a HIP graph and replayed for
-// the following vectors (and later calls on the same record).
+// addressing).  Columns of X are read before the same columns of R are
+// written.  The vectors go through the passes in groups of up to nb_cap_, each
+// group as the batch dimension of every product.  The first pass of a
+// (side, precision, group size) runs eagerly and sizes every buffer; the same
+// sequence is then captured into a HIP graph and replayed for later groups of
+// that size (and later calls on the same record).  A graph is captured again
+// when the pass workspace has moved since its capture.
 void CCSD::transform_batch(int side, int64_t n_vec, const double* X, double* R, Prec prec) {
   require(tape_ != nullptr, side == 0 ? "jacobian_transformation: prepare_for_jacobian has not been called"
                                       : "jacobian_transpose_transformation: prepare_for_jacobian has not been called");
   require(side == 0 || side == 1, "jacobian_transformations: side must be 0 (right) or 1 (left)");
   require(n_vec >= 0, "jacobian_transformations: negative vector count");
   require(n_vec == 0 || (X != nullptr && R != nullptr), "jacobian_transformations: null pointer");
+  if (n_vec == 0) return;
   const int64_t n = n_t1 + n_t2;
-  jin_.resize(sizeof(double) * n);
-  jout_.resize(sizeof(double) * n);
+  if (tape_->dstride(t_.as<double>()) != n || tape_->dstride(omega_.as<double>()) != n)
+    fail("jacobian_transformations: amplitude derivative layout");
   if (graph_tape_ != tape_.get()) {
     drop_transform_graphs();
     graph_tape_ = tape_.get();
   }
+  if (nb_cap_ == 0) nb_cap_ = transform_batch_capacity();
+  const int64_t group = std::min(nb_cap_, n_vec);
+  if (group > reserved_nb_) {
+    // room for `group` vectors everywhere; graphs captured before a move are stale
+    bool moved = tape_->reserve_derivatives(group);
+    moved |= reserve_ladder_vectors(group);
+    moved |= jin_.resize(sizeof(double) * n * group);
+    moved |= jout_.resize(sizeof(double) * n * group);
+    if (moved) drop_transform_graphs();
+    reserved_nb_ = group;
+  }
   const bool graphs = use_graphs_ && !ledger_on_;
   pack_cache_ = false;                  // packs go to fixed workspaces inside the passes
-  for (int64_t k = 0; k < n_vec; ++k) {
-    ETG_HIP(hipMemcpyAsync(jin_.as<double>(), X + k * n, sizeof(double) * n, hipMemcpyDefault, stream_));
+  for (int64_t k0 = 0; k0 < n_vec; k0 += group) {
+    const int64_t nb = std::min(group, n_vec - k0);
+    ETG_HIP(hipMemcpyAsync(jin_.as<double>(), X + k0 * n, sizeof(double) * n * nb, hipMemcpyDefault,
+                           stream_));
     TransformGraph* tg = nullptr;
     for (auto& e : tgraphs_)
-      if (e.side == side && e.prec == prec) tg = &e;
+      if (e.side == side && e.prec == prec && e.nb == nb) tg = &e;
+    if (tg != nullptr && tg->ws_gen != gws_pass_.gen) {   // the workspace moved: capture again
+      drop_transform_graphs();
+      tg = nullptr;
+    }
     if (graphs && tg != nullptr) {
       ETG_HIP(hipGraphLaunch(tg->exec, stream_));
     } else {
-      transform_body(side, prec);
+      transform_body(side, prec, nb);
       if (graphs) {
         synchronize();                  // buffers are sized; capture the same sequence
+        const uint64_t gen = gws_pass_.gen;
         hipGraph_t graph = nullptr;
         ETG_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
         try {
-          transform_body(side, prec);
+          transform_body(side, prec, nb);
         } catch (...) {
           (void)hipStreamEndCapture(stream_, &graph);
           if (graph) (void)hipGraphDestroy(graph);
@@ -840,15 +951,23 @@ void CCSD::transform_batch(int side, int64_t n_vec, const double* X, double* R, 
           throw;
         }
         ETG_HIP(hipStreamEndCapture(stream_, &graph));
+        if (gws_pass_.gen != gen) {     // cannot happen after an identical eager pass
+          (void)hipGraphDestroy(graph);
+          pack_cache_ = true;
+          fail("jacobian_transformations: workspace moved during capture");
+        }
         TransformGraph e;
         e.side = side;
         e.prec = prec;
+        e.nb = nb;
+        e.ws_gen = gen;
         ETG_HIP(hipGraphInstantiate(&e.exec, graph, nullptr, nullptr, 0));
         (void)hipGraphDestroy(graph);
         tgraphs_.push_back(e);
       }
     }
-    ETG_HIP(hipMemcpyAsync(R + k * n, jout_.as<double>(), sizeof(double) * n, hipMemcpyDefault, stream_));
+    ETG_HIP(hipMemcpyAsync(R + k0 * n, jout_.as<double>(), sizeof(double) * n * nb, hipMemcpyDefault,
+                           stream_));
   }
   pack_cache_ = true;
   synchronize();

@@ -72,6 +72,9 @@ class CCSD {
   void transform_batch(int side, int64_t n_vec, const double* X, double* R, Prec prec);
   // replay captured HIP graphs of the transformation passes (on by default)
   void set_transform_graphs(bool on) { use_graphs_ = on; }
+  // at most `nb` vectors per transformation pass (0: as many as HBM allows, <= 16)
+  void set_transform_batch(int64_t nb);
+  int64_t transform_batch() const { return nb_cap_; }
 
   // Davidson eigensolver on the device (eigen_davidson_solver_class.F90:122-318):
   // n_states lowest roots of A (side 0) or A^T (side 1), transforms in `prec`,
@@ -110,10 +113,12 @@ class CCSD {
   void run_gemm(const Gemm& g, Prec prec);
   double* scratch(int slot, int64_t elems, const char* name);
   void ladder_primal(const double* t2, const double* LvvT, double* om2, Prec prec);
-  void ladder_tangent(const double* t2, const double* dt2, const double* LvvT, const double* dLvvT,
-                      double* dom2, Prec prec);
-  void ladder_adjoint(const double* t2, double* t2b, const double* LvvT, double* LvvTb,
-                      const double* om2b, Prec prec);
+  // derivatives for nb vectors, vector k at pointer + k * stride
+  void ladder_tangent(const double* t2, const double* dt2, int64_t s_dt2, const double* LvvT,
+                      const double* dLvvT, int64_t s_dL, double* dom2, int64_t s_dom2, int64_t nb,
+                      Prec prec);
+  void ladder_adjoint(const double* t2, double* t2b, int64_t s_t2b, const double* LvvT, double* LvvTb,
+                      int64_t s_Lb, const double* om2b, int64_t s_om2b, int64_t nb, Prec prec);
 
  private:
   void require(bool ok, const char* what) const;
@@ -146,7 +151,10 @@ class CCSD {
   std::vector<Block> blocks_;
   DeviceBuffer pair_a_, pair_b_, pair_col_, pcol_a_, pcol_b_;
   DeviceBuffer tp_, tm_, wp_, wm_, gblk_, gp_, gm_;
-  DeviceBuffer dtp_, dtm_, dwp_, dwm_, dgblk_, dgp_, dgm_;
+  DeviceBuffer dtp_, dtm_, dwp_, dwm_, dgblk_, dgp_, dgm_;   // per derivative vector, x batch
+  int64_t lsz_tp_ = 0, lsz_tm_ = 0, lsz_wp_ = 0, lsz_wm_ = 0, lsz_gblk_ = 0, lsz_gp_ = 0, lsz_gm_ = 0;
+  int64_t ladder_vector_elems() const;
+  bool reserve_ladder_vectors(int64_t nb);
 
   // DIIS records: errors and parameters, dim x (n_t1 + n_t2) each
   int diis_dim_ = 0;
@@ -167,13 +175,22 @@ class CCSD {
   struct TransformGraph {
     int side = 0;
     Prec prec = Prec::F64;
+    int64_t nb = 1;                 // vectors per pass
+    uint64_t ws_gen = 0;            // gws_pass_ generation the capture saw
     hipGraphExec_t exec = nullptr;
   };
   std::vector<TransformGraph> tgraphs_;
   const Tape* graph_tape_ = nullptr;
   bool use_graphs_ = true, pack_cache_ = true;
   DeviceBuffer jin_, jout_;
-  void transform_body(int side, Prec prec);
+  // vectors per transformation pass: bounded by HBM (derivative buffers of the
+  // record and the ladder's per-vector work) and by transform_batch_limit_
+  static constexpr int64_t kMaxTransformBatch = 16;
+  int64_t transform_batch_limit_ = 0;   // 0: automatic
+  int64_t nb_cap_ = 0;                  // for the current record (0: not yet sized)
+  int64_t reserved_nb_ = 0;             // derivative vectors reserved so far
+  int64_t transform_batch_capacity();
+  void transform_body(int side, Prec prec, int64_t nb);
   void drop_transform_graphs();
   void reset_record();
   GemmWorkspace gws_pass_;                        // products inside transformation passes

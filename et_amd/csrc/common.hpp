@@ -53,13 +53,15 @@ class DeviceBuffer {
   }
   // Grows the allocation (contents are not preserved).  Work queued on any
   // stream may still use the old block, so the device is drained first.
-  void resize(size_t bytes) {
-    if (bytes <= bytes_) return;
+  // Grows to at least `bytes` (contents are not kept); true when the memory moved.
+  bool resize(size_t bytes) {
+    if (bytes <= bytes_) return false;
     if (ptr_) ETG_HIP(hipDeviceSynchronize());
     release();
-    if (bytes == 0) return;
+    if (bytes == 0) return false;
     ETG_HIP(hipMalloc(&ptr_, bytes));
     bytes_ = bytes;
+    return true;
   }
   void release() {
     if (ptr_) (void)hipFree(ptr_);

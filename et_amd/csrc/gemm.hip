@@ -403,7 +403,7 @@ GemmLaunch launch_typed(hipStream_t stream, const Gemm& g, GemmWorkspace* ws) {
     splits = int(ceil_div(g.k, kps));
     p.k_per_split = kps;
     const size_t need = size_t(splits) * size_t(g.m) * size_t(g.n) * size_t(g.batch) * sizeof(double);
-    ws->buf.resize(need);
+    if (ws->buf.resize(need)) ++ws->gen;
     p.W = ws->buf.as<double>();
   }
   GemmLaunch info;

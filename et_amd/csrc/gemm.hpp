@@ -44,12 +44,15 @@ struct PackedOperand {
   DeviceBuffer buf;
 };
 
-// Scratch for split-K partial sums and the split-bf16 operand packs; grown on demand.
+// Scratch for split-K partial sums and the split-bf16 operand packs; grown on
+// demand.  `gen` counts the times one of its buffers moved (launch sequences
+// captured into a graph hold these addresses).
 struct GemmWorkspace {
   DeviceBuffer buf, packA, packB;
   static constexpr int kCached = 4;
   PackedOperand cached[kCached];
   uint64_t clock = 0;
+  uint64_t gen = 0;
 };
 
 // Which kernel instantiation a product ran on (for the per-kernel ledger).

@@ -248,8 +248,10 @@ GemmLaunch gemm_bf16x3(hipStream_t stream, const Gemm& g, GemmWorkspace* ws) {
   if (ws == nullptr) fail("gemm: split-bf16 products need a workspace");
   const int64_t m_p = ceil_div(g.m, kTile) * kTile, n_p = ceil_div(g.n, kTile) * kTile;
   const int64_t Kp = ceil_div(g.k, kDepth) * kDepth;
-  const size_t a_elems = size_t(g.batch) * size_t(m_p) * size_t(Kp);
-  const size_t b_elems = size_t(g.batch) * size_t(n_p) * size_t(Kp);
+  // an operand shared by every batch entry (stride 0) is packed once
+  const int64_t a_batch = g.strideA == 0 ? 1 : g.batch, b_batch = g.strideB == 0 ? 1 : g.batch;
+  const size_t a_elems = size_t(a_batch) * size_t(m_p) * size_t(Kp);
+  const size_t b_elems = size_t(b_batch) * size_t(n_p) * size_t(Kp);
   // A: reuse a pack made under the same caller tag for the same operand
   PackedOperand* hit = nullptr;
   if (g.a_tag != 0)
@@ -273,19 +275,21 @@ GemmLaunch gemm_bf16x3(hipStream_t stream, const Gemm& g, GemmWorkspace* ws) {
       e.lda = g.lda; e.strideA = g.strideA; e.batch = g.batch;
       dst = &e.buf;
     }
-    dst->resize(2 * a_elems * sizeof(__bf16));
+    if (dst->resize(2 * a_elems * sizeof(__bf16))) ++ws->gen;
     Ahi = dst->as<__bf16>();
-    pack(stream, g.A, g.lda, g.strideA, g.ta, g.m, g.k, m_p, Kp, g.batch, Ahi, Ahi + a_elems);
+    pack(stream, g.A, g.lda, g.strideA, g.ta, g.m, g.k, m_p, Kp, a_batch, Ahi, Ahi + a_elems);
   }
   __bf16* Alo = Ahi + a_elems;
-  ws->packB.resize(2 * b_elems * sizeof(__bf16));
+  if (ws->packB.resize(2 * b_elems * sizeof(__bf16))) ++ws->gen;
   __bf16* Bhi = ws->packB.as<__bf16>();
   __bf16* Blo = Bhi + b_elems;
-  pack(stream, g.B, g.ldb, g.strideB, !g.tb, g.n, g.k, n_p, Kp, g.batch, Bhi, Blo);
+  pack(stream, g.B, g.ldb, g.strideB, !g.tb, g.n, g.k, n_p, Kp, b_batch, Bhi, Blo);
 
   Params p{};
   p.Ahi = Ahi; p.Alo = Alo; p.Bhi = Bhi; p.Blo = Blo;
-  p.Kp = Kp; p.strideAp = m_p * Kp; p.strideBp = n_p * Kp;
+  p.Kp = Kp;
+  p.strideAp = g.strideA == 0 ? 0 : m_p * Kp;
+  p.strideBp = g.strideB == 0 ? 0 : n_p * Kp;
   p.m = g.m; p.n = g.n; p.alpha = g.alpha; p.beta = g.beta;
   p.C = g.C; p.ldc = g.ldc; p.strideC = g.strideC;
   p.tiles_m = int(m_p / kTile);
@@ -300,7 +304,8 @@ GemmLaunch gemm_bf16x3(hipStream_t stream, const Gemm& g, GemmWorkspace* ws) {
     const int64_t kps = ceil_div(ceil_div(Kp, splits), kDepth) * kDepth;
     splits = int(ceil_div(Kp, kps));
     p.k_per_split = kps;
-    ws->buf.resize(size_t(splits) * size_t(g.m) * size_t(g.n) * size_t(g.batch) * sizeof(double));
+    if (ws->buf.resize(size_t(splits) * size_t(g.m) * size_t(g.n) * size_t(g.batch) * sizeof(double)))
+      ++ws->gen;
     p.W = ws->buf.as<double>();
   }
   dim3 grid(unsigned(tiles / g.batch), unsigned(splits), unsigned(g.batch));

@@ -115,12 +115,33 @@ double* Tape::deriv(const double* p) {
   return b.d.as<double>() + (p - b.base);
 }
 
-void Tape::zero_derivatives() {
+int64_t Tape::dstride(const double* p) const {
+  return std::max<int64_t>(bufs_[size_t(find(p))]->elems, 1);
+}
+
+int64_t Tape::derivative_elems() const {
+  int64_t e = 0;
+  for (const auto& bp : bufs_)
+    if (!bp->constant) e += std::max<int64_t>(bp->elems, 1);
+  return e;
+}
+
+bool Tape::reserve_derivatives(int64_t nb) {
+  bool moved = false;
   for (auto& bp : bufs_) {
     Buf& b = *bp;
     if (b.constant) continue;
-    b.d.resize(size_t(std::max<int64_t>(b.elems, 1)) * sizeof(double));
-    fill(wf_.stream(), b.d.as<double>(), b.elems, 0.0);
+    moved |= b.d.resize(size_t(nb) * size_t(std::max<int64_t>(b.elems, 1)) * sizeof(double));
+  }
+  return moved;
+}
+
+void Tape::zero_derivatives(int64_t nb) {
+  reserve_derivatives(nb);
+  for (auto& bp : bufs_) {
+    Buf& b = *bp;
+    if (b.constant) continue;
+    fill(wf_.stream(), b.d.as<double>(), nb * std::max<int64_t>(b.elems, 1), 0.0);
   }
 }
 
@@ -267,252 +288,318 @@ void Tape::ladder(const double* t2, const double* LvvT, double* om2, Prec prec) 
 namespace {
 int64_t dense_size(const int64_t d[4]) { return d[0] * d[1] * d[2] * d[3]; }
 
-// C = beta * C over the product's output (and batch)
-void scale_output(CCSD& wf, const Gemm& g, double* C, double beta, Prec prec) {
-  if (beta == 1.0) return;
-  Gemm z = g;
-  z.k = 0;
-  z.C = C;
-  z.beta = beta;
-  wf.run_gemm(z, prec);
-}
 }  // namespace
 
-void Tape::tangent(Prec prec) {
-  hipStream_t s = wf_.stream();
-  for (const Op& op : ops_) {
-    switch (op.kind) {
-      case GEMM: {
-        const Gemm& g = op.g;
-        double* dC = deriv(g.C);
-        const double* dA = deriv(g.A);
-        const double* dB = deriv(g.B);
-        scale_output(wf_, g, dC, g.beta, prec);
-        if (dA) {
-          Gemm h = g;
-          h.A = dA;
-          h.C = dC;
-          h.beta = 1.0;
-          wf_.run_gemm(h, prec);
-        }
-        if (dB) {
-          Gemm h = g;
-          h.B = dB;
-          h.C = dC;
-          h.beta = 1.0;
-          wf_.run_gemm(h, prec);
-        }
-        break;
-      }
-      case SORT4: {
-        double* dy = deriv(op.y);
-        const double* dx = deriv(op.x);
-        if (dx) etg::sort4(s, dx, op.d, op.perm, dy, op.alpha, op.beta);
-        else scale(s, dy, dense_size(op.d), op.beta);
-        break;
-      }
-      case SQUAREUP: {
-        const double* dp = deriv(op.x);
-        double* df = deriv(op.y);
-        if (dp) etg::squareup(s, dp, df, op.N);
-        else fill(s, df, op.N * op.N, 0.0);
-        break;
-      }
-      case PACKIN: {
-        const double* df = deriv(op.x);
-        double* dp = deriv(op.y);
-        if (df) etg::packin(s, df, dp, op.N, op.alpha, op.beta, op.diag);
-        else scale(s, dp, op.N * (op.N + 1) / 2, op.beta);
-        break;
-      }
-      case SYMSUM: {
-        const double* dx = deriv(op.x);
-        double* dy = deriv(op.y);
-        if (dx) etg::symmetric_sum(s, dx, dy, op.N);
-        else fill(s, dy, op.N * op.N, 0.0);
-        break;
-      }
-      case SYMPACK: {
-        const double* dx = deriv(op.x);
-        if (dx) etg::symmetrize_and_pack(s, dx, deriv(op.y), op.nv, op.no, op.alpha, op.f1, op.f2);
-        break;
-      }
-      case COPY_SUB: {
-        const double* dsrc = deriv(op.x);
-        double* ddst = deriv(op.y);
-        if (dsrc) etg::copy_sub(s, dsrc, op.lds, ddst, op.ldd, op.m, op.n, op.alpha, op.beta);
-        else etg::copy_sub(s, ddst, op.ldd, ddst, op.ldd, op.m, op.n, 0.0, op.beta);
-        break;
-      }
-      case FILL0:
-        fill(s, deriv(op.y), op.n, 0.0);
-        break;
-      case T1_XY:
-        etg::t1_xy(s, deriv(op.x), deriv(op.y), deriv(op.y2), wf_.n_o, wf_.n_v, 0.0);
-        break;
-      case TRACE_OO: {
-        const double* dL = deriv(op.x);
-        if (dL) etg::trace_oo(s, dL, deriv(op.y), wf_.n_J, wf_.n_o);
-        else fill(s, deriv(op.y), wf_.n_J, 0.0);
-        break;
-      }
-      case FOCK_OV_VEC:
-        etg::fock_ov_vector(s, deriv(op.x), deriv(op.y), wf_.n_o, wf_.n_v);
-        break;
-      case LADDER:
-        wf_.ladder_tangent(op.x, deriv(op.x), op.x2, deriv(op.x2), deriv(op.y), prec);
-        break;
+// dC_k = beta dC_k + alpha (op(dA_k) op(B) + op(A) op(dB_k)), k < nb.  A product
+// of one matrix pair runs as one strided-batched product over the vectors (the
+// primal operand at stride 0); an already batched product runs once per vector.
+void Tape::tangent_gemm(const Gemm& g, Prec prec, int64_t nb) {
+  double* dC = deriv(g.C);
+  const double* dA = deriv(g.A);
+  const double* dB = deriv(g.B);
+  const int64_t sC = dstride(g.C), sA = dA ? dstride(g.A) : 0, sB = dB ? dstride(g.B) : 0;
+  const bool vec_batch = g.batch == 1 && nb > 1;
+  const int64_t reps = vec_batch ? 1 : nb;
+  for (int64_t k = 0; k < reps; ++k) {
+    Gemm base = g;
+    base.a_tag = 0;
+    base.C = dC + k * sC;
+    if (vec_batch) {
+      base.batch = nb;
+      base.strideC = sC;
+      base.strideA = 0;
+      base.strideB = 0;
+    }
+    if (g.beta != 1.0) {
+      Gemm z = base;
+      z.k = 0;
+      z.beta = g.beta;
+      wf_.run_gemm(z, prec);
+    }
+    if (dA) {
+      Gemm h = base;
+      h.A = dA + k * sA;
+      if (vec_batch) h.strideA = sA;
+      h.beta = 1.0;
+      wf_.run_gemm(h, prec);
+    }
+    if (dB) {
+      Gemm h = base;
+      h.B = dB + k * sB;
+      if (vec_batch) h.strideB = sB;
+      h.beta = 1.0;
+      wf_.run_gemm(h, prec);
     }
   }
 }
 
-void Tape::adjoint(Prec prec) {
+// A_bar_k += alpha C_bar_k op(B)^T (or its transpose), B_bar_k likewise, then
+// C_bar_k *= beta; batched over the vectors as in tangent_gemm.
+void Tape::adjoint_gemm(const Gemm& g, Prec prec, int64_t nb) {
+  double* Cb0 = deriv(g.C);
+  double* Ab0 = deriv(g.A);
+  double* Bb0 = deriv(g.B);
+  const int64_t sC = dstride(g.C), sA = Ab0 ? dstride(g.A) : 0, sB = Bb0 ? dstride(g.B) : 0;
+  const bool vec_batch = g.batch == 1 && nb > 1;
+  const int64_t reps = vec_batch ? 1 : nb;
+  for (int64_t v = 0; v < reps; ++v) {
+    const double* Cb = Cb0 + v * sC;
+    double* Ab = Ab0 ? Ab0 + v * sA : nullptr;
+    double* Bb = Bb0 ? Bb0 + v * sB : nullptr;
+    // product b0 .. b0 + nbat of the recorded batch
+    auto pair = [&](bool for_A, int64_t b0, int64_t nbat) {
+      Gemm h;
+      h.batch = nbat;
+      const double* A = g.A + b0 * g.strideA;
+      const double* B = g.B + b0 * g.strideB;
+      const double* C = Cb + b0 * g.strideC;
+      if (for_A) {
+        double* Ao = Ab + b0 * g.strideA;
+        if (!g.ta) {   // A_bar (m x k) += alpha Cbar op(B)^T
+          h.ta = false; h.tb = !g.tb;
+          h.m = g.m; h.n = g.k; h.k = g.n;
+          h.A = C; h.lda = g.ldc; h.strideA = g.strideC;
+          h.B = B; h.ldb = g.ldb; h.strideB = g.strideB;
+        } else {       // A_bar (k x m) += alpha op(B) Cbar^T
+          h.ta = g.tb; h.tb = true;
+          h.m = g.k; h.n = g.m; h.k = g.n;
+          h.A = B; h.lda = g.ldb; h.strideA = g.strideB;
+          h.B = C; h.ldb = g.ldc; h.strideB = g.strideC;
+        }
+        h.C = Ao; h.ldc = g.lda; h.strideC = g.strideA;
+      } else {
+        double* Bo = Bb + b0 * g.strideB;
+        if (!g.tb) {   // B_bar (k x n) += alpha op(A)^T Cbar
+          h.ta = !g.ta; h.tb = false;
+          h.m = g.k; h.n = g.n; h.k = g.m;
+          h.A = A; h.lda = g.lda; h.strideA = g.strideA;
+          h.B = C; h.ldb = g.ldc; h.strideB = g.strideC;
+        } else {       // B_bar (n x k) += alpha Cbar^T op(A)
+          h.ta = true; h.tb = g.ta;
+          h.m = g.n; h.n = g.k; h.k = g.m;
+          h.A = C; h.lda = g.ldc; h.strideA = g.strideC;
+          h.B = A; h.ldb = g.lda; h.strideB = g.strideA;
+        }
+        h.C = Bo; h.ldc = g.ldb; h.strideC = g.strideB;
+      }
+      h.alpha = g.alpha;
+      h.beta = 1.0;
+      if (vec_batch) {             // one product per vector: adjoints at their strides, primal shared
+        h.batch = nb;
+        h.strideA = (h.A == C) ? sC : 0;
+        h.strideB = (h.B == C) ? sC : 0;
+        h.strideC = for_A ? sA : sB;
+      }
+      wf_.run_gemm(h, prec);
+    };
+    if (Ab) {
+      if (g.batch > 1 && g.strideA == 0) {
+        for (int64_t b = 0; b < g.batch; ++b) pair(true, b, 1);
+      } else {
+        pair(true, 0, g.batch);
+      }
+    }
+    if (Bb) {
+      if (g.batch > 1 && g.strideB == 0) {
+        for (int64_t b = 0; b < g.batch; ++b) pair(false, b, 1);
+      } else {
+        pair(false, 0, g.batch);
+      }
+    }
+    if (g.beta != 1.0) {
+      Gemm z = g;
+      z.a_tag = 0;
+      z.k = 0;
+      z.C = Cb0 + v * sC;
+      z.beta = g.beta;
+      if (vec_batch) {
+        z.batch = nb;
+        z.strideC = sC;
+      }
+      wf_.run_gemm(z, prec);
+    }
+  }
+}
+
+void Tape::tangent(Prec prec, int64_t nb) {
   hipStream_t s = wf_.stream();
+  auto D = [&](const double* p, int64_t k) -> double* {
+    double* d = deriv(p);
+    return d ? d + k * dstride(p) : nullptr;
+  };
+  for (const Op& op : ops_) {
+    if (op.kind == GEMM) {
+      tangent_gemm(op.g, prec, nb);
+      continue;
+    }
+    if (op.kind == LADDER) {
+      wf_.ladder_tangent(op.x, deriv(op.x), dstride(op.x), op.x2, deriv(op.x2), dstride(op.x2),
+                         deriv(op.y), dstride(op.y), nb, prec);
+      continue;
+    }
+    for (int64_t k = 0; k < nb; ++k) {
+      switch (op.kind) {
+        case SORT4: {
+          double* dy = D(op.y, k);
+          const double* dx = D(op.x, k);
+          if (dx) etg::sort4(s, dx, op.d, op.perm, dy, op.alpha, op.beta);
+          else scale(s, dy, dense_size(op.d), op.beta);
+          break;
+        }
+        case SQUAREUP: {
+          const double* dp = D(op.x, k);
+          double* df = D(op.y, k);
+          if (dp) etg::squareup(s, dp, df, op.N);
+          else fill(s, df, op.N * op.N, 0.0);
+          break;
+        }
+        case PACKIN: {
+          const double* df = D(op.x, k);
+          double* dp = D(op.y, k);
+          if (df) etg::packin(s, df, dp, op.N, op.alpha, op.beta, op.diag);
+          else scale(s, dp, op.N * (op.N + 1) / 2, op.beta);
+          break;
+        }
+        case SYMSUM: {
+          const double* dx = D(op.x, k);
+          double* dy = D(op.y, k);
+          if (dx) etg::symmetric_sum(s, dx, dy, op.N);
+          else fill(s, dy, op.N * op.N, 0.0);
+          break;
+        }
+        case SYMPACK: {
+          const double* dx = D(op.x, k);
+          if (dx) etg::symmetrize_and_pack(s, dx, D(op.y, k), op.nv, op.no, op.alpha, op.f1, op.f2);
+          break;
+        }
+        case COPY_SUB: {
+          const double* dsrc = D(op.x, k);
+          double* ddst = D(op.y, k);
+          if (dsrc) etg::copy_sub(s, dsrc, op.lds, ddst, op.ldd, op.m, op.n, op.alpha, op.beta);
+          else etg::copy_sub(s, ddst, op.ldd, ddst, op.ldd, op.m, op.n, 0.0, op.beta);
+          break;
+        }
+        case FILL0:
+          fill(s, D(op.y, k), op.n, 0.0);
+          break;
+        case T1_XY:
+          etg::t1_xy(s, D(op.x, k), D(op.y, k), D(op.y2, k), wf_.n_o, wf_.n_v, 0.0);
+          break;
+        case TRACE_OO: {
+          const double* dL = D(op.x, k);
+          if (dL) etg::trace_oo(s, dL, D(op.y, k), wf_.n_J, wf_.n_o);
+          else fill(s, D(op.y, k), wf_.n_J, 0.0);
+          break;
+        }
+        case FOCK_OV_VEC:
+          etg::fock_ov_vector(s, D(op.x, k), D(op.y, k), wf_.n_o, wf_.n_v);
+          break;
+        case GEMM:
+        case LADDER:
+          break;
+      }
+    }
+  }
+}
+
+void Tape::adjoint(Prec prec, int64_t nb) {
+  hipStream_t s = wf_.stream();
+  auto D = [&](const double* p, int64_t k) -> double* {
+    double* d = deriv(p);
+    return d ? d + k * dstride(p) : nullptr;
+  };
   for (auto it = ops_.rbegin(); it != ops_.rend(); ++it) {
     const Op& op = *it;
-    switch (op.kind) {
-      case GEMM: {
-        const Gemm& g = op.g;
-        const double* Cb = deriv(g.C);
-        double* Ab = deriv(g.A);
-        double* Bb = deriv(g.B);
-        const char ta = g.ta ? 'T' : 'N', tb = g.tb ? 'T' : 'N';
-        (void)ta;
-        (void)tb;
-        auto run_pair = [&](bool for_A, int64_t b0, int64_t nb) {
-          Gemm h;
-          h.batch = nb;
-          const double* A = g.A + b0 * g.strideA;
-          const double* B = g.B + b0 * g.strideB;
-          const double* C = Cb + b0 * g.strideC;
-          if (for_A) {
-            double* Ao = Ab + b0 * g.strideA;
-            if (!g.ta) {   // A_bar (m x k) += alpha Cbar op(B)^T
-              h.ta = false; h.tb = !g.tb;
-              h.m = g.m; h.n = g.k; h.k = g.n;
-              h.A = C; h.lda = g.ldc; h.strideA = g.strideC;
-              h.B = B; h.ldb = g.ldb; h.strideB = g.strideB;
-            } else {       // A_bar (k x m) += alpha op(B) Cbar^T
-              h.ta = g.tb; h.tb = true;
-              h.m = g.k; h.n = g.m; h.k = g.n;
-              h.A = B; h.lda = g.ldb; h.strideA = g.strideB;
-              h.B = C; h.ldb = g.ldc; h.strideB = g.strideC;
+    if (op.kind == GEMM) {
+      adjoint_gemm(op.g, prec, nb);
+      continue;
+    }
+    if (op.kind == LADDER) {
+      wf_.ladder_adjoint(op.x, deriv(op.x), dstride(op.x), op.x2, deriv(op.x2), dstride(op.x2),
+                         deriv(op.y), dstride(op.y), nb, prec);
+      continue;
+    }
+    for (int64_t k = 0; k < nb; ++k) {
+      switch (op.kind) {
+        case SORT4: {
+          double* yb = D(op.y, k);
+          double* xb = D(op.x, k);
+          if (xb) {
+            int64_t e[4];
+            char inv[5] = {0, 0, 0, 0, 0};
+            for (int q = 0; q < 4; ++q) {
+              const int r = op.perm[q] - '1';
+              e[q] = op.d[r];
+              inv[r] = char('1' + q);
             }
-            h.C = Ao; h.ldc = g.lda; h.strideC = g.strideA;
-          } else {
-            double* Bo = Bb + b0 * g.strideB;
-            if (!g.tb) {   // B_bar (k x n) += alpha op(A)^T Cbar
-              h.ta = !g.ta; h.tb = false;
-              h.m = g.k; h.n = g.n; h.k = g.m;
-              h.A = A; h.lda = g.lda; h.strideA = g.strideA;
-              h.B = C; h.ldb = g.ldc; h.strideB = g.strideC;
-            } else {       // B_bar (n x k) += alpha Cbar^T op(A)
-              h.ta = true; h.tb = g.ta;
-              h.m = g.n; h.n = g.k; h.k = g.m;
-              h.A = C; h.lda = g.ldc; h.strideA = g.strideC;
-              h.B = A; h.ldb = g.lda; h.strideB = g.strideA;
-            }
-            h.C = Bo; h.ldc = g.ldb; h.strideC = g.strideB;
+            etg::sort4(s, yb, e, inv, xb, op.alpha, 1.0);
           }
-          h.alpha = g.alpha;
-          h.beta = 1.0;
-          wf_.run_gemm(h, prec);
-        };
-        if (Ab) {
-          if (g.batch > 1 && g.strideA == 0) {
-            for (int64_t b = 0; b < g.batch; ++b) run_pair(true, b, 1);
-          } else {
-            run_pair(true, 0, g.batch);
-          }
+          scale(s, yb, dense_size(op.d), op.beta);
+          break;
         }
-        if (Bb) {
-          if (g.batch > 1 && g.strideB == 0) {
-            for (int64_t b = 0; b < g.batch; ++b) run_pair(false, b, 1);
-          } else {
-            run_pair(false, 0, g.batch);
-          }
+        case SQUAREUP: {
+          double* pb = D(op.x, k);
+          double* fb = D(op.y, k);
+          if (pb) packsum(s, fb, pb, op.N);
+          fill(s, fb, op.N * op.N, 0.0);
+          break;
         }
-        scale_output(wf_, g, deriv(g.C), g.beta, prec);
-        break;
-      }
-      case SORT4: {
-        double* yb = deriv(op.y);
-        double* xb = deriv(op.x);
-        if (xb) {
-          int64_t e[4];
-          char inv[5] = {0, 0, 0, 0, 0};
-          for (int k = 0; k < 4; ++k) {
-            const int q = op.perm[k] - '1';
-            e[k] = op.d[q];
-            inv[q] = char('1' + k);
-          }
-          etg::sort4(s, yb, e, inv, xb, op.alpha, 1.0);
+        case PACKIN: {
+          double* fb = D(op.x, k);
+          double* pb = D(op.y, k);
+          if (fb) unpack_upper_add(s, pb, fb, op.N, op.alpha, op.diag);
+          scale(s, pb, op.N * (op.N + 1) / 2, op.beta);
+          break;
         }
-        scale(s, yb, dense_size(op.d), op.beta);
-        break;
+        case SYMSUM: {
+          double* xb = D(op.x, k);
+          double* yb = D(op.y, k);
+          if (xb) symmetric_sum_acc(s, yb, xb, op.N, 1.0);
+          fill(s, yb, op.N * op.N, 0.0);
+          break;
+        }
+        case SYMPACK: {
+          double* xb = D(op.x, k);
+          if (xb) symmetrize_and_pack_adjoint(s, D(op.y, k), xb, op.nv, op.no, op.alpha, op.f1, op.f2);
+          break;
+        }
+        case COPY_SUB: {
+          double* sb = D(op.x, k);
+          double* db = D(op.y, k);
+          if (sb) etg::copy_sub(s, db, op.ldd, sb, op.lds, op.m, op.n, op.alpha, 1.0);
+          etg::copy_sub(s, db, op.ldd, db, op.ldd, op.m, op.n, 0.0, op.beta);
+          break;
+        }
+        case FILL0:
+          fill(s, D(op.y, k), op.n, 0.0);
+          break;
+        case T1_XY: {
+          double* tb = D(op.x, k);
+          double* Xb = D(op.y, k);
+          double* Yb = D(op.y2, k);
+          if (tb) t1_xy_adjoint(s, Xb, Yb, tb, wf_.n_o, wf_.n_v);
+          fill(s, Xb, wf_.n_mo * wf_.n_mo, 0.0);
+          fill(s, Yb, wf_.n_mo * wf_.n_mo, 0.0);
+          break;
+        }
+        case TRACE_OO: {
+          double* Lb = D(op.x, k);
+          double* Xb = D(op.y, k);
+          if (Lb) trace_oo_adjoint(s, Xb, Lb, wf_.n_J, wf_.n_o);
+          fill(s, Xb, wf_.n_J, 0.0);
+          break;
+        }
+        case FOCK_OV_VEC: {
+          double* Fb = D(op.x, k);
+          double* fb = D(op.y, k);
+          if (Fb) fock_ov_vector_adjoint(s, fb, Fb, wf_.n_o, wf_.n_v);
+          fill(s, fb, wf_.n_o * wf_.n_v, 0.0);
+          break;
+        }
+        case GEMM:
+        case LADDER:
+          break;
       }
-      case SQUAREUP: {
-        double* pb = deriv(op.x);
-        double* fb = deriv(op.y);
-        if (pb) packsum(s, fb, pb, op.N);
-        fill(s, fb, op.N * op.N, 0.0);
-        break;
-      }
-      case PACKIN: {
-        double* fb = deriv(op.x);
-        double* pb = deriv(op.y);
-        if (fb) unpack_upper_add(s, pb, fb, op.N, op.alpha, op.diag);
-        scale(s, pb, op.N * (op.N + 1) / 2, op.beta);
-        break;
-      }
-      case SYMSUM: {
-        double* xb = deriv(op.x);
-        double* yb = deriv(op.y);
-        if (xb) symmetric_sum_acc(s, yb, xb, op.N, 1.0);
-        fill(s, yb, op.N * op.N, 0.0);
-        break;
-      }
-      case SYMPACK: {
-        double* xb = deriv(op.x);
-        if (xb) symmetrize_and_pack_adjoint(s, deriv(op.y), xb, op.nv, op.no, op.alpha, op.f1, op.f2);
-        break;
-      }
-      case COPY_SUB: {
-        double* sb = deriv(op.x);
-        double* db = deriv(op.y);
-        if (sb) etg::copy_sub(s, db, op.ldd, sb, op.lds, op.m, op.n, op.alpha, 1.0);
-        etg::copy_sub(s, db, op.ldd, db, op.ldd, op.m, op.n, 0.0, op.beta);
-        break;
-      }
-      case FILL0:
-        fill(s, deriv(op.y), op.n, 0.0);
-        break;
-      case T1_XY: {
-        double* tb = deriv(op.x);
-        double* Xb = deriv(op.y);
-        double* Yb = deriv(op.y2);
-        if (tb) t1_xy_adjoint(s, Xb, Yb, tb, wf_.n_o, wf_.n_v);
-        fill(s, Xb, wf_.n_mo * wf_.n_mo, 0.0);
-        fill(s, Yb, wf_.n_mo * wf_.n_mo, 0.0);
-        break;
-      }
-      case TRACE_OO: {
-        double* Lb = deriv(op.x);
-        double* Xb = deriv(op.y);
-        if (Lb) trace_oo_adjoint(s, Xb, Lb, wf_.n_J, wf_.n_o);
-        fill(s, Xb, wf_.n_J, 0.0);
-        break;
-      }
-      case FOCK_OV_VEC: {
-        double* Fb = deriv(op.x);
-        double* fb = deriv(op.y);
-        if (Fb) fock_ov_vector_adjoint(s, fb, Fb, wf_.n_o, wf_.n_v);
-        fill(s, fb, wf_.n_o * wf_.n_v, 0.0);
-        break;
-      }
-      case LADDER:
-        wf_.ladder_adjoint(op.x, deriv(op.x), op.x2, deriv(op.x2), deriv(op.y), prec);
-        break;
     }
   }
 }

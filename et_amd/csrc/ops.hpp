@@ -134,16 +134,24 @@ class Tape : public DirectOps {
   // Buffers that exist outside the tape.  Constant ones carry no derivative.
   void register_buffer(const double* base, int64_t elems, bool constant, const char* name);
   void clear();
-  // Forward replay: derivative of every recorded output along the input
-  // derivatives already placed in the derivative buffers (see deriv()).
-  void tangent(Prec prec);
-  // Backward replay: adjoints of every recorded input from the adjoint
+  // Every derivative buffer holds up to `nb` vectors: vector k of the
+  // derivative of primal element p sits at deriv(p) + k * dstride(p).
+  // Forward replay: derivatives of every recorded output along the nb input
+  // derivatives already placed in the derivative buffers.
+  void tangent(Prec prec, int64_t nb = 1);
+  // Backward replay: adjoints of every recorded input from the nb adjoints
   // placed in the output's derivative buffer.
-  void adjoint(Prec prec);
-  // Zeroes every derivative buffer (allocating them on first use).
-  void zero_derivatives();
-  // Derivative buffer matching a primal pointer (nullptr for constants).
+  void adjoint(Prec prec, int64_t nb = 1);
+  // Room for nb derivative vectors per buffer; true when a buffer moved.
+  bool reserve_derivatives(int64_t nb);
+  // Zeroes the first nb derivative vectors of every buffer (allocating on first use).
+  void zero_derivatives(int64_t nb = 1);
+  // Derivative buffer matching a primal pointer (nullptr for constants), vector 0.
   double* deriv(const double* p);
+  // Element distance between consecutive derivative vectors of p's buffer.
+  int64_t dstride(const double* p) const;
+  // Elements of one derivative vector over all buffers (memory per batched vector).
+  int64_t derivative_elems() const;
   bool empty() const { return ops_.empty(); }
 
   enum Kind { GEMM, SORT4, SQUAREUP, PACKIN, SYMSUM, SYMPACK, COPY_SUB, FILL0, T1_XY, TRACE_OO,
@@ -171,6 +179,8 @@ class Tape : public DirectOps {
     DeviceBuffer d;  // derivative
   };
   int find(const double* p) const;
+  void tangent_gemm(const Gemm& g, Prec prec, int64_t nb);
+  void adjoint_gemm(const Gemm& g, Prec prec, int64_t nb);
   std::vector<Op> ops_;
   std::vector<std::unique_ptr<Buf>> bufs_;
   std::map<std::string, DeviceBuffer> named_;

@@ -240,6 +240,14 @@ module etgpu_interface
          integer(c_int64_t), intent(out) :: iterations
       end function etg_davidson
 !
+!     tuning: trial vectors per transformation pass (0: automatic)
+      integer(c_int) function etg_set_transform_batch(handle, max_vectors) &
+                                                   bind(C, name='etg_set_transform_batch')
+         import :: c_int, c_ptr, c_int64_t
+         type(c_ptr), value :: handle
+         integer(c_int64_t), value :: max_vectors
+      end function etg_set_transform_batch
+!
 !     construct_omega with the residual left in HBM (omega_ccsd.F90:39-110)
       integer(c_int) function etg_construct_omega_resident(handle, precision) &
                                                    bind(C, name='etg_construct_omega_resident')

@@ -257,6 +257,13 @@ int etg_kernel_ledger(etg_handle* handle, int64_t index, char* name, int64_t nam
  * default) or run every pass eagerly (off).  Results are identical. */
 int etg_set_transform_graphs(etg_handle* handle, int on);
 
+/* Tuning: at most max_vectors trial vectors per transformation pass (0, the
+ * default: as many as 35 % of the free HBM holds, at most 16).  The vectors of
+ * one pass are the batch dimension of every product, so the ladder's g+/- and
+ * every t-dependent operand are read once per group.  Results agree with
+ * one-vector passes to the products' rounding. */
+int etg_set_transform_batch(etg_handle* handle, int64_t max_vectors);
+
 /* C = alpha op(A) op(B) + beta C on host arrays (copied to the device and
  * back), column-major, op = 'N' or 'T', with the given precision. */
 int etg_test_gemm(char ta, char tb, int64_t m, int64_t n, int64_t k, double alpha,

@@ -163,3 +163,33 @@ def test_graph_replays_equal_eager_transformations():
             E = wf.jacobian_transformations(X, side)
             wf.set_transform_graphs(True)
             assert np.array_equal(R, E)
+
+
+def test_vector_groups_match_one_vector_passes():
+    """Trial vectors as the batch dimension of the passes: 7 vectors in groups of
+    1 (one vector per pass), 3 (3 + 3 + 1, two group sizes) and automatic (one group)
+    agree to FP64 rounding (to the products' rounding for split-bf16); replays of
+    each group size's graph equal the first pass exactly."""
+    s = make(4, 9, 15, 6)
+    n = 4 * 9 + O.n_t2(4, 9)
+    X = np.random.default_rng(44).normal(size=(n, 7))
+    with load(s, 4) as wf:
+        wf.prepare_for_jacobian()
+        ref = {}
+        wf.set_transform_batch(1)
+        for side in ('right', 'left'):
+            for prec in (ccsd.FP64, ccsd.BF16X3):
+                ref[side, prec] = wf.jacobian_transformations(X, side, precision=prec)
+        for limit in (3, 0):
+            wf.set_transform_batch(limit)
+            for side in ('right', 'left'):
+                for prec, tol in ((ccsd.FP64, 1e-13), (ccsd.BF16X3, 1e-5)):
+                    R = wf.jacobian_transformations(X, side, precision=prec)
+                    assert rel(R, ref[side, prec]) < tol, (limit, side, prec)
+                    again = wf.jacobian_transformations(X, side, precision=prec)
+                    wf.construct_omega()          # eager work between the replays
+                    wf.synchronize()
+                    assert np.array_equal(again, R), (limit, side, prec)
+                    # a group smaller than the largest one reuses the reserved buffers
+                    part = wf.jacobian_transformations(X[:, 2:4], side, precision=prec)
+                    assert rel(part, R[:, 2:4]) < tol, (limit, side, prec)
