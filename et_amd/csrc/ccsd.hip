@@ -576,16 +576,13 @@ void CCSD::build_ladder_plan() {
   lsz_tm_ = std::max<int64_t>(1, om * vm);
   lsz_wp_ = op * vp;
   lsz_wm_ = std::max<int64_t>(1, om * vp);
-  lsz_gblk_ = v * v * std::min(nb, v) * std::min(nb, v);
-  lsz_gp_ = vp * maxpairs;
-  lsz_gm_ = std::max<int64_t>(1, vm * maxpairs);
   tp_.resize(sizeof(double) * lsz_tp_);
   tm_.resize(sizeof(double) * lsz_tm_);
   wp_.resize(sizeof(double) * lsz_wp_);
   wm_.resize(sizeof(double) * lsz_wm_);
-  gblk_.resize(sizeof(double) * lsz_gblk_);
-  gp_.resize(sizeof(double) * lsz_gp_);
-  gm_.resize(sizeof(double) * lsz_gm_);
+  gblk_.resize(sizeof(double) * v * v * std::min(nb, v) * std::min(nb, v));
+  gp_.resize(sizeof(double) * vp * maxpairs);
+  gm_.resize(sizeof(double) * std::max<int64_t>(1, vm * maxpairs));
   reserved_nb_ = 0;                     // per-vector sizes changed: size the batches again
   nb_cap_ = 0;
 }
@@ -613,9 +610,9 @@ void CCSD::ladder_primal(const double* t2, const double* LT, double* om2, Prec p
 }
 
 // Per-vector work of the ladder derivatives: t+/- and w+/- of a derivative
-// vector, its g block and g+/- (elements).
+// vector and the two (ov)^2 arrays of its T1 part (elements).
 int64_t CCSD::ladder_vector_elems() const {
-  return lsz_tp_ + lsz_tm_ + lsz_wp_ + lsz_wm_ + lsz_gblk_ + lsz_gp_ + lsz_gm_;
+  return lsz_tp_ + lsz_tm_ + lsz_wp_ + lsz_wm_ + 2 * n_t1 * n_t1;
 }
 
 bool CCSD::reserve_ladder_vectors(int64_t nb) {
@@ -625,22 +622,62 @@ bool CCSD::reserve_ladder_vectors(int64_t nb) {
   grow(dtm_, lsz_tm_);
   grow(dwp_, lsz_wp_);
   grow(dwm_, lsz_wm_);
-  grow(dgblk_, lsz_gblk_);
-  grow(dgp_, lsz_gp_);
-  grow(dgm_, lsz_gm_);
+  grow(lz1_, n_t1 * n_t1);
+  grow(lz2_, n_t1 * n_t1);
   return moved;
 }
 
-// Tangent of the ladder term for nb vectors (derivative k of t2 at dt2 + k s_dt2,
-// of LT at dLT + k s_dL, into dom2 + k s_dom2):
-//   d(om2) = ladder(dt2, g) + ladder(t2, dg),  dg_acbd = dL_ac . L_bd + L_ac . dL_bd.
-// g+/- of each (a, b) block is built once and applied to every vector; the
-// vector dimension is the batch of the products.
+// T1 part of the ladder derivative.  The dressed block depends on t1 through
+// L_t1(J, a, c) = L(J, a, c) - sum_k t_ak L(J, k, c) (ov block undressed), so
+//   sum_cd dg_acbd t_cidj = Z_aibj + Z_bjai,   Z_aibj = -sum_k dt_ak W_kbij,
+//   W_kbij = sum_cd G_kcbd t_cidj,   G_kcbd = sum_J L(J, k, c) L_t1(J, b, d),
+// with W (o^3 v) built once per record (save_jacobian_d2_intermediates,
+// jacobian_ccsd.F90:1984, keeps the same X_kbij).  G is formed in slices of d.
+void CCSD::build_ladder_t1_intermediate() {
+  const int64_t o = n_o, v = n_v, J = n_J, N = n_t1;
+  hipStream_t s = stream_;
+  DirectOps X(*this);
+  lw_.resize(sizeof(double) * size_t(std::max<int64_t>(1, o * v * o * o)));
+  double* W = lw_.as<double>();
+  double* Tfull = scratch(SL_A, N * N, "ladder_t1_t_full");
+  double* Tcd = scratch(SL_B, N * N, "ladder_t1_t_cdij");
+  squareup(s, t_.as<double>() + n_t1, Tfull, N);
+  const int64_t dT[4] = {v, o, v, o};
+  sort4(s, Tfull, dT, "1324", Tcd, 1.0, 0.0);                  // (c, d, i, j)
+  // slices of d sized to the (ov)^2 scratch slots (o v^2 <= (ov)^2, so one d always fits)
+  const int64_t cap = std::max(N * N, o * o * o * std::max(o, v));
+  const int64_t slice = std::max<int64_t>(1, std::min<int64_t>(v, cap / (o * v * v)));
+  double* G = scratch(SL_C, o * v * v * slice, "ladder_t1_G");
+  double* Gs = scratch(SL_D, o * v * v * slice, "ladder_t1_G_kbcd");
+  for (int64_t d0 = 0; d0 < v; d0 += slice) {
+    const int64_t nd = std::min(slice, v - d0);
+    // G(k, c, b, d) = sum_J L(J, k, c) L_t1(J, b, d),  d in [d0, d0 + nd)
+    X.gemm('T', 'N', o * v, v * nd, J, 1.0, Lmo_ov_.as<double>(), J, Lt_vv_.as<double>() + J * v * d0, J,
+           0.0, G, o * v, Prec::F64);
+    const int64_t dG[4] = {o, v, v, nd};
+    sort4(s, G, dG, "1324", Gs, 1.0, 0.0);                      // (k, b, c, d)
+    // W(kb, ij) += G(kb, cd) T(cd, ij) over this slice of d
+    X.gemm('N', 'N', o * v, o * o, v * nd, 1.0, Gs, o * v, Tcd + v * d0, v * v, d0 == 0 ? 0.0 : 1.0, W,
+           o * v, Prec::F64);
+  }
+  have_lw_ = true;
+}
+
+// Tangent of the ladder term for nb vectors (derivative k of t at dt2 - n_t1 +
+// k s_dt2, into dom2 + k s_dom2):
+//   d(om2) = ladder(dt2, g) + [T1 part through W].
+// g+/- of each (a, b) block is built once and applied to every vector (the
+// vector dimension is the batch of the products).  The derivative of the
+// dressed vv block is not read: the T1 part above is the whole of it.
 void CCSD::ladder_tangent(const double* t2, const double* dt2, int64_t s_dt2, const double* LT,
                           const double* dLT, int64_t s_dL, double* dom2, int64_t s_dom2, int64_t nb,
                           Prec prec) {
-  if (dt2 == nullptr || dLT == nullptr || dom2 == nullptr) fail("ladder tangent: missing derivative");
-  const int64_t o = n_o, v = n_v, J = n_J;
+  (void)dLT;
+  (void)s_dL;
+  if (dt2 == nullptr || dom2 == nullptr) fail("ladder tangent: missing derivative");
+  if (t2 != t_.as<double>() + n_t1 || LT != Lt_vvT_.as<double>() || !have_lw_)
+    fail("ladder tangent: the record's ladder operands are required");
+  const int64_t o = n_o, v = n_v, J = n_J, N = n_t1;
   const int64_t op = o * (o + 1) / 2, om = o * (o - 1) / 2;
   const int64_t vp = v * (v + 1) / 2, vm = v * (v - 1) / 2;
   hipStream_t s = stream_;
@@ -650,9 +687,6 @@ void CCSD::ladder_tangent(const double* t2, const double* dt2, int64_t s_dt2, co
   double* dtm = dtm_.as<double>();
   double* dwp = dwp_.as<double>();
   double* dwm = dwm_.as<double>();
-  double* dgb = dgblk_.as<double>();
-  double* dgp = dgp_.as<double>();
-  double* dgm = dgm_.as<double>();
   auto batched = [&](Gemm g, int64_t sA, int64_t sB, int64_t sC) {
     g.batch = nb;
     g.strideA = sA;
@@ -667,37 +701,40 @@ void CCSD::ladder_tangent(const double* t2, const double* dt2, int64_t s_dt2, co
     const double* LA = LT + J * v * b.a0;
     const double* LB = LT + J * v * b.b0;
     X.gemm('T', 'N', v * b.la, v * b.lb, J, 1.0, LA, J, LB, J, 0.0, gblk_.as<double>(), v * b.la, prec);
-    batched(make_gemm('T', 'N', v * b.la, v * b.lb, J, 1.0, dLT + J * v * b.a0, J, LB, J, 0.0, dgb,
-                      v * b.la), s_dL, 0, lsz_gblk_);
-    batched(make_gemm('T', 'N', v * b.la, v * b.lb, J, 1.0, LA, J, dLT + J * v * b.b0, J, 1.0, dgb,
-                      v * b.la), 0, s_dL, lsz_gblk_);
     ladder_g_pm(s, gblk_.as<double>(), v, b.la, b.lb, pair_a_.as<int32_t>() + b.col0,
                 pair_b_.as<int32_t>() + b.col0, b.npairs, gp_.as<double>(), gm_.as<double>());
-    for (int64_t k = 0; k < nb; ++k)
-      ladder_g_pm(s, dgb + k * lsz_gblk_, v, b.la, b.lb, pair_a_.as<int32_t>() + b.col0,
-                  pair_b_.as<int32_t>() + b.col0, b.npairs, dgp + k * lsz_gp_, dgm + k * lsz_gm_);
     batched(make_gemm('N', 'N', op, b.npairs, vp, 0.5, dtp, op, gp_.as<double>(), vp, 0.0,
                       dwp + op * b.col0, op), lsz_tp_, 0, lsz_wp_);
-    batched(make_gemm('N', 'N', op, b.npairs, vp, 0.5, tp_.as<double>(), op, dgp, vp, 1.0,
-                      dwp + op * b.col0, op), 0, lsz_gp_, lsz_wp_);
-    if (om > 0 && vm > 0) {
+    if (om > 0 && vm > 0)
       batched(make_gemm('N', 'N', om, b.npairs, vm, 0.5, dtm, om, gm_.as<double>(), vm, 0.0,
                         dwm + om * b.col0, om), lsz_tm_, 0, lsz_wm_);
-      batched(make_gemm('N', 'N', om, b.npairs, vm, 0.5, tm_.as<double>(), om, dgm, vm, 1.0,
-                        dwm + om * b.col0, om), 0, lsz_gm_, lsz_wm_);
-    }
   }
   for (int64_t k = 0; k < nb; ++k)
     ladder_gather(s, dwp + k * lsz_wp_, dwm + k * lsz_wm_, pair_col_.as<int32_t>(), dom2 + k * s_dom2,
                   o, v);
+  // T1 part: Z(a, b, i, j) = -dt1(a, k) W(k, bij), then (a, i, b, j), then symmetrised into packed
+  double* Z = lz1_.as<double>();
+  double* Zf = lz2_.as<double>();
+  batched(make_gemm('N', 'N', v, v * o * o, o, -1.0, dt2 - n_t1, v, lw_.as<double>(), o, 0.0, Z, v),
+          s_dt2, 0, N * N);
+  const int64_t dZ[4] = {v, v, o, o};
+  for (int64_t k = 0; k < nb; ++k) {
+    sort4(s, Z + k * N * N, dZ, "1324", Zf + k * N * N, 1.0, 0.0);
+    symmetrize_and_pack(s, Zf + k * N * N, dom2 + k * s_dom2, v, o, 1.0, 1.0, 0.0);
+  }
 }
 
 // Adjoint of the ladder term for nb vectors: from om2b + k s_om2b, accumulates
-// into t2b + k s_t2b and LTb + k s_Lb; g+/- recomputed once per block.
+// into t2b + k s_t2b (doubles) and the t1 adjoint ahead of it (T1 part through W);
+// the dressed vv block receives nothing.  g+/- recomputed once per block.
 void CCSD::ladder_adjoint(const double* t2, double* t2b, int64_t s_t2b, const double* LT, double* LTb,
                           int64_t s_Lb, const double* om2b, int64_t s_om2b, int64_t nb, Prec prec) {
-  if (t2b == nullptr || LTb == nullptr || om2b == nullptr) fail("ladder adjoint: missing derivative");
-  const int64_t o = n_o, v = n_v, J = n_J;
+  (void)LTb;
+  (void)s_Lb;
+  if (t2b == nullptr || om2b == nullptr) fail("ladder adjoint: missing derivative");
+  if (t2 != t_.as<double>() + n_t1 || LT != Lt_vvT_.as<double>() || !have_lw_)
+    fail("ladder adjoint: the record's ladder operands are required");
+  const int64_t o = n_o, v = n_v, J = n_J, N = n_t1;
   const int64_t op = o * (o + 1) / 2, om = o * (o - 1) / 2;
   const int64_t vp = v * (v + 1) / 2, vm = v * (v - 1) / 2;
   hipStream_t s = stream_;
@@ -707,9 +744,6 @@ void CCSD::ladder_adjoint(const double* t2, double* t2b, int64_t s_t2b, const do
   double* tmb = dtm_.as<double>();
   double* wpb = dwp_.as<double>();
   double* wmb = dwm_.as<double>();
-  double* dgb = dgblk_.as<double>();
-  double* dgp = dgp_.as<double>();
-  double* dgm = dgm_.as<double>();
   auto batched = [&](Gemm g, int64_t sA, int64_t sB, int64_t sC) {
     g.batch = nb;
     g.strideA = sA;
@@ -729,30 +763,26 @@ void CCSD::ladder_adjoint(const double* t2, double* t2b, int64_t s_t2b, const do
     X.gemm('T', 'N', v * b.la, v * b.lb, J, 1.0, LA, J, LB, J, 0.0, gblk_.as<double>(), v * b.la, prec);
     ladder_g_pm(s, gblk_.as<double>(), v, b.la, b.lb, pair_a_.as<int32_t>() + b.col0,
                 pair_b_.as<int32_t>() + b.col0, b.npairs, gp_.as<double>(), gm_.as<double>());
-    // w+ = 1/2 t+ g+  ->  t+_bar += 1/2 w+_bar g+^T,  g+_bar = 1/2 t+^T w+_bar
+    // w+ = 1/2 t+ g+  ->  t+_bar += 1/2 w+_bar g+^T
     batched(make_gemm('N', 'T', op, vp, b.npairs, 0.5, wpb + op * b.col0, op, gp_.as<double>(), vp,
                       1.0, tpb, op), lsz_wp_, 0, lsz_tp_);
-    batched(make_gemm('T', 'N', vp, b.npairs, op, 0.5, tp_.as<double>(), op, wpb + op * b.col0, op,
-                      0.0, dgp, vp), 0, lsz_wp_, lsz_gp_);
-    if (om > 0 && vm > 0) {
+    if (om > 0 && vm > 0)
       batched(make_gemm('N', 'T', om, vm, b.npairs, 0.5, wmb + om * b.col0, om, gm_.as<double>(), vm,
                         1.0, tmb, om), lsz_wm_, 0, lsz_tm_);
-      batched(make_gemm('T', 'N', vm, b.npairs, om, 0.5, tm_.as<double>(), om, wmb + om * b.col0, om,
-                        0.0, dgm, vm), 0, lsz_wm_, lsz_gm_);
-    } else {
-      fill(s, dgm, nb * lsz_gm_, 0.0);
-    }
-    for (int64_t k = 0; k < nb; ++k)
-      ladder_g_pm_adjoint(s, dgp + k * lsz_gp_, dgm + k * lsz_gm_, dgb + k * lsz_gblk_, v, b.a0, b.la,
-                          b.b0, b.lb, pair_col_.as<int32_t>(), b.col0);
-    // G = LA^T LB  ->  LA_bar += LB G_bar^T,  LB_bar += LA G_bar
-    batched(make_gemm('N', 'T', J, v * b.la, v * b.lb, 1.0, LB, J, dgb, v * b.la, 1.0,
-                      LTb + J * v * b.a0, J), 0, lsz_gblk_, s_Lb);
-    batched(make_gemm('N', 'N', J, v * b.lb, v * b.la, 1.0, LA, J, dgb, v * b.la, 1.0,
-                      LTb + J * v * b.b0, J), 0, lsz_gblk_, s_Lb);
   }
   for (int64_t k = 0; k < nb; ++k)
     ladder_t_pm_adjoint(s, tpb + k * lsz_tp_, tmb + k * lsz_tm_, t2b + k * s_t2b, o, v);
+  // T1 part: Zf_bar from the packed adjoint, to (a, b, i, j), then t1_bar -= Z_bar W^T
+  double* Zb = lz1_.as<double>();
+  double* Zfb = lz2_.as<double>();
+  fill(s, Zfb, nb * N * N, 0.0);
+  const int64_t dZf[4] = {v, o, v, o};
+  for (int64_t k = 0; k < nb; ++k) {
+    symmetrize_and_pack_adjoint(s, om2b + k * s_om2b, Zfb + k * N * N, v, o, 1.0, 1.0, 0.0);
+    sort4(s, Zfb + k * N * N, dZf, "1324", Zb + k * N * N, 1.0, 0.0);
+  }
+  batched(make_gemm('N', 'T', v, o, v * o * o, -1.0, Zb, v, lw_.as<double>(), o, 1.0, t2b - n_t1, v),
+          N * N, 0, s_t2b);
 }
 
 // ---------------------------------------------------------------------------
@@ -796,6 +826,7 @@ void CCSD::prepare_for_jacobian() {
   program_t1_cholesky(*tape);
   program_fock(*tape);
   program_omega(*tape, Prec::F64);
+  build_ladder_t1_intermediate();       // t and L_t1 of this record
   synchronize();
   drop_transform_graphs();
   tape_ = std::move(tape);
@@ -876,6 +907,7 @@ void CCSD::transform_body(int side, Prec prec, int64_t nb) {
 void CCSD::reset_record() {
   drop_transform_graphs();
   tape_.reset();
+  have_lw_ = false;
   nb_cap_ = 0;
   reserved_nb_ = 0;
 }

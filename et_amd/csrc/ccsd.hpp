@@ -151,10 +151,15 @@ class CCSD {
   std::vector<Block> blocks_;
   DeviceBuffer pair_a_, pair_b_, pair_col_, pcol_a_, pcol_b_;
   DeviceBuffer tp_, tm_, wp_, wm_, gblk_, gp_, gm_;
-  DeviceBuffer dtp_, dtm_, dwp_, dwm_, dgblk_, dgp_, dgm_;   // per derivative vector, x batch
-  int64_t lsz_tp_ = 0, lsz_tm_ = 0, lsz_wp_ = 0, lsz_wm_ = 0, lsz_gblk_ = 0, lsz_gp_ = 0, lsz_gm_ = 0;
+  DeviceBuffer dtp_, dtm_, dwp_, dwm_, lz1_, lz2_;   // per derivative vector, x batch
+  int64_t lsz_tp_ = 0, lsz_tm_ = 0, lsz_wp_ = 0, lsz_wm_ = 0;
   int64_t ladder_vector_elems() const;
   bool reserve_ladder_vectors(int64_t nb);
+  // W_kbij = sum_cd (sum_J L_kc L_t1,bd) t_cidj of the current record (T1 part of
+  // the ladder derivative)
+  DeviceBuffer lw_;
+  bool have_lw_ = false;
+  void build_ladder_t1_intermediate();
 
   // DIIS records: errors and parameters, dim x (n_t1 + n_t2) each
   int diis_dim_ = 0;
