@@ -135,6 +135,11 @@ Gemm tagged(Gemm g, uint64_t tag, bool on) {
 }
 }  // namespace
 
+double* CCSD::gemm_partials(int64_t elems) {
+  if (gws_cur_->partials.resize(sizeof(double) * size_t(std::max<int64_t>(1, elems)))) ++gws_cur_->gen;
+  return gws_cur_->partials.as<double>();
+}
+
 void CCSD::run_gemm(const Gemm& g, Prec prec) {
   hipEvent_t a = nullptr, b = nullptr;
   if (ledger_on_) {

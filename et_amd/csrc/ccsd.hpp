@@ -111,6 +111,8 @@ class CCSD {
 
   // used by the ops layer
   void run_gemm(const Gemm& g, Prec prec);
+  // `elems` doubles in the current product workspace (moves count as a workspace move)
+  double* gemm_partials(int64_t elems);
   double* scratch(int slot, int64_t elems, const char* name);
   void ladder_primal(const double* t2, const double* LvvT, double* om2, Prec prec);
   // derivatives for nb vectors, vector k at pointer + k * stride

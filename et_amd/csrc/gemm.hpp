@@ -49,6 +49,7 @@ struct PackedOperand {
 // captured into a graph hold these addresses).
 struct GemmWorkspace {
   DeviceBuffer buf, packA, packB;
+  DeviceBuffer partials;             // per-entry products summed by the caller
   static constexpr int kCached = 4;
   PackedOperand cached[kCached];
   uint64_t clock = 0;

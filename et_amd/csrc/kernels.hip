@@ -665,9 +665,27 @@ __global__ void copy_sub_kernel2(const double* src, int64_t lds, double* dst, in
   }
 }
 
+// dst(m x n, ld ldd) += sum_b part[b m n + r + m c], b < nbatch
+__global__ void batch_sum_acc_kernel(const double* __restrict__ part, int64_t nbatch, int64_t m,
+                                     int64_t n, double* __restrict__ dst, int64_t ldd) {
+  const int64_t mn = m * n;
+  GRID_LOOP(k, mn) {
+    double v = 0.0;
+    for (int64_t b = 0; b < nbatch; ++b) v += part[k + b * mn];
+    dst[(k % m) + ldd * (k / m)] += v;
+  }
+}
+
 inline unsigned g2(int64_t n) { return grid_for(n); }
 
 }  // namespace
+
+void batch_sum_acc(hipStream_t s, const double* part, int64_t nbatch, int64_t m, int64_t n, double* dst,
+                   int64_t ldd) {
+  if (m * n == 0) return;
+  batch_sum_acc_kernel<<<g2(m * n), kBlock, 0, s>>>(part, nbatch, m, n, dst, ldd);
+  ETG_LAUNCH("batch_sum_acc");
+}
 
 void scale(hipStream_t s, double* x, int64_t n, double beta) {
   if (n == 0 || beta == 1.0) return;

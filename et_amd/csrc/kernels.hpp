@@ -109,6 +109,9 @@ void fock_ov_vector_adjoint(hipStream_t s, const double* fb, double* Fb, int64_t
 // e = -residual / eps (x diag_factor on the packed doubles diagonal); x += e
 void quasi_newton_step(hipStream_t s, const double* eps, const double* residual, double* x, double* e,
                        int64_t n_o, int64_t n_v, int64_t n, double diag_factor);
+// dst (m x n, ld ldd) += sum over b < nbatch of the contiguous m x n blocks part_b
+void batch_sum_acc(hipStream_t s, const double* part, int64_t nbatch, int64_t m, int64_t n, double* dst,
+                   int64_t ldd);
 // dst (m x n) = alpha * src + beta * dst
 void copy_sub(hipStream_t s, const double* src, int64_t lds, double* dst, int64_t ldd, int64_t m,
               int64_t n, double alpha, double beta);

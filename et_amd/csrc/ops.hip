@@ -346,7 +346,7 @@ void Tape::adjoint_gemm(const Gemm& g, Prec prec, int64_t nb) {
     const double* Cb = Cb0 + v * sC;
     double* Ab = Ab0 ? Ab0 + v * sA : nullptr;
     double* Bb = Bb0 ? Bb0 + v * sB : nullptr;
-    // product b0 .. b0 + nbat of the recorded batch
+    // adjoint product for entries b0 .. b0 + nbat of the recorded batch
     auto pair = [&](bool for_A, int64_t b0, int64_t nbat) {
       Gemm h;
       h.batch = nbat;
@@ -390,21 +390,35 @@ void Tape::adjoint_gemm(const Gemm& g, Prec prec, int64_t nb) {
         h.strideB = (h.B == C) ? sC : 0;
         h.strideC = for_A ? sA : sB;
       }
+      return h;
+    };
+    // An operand shared by every entry of a recorded batch (stride 0) collects
+    // the sum of the entries' adjoints: one batched product into per-entry
+    // partials, then their sum (or entry by entry when the partials would be large).
+    auto shared = [&](bool for_A) {
+      Gemm h = pair(for_A, 0, g.batch);
+      const int64_t mn = h.m * h.n;
+      if (double(g.batch) * double(mn) > double(int64_t(1) << 27)) {
+        for (int64_t b = 0; b < g.batch; ++b) wf_.run_gemm(pair(for_A, b, 1), prec);
+        return;
+      }
+      double* part = wf_.gemm_partials(g.batch * mn);
+      double* out = h.C;
+      const int64_t ldo = h.ldc;
+      h.C = part;
+      h.ldc = h.m;
+      h.strideC = mn;
+      h.beta = 0.0;
       wf_.run_gemm(h, prec);
+      batch_sum_acc(wf_.stream(), part, g.batch, h.m, h.n, out, ldo);
     };
     if (Ab) {
-      if (g.batch > 1 && g.strideA == 0) {
-        for (int64_t b = 0; b < g.batch; ++b) pair(true, b, 1);
-      } else {
-        pair(true, 0, g.batch);
-      }
+      if (g.batch > 1 && g.strideA == 0) shared(true);
+      else wf_.run_gemm(pair(true, 0, g.batch), prec);
     }
     if (Bb) {
-      if (g.batch > 1 && g.strideB == 0) {
-        for (int64_t b = 0; b < g.batch; ++b) pair(false, b, 1);
-      } else {
-        pair(false, 0, g.batch);
-      }
+      if (g.batch > 1 && g.strideB == 0) shared(false);
+      else wf_.run_gemm(pair(false, 0, g.batch), prec);
     }
     if (g.beta != 1.0) {
       Gemm z = g;
