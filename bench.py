@@ -247,6 +247,34 @@ def eom_time_to_solution(copies=4, n_states=12, threshold=1e-6):
     return out
 
 
+def transforms_profile(side, precision, copies=4, n_vec=12):
+    """One group of n_vec right or left transformations of `copies` H2O-He copies
+    (eager passes, graphs off), after a short FP64 ground state: the launch
+    sequence the EOM Davidson replays, for rocprofv3 kernel traces."""
+    from et_amd import solvers, workloads
+    prec = ccsd.PRECISIONS[precision]
+    r = workloads.replicate(workloads.h2o_he_inputs(), copies)
+    n_o = r['n_o']
+    n_v = r['L_mo'].shape[1] - n_o
+    with ccsd.CCSD(n_o, n_v, r['n_J']) as wf:
+        wf.set_cholesky(r['L_mo'])
+        wf.set_oei(r['h_mo'])
+        wf.set_orbital_energies(r['eps'])
+        del r
+        solvers.DiisCCGroundState(wf, residual_threshold=1e-4).run()
+        wf.set_transform_graphs(False)
+        wf.prepare_for_jacobian(side)
+        X = np.random.default_rng(0).standard_normal((wf.n_gs_amplitudes, n_vec))
+        wf.jacobian_transformations(X, side, prec)            # sizes the buffers
+        wf.synchronize()
+        t0 = time.perf_counter()
+        wf.jacobian_transformations(X, side, prec)
+        wf.synchronize()
+        dt = time.perf_counter() - t0
+    print(json.dumps({'transforms_profile': side, 'precision': precision, 'copies': copies, 'n_vec': n_vec,
+                      'ms_per_transform': dt / n_vec * 1e3}), flush=True)
+
+
 def copy_calibration(nbytes=1 << 30, reps=3):
     """Plain copies of known size: each reads and writes `nbytes` of HBM once.
 
@@ -273,11 +301,17 @@ def main():
     ap.add_argument('--no-mixed', action='store_true')
     ap.add_argument('--no-tts', action='store_true', help='skip the time-to-solution comparison')
     ap.add_argument('--no-jacobian', action='store_true', help='skip the Jacobian transformation rates')
+    ap.add_argument('--transforms-profile', nargs=2, metavar=('SIDE', 'PRECISION'),
+                    help='only run two groups of 12 EOM transformations (right|left, fp64|fp32|bf16x3) '
+                         'on four H2O-He copies, graphs off (for kernel traces)')
     ap.add_argument('--copy-calibration', action='store_true',
                     help='only run three 1 GiB device-to-device copies (HBM counter calibration)')
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.transforms_profile:
+        transforms_profile(*args.transforms_profile)
         return
     if args.gpus != 1:
         print('bench.py measures one GPU: every box in this setup has a single MI355X, so '
