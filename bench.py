@@ -141,7 +141,8 @@ def time_to_solution(copies=8, threshold=1e-10):
         wf.set_orbital_energies(r['eps'])
         del r
         wf.set_initial_amplitudes_guess()
-        wf.construct_omega(ccsd.BF16X3, copy_out=False)      # size the workspaces once
+        for p in (ccsd.FP64, ccsd.FP32, ccsd.BF16X3):        # size workspaces, choose tiles once
+            wf.construct_omega(p, copy_out=False)
         results = {}
         for name in ('fp64', 'mixed'):
             wf.synchronize()
@@ -211,7 +212,11 @@ def eom_time_to_solution(copies=4, n_states=12, threshold=1e-6):
         solvers.DiisCCGroundState(wf, residual_threshold=1e-10).run()
         for side in ('right', 'left'):
             wf.prepare_for_jacobian(side)
-            wf.jacobian_transformations(np.ones((wf.n_gs_amplitudes, 1)), side, ccsd.BF16X3)
+            # size the group buffers and choose tiles for every group size before timing
+            ones = np.ones((wf.n_gs_amplitudes, n_states))
+            for p in (ccsd.FP64, ccsd.BF16X3):
+                for nv in (n_states, 8, 4, 2, 1):
+                    wf.jacobian_transformations(ones[:, :nv], side, p)
             res = {}
             for name in ('fp64', 'mixed'):
                 wf.synchronize()

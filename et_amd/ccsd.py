@@ -79,6 +79,7 @@ _SIGNATURES = {
                           _c_dbl_p, _c_dbl_p, _c_dbl_p, ctypes.POINTER(_i64)],
     'etg_test_gemm': [ctypes.c_char, ctypes.c_char, _i64, _i64, _i64, ctypes.c_double, _c_dbl_p,
                       _i64, _c_dbl_p, _i64, ctypes.c_double, _c_dbl_p, _i64, ctypes.c_int],
+    'etg_set_gemm_autotune': [ctypes.c_int, ctypes.POINTER(_i64)],
     'etg_bench_gemm': [ctypes.c_char, ctypes.c_char, _i64, _i64, _i64, ctypes.c_int, ctypes.c_int,
                        _c_dbl_p],
 }
@@ -132,6 +133,14 @@ def reduced_eigen(G, n_want):
     X = np.empty((k, max(n_want, 1)), order='F')
     _check(lib.etg_test_reduced_eigen(k, _ptr(G), n_want, _ptr(wr), _ptr(wi), _ptr(X)))
     return wr, wi, X[:, :n_want]
+
+
+def set_gemm_autotune(on):
+    """Measured tile choice of the FP64/FP32 products on or off (process-wide);
+    returns the number of product shapes measured so far."""
+    n = _i64()
+    _check(load_library().etg_set_gemm_autotune(1 if on else 0, ctypes.byref(n)))
+    return n.value
 
 
 def bench_gemm(m, n, k, precision=FP64, reps=5, ta='N', tb='N'):

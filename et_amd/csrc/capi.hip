@@ -362,4 +362,11 @@ int etg_bench_gemm(char ta, char tb, int64_t m, int64_t n, int64_t k, int precis
   });
 }
 
+int etg_set_gemm_autotune(int on, int64_t* tuned) {
+  return guarded([&] {
+    etg::set_gemm_autotune(on != 0);
+    if (tuned) *tuned = etg::gemm_tuned_shapes();
+  });
+}
+
 }  // extern "C"

@@ -20,7 +20,12 @@
 #include "gemm.hpp"
 
 #include <algorithm>
+#include <atomic>
+#include <map>
+#include <mutex>
 #include <string>
+#include <tuple>
+#include <vector>
 
 namespace etg {
 namespace {
@@ -350,7 +355,8 @@ struct Choice {
   int bm = 128, splits = 1;
 };
 
-Choice choose_tiling(const Gemm& g, double peak, bool can_split) {
+// only_bm != 0 restricts the search to that tile size
+Choice choose_tiling(const Gemm& g, double peak, bool can_split, int only_bm = 0) {
   struct Cfg { int bm; int64_t slots; double eff; };
   const Cfg cfgs[2] = {{128, 512, 0.83}, {64, 1024, 0.62}};
   const double hbm = 5.0e12;                       // sustained bytes/s for the split-K round trip
@@ -358,6 +364,7 @@ Choice choose_tiling(const Gemm& g, double peak, bool can_split) {
   Choice best;
   double best_t = 1e300;
   for (const Cfg& c : cfgs) {
+    if (only_bm != 0 && c.bm != only_bm) continue;
     const int64_t tiles = ceil_div(g.m, c.bm) * ceil_div(g.n, c.bm) * g.batch;
     static const int kSplits[] = {1,  2,  3,  4,  5,  6,  7,  8,  9,   10,  11,  12,  13,  14,  16,
                                   18, 20, 24, 28, 32, 40, 48, 64, 96, 128, 192, 256, 384, 512, 1024};
@@ -380,8 +387,9 @@ Choice choose_tiling(const Gemm& g, double peak, bool can_split) {
   return best;
 }
 
+// Runs the product with a given tile size and split-K count.
 template <typename T>
-GemmLaunch launch_typed(hipStream_t stream, const Gemm& g, GemmWorkspace* ws) {
+GemmLaunch run_choice(hipStream_t stream, const Gemm& g, GemmWorkspace* ws, Choice ch) {
   Params p{};
   p.m = g.m; p.n = g.n; p.k = g.k;
   p.alpha = g.alpha; p.beta = g.beta;
@@ -390,13 +398,6 @@ GemmLaunch launch_typed(hipStream_t stream, const Gemm& g, GemmWorkspace* ws) {
   p.C = g.C; p.ldc = g.ldc; p.strideC = g.strideC;
   p.W = nullptr;
   p.k_per_split = g.k;
-
-  // Tile size and split-K count from a cost model: a launch runs in rounds of
-  // `slots` concurrent tiles (two 128 x 128 or four 64 x 64 workgroups per CU
-  // on 256 CUs), each round at the tile's sustained matrix-core efficiency;
-  // a split adds its partial sums' round trip through HBM.
-  const Choice ch = choose_tiling(g, sizeof(T) == 8 ? 78.6e12 : 157.3e12, ws != nullptr);
-  const bool big = ch.bm == 128;
   int splits = ch.splits;
   if (splits > 1) {
     const int64_t kps = ceil_div(ceil_div(g.k, splits), 16) * 16;
@@ -408,7 +409,7 @@ GemmLaunch launch_typed(hipStream_t stream, const Gemm& g, GemmWorkspace* ws) {
   }
   GemmLaunch info;
   info.splits = splits;
-  if (big) info.kernel = launch_tile<T, 128, 128, 16>(stream, g, p, splits);
+  if (ch.bm == 128) info.kernel = launch_tile<T, 128, 128, 16>(stream, g, p, splits);
   else info.kernel = launch_tile<T, 64, 64, 16>(stream, g, p, splits);
   if (splits > 1) {
     const int64_t mn = g.m * g.n;
@@ -420,7 +421,113 @@ GemmLaunch launch_typed(hipStream_t stream, const Gemm& g, GemmWorkspace* ws) {
   return info;
 }
 
+// ---- measured tile choice ---------------------------------------------------
+// The cost model below ranks tile size and split-K count; the first time a
+// product shape is seen outside graph capture, the model's pick and its
+// neighbours (the other tile size's best, half and twice the split count) are
+// timed on the real operands into a scratch output, and the fastest is kept
+// for the rest of the process.  Products under 2 GFLOP (tens of microseconds)
+// and shapes whose output exceeds 1 GiB keep the model's pick; batch counts
+// share a choice within a power-of-two bucket (the trial-vector groups of a
+// Davidson solve shrink one by one).
+struct TuneKey {
+  int prec;
+  bool ta, tb, wide, shared_a, shared_b;
+  int64_t m, n, k, batch;
+  bool operator<(const TuneKey& o) const {
+    return std::tie(prec, ta, tb, wide, shared_a, shared_b, m, n, k, batch) <
+           std::tie(o.prec, o.ta, o.tb, o.wide, o.shared_a, o.shared_b, o.m, o.n, o.k, o.batch);
+  }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, Choice> g_tuned;
+std::atomic<bool> g_autotune{true};
+
+bool capturing(hipStream_t s) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  ETG_HIP(hipStreamIsCapturing(s, &st));
+  return st != hipStreamCaptureStatusNone;
+}
+
+template <typename T>
+Choice tuned_choice(hipStream_t stream, const Gemm& g, GemmWorkspace* ws, Choice model, double peak) {
+  if (ws == nullptr || !g_autotune.load()) return model;
+  if (2.0 * double(g.m) * double(g.n) * double(g.k) * double(g.batch) < 2.0e9) return model;
+  int64_t bucket = 1;
+  while (bucket < g.batch) bucket <<= 1;
+  const TuneKey key{int(sizeof(T)), g.ta, g.tb, std::max(g.lda, g.ldb) >= (int64_t(1) << 21),
+                    g.batch > 1 && g.strideA == 0, g.batch > 1 && g.strideB == 0, g.m, g.n, g.k, bucket};
+  {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    auto it = g_tuned.find(key);
+    if (it != g_tuned.end()) return it->second;
+  }
+  const double out_bytes = 8.0 * double(g.m) * double(g.n) * double(g.batch);
+  if (capturing(stream) || out_bytes > double(int64_t(1) << 30)) return model;
+  std::vector<Choice> cand{model};
+  auto add = [&](Choice c) {
+    for (const Choice& e : cand)
+      if (e.bm == c.bm && e.splits == c.splits) return;
+    cand.push_back(c);
+  };
+  add(choose_tiling(g, peak, true, model.bm == 128 ? 64 : 128));
+  if (model.splits > 1) add(Choice{model.bm, std::max(1, model.splits / 2)});
+  if (g.k >= int64_t(2 * model.splits) * 256 && 2.0 * model.splits * out_bytes <= 2.0e9)
+    add(Choice{model.bm, 2 * model.splits});
+  Choice best = model;
+  if (cand.size() > 1) {
+    if (ws->tune.resize(size_t(out_bytes))) ++ws->gen;
+    Gemm t = g;
+    t.C = ws->tune.as<double>();
+    t.ldc = g.m;
+    t.strideC = g.m * g.n;
+    t.beta = 0.0;
+    std::vector<hipEvent_t> ev(2 * cand.size());
+    for (auto& e : ev) ETG_HIP(hipEventCreate(&e));
+    run_choice<T>(stream, t, ws, cand[0]);            // warm caches and page tables
+    for (size_t c = 0; c < cand.size(); ++c) {
+      ETG_HIP(hipEventRecord(ev[2 * c], stream));
+      run_choice<T>(stream, t, ws, cand[c]);
+      ETG_HIP(hipEventRecord(ev[2 * c + 1], stream));
+    }
+    ETG_HIP(hipEventSynchronize(ev.back()));
+    float best_ms = 0.0f;
+    for (size_t c = 0; c < cand.size(); ++c) {
+      float ms = 0.0f;
+      ETG_HIP(hipEventElapsedTime(&ms, ev[2 * c], ev[2 * c + 1]));
+      if (c == 0 || ms < best_ms) {
+        best_ms = ms;
+        best = cand[c];
+      }
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+  }
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  g_tuned[key] = best;
+  return best;
+}
+
+template <typename T>
+GemmLaunch launch_typed(hipStream_t stream, const Gemm& g, GemmWorkspace* ws) {
+  // Tile size and split-K count from a cost model: a launch runs in rounds of
+  // `slots` concurrent tiles (two 128 x 128 or four 64 x 64 workgroups per CU
+  // on 256 CUs), each round at the tile's sustained matrix-core efficiency;
+  // a split adds its partial sums' round trip through HBM.  The model's pick
+  // is then checked against its neighbours by measurement (tuned_choice).
+  const double peak = sizeof(T) == 8 ? 78.6e12 : 157.3e12;
+  Choice ch = choose_tiling(g, peak, ws != nullptr);
+  ch = tuned_choice<T>(stream, g, ws, ch, peak);
+  return run_choice<T>(stream, g, ws, ch);
+}
+
 }  // namespace
+
+void set_gemm_autotune(bool on) { g_autotune.store(on); }
+
+int64_t gemm_tuned_shapes() {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  return int64_t(g_tuned.size());
+}
 
 GemmLaunch gemm(hipStream_t stream, const Gemm& g, Prec prec, GemmWorkspace* ws) {
   if (g.m < 0 || g.n < 0 || g.k < 0 || g.batch < 1) fail("gemm: negative dimension");

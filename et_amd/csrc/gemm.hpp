@@ -50,6 +50,7 @@ struct PackedOperand {
 struct GemmWorkspace {
   DeviceBuffer buf, packA, packB;
   DeviceBuffer partials;             // per-entry products summed by the caller
+  DeviceBuffer tune;                 // scratch output of the tile measurements
   static constexpr int kCached = 4;
   PackedOperand cached[kCached];
   uint64_t clock = 0;
@@ -61,6 +62,12 @@ struct GemmLaunch {
   const char* kernel = "";   // e.g. "gemm_mfma16<f64,NN,128x128>"
   int splits = 1;            // > 1: split-K partial sums + reduction kernel
 };
+
+// Measured tile choice for FP64/FP32 products (on by default, process-wide):
+// the first product of a shape outside graph capture times the cost model's
+// pick against its neighbours and keeps the fastest.
+void set_gemm_autotune(bool on);
+int64_t gemm_tuned_shapes();
 
 // Launches the product on `stream`.  Validates shapes and leading dimensions
 // on the host (throws etg::Error on inconsistent arguments).

@@ -281,6 +281,14 @@ int etg_test_reduced_eigen(int64_t k, const double* G, int64_t n_want, double* w
 int etg_bench_gemm(char ta, char tb, int64_t m, int64_t n, int64_t k, int precision, int reps,
                    double* seconds);
 
+/* Tuning, process-wide: measured tile choice for the FP64/FP32 products (on by
+ * default).  The first product of a shape outside graph capture times the
+ * cost model's tile size and split-K count against its neighbours (the other
+ * tile size's best, half and twice the split count) on the real operands and
+ * keeps the fastest; off keeps the model's pick.  `tuned` (may be NULL)
+ * receives the number of shapes measured so far. */
+int etg_set_gemm_autotune(int on, int64_t* tuned);
+
 #ifdef __cplusplus
 }
 #endif

@@ -88,3 +88,29 @@ def test_bf16x3_gemm(ta, tb, m, n, k):
     ref = 0.7 * opA @ opB - 0.3 * C
     bound = 0.7 * (4 * 2.0 ** -16 + k * 2.0 ** -24) * (np.abs(opA) @ np.abs(opB)) + 1e-15 * np.abs(C)
     assert np.all(np.abs(out - ref) <= bound)
+
+
+@pytest.mark.parametrize('ta,tb,m,n,k', [('N', 'N', 48, 48, 4000), ('T', 'N', 300, 200, 500),
+                                         ('N', 'T', 640, 520, 130), ('T', 'T', 129, 257, 3000)])
+def test_measured_tile_choice_keeps_results(ta, tb, m, n, k):
+    """The measured tile choice times candidates into a scratch output: the product itself
+    (with beta != 0) is the same as with the cost model's pick, to FP64 rounding."""
+    rng = np.random.default_rng(m + 3 * n + 7 * k)
+    A = rng.normal(size=(k, m) if ta == 'T' else (m, k))
+    B = rng.normal(size=(n, k) if tb == 'T' else (k, n))
+    C = rng.normal(size=(m, n))
+    ref = 0.7 * (A.T if ta == 'T' else A) @ (B.T if tb == 'T' else B) - 0.3 * C
+    scale = 4e-14 * (np.sqrt(k) + 1.0)
+    try:
+        ccsd.set_gemm_autotune(False)
+        model = ccsd.device_gemm(ta, tb, 0.7, A, B, -0.3, C)
+        before = ccsd.set_gemm_autotune(True)
+        tuned = ccsd.device_gemm(ta, tb, 0.7, A, B, -0.3, C)
+        again = ccsd.device_gemm(ta, tb, 0.7, A, B, -0.3, C)
+        after = ccsd.set_gemm_autotune(True)
+    finally:
+        ccsd.set_gemm_autotune(True)
+    assert np.abs(model - ref).max() < scale
+    assert np.abs(tuned - ref).max() < scale
+    assert np.array_equal(tuned, again)          # the kept choice is reused
+    assert after >= before
