@@ -59,7 +59,8 @@ def build(verbose=True):
                 print(m, file=sys.stderr)
     objs = [o for o, _ in results]
     tmp = LIB + '.tmp'
-    cmd = [hipcc(), f'--offload-arch={ARCH}', '-shared', '-fPIC', '-o', tmp, *objs]
+    cmd = [hipcc(), f'--offload-arch={ARCH}', '-shared', '-fPIC', '-o', tmp, *objs,
+           '-L/opt/rocm/lib', '-lrocsolver', '-lrocblas', '-Wl,-rpath,/opt/rocm/lib']
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError('link failed: %s\n%s%s' % (' '.join(cmd), res.stdout, res.stderr))

@@ -80,6 +80,15 @@ _SIGNATURES = {
     'etg_test_gemm': [ctypes.c_char, ctypes.c_char, _i64, _i64, _i64, ctypes.c_double, _c_dbl_p,
                       _i64, _c_dbl_p, _i64, ctypes.c_double, _c_dbl_p, _i64, ctypes.c_int],
     'etg_set_gemm_autotune': [ctypes.c_int, ctypes.POINTER(_i64)],
+    'etg_molecule_create': [_i64, _c_dbl_p, _c_dbl_p, _i64, ctypes.POINTER(_i64), ctypes.POINTER(_i64),
+                            ctypes.POINTER(_i64), _c_dbl_p, _c_dbl_p, _i64, ctypes.POINTER(ctypes.c_void_p)],
+    'etg_molecule_run': [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, _i64],
+    'etg_molecule_dims': [ctypes.c_void_p, ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_i64),
+                          ctypes.POINTER(_i64)],
+    'etg_molecule_results': [ctypes.c_void_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p],
+    'etg_molecule_one_electron': [ctypes.c_void_p, _c_dbl_p, _c_dbl_p, _c_dbl_p],
+    'etg_molecule_ao_eri': [ctypes.c_void_p, _c_dbl_p],
+    'etg_molecule_destroy': [ctypes.c_void_p],
     'etg_bench_gemm': [ctypes.c_char, ctypes.c_char, _i64, _i64, _i64, ctypes.c_int, ctypes.c_int,
                        _c_dbl_p],
 }

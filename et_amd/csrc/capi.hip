@@ -8,9 +8,14 @@
 
 #include "../../include/etgpu.h"
 #include "ccsd.hpp"
+#include "frontend.hpp"
 
 struct etg_handle {
   etg::CCSD* wf;
+};
+
+struct etg_molecule {
+  etg::Molecule* m;
 };
 
 namespace {
@@ -34,6 +39,11 @@ int guarded(F&& f) {
 etg::CCSD& wf_of(etg_handle* h) {
   if (h == nullptr || h->wf == nullptr) etg::fail("null etg handle");
   return *h->wf;
+}
+
+etg::Molecule& mol_of(etg_molecule* h) {
+  if (h == nullptr || h->m == nullptr) etg::fail("null molecule handle");
+  return *h->m;
 }
 
 etg::Prec prec_of(int p) {
@@ -359,6 +369,66 @@ int etg_bench_gemm(char ta, char tb, int64_t m, int64_t n, int64_t k, int precis
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     *seconds = double(ms) * 1e-3 / reps;
+  });
+}
+
+int etg_molecule_create(int64_t n_atoms, const double* charges, const double* xyz_bohr, int64_t n_shells,
+                        const int64_t* shell_atom, const int64_t* shell_l, const int64_t* shell_nprim,
+                        const double* exponents, const double* coefficients, int64_t n_electrons,
+                        etg_molecule** molecule) {
+  return guarded([&] {
+    if (molecule == nullptr) etg::fail("etg_molecule_create: null handle pointer");
+    *molecule = nullptr;
+    if (!charges || !xyz_bohr || !shell_atom || !shell_l || !shell_nprim || !exponents || !coefficients)
+      etg::fail("etg_molecule_create: null array");
+    etg::Molecule* m = new etg::Molecule(n_atoms, charges, xyz_bohr, n_shells, shell_atom, shell_l, shell_nprim,
+                                         exponents, coefficients, n_electrons);
+    *molecule = new etg_molecule{m};
+  });
+}
+
+int etg_molecule_run(etg_molecule* molecule, double cholesky_threshold, double scf_threshold,
+                     int64_t max_scf_iterations) {
+  return guarded([&] { mol_of(molecule).run(cholesky_threshold, scf_threshold, max_scf_iterations); });
+}
+
+int etg_molecule_dims(etg_molecule* molecule, int64_t* n_ao, int64_t* n_occ, int64_t* n_J,
+                      int64_t* scf_iterations) {
+  return guarded([&] {
+    etg::Molecule& m = mol_of(molecule);
+    if (n_ao) *n_ao = m.n_ao();
+    if (n_occ) *n_occ = m.n_occ();
+    if (n_J) *n_J = m.n_J();
+    if (scf_iterations) *scf_iterations = m.scf_iterations();
+  });
+}
+
+int etg_molecule_results(etg_molecule* molecule, double* e_nuc, double* e_hf, double* eps, double* h_mo,
+                         double* L_mo, double* C) {
+  return guarded([&] { mol_of(molecule).results(e_nuc, e_hf, eps, h_mo, L_mo, C); });
+}
+
+int etg_molecule_one_electron(etg_molecule* molecule, double* S, double* T, double* V) {
+  return guarded([&] {
+    if (!S || !T || !V) etg::fail("etg_molecule_one_electron: null array");
+    mol_of(molecule).one_electron_integrals(S, T, V);
+  });
+}
+
+int etg_molecule_ao_eri(etg_molecule* molecule, double* G) {
+  return guarded([&] {
+    if (!G) etg::fail("etg_molecule_ao_eri: null array");
+    etg::Molecule& m = mol_of(molecule);
+    m.build_eri();
+    m.copy_eri(G);
+  });
+}
+
+int etg_molecule_destroy(etg_molecule* molecule) {
+  return guarded([&] {
+    if (molecule == nullptr) return;
+    delete molecule->m;
+    delete molecule;
   });
 }
 

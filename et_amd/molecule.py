@@ -1,0 +1,170 @@
+"""Molecular producer of the CCSD inputs over libetgpu.so: AO integrals -> RHF
+-> pivoted Cholesky -> MO Cholesky vectors, all on the device.
+
+Replaces the producers eT runs before its CC solvers: the Libint integral
+interface (src/libint/eri.cpp:33-89), the RHF solver
+(src/solvers/hf/scf_solver_class.F90) and the Cholesky decomposition of the
+AO integrals with the AO->MO transformation
+(src/solvers/cholesky/eri_cd_class.F90:241, :3132).  ``Molecule.run`` returns
+the arrays ``ccsd.CCSD`` takes (L_mo in eT's (J, p, q) layout, h_mo, orbital
+energies) together with E_nuc, E_HF and the MO coefficients.
+
+Also here: the cc-pVDZ basis (et_amd/basis/cc-pvdz.json, from the reference's
+ao_basis/cc-pvdz.g94) and the seeded water lattices of the (H2O)_n systems.
+"""
+import ctypes
+import json
+import math
+import os
+
+import numpy as np
+
+from . import ccsd as _ccsd
+
+# eT's Bohr radius in Angstrom (CODATA 2014, src/various/parameters.F90:105)
+BOHR_TO_ANGSTROM = 0.52917721067
+Z_OF = {'H': 1, 'He': 2, 'Li': 3, 'Be': 4, 'B': 5, 'C': 6, 'N': 7, 'O': 8, 'F': 9, 'Ne': 10}
+_BASIS_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'basis')
+
+# the water molecule of the (H2O)_n systems (Angstrom): O on the z axis, H in the yz plane
+WATER = [('O', (0.0, 0.0, 0.1173)), ('H', (0.0, 0.7572, -0.4692)), ('H', (0.0, -0.7572, -0.4692))]
+
+
+def load_basis(name='cc-pvdz'):
+    with open(os.path.join(_BASIS_DIR, name + '.json')) as f:
+        return json.load(f)['basis']
+
+
+def _splitmix64(state):
+    state = (state + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+    z = state
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+    return state, z ^ (z >> 31)
+
+
+class _Uniform:
+    """Uniform doubles in [0, 1) from a splitmix64 stream of `seed` (53-bit mantissas)."""
+
+    def __init__(self, seed):
+        self.state = int(seed) & 0xFFFFFFFFFFFFFFFF
+
+    def __call__(self):
+        self.state, z = _splitmix64(self.state)
+        return (z >> 11) * (1.0 / (1 << 53))
+
+
+def random_rotation(u):
+    """Uniformly distributed rotation from three uniforms (unit quaternion of
+    K. Shoemake, "Uniform random rotations", Graphics Gems III, 1992)."""
+    u1, u2, u3 = u(), u(), u()
+    a, b = math.sqrt(1.0 - u1), math.sqrt(u1)
+    x, y = a * math.sin(2 * math.pi * u2), a * math.cos(2 * math.pi * u2)
+    z, w = b * math.sin(2 * math.pi * u3), b * math.cos(2 * math.pi * u3)
+    return np.array([
+        [1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+        [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+        [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)],
+    ])
+
+
+def water_lattice(shape=(2, 2, 2), spacing=2.9, seed=0, min_distance=1.5, max_draws=1000):
+    """(H2O)_n on a cubic lattice: each water (geometry WATER, rotated about its
+    origin) is placed on a lattice point, x fastest, with a uniform random
+    rotation drawn from the seed; a rotation that brings any atom closer than
+    `min_distance` Angstrom to an atom of an earlier molecule is redrawn.
+    Returns [(symbol, (x, y, z) Angstrom)]."""
+    u = _Uniform(seed)
+    atoms, placed = [], []
+    for iz in range(shape[2]):
+        for iy in range(shape[1]):
+            for ix in range(shape[0]):
+                origin = spacing * np.array([ix, iy, iz], dtype=float)
+                for _ in range(max_draws):
+                    R = random_rotation(u)
+                    mol = [(s, origin + R @ np.array(x)) for s, x in WATER]
+                    if all(np.linalg.norm(p - q) >= min_distance for _, p in mol for q in placed):
+                        break
+                else:
+                    raise RuntimeError('water_lattice: no admissible rotation')
+                placed.extend(p for _, p in mol)
+                atoms.extend((s, tuple(p)) for s, p in mol)
+    return atoms
+
+
+class Molecule:
+    """A closed-shell molecule in a Gaussian basis on the device (libetgpu.so)."""
+
+    def __init__(self, atoms, basis=None, bohr_to_angstrom=BOHR_TO_ANGSTROM, charge=0):
+        basis = load_basis() if basis is None else basis
+        self._lib = _ccsd.load_library()
+        Z, xyz, sh_atom, sh_l, sh_np, exps, coefs = [], [], [], [], [], [], []
+        for ia, (sym, x) in enumerate(atoms):
+            Z.append(float(Z_OF[sym]))
+            xyz.extend(float(c) / bohr_to_angstrom for c in x)
+            for sh in basis[sym]:
+                sh_atom.append(ia)
+                sh_l.append(int(sh['l']))
+                sh_np.append(len(sh['exps']))
+                exps.extend(sh['exps'])
+                coefs.extend(sh['coefs'])
+        self.charges = np.array(Z)
+        self.xyz_bohr = np.array(xyz).reshape(-1, 3)
+        n_el = int(round(sum(Z))) - int(charge)
+        arr = lambda v, t: np.ascontiguousarray(v, dtype=t)
+        self._keep = [arr(Z, np.float64), arr(xyz, np.float64), arr(sh_atom, np.int64), arr(sh_l, np.int64),
+                      arr(sh_np, np.int64), arr(exps, np.float64), arr(coefs, np.float64)]
+        i64p = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+        h = ctypes.c_void_p()
+        k = self._keep
+        _ccsd._check(self._lib.etg_molecule_create(len(atoms), _ccsd._ptr(k[0]), _ccsd._ptr(k[1]), len(sh_l),
+                                                   i64p(k[2]), i64p(k[3]), i64p(k[4]), _ccsd._ptr(k[5]),
+                                                   _ccsd._ptr(k[6]), n_el, ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, '_h', None) is not None and self._h.value:
+            self._lib.etg_molecule_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def dims(self):
+        v = [ctypes.c_int64() for _ in range(4)]
+        _ccsd._check(self._lib.etg_molecule_dims(self._h, *[ctypes.byref(x) for x in v]))
+        return dict(n_ao=v[0].value, n_o=v[1].value, n_J=v[2].value, scf_iterations=v[3].value)
+
+    def one_electron(self):
+        n = self.dims()['n_ao']
+        S, T, V = (np.empty((n, n), order='F') for _ in range(3))
+        _ccsd._check(self._lib.etg_molecule_one_electron(self._h, _ccsd._ptr(S), _ccsd._ptr(T), _ccsd._ptr(V)))
+        return S, T, V
+
+    def ao_eri(self):
+        """(pq|rs) as an n^2 x n^2 matrix (small molecules)."""
+        n = self.dims()['n_ao']
+        G = np.empty((n * n, n * n), order='F')
+        _ccsd._check(self._lib.etg_molecule_ao_eri(self._h, _ccsd._ptr(G)))
+        return G
+
+    def run(self, cholesky_threshold=1e-8, scf_threshold=1e-10, max_scf_iterations=200):
+        """Integrals, decomposition, RHF, MO vectors; returns the CCSD inputs."""
+        _ccsd._check(self._lib.etg_molecule_run(self._h, cholesky_threshold, scf_threshold, max_scf_iterations))
+        d = self.dims()
+        n, nJ = d['n_ao'], d['n_J']
+        e_nuc, e_hf = ctypes.c_double(), ctypes.c_double()
+        eps = np.empty(n)
+        h_mo = np.empty((n, n), order='F')
+        L_mo = np.empty((nJ, n, n), order='F')
+        C = np.empty((n, n), order='F')
+        _ccsd._check(self._lib.etg_molecule_results(self._h, ctypes.byref(e_nuc), ctypes.byref(e_hf), _ccsd._ptr(eps),
+                                                    _ccsd._ptr(h_mo), _ccsd._ptr(L_mo), _ccsd._ptr(C)))
+        return dict(n_o=d['n_o'], n_J=nJ, e_nuc=e_nuc.value, e_hf=e_hf.value, eps=eps, h_mo=h_mo, L_mo=L_mo, C=C,
+                    scf_iterations=d['scf_iterations'])

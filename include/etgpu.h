@@ -281,6 +281,49 @@ int etg_test_reduced_eigen(int64_t k, const double* G, int64_t n_want, double* w
 int etg_bench_gemm(char ta, char tb, int64_t m, int64_t n, int64_t k, int precision, int reps,
                    double* seconds);
 
+/* ---- molecular producer of the inputs: AO integrals -> RHF -> Cholesky -> MO -- */
+
+typedef struct etg_molecule etg_molecule;
+
+/* A closed-shell molecule: n_atoms nuclei (charges, xyz in bohr, 3 per atom),
+ * n_shells contracted Gaussian shells in AO order (shell_atom, angular momentum
+ * shell_l <= 2 with spherical d, shell_nprim primitives whose exponents and
+ * unnormalised contraction coefficients are consecutive in the flat arrays),
+ * n_electrons even.
+ * Replaces: eT's molecular system and AO basis set-up feeding Libint
+ * (src/libint/eri.cpp:33-89). */
+int etg_molecule_create(int64_t n_atoms, const double* charges, const double* xyz_bohr, int64_t n_shells,
+                        const int64_t* shell_atom, const int64_t* shell_l, const int64_t* shell_nprim,
+                        const double* exponents, const double* coefficients, int64_t n_electrons,
+                        etg_molecule** molecule);
+
+/* On the device: the AO integrals (McMurchie-Davidson; the dense (pq|rs)
+ * matrix in HBM), its pivoted Cholesky decomposition to cholesky_threshold on
+ * the largest remaining diagonal, RHF with DIIS (core-Hamiltonian guess, J and
+ * K from the Cholesky vectors; converged when max|FDS - SDF| < scf_threshold
+ * and |dE| < 1e-13) and the AO->MO transformation of the Cholesky vectors.
+ * Replaces: eri_cd (src/solvers/cholesky/eri_cd_class.F90:241, AO->MO :3132)
+ * and the RHF solver (src/solvers/hf/scf_solver_class.F90). */
+int etg_molecule_run(etg_molecule* molecule, double cholesky_threshold, double scf_threshold,
+                     int64_t max_scf_iterations);
+
+/* Sizes: AO (= MO) count, occupied orbitals, Cholesky vectors, RHF iterations. */
+int etg_molecule_dims(etg_molecule* molecule, int64_t* n_ao, int64_t* n_occ, int64_t* n_J,
+                      int64_t* scf_iterations);
+
+/* After etg_molecule_run (each may be NULL): E_nuc, E_HF, orbital energies
+ * (n), h_mo (n x n), L_mo (n_J x n x n, J fastest: eT's layout, the input of
+ * etg_set_cholesky; host or device pointer), MO coefficients C (n x n). */
+int etg_molecule_results(etg_molecule* molecule, double* e_nuc, double* e_hf, double* eps, double* h_mo,
+                         double* L_mo, double* C);
+
+/* Checks: S, T, V (n x n) and the dense (pq|rs) matrix (n^2 x n^2; small
+ * molecules) in the spherical AO basis. */
+int etg_molecule_one_electron(etg_molecule* molecule, double* S, double* T, double* V);
+int etg_molecule_ao_eri(etg_molecule* molecule, double* G);
+
+int etg_molecule_destroy(etg_molecule* molecule);
+
 /* Tuning, process-wide: measured tile choice for the FP64/FP32 products (on by
  * default).  The first product of a shape outside graph capture times the
  * cost model's tile size and split-K count against its neighbours (the other
