@@ -558,16 +558,16 @@ int64_t CCSD::davidson(int side, int64_t n_states, double threshold, int64_t max
     }
     // With reduced-precision transformations the residuals bottom out at the
     // transforms' rounding level: the solve also ends once the largest residual
-    // has not halved in 4 iterations (returned as -iterations).
+    // has not improved by 10 % in 6 iterations (returned as -iterations).
     bool stalled = false;
     if (prec != Prec::F64) {
       double worst = 0.0;
       for (int64_t j = 0; j < s; ++j) worst = std::max(worst, norms[j]);
-      if (worst < 0.5 * best) {
+      if (worst < 0.9 * best) {
         best = worst;
         since_best = 0;
       } else {
-        stalled = ++since_best >= 4;
+        stalled = ++since_best >= 6;
       }
     }
     if (converged || stalled) {

@@ -520,6 +520,128 @@ GemmLaunch launch_typed(hipStream_t stream, const Gemm& g, GemmWorkspace* ws) {
   return run_choice<T>(stream, g, ws, ch);
 }
 
+// ---- tall products with leading dimensions beyond the matrix-core kernel ----
+// Davidson's trial space at o = 40, v = 152 has n_t1 + n_t2 > 2^24 rows, more
+// than the 31-bit per-thread buffer offsets of the tile kernel span over 16
+// columns.  Its products are tall and skinny and bound by HBM, so two plain
+// FP64 kernels with 64-bit addressing serve them:
+//   gemm_tall_tn: C (m x n) = alpha A^T B + beta C, A (K x m), B (K x n), K long:
+//     per-block partial sums over row chunks (A and B rows staged through LDS),
+//     then a deterministic sum over the blocks;
+//   gemm_tall_nn: C (M x n) = alpha A B + beta C, A (M x k), B (k x n) small:
+//     B in LDS, one row of C per thread, columns of A read coalesced.
+constexpr int kTallRows = 32;
+constexpr int kTallThreads = 256;
+constexpr int kTallOut = 16384;                    // m n <= 16384 (64 outputs per thread)
+
+__global__ __launch_bounds__(kTallThreads) void gemm_tall_tn_kernel(
+    const double* __restrict__ A, int64_t lda, const double* __restrict__ B, int64_t ldb, int64_t K, int m,
+    int n, int64_t rows_per_block, double* __restrict__ part) {
+  extern __shared__ double sm[];
+  double* As = sm;                                  // [m][kTallRows]
+  double* Bs = sm + int64_t(m) * kTallRows;         // [n][kTallRows]
+  const int tid = threadIdx.x, nout = m * n;
+  const int64_t r0 = int64_t(blockIdx.x) * rows_per_block;
+  const int64_t r1 = min(K, r0 + rows_per_block);
+  double acc[kTallOut / kTallThreads];
+#pragma unroll
+  for (int q = 0; q < kTallOut / kTallThreads; ++q) acc[q] = 0.0;
+  for (int64_t rb = r0; rb < r1; rb += kTallRows) {
+    const int nr = int(min<int64_t>(kTallRows, r1 - rb));
+    for (int e = tid; e < kTallRows * m; e += kTallThreads) {
+      const int rr = e % kTallRows, i = e / kTallRows;
+      As[e] = rr < nr ? A[(rb + rr) + int64_t(i) * lda] : 0.0;
+    }
+    for (int e = tid; e < kTallRows * n; e += kTallThreads) {
+      const int rr = e % kTallRows, j = e / kTallRows;
+      Bs[e] = rr < nr ? B[(rb + rr) + int64_t(j) * ldb] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kTallOut / kTallThreads; ++q) {
+      const int o = tid + q * kTallThreads;
+      if (o < nout) {
+        const double* a = As + (o % m) * kTallRows;
+        const double* b = Bs + (o / m) * kTallRows;
+        double s = 0.0;
+#pragma unroll 8
+        for (int rr = 0; rr < kTallRows; ++rr) s += a[rr] * b[rr];
+        acc[q] += s;
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < kTallOut / kTallThreads; ++q) {
+    const int o = tid + q * kTallThreads;
+    if (o < nout) part[int64_t(blockIdx.x) * nout + o] = acc[q];
+  }
+}
+
+__global__ void gemm_tall_reduce_kernel(const double* __restrict__ part, int64_t nblocks, int m, int n,
+                                        double alpha, double beta, double* __restrict__ C, int64_t ldc) {
+  const int o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= m * n) return;
+  double s = 0.0;
+  for (int64_t b = 0; b < nblocks; ++b) s += part[b * m * n + o];
+  double* c = C + (o % m) + int64_t(o / m) * ldc;
+  *c = (beta == 0.0) ? alpha * s : alpha * s + beta * *c;
+}
+
+__global__ __launch_bounds__(kTallThreads) void gemm_tall_nn_kernel(
+    const double* __restrict__ A, int64_t lda, const double* __restrict__ B, int64_t ldb, double* __restrict__ C,
+    int64_t ldc, int64_t M, int k, int n, double alpha, double beta) {
+  extern __shared__ double Bs[];                    // k x n
+  for (int e = threadIdx.x; e < k * n; e += kTallThreads) Bs[e] = B[(e % k) + int64_t(e / k) * ldb];
+  __syncthreads();
+  for (int64_t row = int64_t(blockIdx.x) * kTallThreads + threadIdx.x; row < M;
+       row += int64_t(gridDim.x) * kTallThreads) {
+    for (int j0 = 0; j0 < n; j0 += 16) {
+      const int nj = min(16, n - j0);
+      double acc[16];
+#pragma unroll
+      for (int jj = 0; jj < 16; ++jj) acc[jj] = 0.0;
+      for (int i = 0; i < k; ++i) {
+        const double a = A[row + int64_t(i) * lda];
+#pragma unroll
+        for (int jj = 0; jj < 16; ++jj)
+          if (jj < nj) acc[jj] += a * Bs[i + k * (j0 + jj)];
+      }
+      for (int jj = 0; jj < nj; ++jj) {
+        double* c = C + row + int64_t(j0 + jj) * ldc;
+        *c = (beta == 0.0) ? alpha * acc[jj] : alpha * acc[jj] + beta * *c;
+      }
+    }
+  }
+}
+
+GemmLaunch gemm_tall(hipStream_t stream, const Gemm& g, GemmWorkspace* ws) {
+  if (g.batch != 1) fail("gemm: leading dimension too large for a batched product");
+  if (g.ta && !g.tb && g.m * g.n <= kTallOut) {
+    if (ws == nullptr) fail("gemm: tall products need a workspace");
+    const int64_t nblocks = std::max<int64_t>(1, std::min<int64_t>(1024, ceil_div(g.k, 4096)));
+    const int64_t rows = ceil_div(ceil_div(g.k, nblocks), kTallRows) * kTallRows;
+    const int64_t nb = ceil_div(g.k, rows);
+    if (ws->buf.resize(sizeof(double) * size_t(nb * g.m * g.n))) ++ws->gen;
+    const size_t lds = sizeof(double) * size_t(kTallRows * (g.m + g.n));
+    gemm_tall_tn_kernel<<<unsigned(nb), kTallThreads, lds, stream>>>(g.A, g.lda, g.B, g.ldb, g.k, int(g.m),
+                                                                      int(g.n), rows, ws->buf.as<double>());
+    ETG_LAUNCH("gemm_tall_tn");
+    gemm_tall_reduce_kernel<<<unsigned(ceil_div(g.m * g.n, 256)), 256, 0, stream>>>(
+        ws->buf.as<double>(), nb, int(g.m), int(g.n), g.alpha, g.beta, g.C, g.ldc);
+    ETG_LAUNCH("gemm_tall_reduce");
+    return GemmLaunch{"gemm_tall_tn", int(nb)};
+  }
+  if (!g.ta && !g.tb && g.k * g.n <= 8192) {
+    const unsigned grid = unsigned(std::min<int64_t>(ceil_div(g.m, kTallThreads), 4096));
+    gemm_tall_nn_kernel<<<grid, kTallThreads, sizeof(double) * size_t(g.k * g.n), stream>>>(
+        g.A, g.lda, g.B, g.ldb, g.C, g.ldc, g.m, int(g.k), int(g.n), g.alpha, g.beta);
+    ETG_LAUNCH("gemm_tall_nn");
+    return GemmLaunch{"gemm_tall_nn", 1};
+  }
+  fail("gemm: leading dimension too large");
+}
+
 }  // namespace
 
 void set_gemm_autotune(bool on) { g_autotune.store(on); }
@@ -545,9 +667,10 @@ GemmLaunch gemm(hipStream_t stream, const Gemm& g, Prec prec, GemmWorkspace* ws)
     ETG_LAUNCH("scale_matrix");
     return GemmLaunch{"scale_matrix", 1};
   }
+  // per-thread byte offsets inside a tile span 16 operand columns and are 31-bit:
+  // longer leading dimensions go to the tall FP64 kernels (any precision asked)
+  if (std::max(g.lda, g.ldb) >= (int64_t(1) << 24)) return gemm_tall(stream, g, ws);
   if (prec == Prec::BF16X3) return gemm_bf16x3(stream, g, ws);
-  // per-thread byte offsets inside a tile span 16 operand columns and are 31-bit
-  if (std::max(g.lda, g.ldb) >= (int64_t(1) << 24)) fail("gemm: leading dimension too large");
   if (prec == Prec::F32) return launch_typed<float>(stream, g, ws);
   return launch_typed<double>(stream, g, ws);
 }

@@ -211,7 +211,7 @@ class EigenDavidson:
 class MixedPrecisionDavidson:
     """EOM roots from reduced-precision transformations, finished in FP64.
 
-    Each precision of `schedule` (e.g. split-bf16) runs the device Davidson with
+    Each precision of `schedule` (split-bf16, then FP32) runs the device Davidson with
     the transformations in that precision down to `margin` times its noise floor
     ||A_p x - A_FP64 x|| (measured once on a normalised random probe), or until
     its residuals stop decreasing; its Ritz vectors start the next phase.  The
@@ -221,7 +221,7 @@ class MixedPrecisionDavidson:
     """
 
     def __init__(self, wf, n_states, side='right', residual_threshold=1e-6, max_iterations=100,
-                 max_dim=0, schedule=(_ccsd.BF16X3,), margin=10.0, log=None, seed=7, start=None):
+                 max_dim=0, schedule=(_ccsd.BF16X3, _ccsd.FP32), margin=10.0, log=None, seed=7, start=None):
         self.wf = wf
         self.n_states = n_states
         self.side = side

@@ -163,8 +163,8 @@ int etg_diis_update(etg_handle* handle);
  * n_states, host or device, may be NULL; residual_norms: max_iterations x
  * n_states per-iteration norms, may be NULL; iterations: may be NULL.  With
  * precision FP32 or BF16X3 the solve also ends, with *iterations = -(count),
- * when the largest residual has not halved in 4 iterations (the transforms'
- * rounding floor); vectors then hold the current Ritz vectors, the start of an
+ * when the largest residual has not improved by 10 % in 6 iterations (the
+ * transforms' rounding floor); vectors then hold the current Ritz vectors, the start of an
  * FP64 solve.
  * Replaces: run_eigen_davidson_solver
  * (src/solvers/eigen_davidson_solver_class.F90:122-318) with eT's eigen

@@ -207,8 +207,8 @@ def test_gpu_mixed_precision_eom_reproduces_et():
             sv = solvers.MixedPrecisionDavidson(wf, 3, side, residual_threshold=1e-9)
             w = sv.run()
             np.testing.assert_allclose(w, w_ref, rtol=0, atol=1e-10)
-            assert [p for p, *_ in sv.phases] == [ccsd.BF16X3, ccsd.FP64]
-            assert sv.noise[ccsd.BF16X3] > 0
+            assert [p for p, *_ in sv.phases] == [ccsd.BF16X3, ccsd.FP32, ccsd.FP64]
+            assert sv.noise[ccsd.BF16X3] > sv.noise[ccsd.FP32] > 0
             again, hist = wf.davidson(3, side, 1e-8, 10, 0, ccsd.FP64, start=sv.eigenvectors)
             assert len(hist) == 1 and not wf.davidson_stalled
             np.testing.assert_allclose(again, w_ref, rtol=0, atol=1e-10)

@@ -114,3 +114,17 @@ def test_measured_tile_choice_keeps_results(ta, tb, m, n, k):
     assert np.abs(tuned - ref).max() < scale
     assert np.array_equal(tuned, again)          # the kept choice is reused
     assert after >= before
+
+
+@pytest.mark.parametrize('ta,tb,m,n,k', [('T', 'N', 3, 2, (1 << 24) + 3),       # inner products
+                                         ('N', 'N', (1 << 24) + 3, 2, 3)])      # tall output
+def test_tall_products_beyond_the_tile_kernel_offsets(ta, tb, m, n, k):
+    """Leading dimensions >= 2^24 (the Davidson space of (H2O)_8) go to the tall
+    FP64 kernels: same result as numpy, with beta != 0."""
+    rng = np.random.default_rng(11)
+    A = rng.normal(size=(k, m) if ta == 'T' else (m, k))
+    B = rng.normal(size=(k, n))
+    C = rng.normal(size=(m, n))
+    out = ccsd.device_gemm(ta, tb, 0.5, A, B, 2.0, C)
+    ref = 0.5 * (A.T if ta == 'T' else A) @ B + 2.0 * C
+    assert np.abs(out - ref).max() < 1e-13 * (np.sqrt(max(k, 1)) + 1.0) * np.abs(ref).max()
