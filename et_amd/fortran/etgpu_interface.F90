@@ -256,6 +256,46 @@ module etgpu_interface
          integer(c_int), value :: precision
       end function etg_construct_omega_resident
 !
+!     molecular producer: integrals -> Cholesky -> RHF -> MO vectors
+!     (src/libint/eri.cpp:33-89, eri_cd_class.F90:241,3132, scf_solver_class.F90)
+      integer(c_int) function etg_molecule_create(n_atoms, charges, xyz_bohr, n_shells, shell_atom, shell_l, &
+                                                  shell_nprim, exponents, coefficients, n_electrons, molecule) &
+                                                  bind(C, name='etg_molecule_create')
+         import :: c_int, c_ptr, c_double, c_int64_t
+         integer(c_int64_t), value :: n_atoms, n_shells, n_electrons
+         real(c_double), dimension(*), intent(in) :: charges, xyz_bohr, exponents, coefficients
+         integer(c_int64_t), dimension(*), intent(in) :: shell_atom, shell_l, shell_nprim
+         type(c_ptr), intent(out) :: molecule
+      end function etg_molecule_create
+!
+      integer(c_int) function etg_molecule_run(molecule, cholesky_threshold, scf_threshold, max_scf_iterations) &
+                                               bind(C, name='etg_molecule_run')
+         import :: c_int, c_ptr, c_double, c_int64_t
+         type(c_ptr), value :: molecule
+         real(c_double), value :: cholesky_threshold, scf_threshold
+         integer(c_int64_t), value :: max_scf_iterations
+      end function etg_molecule_run
+!
+      integer(c_int) function etg_molecule_dims(molecule, n_ao, n_occ, n_J, scf_iterations) &
+                                                bind(C, name='etg_molecule_dims')
+         import :: c_int, c_ptr, c_int64_t
+         type(c_ptr), value :: molecule
+         integer(c_int64_t), intent(out) :: n_ao, n_occ, n_J, scf_iterations
+      end function etg_molecule_dims
+!
+      integer(c_int) function etg_molecule_results(molecule, e_nuc, e_hf, eps, h_mo, L_mo, C) &
+                                                   bind(C, name='etg_molecule_results')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: molecule
+         real(c_double), intent(out) :: e_nuc, e_hf
+         real(c_double), dimension(*), intent(out) :: eps, h_mo, L_mo, C
+      end function etg_molecule_results
+!
+      integer(c_int) function etg_molecule_destroy(molecule) bind(C, name='etg_molecule_destroy')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: molecule
+      end function etg_molecule_destroy
+!
    end interface
 !
 contains
