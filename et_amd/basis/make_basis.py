@@ -1,16 +1,16 @@
-"""Writes cc-pvdz.json (H-Ne) from the reference's Gaussian-94 basis file.
+"""Writes cc-pvdz.json and sto-3g.json (H-Ne) from the reference's Gaussian-94 basis files.
 
-Run in a checkout that has the reference next to it; the JSON is committed and
-is what the package reads.  Data source: ao_basis/cc-pvdz.g94 (eT's basis
-library, the cc-pVDZ sets of Dunning).  Each element maps to a list of shells
-{"l": 0|1|2, "exps": [...], "coefs": [...]} in file order, which is eT's AO
-shell order for the element.
+Run in a checkout that has the reference next to it; the JSON files are
+committed and are what the package reads.  Data sources: ao_basis/cc-pvdz.g94
+and ao_basis/sto-3g.g94 (eT's basis library).  Each element maps to a list of
+shells {"l": 0|1|2, "exps": [...], "coefs": [...]} in file order; an "SP" shell
+(shared exponents) becomes an s shell followed by a p shell.
 """
 import json
 import os
 import sys
 
-SRC = os.path.join(sys.argv[1] if len(sys.argv) > 1 else '/root/reference', 'ao_basis', 'cc-pvdz.g94')
+REF = sys.argv[1] if len(sys.argv) > 1 else '/root/reference'
 ELEMENTS = ('H', 'He', 'Li', 'Be', 'B', 'C', 'N', 'O', 'F', 'Ne')
 
 
@@ -23,15 +23,20 @@ def parse(path):
         element, shells, i = rows[0].split()[0], [], 1
         while i < len(rows):
             kind, count = rows[i].split()[0], int(rows[i].split()[1])
-            pairs = [r.replace('D', 'E').split()[:2] for r in rows[i + 1:i + 1 + count]]
-            shells.append({'l': 'SPD'.index(kind), 'exps': [float(a) for a, _ in pairs],
-                           'coefs': [float(b) for _, b in pairs]})
+            cols = [r.replace('D', 'E').split() for r in rows[i + 1:i + 1 + count]]
+            exps = [float(c[0]) for c in cols]
+            if kind == 'SP':
+                shells.append({'l': 0, 'exps': exps, 'coefs': [float(c[1]) for c in cols]})
+                shells.append({'l': 1, 'exps': exps, 'coefs': [float(c[2]) for c in cols]})
+            else:
+                shells.append({'l': 'SPD'.index(kind), 'exps': exps, 'coefs': [float(c[1]) for c in cols]})
             i += 1 + count
         out[element] = shells
     return out
 
 
 if __name__ == '__main__':
-    data = {'source': 'ao_basis/cc-pvdz.g94', 'basis': parse(SRC)}
-    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), 'cc-pvdz.json'), 'w') as f:
-        json.dump(data, f, indent=0)
+    for name in ('cc-pvdz', 'sto-3g'):
+        data = {'source': 'ao_basis/%s.g94' % name, 'basis': parse(os.path.join(REF, 'ao_basis', name + '.g94'))}
+        with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), name + '.json'), 'w') as f:
+            json.dump(data, f, indent=0)

@@ -73,6 +73,8 @@ _SIGNATURES = {
     'etg_multiplier_quasi_newton_update': [ctypes.c_void_p],
     'etg_multiplier_diis_update': [ctypes.c_void_p],
     'etg_set_transform_graphs': [ctypes.c_void_p, ctypes.c_int],
+    'etg_one_electron_expectation': [ctypes.c_void_p, _c_dbl_p, _c_dbl_p],
+    'etg_molecule_dipole_integrals': [ctypes.c_void_p, _c_dbl_p, _c_dbl_p],
     'etg_set_transform_batch': [ctypes.c_void_p, _i64],
     'etg_set_kernel_ledger': [ctypes.c_void_p, ctypes.c_int],
     'etg_kernel_ledger': [ctypes.c_void_p, _i64, ctypes.c_char_p, _i64, ctypes.POINTER(_i64),
@@ -392,6 +394,17 @@ class CCSD:
 
     def multiplier_diis_update(self):
         _check(self._lib.etg_multiplier_diis_update(self._h))
+
+    def one_electron_expectation(self, X):
+        """CC mean value of the one-electron operator X (n_mo x n_mo, MO basis) with the
+        Lagrangian density (needs prepare_for_jacobian at the converged amplitudes and
+        the multipliers)."""
+        X = np.asfortranarray(X, dtype=np.float64)
+        if X.shape != (self.n_mo, self.n_mo):
+            raise ETGError('operator must be n_mo x n_mo')
+        v = ctypes.c_double()
+        _check(self._lib.etg_one_electron_expectation(self._h, _ptr(X), ctypes.byref(v)))
+        return v.value
 
     def set_transform_graphs(self, on):
         """Replay captured HIP graphs of the transformation passes (default) or run them eagerly."""

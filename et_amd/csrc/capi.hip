@@ -251,6 +251,13 @@ int etg_multiplier_diis_update(etg_handle* h) {
   return guarded([&] { wf_of(h).multiplier_diis_update(); });
 }
 
+int etg_one_electron_expectation(etg_handle* h, const double* X_mo, double* value) {
+  return guarded([&] {
+    if (value == nullptr) etg::fail("etg_one_electron_expectation: null result");
+    *value = wf_of(h).one_electron_expectation(X_mo);
+  });
+}
+
 int etg_set_transform_graphs(etg_handle* h, int on) {
   return guarded([&] { wf_of(h).set_transform_graphs(on != 0); });
 }
@@ -412,6 +419,13 @@ int etg_molecule_one_electron(etg_molecule* molecule, double* S, double* T, doub
   return guarded([&] {
     if (!S || !T || !V) etg::fail("etg_molecule_one_electron: null array");
     mol_of(molecule).one_electron_integrals(S, T, V);
+  });
+}
+
+int etg_molecule_dipole_integrals(etg_molecule* molecule, const double* origin, double* D) {
+  return guarded([&] {
+    if (!origin || !D) etg::fail("etg_molecule_dipole_integrals: null array");
+    mol_of(molecule).dipole_integrals(origin, D);
   });
 }
 

@@ -819,7 +819,7 @@ void CCSD::prepare_for_jacobian() {
   tape->register_buffer(Lmo_ov_.as<double>(), J * o * v, true, "L_mo ov");
   tape->register_buffer(Lmo_vo_.as<double>(), J * v * o, true, "L_mo vo");
   tape->register_buffer(Lmo_vv_.as<double>(), J * v * v, true, "L_mo vv");
-  tape->register_buffer(h_.as<double>(), n * n, true, "h");
+  tape->register_buffer(h_.as<double>(), n * n, false, "h");   // one-electron perturbations
   tape->register_buffer(g_ovov_.as<double>(), n_t1 * n_t1, true, "g_ovov");
   tape->register_buffer(t_.as<double>(), n_t1 + n_t2, false, "t");
   tape->register_buffer(F_.as<double>(), n * n, false, "F");

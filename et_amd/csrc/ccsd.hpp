@@ -63,6 +63,9 @@ class CCSD {
   double multiplier_residual_norm();
   void multiplier_quasi_newton_update();          // e = -R/eps; t-bar += e
   void multiplier_diis_update();                  // records (e, t-bar); t-bar extrapolated
+  // CC mean value <X> of a one-electron operator X (host, MO basis) with the
+  // Lagrangian density at the current amplitudes, multipliers and record
+  double one_electron_expectation(const double* X_host);
 
   // Jacobian transformations at the current amplitudes (A = dOmega/dt D2)
   void prepare_for_jacobian();                    // records the residual program

@@ -293,6 +293,51 @@ void one_electron(const std::vector<HostShell>& sh, const std::vector<double>& Z
     }
 }
 
+// <mu| r_c - O_c |nu>: along c the primitive pair gives sqrt(pi/p)(E^{ij}_1 + X_PC E^{ij}_0),
+// along the other two directions the overlaps.  D: 3 matrices n x n (x, y, z).
+void dipole(const std::vector<HostShell>& sh, const double* O, int64_t n, double* D) {
+  for (const HostShell& A : sh)
+    for (const HostShell& B : sh) {
+      const int la = A.l, lb = B.l, na = ncart(la), nb = ncart(lb);
+      double blk[3][kMaxCart][kMaxCart] = {};
+      double E[3][(kMaxL + 1) * (kMaxL + 1) * kMaxHerm];
+      const int Lh = la + lb + 1;
+      for (size_t i = 0; i < A.exps.size(); ++i)
+        for (size_t j = 0; j < B.exps.size(); ++j) {
+          const double a = A.exps[i], b = B.exps[j], p = a + b;
+          const double pref = A.coefs[i] * B.coefs[j] * std::pow(M_PI / p, 1.5);
+          for (int d = 0; d < 3; ++d) hermite_e(la, lb, a, b, A.c[d], B.c[d], E[d]);
+          for (int ca = 0; ca < na; ++ca)
+            for (int cb = 0; cb < nb; ++cb) {
+              int ax[3], bx[3];
+              cart(la, ca, ax[0], ax[1], ax[2]);
+              cart(lb, cb, bx[0], bx[1], bx[2]);
+              double s0[3], m1[3];
+              for (int d = 0; d < 3; ++d) {
+                const double* e = &E[d][(ax[d] * (lb + 1) + bx[d]) * Lh];
+                const double XPC = (a * A.c[d] + b * B.c[d]) / p - O[d];
+                s0[d] = e[0];
+                m1[d] = (ax[d] + bx[d] >= 1 ? e[1] : 0.0) + XPC * e[0];
+              }
+              blk[0][ca][cb] += pref * m1[0] * s0[1] * s0[2];
+              blk[1][ca][cb] += pref * s0[0] * m1[1] * s0[2];
+              blk[2][ca][cb] += pref * s0[0] * s0[1] * m1[2];
+            }
+        }
+      for (int c = 0; c < 3; ++c)
+        for (int i = 0; i < nsph(la); ++i)
+          for (int j = 0; j < nsph(lb); ++j) {
+            double v = 0.0;
+            for (int ca = 0; ca < na; ++ca) {
+              const double ta = sph_coef(la, i, ca) * comp_scale(la, ca);
+              if (ta == 0.0) continue;
+              for (int cb = 0; cb < nb; ++cb) v += ta * sph_coef(lb, j, cb) * comp_scale(lb, cb) * blk[c][ca][cb];
+            }
+            D[int64_t(c) * n * n + (A.f0 + i) + n * (B.f0 + j)] = v;
+          }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // two-electron integrals (device): one wavefront per unique shell quartet
 // ---------------------------------------------------------------------------
@@ -624,6 +669,10 @@ Molecule::~Molecule() {
 
 void Molecule::one_electron_integrals(double* S, double* T, double* V) const {
   fe::one_electron(shells_, Z_, xyz_, n_, S, T, V);
+}
+
+void Molecule::dipole_integrals(const double* origin, double* D) const {
+  fe::dipole(shells_, origin, n_, D);
 }
 
 void Molecule::build_eri() {

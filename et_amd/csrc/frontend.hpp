@@ -44,6 +44,8 @@ class Molecule {
 
   // parts, for checks on small molecules
   void one_electron_integrals(double* S, double* T, double* V) const;
+  // <mu| r_c - origin_c |nu>, c = x, y, z: 3 consecutive n x n matrices
+  void dipole_integrals(const double* origin, double* D) const;
   void build_eri();
   void copy_eri(double* host) const;     // (pq|rs) as an n^2 x n^2 matrix
 

@@ -158,4 +158,35 @@ void CCSD::multiplier_diis_update() {
   have_mupdate_ = false;
 }
 
+// CC mean value of a one-electron operator X (MO basis, n_mo x n_mo) with the
+// Lagrangian density: <X> = <HF|(1 + Lambda) e^-T X e^T|HF> (construct_gs_density,
+// mean_value_ccs.F90:94-122, mean_value_doubles.F90), which is dL/de at e = 0 for
+// h -> h + e X, L = E + t-bar . Omega at fixed t and t-bar:
+//   dE/de = 2 sum_i X_ii + 2 sum_ai X_ia t_ai   (F_ia(e) = e X_ia on the HF reference),
+//   t-bar . dOmega/de is one forward replay of the recorded residual program
+//   along dh = X (the record keeps h differentiable).
+double CCSD::one_electron_expectation(const double* X) {
+  require(X != nullptr, "one_electron_expectation: null operator");
+  require(tape_ != nullptr, "one_electron_expectation: prepare_for_jacobian has not been called at these amplitudes");
+  require(have_tbar_, "one_electron_expectation: multipliers required");
+  const int64_t n = n_t1 + n_t2, nm = n_mo;
+  std::vector<double> t1(size_t(std::max<int64_t>(1, n_t1)));
+  ETG_HIP(hipMemcpyAsync(t1.data(), t_.as<double>(), sizeof(double) * n_t1, hipMemcpyDeviceToHost, stream_));
+  tape_->zero_derivatives(1);
+  double* dh = tape_->deriv(h_.as<double>());
+  require(dh != nullptr, "one_electron_expectation: the record holds h as a constant");
+  ETG_HIP(hipMemcpyAsync(dh, X, sizeof(double) * nm * nm, hipMemcpyDefault, stream_));
+  tape_->tangent(Prec::F64, 1);
+  double* res = red_result_.as<double>() + 3;
+  dot(stream_, tbar_.as<double>(), tape_->deriv(omega_.as<double>()), n, red_partial_.as<double>(), res);
+  double lag = 0.0;
+  ETG_HIP(hipMemcpyAsync(&lag, res, sizeof(double), hipMemcpyDeviceToHost, stream_));
+  synchronize();
+  double e = 0.0;
+  for (int64_t i = 0; i < n_o; ++i) e += 2.0 * X[i + nm * i];
+  for (int64_t i = 0; i < n_o; ++i)
+    for (int64_t a = 0; a < n_v; ++a) e += 2.0 * X[i + nm * (n_o + a)] * t1[size_t(a + n_v * i)];
+  return e + lag;
+}
+
 }  // namespace etg

@@ -256,6 +256,16 @@ module etgpu_interface
          integer(c_int), value :: precision
       end function etg_construct_omega_resident
 !
+!     CC mean value of a one-electron operator with the Lagrangian density
+!     (mean_value_ccs.F90:94-122, cc_mean_values_task_class.F90:88-92)
+      integer(c_int) function etg_one_electron_expectation(handle, X_mo, value) &
+                                                           bind(C, name='etg_one_electron_expectation')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: handle
+         real(c_double), dimension(*), intent(in) :: X_mo
+         real(c_double), intent(out) :: value
+      end function etg_one_electron_expectation
+!
 !     molecular producer: integrals -> Cholesky -> RHF -> MO vectors
 !     (src/libint/eri.cpp:33-89, eri_cd_class.F90:241,3132, scf_solver_class.F90)
       integer(c_int) function etg_molecule_create(n_atoms, charges, xyz_bohr, n_shells, shell_atom, shell_l, &

@@ -147,6 +147,14 @@ class Molecule:
         _ccsd._check(self._lib.etg_molecule_one_electron(self._h, _ccsd._ptr(S), _ccsd._ptr(T), _ccsd._ptr(V)))
         return S, T, V
 
+    def dipole_integrals(self, origin=(0.0, 0.0, 0.0)):
+        """<mu| r_c - origin_c |nu> (c = x, y, z) in the AO basis, shape (3, n, n)."""
+        n = self.dims()['n_ao']
+        D = np.empty((n, n, 3), order='F')
+        o = np.asarray(origin, dtype=np.float64)
+        _ccsd._check(self._lib.etg_molecule_dipole_integrals(self._h, _ccsd._ptr(o), _ccsd._ptr(D)))
+        return np.moveaxis(D, 2, 0)
+
     def ao_eri(self):
         """(pq|rs) as an n^2 x n^2 matrix (small molecules)."""
         n = self.dims()['n_ao']

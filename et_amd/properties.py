@@ -1,0 +1,23 @@
+"""CC mean-value properties over the device wavefunction.
+
+dipole_moment follows eT's CC mean-value dipole (cc_mean_values_task_class.F90:88-92,
+calculate_and_print_dipole): electronic part -<r_c> with the CC Lagrangian
+(Lambda) density, nuclear part sum_A Z_A (R_A - origin), in atomic units and
+Debye (eT's factor, CODATA 2018, src/various/parameters.F90:110).
+"""
+import numpy as np
+
+AU_TO_DEBYE = 2.541746473
+
+
+def dipole_moment(wf, molecule, C, origin=(0.0, 0.0, 0.0)):
+    """wf: converged ground state with its multipliers and the record of the residual
+    program at those amplitudes (prepare_for_jacobian or prepare_for_multiplier_equation);
+    molecule: the et_amd.molecule.Molecule it came from; C: its MO coefficients."""
+    o = np.asarray(origin, dtype=float)
+    D = molecule.dipole_integrals(o)
+    electronic = np.array([-wf.one_electron_expectation(C.T @ D[c] @ C) for c in range(3)])
+    nuclear = (molecule.charges[:, None] * (molecule.xyz_bohr - o[None, :])).sum(axis=0)
+    total = electronic + nuclear
+    return dict(electronic=electronic, nuclear=nuclear, total=total, electronic_debye=electronic * AU_TO_DEBYE,
+                nuclear_debye=nuclear * AU_TO_DEBYE, total_debye=total * AU_TO_DEBYE)

@@ -281,6 +281,18 @@ int etg_test_reduced_eigen(int64_t k, const double* G, int64_t n_want, double* w
 int etg_bench_gemm(char ta, char tb, int64_t m, int64_t n, int64_t k, int precision, int reps,
                    double* seconds);
 
+/* CC mean value of a one-electron operator X (n_mo x n_mo, MO basis, host)
+ * with the CC Lagrangian (Lambda) density at the current amplitudes and
+ * multipliers: <X> = <HF|(1 + Lambda) e^-T X e^T|HF>, evaluated as dL/de for
+ * h -> h + e X (one forward replay of the recorded residual program plus the
+ * reference and t1 terms of the energy).  Needs etg_prepare_for_jacobian (or
+ * etg_prepare_for_multiplier_equation) at the converged amplitudes and the
+ * multipliers.  The electronic dipole is -<r_c> (Debye: times 2.541746473).
+ * Replaces: construct_gs_density (src/wavefunctions/ccs/mean_value_ccs.F90:94-122,
+ * src/wavefunctions/doubles/mean_value_doubles.F90) with the CC mean values
+ * (src/engines/cc/tasks/cc_mean_values_task_class.F90:88-92). */
+int etg_one_electron_expectation(etg_handle* handle, const double* X_mo, double* value);
+
 /* ---- molecular producer of the inputs: AO integrals -> RHF -> Cholesky -> MO -- */
 
 typedef struct etg_molecule etg_molecule;
@@ -320,6 +332,11 @@ int etg_molecule_results(etg_molecule* molecule, double* e_nuc, double* e_hf, do
 /* Checks: S, T, V (n x n) and the dense (pq|rs) matrix (n^2 x n^2; small
  * molecules) in the spherical AO basis. */
 int etg_molecule_one_electron(etg_molecule* molecule, double* S, double* T, double* V);
+
+/* Dipole integrals <mu| r_c - origin_c |nu>, c = x, y, z, in the AO basis: D
+ * holds three n x n matrices.  Replaces: the AO dipole integrals eT takes from
+ * Libint for its CC mean values (src/libint/eri.cpp). */
+int etg_molecule_dipole_integrals(etg_molecule* molecule, const double* origin, double* D);
 int etg_molecule_ao_eri(etg_molecule* molecule, double* G);
 
 int etg_molecule_destroy(etg_molecule* molecule);

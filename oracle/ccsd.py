@@ -536,6 +536,22 @@ def multipliers_dense(L_mo, h_mo, t1, t2, n_o):
     return np.linalg.solve(A.T, -construct_eta(L_mo, h_mo, t1, n_o))
 
 
+def one_electron_expectation(L_mo, h_mo, t1, t2, tbar, n_o, X, step=1e-30):
+    """CC mean value of a one-electron operator X (MO basis) with the Lagrangian density.
+
+    eT's ground-state density (construct_gs_density, mean_value_ccs.F90:94-122 and
+    mean_value_doubles.F90) gives <X> = <HF|(1 + Lambda) e^-T X e^T|HF>, which is
+    dL/de at e = 0 for h -> h + e X with L = E + tbar . Omega at fixed t and tbar:
+      dE/de = 2 sum_i X_ii + 2 sum_ai X_ia t_ai   (E = E_ref + 2 F_ia t_ai + ..., F_ia(e) = e X_ia)
+      tbar . dOmega/de by the complex step of the residual along X.
+    """
+    o = slice(0, n_o)
+    v = slice(n_o, X.shape[0])
+    dE = 2.0 * np.trace(X[o, o]) + 2.0 * np.sum(X[o, v] * t1.T)
+    dOm = residual(L_mo, h_mo + 1j * step * X, t1, t2, n_o).imag / step
+    return float(dE + np.dot(tbar, dOm))
+
+
 def jacobian_matrix(L_mo, h_mo, t1, t2, n_o):
     """Full A (n_t1 + n_t2 square) column by column (tiny systems only)."""
     n_v = L_mo.shape[1] - n_o

@@ -225,6 +225,40 @@ def one_electron(shells, atoms):
     return S, T, V
 
 
+def dipole_integrals(shells, origin=(0.0, 0.0, 0.0)):
+    """<mu| r_c - O_c |nu> for c = x, y, z in the spherical AO basis, (3, n, n).
+
+    Along c the primitive pair gives sqrt(pi/p) (E^{ij}_1 + X_PC E^{ij}_0) (the first
+    Hermite moment), the other two directions their overlaps (eT's dipole
+    integrals come from Libint, src/libint/eri.cpp; this is the same quantity).
+    """
+    nf = sum(s.nfunc for s in shells)
+    offs = np.cumsum([0] + [s.nfunc for s in shells])
+    D = np.zeros((3, nf, nf))
+    O = np.asarray(origin, dtype=float)
+    for ia, A in enumerate(shells):
+        for ib, B in enumerate(shells):
+            blk = np.zeros((3, len(A.carts), len(B.carts)))
+            for i_a, a in enumerate(A.exps):
+                for i_b, b in enumerate(B.exps):
+                    p = a + b
+                    P = (a * A.center + b * B.center) / p
+                    cc = A.coefs[i_a] * B.coefs[i_b] * (np.pi / p) ** 1.5
+                    E = [hermite_E(A.l, B.l, a, b, A.center[d], B.center[d]) for d in range(3)]
+                    for ka, ca in enumerate(A.carts):
+                        for kb, cb in enumerate(B.carts):
+                            s0 = [E[d][ca[d], cb[d], 0] for d in range(3)]
+                            m1 = [(E[d][ca[d], cb[d], 1] if ca[d] + cb[d] >= 1 else 0.0)
+                                  + (P[d] - O[d]) * s0[d] for d in range(3)]
+                            for c in range(3):
+                                others = [s0[d] for d in range(3) if d != c]
+                                blk[c, ka, kb] += cc * m1[c] * others[0] * others[1]
+            sa, sb = slice(offs[ia], offs[ia + 1]), slice(offs[ib], offs[ib + 1])
+            for c in range(3):
+                D[c, sa, sb] = A.T @ blk[c] @ B.T.T
+    return D
+
+
 def eri(shells):
     """(pq|rs) in the spherical AO basis, full 4-index array."""
     nf = sum(s.nfunc for s in shells)
