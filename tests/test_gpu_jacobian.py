@@ -193,3 +193,43 @@ def test_vector_groups_match_one_vector_passes():
                     # a group smaller than the largest one reuses the reserved buffers
                     part = wf.jacobian_transformations(X[:, 2:4], side, precision=prec)
                     assert rel(part, R[:, 2:4]) < tol, (limit, side, prec)
+
+
+def test_left_transformation_against_central_differences_on_water_dimer():
+    """(H2O)_2 / cc-pVDZ from the device producer at its converged ground state: along 32
+    seeded random directions c (with random b), <A^T b, c> from the left transformation
+    equals b . (Omega(t + h D2 c) - Omega(t - h D2 c)) / 2h with h = 1e-4, to the O(h^2)
+    error of the central difference."""
+    from et_amd import molecule as M, solvers
+    with M.Molecule(M.water_lattice((2, 1, 1), 2.9, 0)) as mol:
+        r = mol.run(cholesky_threshold=1e-8, scf_threshold=1e-10)
+    n_o = r['n_o']
+    n_v = r['L_mo'].shape[1] - n_o
+    with ccsd.CCSD(n_o, n_v, r['n_J']) as wf:
+        wf.set_cholesky(r['L_mo'])
+        wf.set_oei(r['h_mo'])
+        wf.set_orbital_energies(r['eps'])
+        solvers.DiisCCGroundState(wf, residual_threshold=1e-10).run()
+        t0 = wf.get_amplitudes()
+        n_t1 = wf.n_t1
+        diag = n_t1 + np.arange(n_t1) * (np.arange(n_t1) + 3) // 2     # packed (ai, ai)
+        wf.prepare_for_jacobian('left')
+        rng = np.random.default_rng(2024)
+        C = rng.normal(size=(wf.n_gs_amplitudes, 32))
+        B = rng.normal(size=(wf.n_gs_amplitudes, 32))
+        AtB = wf.jacobian_transformations(B, 'left')
+        h = 1e-4
+        worst = 0.0
+        for k in range(32):
+            d = C[:, k].copy()
+            d[diag] *= 2.0                                              # D2 c
+            om = []
+            for sgn in (1.0, -1.0):
+                wf.set_amplitudes(t0 + sgn * h * d)
+                wf.construct_t1_cholesky()
+                wf.construct_fock()
+                om.append(wf.construct_omega())
+            fd = B[:, k] @ (om[0] - om[1]) / (2 * h)
+            an = AtB[:, k] @ C[:, k]
+            worst = max(worst, abs(fd - an) / max(abs(an), 1e-12))
+        assert worst < 1e-6, worst
