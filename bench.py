@@ -44,10 +44,10 @@ HARTREE_TO_EV = 27.21138602          # CODATA 2014, the factor eT prints
 WORKLOAD_NAME = 'omega_o48_v400_J1600_seed0'
 # HBM bytes per launch of the residual's dominant kernel on this workload: rocprofv3
 # --pmc FETCH_SIZE and --pmc WRITE_SIZE (KiB per dispatch, averaged over the kernel's
-# launches in two residual evaluations), each scaled by what the counter reads for a
+# launches in the last of three residual evaluations), each scaled by what the counter reads for a
 # plain 1 GiB device copy (bench.py --copy-calibration: FETCH_SIZE x 2.000,
 # WRITE_SIZE x 1.000).  Summary and commands: profiles/omega_fp64_o48_v400_J1600_hbm_traffic.md
-MEASURED_TRAFFIC = {(WORKLOAD_NAME, 'gemm_mfma16<f64,NN,128x128>'): 1.0175e11}
+MEASURED_TRAFFIC = {(WORKLOAD_NAME, 'gemm_mfma16<f64,NN,128x128>'): 1.1225e11}
 
 
 def log(*a):
