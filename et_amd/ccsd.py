@@ -12,6 +12,8 @@ import os
 
 import numpy as np
 
+from . import restart as _restart
+
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, 'libetgpu.so')
 
@@ -85,6 +87,7 @@ _SIGNATURES = {
     'etg_molecule_create': [_i64, _c_dbl_p, _c_dbl_p, _i64, ctypes.POINTER(_i64), ctypes.POINTER(_i64),
                             ctypes.POINTER(_i64), _c_dbl_p, _c_dbl_p, _i64, ctypes.POINTER(ctypes.c_void_p)],
     'etg_molecule_run': [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, _i64],
+    'etg_molecule_set_orbitals': [ctypes.c_void_p, _i64, _i64, _c_dbl_p, _c_dbl_p],
     'etg_molecule_dims': [ctypes.c_void_p, ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_i64),
                           ctypes.POINTER(_i64)],
     'etg_molecule_results': [ctypes.c_void_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p],
@@ -182,6 +185,7 @@ class CCSD:
         self.correlation_energy = 0.0
         self.energy = 0.0
         self.davidson_stalled = False
+        self.files = None
 
     def close(self):
         if getattr(self, '_h', None) is not None and self._h.value:
@@ -237,12 +241,108 @@ class CCSD:
     def set_t2_to_cc2_guess(self):
         _check(self._lib.etg_set_t2_to_mp2_guess(self._h))
 
-    def set_initial_amplitudes_guess(self):
-        """t1 = 0, L_t1 = L_mo, t2 = MP2 guess (ccsd_class.F90:357-420, no restart)."""
+    def set_initial_amplitudes_guess(self, restart=False):
+        """set_initial_amplitudes_guess_ccsd (ccsd_class.F90:357-420).
+
+        Without restart (or with no t file): t1 = 0, L_t1 = L_mo, t2 = MP2 guess.  With
+        restart, the t file of initialize_files' directory: a whole vector is taken as
+        is; a singles-only one (a CCS / CC2 file) is dropped for the no-restart guess,
+        as eT does; any other length is an error.
+        """
+        if restart and self.files is not None and self.files.t.exists():
+            n_read = self.read_amplitudes()
+            if n_read == self.n_gs_amplitudes:
+                self.construct_t1_cholesky()
+                return
+            if n_read != self.n_t1:
+                raise ETGError('Did not recognize number of t-amplitudes on file expected %d or %d '
+                               'found %d amplitudes.' % (self.n_gs_amplitudes, self.n_t1, n_read))
         t = np.zeros(self.n_gs_amplitudes)
         self.set_amplitudes(t)
         self.construct_t1_cholesky()
         self.set_t2_to_cc2_guess()
+
+    # ---- restart files (file_handling_ccs.F90) ------------------------------
+    def initialize_files(self, directory):
+        """Restart files of this wavefunction in `directory` (file_handling_ccs.F90:36-99):
+        t, tbar, r_00k, l_00k in eT's amplitude file format."""
+        self.files = _restart.CCSDFiles(directory, self.n_o, self.n_v)
+        return self.files
+
+    def _require_files(self):
+        if self.files is None:
+            raise ETGError('restart files not initialised (initialize_files)')
+        return self.files
+
+    def save_amplitudes(self):
+        """save_amplitudes_ccs (file_handling_ccs.F90:147-167): t and the energy."""
+        f = self._require_files()
+        f.make_directory()
+        f.t.save(self.get_amplitudes(), self.energy)
+
+    def read_amplitudes(self):
+        """read_amplitudes_ccs (file_handling_ccs.F90:170-195); returns the number read."""
+        v, _, n_read = self._require_files().t.read()
+        self.set_amplitudes(v)
+        return n_read
+
+    def save_multipliers(self):
+        """save_multipliers_ccs (file_handling_ccs.F90:198-218): tbar and the energy."""
+        f = self._require_files()
+        f.make_directory()
+        f.tbar.save(self.get_multipliers(), self.energy)
+
+    def read_multipliers(self):
+        """read_multipliers_ccs (file_handling_ccs.F90:221-246); returns the number read."""
+        v, _, n_read = self._require_files().tbar.read()
+        self.set_multipliers(v)
+        return n_read
+
+    def save_excited_state(self, X, first, last, side, energies):
+        """save_excited_state_ccs (file_handling_ccs.F90:249-286): columns of X are the
+        states first..last (1-based) of `side`, each file with its excitation energy."""
+        f = self._require_files()
+        f.make_directory()
+        X = np.asarray(X, dtype=np.float64).reshape(self.n_gs_amplitudes, -1)
+        if X.shape[1] != last - first + 1 or len(energies) != X.shape[1]:
+            raise ETGError('save_excited_state: states %d..%d need as many vectors and energies'
+                           % (first, last))
+        for c, k in enumerate(range(first, last + 1)):
+            f.side(side, k).save(X[:, c], energies[c])
+
+    def read_excited_state(self, first, last, side):
+        """read_excited_state_ccs (file_handling_ccs.F90:329-373): (X, energies)."""
+        f = self._require_files()
+        X = np.empty((self.n_gs_amplitudes, last - first + 1), order='F')
+        e = np.empty(last - first + 1)
+        for c, k in enumerate(range(first, last + 1)):
+            X[:, c], e[c], _ = f.side(side, k).read()
+        return X, e
+
+    def restart_start_vectors(self, n_states, side='right'):
+        """Davidson start vectors on restart (es_start_vector_tool_class.F90:60-111).
+
+        State k comes from its own file, or else from the other side's through
+        get_restart_vector_doubles, normalised; a state with neither file gets the unit
+        vector on its orbital difference (the k-th lowest).  Returns (start, energies,
+        found) with start (n_t1 + n_t2) x n_states.
+        """
+        f = self._require_files()
+        S = np.zeros((self.n_gs_amplitudes, n_states), order='F')
+        e = np.zeros(n_states)
+        found = np.zeros(n_states, dtype=bool)
+        lowest = None
+        for k in range(n_states):
+            got = f.check_and_get_restart_vector(k + 1, side)
+            if got is not None:
+                v, e[k] = got
+                S[:, k] = v / np.linalg.norm(v)
+                found[k] = True
+            else:
+                if lowest is None:
+                    lowest = np.argsort(self.get_orbital_differences(), kind='stable')
+                S[lowest[k], k] = 1.0
+        return S, e, found
 
     # ---- residual path --------------------------------------------------
     def construct_t1_cholesky(self):
@@ -360,7 +460,26 @@ class CCSD:
     def prepare_for_multiplier_equation(self):
         _check(self._lib.etg_prepare_for_multiplier_equation(self._h))
 
-    def set_initial_multipliers_guess(self):
+    def set_initial_multipliers_guess(self, restart=False):
+        """set_initial_multipliers_guess_ccsd (ccsd_class.F90:423-470).
+
+        Without restart (or with no tbar file): tbar = (t1, 2 t_aibj - t_ajbi) on the
+        device.  With restart, the tbar file: a whole vector is taken as is; a
+        singles-only one keeps its singles and gets the doubles of the guess.
+        """
+        if restart and self.files is not None and self.files.tbar.exists():
+            v, _, n_read = self.files.tbar.read()
+            if n_read == self.n_gs_amplitudes:
+                self.set_multipliers(v)
+                return
+            if n_read != self.n_t1:
+                raise ETGError('Did not recognize number of multipliers on file expected %d or %d '
+                               'found %d' % (self.n_gs_amplitudes, self.n_t1, n_read))
+            _check(self._lib.etg_set_initial_multipliers_guess(self._h))
+            guess = self.get_multipliers()
+            guess[:self.n_t1] = v[:self.n_t1]
+            self.set_multipliers(guess)
+            return
         _check(self._lib.etg_set_initial_multipliers_guess(self._h))
 
     def set_multipliers(self, tbar):

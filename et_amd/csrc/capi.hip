@@ -399,6 +399,16 @@ int etg_molecule_run(etg_molecule* molecule, double cholesky_threshold, double s
   return guarded([&] { mol_of(molecule).run(cholesky_threshold, scf_threshold, max_scf_iterations); });
 }
 
+int etg_molecule_set_orbitals(etg_molecule* molecule, int64_t n_ao, int64_t n_mo, const double* C,
+                              const double* eps) {
+  return guarded([&] {
+    etg::Molecule& m = mol_of(molecule);
+    if (n_ao != m.n_ao() || n_mo != m.n_ao())
+      etg::fail(n_ao != m.n_ao() ? "Number of AOs does not match" : "Number of MOs does not match");
+    m.set_orbitals(C, eps);
+  });
+}
+
 int etg_molecule_dims(etg_molecule* molecule, int64_t* n_ao, int64_t* n_occ, int64_t* n_J,
                       int64_t* scf_iterations) {
   return guarded([&] {

@@ -817,8 +817,10 @@ void Molecule::scf(const double* S_host, const double* H_host, double conv, int6
   std::vector<double> hF(nn), hD(nn), hH(H_host, H_host + nn), hErr(nn);
   double e_old = 0.0;
   const double* Lao = Lao_.as<double>();
+  if (given_orbitals_)
+    ETG_HIP(hipMemcpyAsync(Cd.as<double>(), C_.data(), sizeof(double) * nn, hipMemcpyHostToDevice, stream_));
   for (int64_t it = 0; it < max_iterations; ++it) {
-    diag_fock();
+    if (!given_orbitals_) diag_fock();
     // D = C_occ C_occ^T
     etg::gemm(stream_, 'N', 'T', n, n, o, 1.0, Cd.as<double>(), n, Cd.as<double>(), n, 0.0, D.as<double>(), n,
               Prec::F64, &ws);
@@ -853,6 +855,12 @@ void Molecule::scf(const double* S_host, const double* H_host, double conv, int6
       emax = std::max(emax, std::fabs(hErr[k]));
     }
     ++scf_iterations_;
+    if (given_orbitals_) {   // skipped SCF: the energy of the given orbitals, which must be converged
+      if (!(emax < conv)) fail("scf: cannot skip scf when gradient has not converged");
+      e_hf_ = e;
+      h_ao_.assign(hH.begin(), hH.end());
+      return;
+    }
     if (emax < conv && std::fabs(e - e_old) < 1e-13) {
       e_hf_ = e;
       diag_fock();
@@ -925,6 +933,14 @@ void Molecule::run(double cholesky_threshold, double scf_threshold, int64_t max_
   scf(S.data(), H.data(), scf_threshold, max_scf_iterations);
   mo_transform();
   done_ = true;
+}
+
+void Molecule::set_orbitals(const double* C, const double* eps) {
+  if (!C || !eps) fail("set_orbitals: null pointer");
+  const size_t nn = size_t(n_) * size_t(n_);
+  C_.assign(C, C + nn);
+  eps_.assign(eps, eps + size_t(n_));
+  given_orbitals_ = true;
 }
 
 void Molecule::mo_transform() {

@@ -39,6 +39,10 @@ class Molecule {
 
   // the whole producer: integrals, decomposition, RHF, MO vectors
   void run(double cholesky_threshold, double scf_threshold, int64_t max_scf_iterations);
+  // restart orbitals (n x n column-major C, n orbital energies) for the next run:
+  // the RHF iterations are skipped, one Fock build gives E_HF and checks that
+  // max|FDS - SDF| < scf_threshold (eT's `skip` of a restarted SCF)
+  void set_orbitals(const double* C, const double* eps);
   // after run: E_nuc, E_HF, eps (n), h_mo (n x n), L_mo (n_J x n x n, J fastest), C (n x n); any may be null
   void results(double* e_nuc, double* e_hf, double* eps, double* h_mo, double* L_mo, double* C) const;
 
@@ -58,7 +62,7 @@ class Molecule {
   std::vector<double> Z_, xyz_;
   int64_t n_ = 0, n_occ_ = 0, n_J_ = 0, scf_iterations_ = 0;
   double e_nuc_ = 0.0, e_hf_ = 0.0;
-  bool done_ = false;
+  bool done_ = false, given_orbitals_ = false;
   hipStream_t stream_ = nullptr;
   DeviceBuffer G_, Lao_, Lmo_;
   std::vector<double> C_, eps_, h_ao_, h_mo_;

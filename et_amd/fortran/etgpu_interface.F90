@@ -286,6 +286,14 @@ module etgpu_interface
          integer(c_int64_t), value :: max_scf_iterations
       end function etg_molecule_run
 !
+      integer(c_int) function etg_molecule_set_orbitals(molecule, n_ao, n_mo, C, eps) &
+                                                        bind(C, name='etg_molecule_set_orbitals')
+         import :: c_int, c_ptr, c_double, c_int64_t
+         type(c_ptr), value :: molecule
+         integer(c_int64_t), value :: n_ao, n_mo
+         real(c_double), dimension(*), intent(in) :: C, eps
+      end function etg_molecule_set_orbitals
+!
       integer(c_int) function etg_molecule_dims(molecule, n_ao, n_occ, n_J, scf_iterations) &
                                                 bind(C, name='etg_molecule_dims')
          import :: c_int, c_ptr, c_int64_t

@@ -20,6 +20,7 @@ import os
 import numpy as np
 
 from . import ccsd as _ccsd
+from . import restart as _restart
 
 # eT's Bohr radius in Angstrom (CODATA 2014, src/various/parameters.F90:105)
 BOHR_TO_ANGSTROM = 0.52917721067
@@ -161,6 +162,23 @@ class Molecule:
         G = np.empty((n * n, n * n), order='F')
         _ccsd._check(self._lib.etg_molecule_ao_eri(self._h, _ccsd._ptr(G)))
         return G
+
+    def set_orbitals(self, C, eps):
+        """Restart orbitals for run(): C (n_ao, n_mo) and their energies are used as they
+        are, the RHF iterations skipped (eT's restarted SCF with `skip`,
+        hf_class.F90:1525-1537); run() then checks they are converged."""
+        C = np.asfortranarray(C, dtype=np.float64)
+        eps = np.ascontiguousarray(eps, dtype=np.float64)
+        if C.ndim != 2 or eps.shape != (C.shape[1],):
+            raise _ccsd.ETGError('orbitals: C must be (n_ao, n_mo) with n_mo orbital energies')
+        _ccsd._check(self._lib.etg_molecule_set_orbitals(self._h, C.shape[0], C.shape[1], _ccsd._ptr(C),
+                                                         _ccsd._ptr(eps)))
+
+    def read_orbitals(self, path):
+        """set_orbitals from an eT orbital_coefficients file (file_handling_hf.F90:55-81)."""
+        n = self.dims()['n_ao']
+        C, eps = _restart.OrbitalFile(path).read(n_ao=n, n_mo=n)
+        self.set_orbitals(C, eps)
 
     def run(self, cholesky_threshold=1e-8, scf_threshold=1e-10, max_scf_iterations=200):
         """Integrals, decomposition, RHF, MO vectors; returns the CCSD inputs."""

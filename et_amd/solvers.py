@@ -36,9 +36,9 @@ class DiisCCGroundState:
         self.history = []
 
     def run(self, restart=False):
+        """restart: start from the wavefunction's t file (initialize_files) where it has one."""
         wf = self.wf
-        if not restart:
-            wf.set_initial_amplitudes_guess()
+        wf.set_initial_amplitudes_guess(restart)
         wf.diis_initialize(self.diis_dimension)
         previous = 0.0
         for it in range(1, self.max_iterations + 1):
@@ -149,11 +149,12 @@ class DiisCCMultipliers:
         self.log = log
         self.history = []
 
-    def run(self):
+    def run(self, restart=False):
+        """restart: start from the wavefunction's tbar file (initialize_files) where it has one."""
         wf = self.wf
         wf.prepare_for_multiplier_equation()
         wf.diis_initialize(self.diis_dimension)
-        wf.set_initial_multipliers_guess()
+        wf.set_initial_multipliers_guess(restart)
         for it in range(1, self.max_iterations + 1):
             wf.construct_multiplier_equation(copy_out=False)
             norm = wf.multiplier_residual_norm()
@@ -175,7 +176,8 @@ class EigenDavidson:
     """Lowest eigenpairs of the CC Jacobian, eT's eigen Davidson solver on the device.
 
     run_eigen_davidson_solver (src/solvers/eigen_davidson_solver_class.F90:122-318):
-    start vectors on the lowest orbital differences, each new trial vector
+    start vectors on the lowest orbital differences (with `restart`, the states the
+    wavefunction's restart files hold), each new trial vector
     transformed by the Jacobian (right) or its transpose (left), the reduced
     matrix X^T (A X) and its non-symmetric eigenproblem, residuals preconditioned
     by the orbital differences and orthonormalised against the space.  Trial
@@ -184,8 +186,9 @@ class EigenDavidson:
     """
 
     def __init__(self, wf, n_states, side='right', residual_threshold=1e-6, max_iterations=100,
-                 max_dim=0, precision=_ccsd.FP64, log=None, start=None):
+                 max_dim=0, precision=_ccsd.FP64, log=None, start=None, restart=False):
         self.wf = wf
+        self.restart = restart
         self.n_states = n_states
         self.side = side
         self.residual_threshold = residual_threshold
@@ -198,8 +201,11 @@ class EigenDavidson:
     def run(self):
         wf = self.wf
         wf.prepare_for_jacobian(self.side)
+        start = self.start
+        if start is None and self.restart:
+            start = wf.restart_start_vectors(self.n_states, self.side)[0]
         w, hist, X = wf.davidson(self.n_states, self.side, self.residual_threshold, self.max_iterations,
-                                 self.max_dim, self.precision, vectors=True, start=self.start)
+                                 self.max_dim, self.precision, vectors=True, start=start)
         self.history = hist
         self.eigenvectors = X
         if self.log:
@@ -221,8 +227,10 @@ class MixedPrecisionDavidson:
     """
 
     def __init__(self, wf, n_states, side='right', residual_threshold=1e-6, max_iterations=100,
-                 max_dim=0, schedule=(_ccsd.BF16X3, _ccsd.FP32), margin=10.0, log=None, seed=7, start=None):
+                 max_dim=0, schedule=(_ccsd.BF16X3, _ccsd.FP32), margin=10.0, log=None, seed=7, start=None,
+                 restart=False):
         self.wf = wf
+        self.restart = restart
         self.n_states = n_states
         self.side = side
         self.residual_threshold = residual_threshold
@@ -249,6 +257,8 @@ class MixedPrecisionDavidson:
         wf.prepare_for_jacobian(self.side)
         self._measure_noise()
         start = self.start
+        if start is None and self.restart:
+            start = wf.restart_start_vectors(self.n_states, self.side)[0]
         phases = [(p, max(self.residual_threshold, self.margin * self.noise[p])) for p in self.schedule]
         phases.append((_ccsd.FP64, self.residual_threshold))
         for p, threshold in phases:

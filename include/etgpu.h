@@ -319,6 +319,16 @@ int etg_molecule_create(int64_t n_atoms, const double* charges, const double* xy
 int etg_molecule_run(etg_molecule* molecule, double cholesky_threshold, double scf_threshold,
                      int64_t max_scf_iterations);
 
+/* Restart orbitals for etg_molecule_run: C (n_ao x n_mo, column-major) and the
+ * orbital energies, as eT's orbital_coefficients file holds them; n_ao and n_mo
+ * must equal the molecule's AO count.  The run then takes these orbitals as they
+ * are (no RHF iterations): one Fock build gives E_HF, and the run fails unless
+ * max|FDS - SDF| < scf_threshold.
+ * Replaces: read_for_scf_restart_hf with the SCF skipped (src/wavefunctions/hf/
+ * hf_class.F90:326-338, 1525-1537; read_orbitals_hf, file_handling_hf.F90:55-81). */
+int etg_molecule_set_orbitals(etg_molecule* molecule, int64_t n_ao, int64_t n_mo, const double* C,
+                              const double* eps);
+
 /* Sizes: AO (= MO) count, occupied orbitals, Cholesky vectors, RHF iterations. */
 int etg_molecule_dims(etg_molecule* molecule, int64_t* n_ao, int64_t* n_occ, int64_t* n_J,
                       int64_t* scf_iterations);
