@@ -41,6 +41,7 @@ METRIC = 'CCSD omega evals/s & time-to-solution, mixed vs FP64 GPU at |ΔE|≤1e
 PEAKS = {'fp64': 78.6e12, 'fp32': 157.3e12, 'bf16x3': 2.5e15 / 3}
 WORKLOAD = dict(n_o=48, n_v=400, n_J=1600, seed=0)
 HARTREE_TO_EV = 27.21138602          # CODATA 2014, the factor eT prints
+_PN = {ccsd.BF16X3: 'bf16x3', ccsd.FP32: 'fp32', ccsd.FP64: 'fp64'}
 WORKLOAD_NAME = 'omega_o48_v400_J1600_seed0'
 # HBM bytes per launch of the residual's dominant kernel on this workload: rocprofv3
 # --pmc FETCH_SIZE and --pmc WRITE_SIZE (KiB per dispatch, averaged over the kernel's
@@ -258,6 +259,78 @@ def transforms_profile(side, precision, copies=4, n_vec=12):
                       'ms_per_transform': dt / n_vec * 1e3}), flush=True)
 
 
+def lambda_dipole(shape=(4, 2, 2), seed=0, gs_threshold=1e-10, lambda_threshold=1e-8, fp64=False):
+    """Ground state, Lambda equations and the CC-Lagrangian dipole of a seeded water
+    lattice ((H2O)_16 / cc-pVDZ for the 4 x 2 x 2 default: o = 80, v = 304): the
+    device producer, then split-bf16 -> FP32 -> FP64 ground-state and multiplier
+    iterations (FP64 decides convergence), then <mu> from one tangent replay.  With
+    fp64, the same ground state and multipliers with FP64 products throughout, and
+    the differences."""
+    from et_amd import molecule, properties, solvers
+    n_mol = shape[0] * shape[1] * shape[2]
+    out = {'workload': 'h2o%d_ccpvdz_seed%d' % (n_mol, seed), 'gs_threshold': gs_threshold,
+           'lambda_threshold': lambda_threshold}
+    t0 = time.perf_counter()
+    mol = molecule.Molecule(molecule.water_lattice(shape, 2.9, seed))
+    r = mol.run(cholesky_threshold=1e-8, scf_threshold=1e-10)
+    n_o = r['n_o']
+    n_v = r['L_mo'].shape[1] - n_o
+    out.update(n_o=n_o, n_v=n_v, n_J=r['n_J'], e_hf=r['e_hf'], producer_s=time.perf_counter() - t0)
+    log('(H2O)%d producer: %.1f s, o %d v %d n_J %d, E_HF %.10f' % (n_mol, out['producer_s'], n_o, n_v,
+                                                                   r['n_J'], r['e_hf']))
+    runs = [('mixed', True)] + ([('fp64', False)] if fp64 else [])
+    tbar_mixed = None
+    for name, mixed in runs:
+        res = {}
+        with ccsd.CCSD(n_o, n_v, r['n_J']) as wf:
+            load(wf, r)
+            t0 = time.perf_counter()
+            if mixed:
+                gs = solvers.MixedPrecisionGroundState(wf, residual_threshold=gs_threshold, log=log)
+                gs.run()
+                res['gs_iterations'] = {_PN[p]: sum(1 for h in gs.history if h[0] == p) for p in _PN}
+            else:
+                gs = solvers.DiisCCGroundState(wf, residual_threshold=gs_threshold, log=log)
+                gs.run()
+                res['gs_iterations'] = {'fp64': len(gs.history)}
+            wf.synchronize()
+            res['gs_s'] = time.perf_counter() - t0
+            res['correlation_energy'] = wf.correlation_energy
+            log('%s ground state: %.1f s, E_corr %.12f' % (name, res['gs_s'], wf.correlation_energy))
+            t0 = time.perf_counter()
+            if mixed:
+                lam = solvers.MixedPrecisionMultipliers(wf, residual_threshold=lambda_threshold, log=log)
+                tbar = lam.run()
+                res['lambda_iterations'] = {_PN[p]: sum(1 for h in lam.history if h[0] == p) for p in _PN}
+                res['lambda_noise'] = {_PN[p]: v for p, v in lam.noise.items()}
+            else:
+                lam = solvers.DiisCCMultipliers(wf, residual_threshold=lambda_threshold, log=log)
+                tbar = lam.run()
+                res['lambda_iterations'] = {'fp64': len(lam.history)}
+            wf.synchronize()
+            res['lambda_s'] = time.perf_counter() - t0
+            res['jacobian_memory_gb'] = {k: v / 1e9 for k, v in wf.jacobian_memory().items()}
+            t0 = time.perf_counter()
+            mu = properties.dipole_moment(wf, mol, r['C'])
+            res['dipole_s'] = time.perf_counter() - t0
+            res['dipole_debye'] = [float(x) for x in mu['total_debye']]
+            log('%s Lambda: %.1f s; dipole %s D (%.1f s)' % (name, res['lambda_s'], res['dipole_debye'],
+                                                             res['dipole_s']))
+        if tbar_mixed is None:
+            tbar_mixed = tbar
+        else:
+            res['max_abs_tbar_diff_vs_mixed'] = float(np.abs(tbar - tbar_mixed).max())
+        out[name] = res
+    mol.close()
+    if fp64:
+        out['abs_e_corr_diff'] = abs(out['mixed']['correlation_energy'] - out['fp64']['correlation_energy'])
+        out['max_abs_dipole_diff_debye'] = max(abs(a - b) for a, b in zip(out['mixed']['dipole_debye'],
+                                                                         out['fp64']['dipole_debye']))
+        for k in ('gs_s', 'lambda_s'):
+            out['speedup_' + k[:-2]] = out['fp64'][k] / out['mixed'][k]
+    return out
+
+
 def copy_calibration(nbytes=1 << 30, reps=3):
     """Plain copies of known size: each reads and writes `nbytes` of HBM once.
 
@@ -287,6 +360,11 @@ def main():
     ap.add_argument('--transforms-profile', nargs=2, metavar=('SIDE', 'PRECISION'),
                     help='only run two groups of 12 EOM transformations (right|left, fp64|fp32|bf16x3) '
                          'on four H2O-He copies, graphs off (for kernel traces)')
+    ap.add_argument('--lambda-dipole', action='store_true',
+                    help='only run the (H2O)16 / cc-pVDZ ground state, Lambda equations and dipole '
+                         '(mixed precision; a few minutes)')
+    ap.add_argument('--fp64-reference', action='store_true',
+                    help='with --lambda-dipole: also the FP64 ground state and Lambda, and the differences')
     ap.add_argument('--copy-calibration', action='store_true',
                     help='only run three 1 GiB device-to-device copies (HBM counter calibration)')
     args = ap.parse_args()
@@ -295,6 +373,9 @@ def main():
         return
     if args.transforms_profile:
         transforms_profile(*args.transforms_profile)
+        return
+    if args.lambda_dipole:
+        print(json.dumps({'lambda_dipole': lambda_dipole(fp64=args.fp64_reference)}), flush=True)
         return
     if args.gpus != 1:
         print('bench.py measures one GPU: every box in this setup has a single MI355X, so '

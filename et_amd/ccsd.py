@@ -70,7 +70,7 @@ _SIGNATURES = {
     'etg_set_multipliers': [ctypes.c_void_p, _c_dbl_p],
     'etg_get_multipliers': [ctypes.c_void_p, _c_dbl_p],
     'etg_construct_eta': [ctypes.c_void_p, _c_dbl_p],
-    'etg_construct_multiplier_equation': [ctypes.c_void_p, _c_dbl_p],
+    'etg_construct_multiplier_equation': [ctypes.c_void_p, _c_dbl_p, ctypes.c_int],
     'etg_multiplier_residual_norm': [ctypes.c_void_p, _c_dbl_p],
     'etg_multiplier_quasi_newton_update': [ctypes.c_void_p],
     'etg_multiplier_diis_update': [ctypes.c_void_p],
@@ -79,6 +79,7 @@ _SIGNATURES = {
     'etg_molecule_dipole_integrals': [ctypes.c_void_p, _c_dbl_p, _c_dbl_p],
     'etg_set_transform_batch': [ctypes.c_void_p, _i64],
     'etg_set_kernel_ledger': [ctypes.c_void_p, ctypes.c_int],
+    'etg_jacobian_memory': [ctypes.c_void_p, ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_i64)],
     'etg_kernel_ledger': [ctypes.c_void_p, _i64, ctypes.c_char_p, _i64, ctypes.POINTER(_i64),
                           _c_dbl_p, _c_dbl_p, _c_dbl_p, ctypes.POINTER(_i64)],
     'etg_test_gemm': [ctypes.c_char, ctypes.c_char, _i64, _i64, _i64, ctypes.c_double, _c_dbl_p,
@@ -498,9 +499,11 @@ class CCSD:
         _check(self._lib.etg_construct_eta(self._h, _ptr(out) if copy_out else None))
         return out
 
-    def construct_multiplier_equation(self, copy_out=True):
+    def construct_multiplier_equation(self, copy_out=True, precision=FP64):
+        """R = A^T tbar + eta, the left transformation in `precision`."""
         out = np.empty(self.n_gs_amplitudes) if copy_out else None
-        _check(self._lib.etg_construct_multiplier_equation(self._h, _ptr(out) if copy_out else None))
+        _check(self._lib.etg_construct_multiplier_equation(self._h, _ptr(out) if copy_out else None,
+                                                           int(precision)))
         return out
 
     def multiplier_residual_norm(self):
@@ -529,10 +532,16 @@ class CCSD:
         """Replay captured HIP graphs of the transformation passes (default) or run them eagerly."""
         _check(self._lib.etg_set_transform_graphs(self._h, 1 if on else 0))
 
-
     def set_transform_batch(self, max_vectors):
         """At most max_vectors trial vectors per transformation pass (0: automatic)."""
         _check(self._lib.etg_set_transform_batch(self._h, int(max_vectors)))
+
+    def jacobian_memory(self):
+        """Bytes of the Jacobian record and per trial vector of a pass (after prepare_for_jacobian)."""
+        v = [_i64(), _i64(), _i64()]
+        _check(self._lib.etg_jacobian_memory(self._h, *[ctypes.byref(x) for x in v]))
+        return dict(record=v[0].value, per_vector=v[1].value, per_vector_unshared=v[2].value)
+
     def set_kernel_ledger(self, on):
         _check(self._lib.etg_set_kernel_ledger(self._h, 1 if on else 0))
 

@@ -232,8 +232,8 @@ int etg_construct_eta(etg_handle* h, double* eta) {
   return guarded([&] { wf_of(h).construct_eta(eta); });
 }
 
-int etg_construct_multiplier_equation(etg_handle* h, double* residual) {
-  return guarded([&] { wf_of(h).construct_multiplier_equation(residual); });
+int etg_construct_multiplier_equation(etg_handle* h, double* residual, int precision) {
+  return guarded([&] { wf_of(h).construct_multiplier_equation(residual, prec_of(precision)); });
 }
 
 int etg_multiplier_residual_norm(etg_handle* h, double* norm) {
@@ -264,6 +264,11 @@ int etg_set_transform_graphs(etg_handle* h, int on) {
 
 int etg_set_transform_batch(etg_handle* h, int64_t max_vectors) {
   return guarded([&] { wf_of(h).set_transform_batch(max_vectors); });
+}
+
+int etg_jacobian_memory(etg_handle* h, int64_t* record_bytes, int64_t* bytes_per_vector,
+                        int64_t* unshared_bytes_per_vector) {
+  return guarded([&] { wf_of(h).jacobian_memory(record_bytes, bytes_per_vector, unshared_bytes_per_vector); });
 }
 
 int etg_set_kernel_ledger(etg_handle* h, int on) {

@@ -822,17 +822,20 @@ void CCSD::prepare_for_jacobian() {
   tape->register_buffer(h_.as<double>(), n * n, false, "h");   // one-electron perturbations
   tape->register_buffer(g_ovov_.as<double>(), n_t1 * n_t1, true, "g_ovov");
   tape->register_buffer(t_.as<double>(), n_t1 + n_t2, false, "t");
-  tape->register_buffer(F_.as<double>(), n * n, false, "F");
+  tape->register_buffer(F_.as<double>(), n * n, false, "F", true);
   tape->register_buffer(omega_.as<double>(), n_t1 + n_t2, false, "omega");
-  tape->register_buffer(Lt_oo_.as<double>(), J * o * o, false, "L_t1 oo");
-  tape->register_buffer(Lt_vo_.as<double>(), J * v * o, false, "L_t1 vo");
-  tape->register_buffer(Lt_vv_.as<double>(), J * v * v, false, "L_t1 vv");
-  tape->register_buffer(Lt_vvT_.as<double>(), J * v * v, false, "L_t1 vvT");
+  tape->register_buffer(Lt_oo_.as<double>(), J * o * o, false, "L_t1 oo", true);
+  tape->register_buffer(Lt_vo_.as<double>(), J * v * o, false, "L_t1 vo", true);
+  tape->register_buffer(Lt_vv_.as<double>(), J * v * v, false, "L_t1 vv", true);
+  tape->register_buffer(Lt_vvT_.as<double>(), J * v * v, false, "L_t1 vvT", true);
   program_t1_cholesky(*tape);
   program_fock(*tape);
   program_omega(*tape, Prec::F64);
   build_ladder_t1_intermediate();       // t and L_t1 of this record
   synchronize();
+  // the record holds its own intermediates: the direct program's (ov)^2 slots
+  // are given back (and taken again by the next direct residual)
+  for (DeviceBuffer& w : work_) w.release();
   drop_transform_graphs();
   tape_ = std::move(tape);
   have_F_ = true;
@@ -849,11 +852,6 @@ void CCSD::jacobian_transpose_transform(const double* b, double* sigma) {
   transform_batch(1, 1, b, sigma, Prec::F64);
 }
 
-// sigma = A^T b on device vectors (b is read before sigma is written; they may alias).
-void CCSD::transpose_transform_device(const double* b, double* sigma) {
-  transform_batch(1, 1, b, sigma, Prec::F64);
-}
-
 // Vectors per transformation pass for the current record: as many as fit in
 // 35 % of the free HBM (the record's derivative buffers and the ladder's
 // per-vector work, per vector), at most kMaxTransformBatch or the caller's limit.
@@ -865,6 +863,14 @@ int64_t CCSD::transform_batch_capacity() {
   nb = std::max<int64_t>(1, std::min<int64_t>(nb, kMaxTransformBatch));
   if (transform_batch_limit_ > 0) nb = std::min(nb, transform_batch_limit_);
   return nb;
+}
+
+void CCSD::jacobian_memory(int64_t* record, int64_t* per_vector, int64_t* per_vector_unshared) {
+  require(tape_ != nullptr, "jacobian_memory: prepare_for_jacobian has not been called");
+  const int64_t extra = ladder_vector_elems() + 2 * (n_t1 + n_t2);
+  if (record) *record = 8 * tape_->record_elems();
+  if (per_vector) *per_vector = 8 * (tape_->derivative_elems() + extra);
+  if (per_vector_unshared) *per_vector_unshared = 8 * (tape_->unshared_derivative_elems() + extra);
 }
 
 void CCSD::set_transform_batch(int64_t nb) {

@@ -59,7 +59,8 @@ class CCSD {
   void set_multipliers(const double* host);
   void get_multipliers(double* host);
   void construct_eta(double* host);               // eta (host may be null: kept in HBM)
-  void construct_multiplier_equation(double* host);   // R = A^T t-bar + eta (host may be null)
+  // R = A^T t-bar + eta (host may be null), the left transformation in `prec`
+  void construct_multiplier_equation(double* host, Prec prec = Prec::F64);
   double multiplier_residual_norm();
   void multiplier_quasi_newton_update();          // e = -R/eps; t-bar += e
   void multiplier_diis_update();                  // records (e, t-bar); t-bar extrapolated
@@ -78,6 +79,9 @@ class CCSD {
   // at most `nb` vectors per transformation pass (0: as many as HBM allows, <= 16)
   void set_transform_batch(int64_t nb);
   int64_t transform_batch() const { return nb_cap_; }
+  // bytes of the record's intermediates, and of one trial vector's derivatives
+  // (shared slots, and as one buffer per intermediate would need)
+  void jacobian_memory(int64_t* record, int64_t* per_vector, int64_t* per_vector_unshared);
 
   // Davidson eigensolver on the device (eigen_davidson_solver_class.F90:122-318):
   // n_states lowest roots of A (side 0) or A^T (side 1), transforms in `prec`,
@@ -129,7 +133,6 @@ class CCSD {
   void require(bool ok, const char* what) const;
   void build_ladder_plan();
   void diis_step(const double* e, double* x);
-  void transpose_transform_device(const double* b, double* sigma);  // device pointers
   // the residual pipeline, written once against Ops
   void program_t1_cholesky(Ops& X);
   void program_fock(Ops& X);

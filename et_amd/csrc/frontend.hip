@@ -27,6 +27,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -358,7 +359,8 @@ __global__ __launch_bounds__(kEriThreads) void eri_kernel(const DevShell* __rest
                                                           const int32_t* __restrict__ pair_b,
                                                           const double* __restrict__ exps,
                                                           const double* __restrict__ coefs,
-                                                          int64_t npairs, int64_t n, double* __restrict__ G) {
+                                                          int64_t npairs, int64_t n, int64_t q0,
+                                                          double* __restrict__ G) {
   __shared__ double Eab[3][kEsz], Ecd[3][kEsz];
   __shared__ double Rc[2][kCube * kCube * kCube];
   __shared__ double acc[kMaxCart * kMaxCart * kMaxCart * kMaxCart];
@@ -366,7 +368,7 @@ __global__ __launch_bounds__(kEriThreads) void eri_kernel(const DevShell* __rest
   __shared__ double Fb[kMaxLtot + 1];
   __shared__ double sc[4];
   const int tid = threadIdx.x;
-  const int64_t q = blockIdx.x;
+  const int64_t q = q0 + int64_t(blockIdx.x);
   const int64_t P = tri_root_f(q), Q = q - P * (P + 1) / 2;     // P >= Q
   if (P >= npairs) return;
   const DevShell A = shells[pair_a[P]], B = shells[pair_b[P]];
@@ -694,7 +696,6 @@ void Molecule::build_eri() {
     }
   const int64_t npairs = int64_t(pa.size());
   const int64_t nquart = npairs * (npairs + 1) / 2;
-  if (nquart > int64_t(0x7fffffff)) fail("molecule: too many shell quartets for one launch");
   DeviceBuffer dshell(sizeof(DevShell) * ds.size()), dpa(sizeof(int32_t) * pa.size()), dpb(sizeof(int32_t) * pb.size());
   DeviceBuffer dex(sizeof(double) * ex.size()), dco(sizeof(double) * co.size());
   ETG_HIP(hipMemcpy(dshell.as<void>(), ds.data(), sizeof(DevShell) * ds.size(), hipMemcpyHostToDevice));
@@ -703,10 +704,21 @@ void Molecule::build_eri() {
   ETG_HIP(hipMemcpy(dex.as<void>(), ex.data(), sizeof(double) * ex.size(), hipMemcpyHostToDevice));
   ETG_HIP(hipMemcpy(dco.as<void>(), co.data(), sizeof(double) * co.size(), hipMemcpyHostToDevice));
   G_.resize(sizeof(double) * size_t(n2) * size_t(n2));
-  eri_kernel<<<unsigned(nquart), kEriThreads, 0, stream_>>>(dshell.as<DevShell>(), dpa.as<int32_t>(),
-                                                           dpb.as<int32_t>(), dex.as<double>(), dco.as<double>(),
-                                                           npairs, n_, G_.as<double>());
-  ETG_LAUNCH("eri");
+  // one workgroup per quartet, in launches of at most 2^25 workgroups (a
+  // dispatch counts its work-items in 32 bits: 2^25 x 64 stays below 2^32);
+  // ETG_ERI_LAUNCH_GROUPS lowers the limit (checks of the chunked launches)
+  int64_t max_groups = int64_t(1) << 25;
+  if (const char* env = std::getenv("ETG_ERI_LAUNCH_GROUPS")) {
+    const long long v = std::atoll(env);
+    if (v > 0) max_groups = std::min<int64_t>(max_groups, v);
+  }
+  for (int64_t q0 = 0; q0 < nquart; q0 += max_groups) {
+    const int64_t groups = std::min(max_groups, nquart - q0);
+    eri_kernel<<<unsigned(groups), kEriThreads, 0, stream_>>>(dshell.as<DevShell>(), dpa.as<int32_t>(),
+                                                             dpb.as<int32_t>(), dex.as<double>(), dco.as<double>(),
+                                                             npairs, n_, q0, G_.as<double>());
+    ETG_LAUNCH("eri");
+  }
   ETG_HIP(hipStreamSynchronize(stream_));
 }
 

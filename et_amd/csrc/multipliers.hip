@@ -116,14 +116,14 @@ void CCSD::construct_eta(double* host) {
   synchronize();
 }
 
-void CCSD::construct_multiplier_equation(double* host) {
+void CCSD::construct_multiplier_equation(double* host, Prec prec) {
   require(have_tbar_, "construct_multiplier_equation: no multipliers (set an initial guess first)");
   require(tape_ != nullptr,
           "construct_multiplier_equation: prepare_for_jacobian has not been called at these amplitudes");
   const int64_t n = n_t1 + n_t2;
   construct_eta(nullptr);
   mres_.resize(sizeof(double) * n);
-  transpose_transform_device(tbar_.as<double>(), mres_.as<double>());
+  transform_batch(1, 1, tbar_.as<double>(), mres_.as<double>(), prec);
   axpy(stream_, n, 1.0, eta_.as<double>(), mres_.as<double>());
   if (host)
     ETG_HIP(hipMemcpyAsync(host, mres_.as<double>(), sizeof(double) * n, hipMemcpyDeviceToHost, stream_));

@@ -81,15 +81,17 @@ double* Tape::buf(int slot, const char* name, int64_t elems) {
   DeviceBuffer b(size_t(std::max<int64_t>(elems, 1)) * sizeof(double));
   double* p = b.as<double>();
   named_.emplace(name, std::move(b));
-  register_buffer(p, elems, false, name);
+  register_buffer(p, elems, false, name, true);
   return p;
 }
 
-void Tape::register_buffer(const double* base, int64_t elems, bool constant, const char* name) {
+void Tape::register_buffer(const double* base, int64_t elems, bool constant, const char* name,
+                           bool internal) {
   auto b = std::make_unique<Buf>();
   b->base = base;
   b->elems = elems;
   b->constant = constant;
+  b->intermediate = internal && !constant;
   b->name = name;
   bufs_.push_back(std::move(b));
 }
@@ -98,6 +100,9 @@ void Tape::clear() {
   ops_.clear();
   bufs_.clear();
   named_.clear();
+  slots_.clear();
+  slot_elems_.clear();
+  planned_ops_ = size_t(-1);
 }
 
 int Tape::find(const double* p) const {
@@ -111,7 +116,12 @@ int Tape::find(const double* p) const {
 double* Tape::deriv(const double* p) {
   Buf& b = *bufs_[size_t(find(p))];
   if (b.constant) return nullptr;
-  if (b.d.bytes() < size_t(b.elems) * sizeof(double)) fail("tape: derivatives not initialised");
+  const size_t need = size_t(std::max<int64_t>(b.elems, 1)) * sizeof(double);
+  if (b.intermediate) {
+    if (b.slot < 0 || slots_[size_t(b.slot)].bytes() < need) fail("tape: derivatives not initialised");
+    return slots_[size_t(b.slot)].as<double>() + (p - b.base);
+  }
+  if (b.d.bytes() < need) fail("tape: derivatives not initialised");
   return b.d.as<double>() + (p - b.base);
 }
 
@@ -119,20 +129,102 @@ int64_t Tape::dstride(const double* p) const {
   return std::max<int64_t>(bufs_[size_t(find(p))]->elems, 1);
 }
 
-int64_t Tape::derivative_elems() const {
+// Lifetimes of the intermediates' derivatives and their slots: intervals in
+// order of their first operation, each placed in the smallest free slot that
+// holds it, else the largest free slot (grown), else a new slot.
+void Tape::plan_derivatives() {
+  if (planned_ops_ == ops_.size()) return;
+  for (auto& bp : bufs_) bp->first = bp->last = -1;
+  auto touch = [&](const double* p, int i) {
+    if (p == nullptr) return;
+    Buf& b = *bufs_[size_t(find(p))];
+    if (b.constant || !b.intermediate) return;
+    if (b.first < 0) b.first = i;
+    b.last = i;
+  };
+  for (size_t i = 0; i < ops_.size(); ++i) {
+    const Op& op = ops_[i];
+    if (op.kind == GEMM) {
+      touch(op.g.A, int(i));
+      touch(op.g.B, int(i));
+      touch(op.g.C, int(i));
+    } else {
+      touch(op.x, int(i));
+      touch(op.x2, int(i));
+      touch(op.y, int(i));
+      touch(op.y2, int(i));
+    }
+  }
+  std::vector<int> order;
+  for (size_t k = 0; k < bufs_.size(); ++k)
+    if (bufs_[k]->intermediate && bufs_[k]->first >= 0) order.push_back(int(k));
+  std::sort(order.begin(), order.end(), [&](int a, int b) { return bufs_[size_t(a)]->first < bufs_[size_t(b)]->first; });
+  slots_.clear();
+  slot_elems_.clear();
+  std::vector<int> busy_until;          // last operation of each slot's current occupant
+  starts_fwd_.assign(ops_.size(), {});
+  starts_bwd_.assign(ops_.size(), {});
+  for (auto& bp : bufs_) bp->slot = -1;
+  for (int k : order) {
+    Buf& b = *bufs_[size_t(k)];
+    const int64_t e = std::max<int64_t>(b.elems, 1);
+    int fit = -1, grow = -1;
+    for (size_t s = 0; s < slot_elems_.size(); ++s) {
+      if (busy_until[s] >= b.first) continue;
+      if (slot_elems_[s] >= e) {
+        if (fit < 0 || slot_elems_[s] < slot_elems_[size_t(fit)]) fit = int(s);
+      } else if (grow < 0 || slot_elems_[s] > slot_elems_[size_t(grow)]) {
+        grow = int(s);
+      }
+    }
+    int s = fit >= 0 ? fit : grow;
+    if (s < 0) {
+      s = int(slot_elems_.size());
+      slot_elems_.push_back(0);
+      busy_until.push_back(-1);
+    }
+    slot_elems_[size_t(s)] = std::max(slot_elems_[size_t(s)], e);
+    busy_until[size_t(s)] = b.last;
+    b.slot = s;
+    starts_fwd_[size_t(b.first)].push_back(k);
+    starts_bwd_[size_t(b.last)].push_back(k);
+  }
+  slots_.resize(slot_elems_.size());
+  planned_ops_ = ops_.size();
+}
+
+int64_t Tape::derivative_elems() {
+  plan_derivatives();
+  int64_t e = 0;
+  for (const auto& bp : bufs_)
+    if (!bp->constant && !bp->intermediate) e += std::max<int64_t>(bp->elems, 1);
+  for (int64_t s : slot_elems_) e += s;
+  return e;
+}
+
+int64_t Tape::unshared_derivative_elems() const {
   int64_t e = 0;
   for (const auto& bp : bufs_)
     if (!bp->constant) e += std::max<int64_t>(bp->elems, 1);
   return e;
 }
 
+int64_t Tape::record_elems() const {
+  int64_t e = 0;
+  for (const auto& kv : named_) e += int64_t(kv.second.bytes() / sizeof(double));
+  return e;
+}
+
 bool Tape::reserve_derivatives(int64_t nb) {
+  plan_derivatives();
   bool moved = false;
   for (auto& bp : bufs_) {
     Buf& b = *bp;
-    if (b.constant) continue;
+    if (b.constant || b.intermediate) continue;
     moved |= b.d.resize(size_t(nb) * size_t(std::max<int64_t>(b.elems, 1)) * sizeof(double));
   }
+  for (size_t s = 0; s < slots_.size(); ++s)
+    moved |= slots_[s].resize(size_t(nb) * size_t(slot_elems_[s]) * sizeof(double));
   return moved;
 }
 
@@ -140,8 +232,16 @@ void Tape::zero_derivatives(int64_t nb) {
   reserve_derivatives(nb);
   for (auto& bp : bufs_) {
     Buf& b = *bp;
-    if (b.constant) continue;
+    if (b.constant || b.intermediate) continue;
     fill(wf_.stream(), b.d.as<double>(), nb * std::max<int64_t>(b.elems, 1), 0.0);
+  }
+}
+
+// nb derivative vectors of each listed intermediate (contiguous from its slot's start).
+void Tape::zero_starting(const std::vector<int>& bufs, int64_t nb) {
+  for (int k : bufs) {
+    const Buf& b = *bufs_[size_t(k)];
+    fill(wf_.stream(), slots_[size_t(b.slot)].as<double>(), nb * std::max<int64_t>(b.elems, 1), 0.0);
   }
 }
 
@@ -441,7 +541,10 @@ void Tape::tangent(Prec prec, int64_t nb) {
     double* d = deriv(p);
     return d ? d + k * dstride(p) : nullptr;
   };
-  for (const Op& op : ops_) {
+  plan_derivatives();
+  for (size_t i = 0; i < ops_.size(); ++i) {
+    const Op& op = ops_[i];
+    zero_starting(starts_fwd_[i], nb);
     if (op.kind == GEMM) {
       tangent_gemm(op.g, prec, nb);
       continue;
@@ -522,8 +625,10 @@ void Tape::adjoint(Prec prec, int64_t nb) {
     double* d = deriv(p);
     return d ? d + k * dstride(p) : nullptr;
   };
-  for (auto it = ops_.rbegin(); it != ops_.rend(); ++it) {
-    const Op& op = *it;
+  plan_derivatives();
+  for (size_t i = ops_.size(); i-- > 0;) {
+    const Op& op = ops_[i];
+    zero_starting(starts_bwd_[i], nb);
     if (op.kind == GEMM) {
       adjoint_gemm(op.g, prec, nb);
       continue;

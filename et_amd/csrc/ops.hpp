@@ -131,8 +131,11 @@ class Tape : public DirectOps {
   void fock_ov_vec(const double* F, double* f) override;
   void ladder(const double* t2, const double* LvvT, double* om2, Prec prec) override;
 
-  // Buffers that exist outside the tape.  Constant ones carry no derivative.
-  void register_buffer(const double* base, int64_t elems, bool constant, const char* name);
+  // Buffers that exist outside the tape.  Constant ones carry no derivative;
+  // `internal` ones are written by the recorded program itself, so their
+  // derivatives live only inside a pass, as those of the tape's own buffers.
+  void register_buffer(const double* base, int64_t elems, bool constant, const char* name,
+                       bool internal = false);
   void clear();
   // Every derivative buffer holds up to `nb` vectors: vector k of the
   // derivative of primal element p sits at deriv(p) + k * dstride(p).
@@ -144,14 +147,20 @@ class Tape : public DirectOps {
   void adjoint(Prec prec, int64_t nb = 1);
   // Room for nb derivative vectors per buffer; true when a buffer moved.
   bool reserve_derivatives(int64_t nb);
-  // Zeroes the first nb derivative vectors of every buffer (allocating on first use).
+  // Zeroes the first nb derivative vectors of the buffers that exist outside
+  // the tape (allocating on first use); the passes zero each intermediate's
+  // derivative where its lifetime starts.
   void zero_derivatives(int64_t nb = 1);
   // Derivative buffer matching a primal pointer (nullptr for constants), vector 0.
   double* deriv(const double* p);
   // Element distance between consecutive derivative vectors of p's buffer.
   int64_t dstride(const double* p) const;
   // Elements of one derivative vector over all buffers (memory per batched vector).
-  int64_t derivative_elems() const;
+  int64_t derivative_elems();
+  // The same without sharing (one derivative buffer per intermediate).
+  int64_t unshared_derivative_elems() const;
+  // Elements of the recorded intermediates (the primal record).
+  int64_t record_elems() const;
   bool empty() const { return ops_.empty(); }
 
   enum Kind { GEMM, SORT4, SQUAREUP, PACKIN, SYMSUM, SYMPACK, COPY_SUB, FILL0, T1_XY, TRACE_OO,
@@ -171,19 +180,34 @@ class Tape : public DirectOps {
   };
 
  private:
+  // The derivative of an intermediate (a buffer the tape allocated) is live
+  // from the first to the last operation that touches the buffer; that
+  // interval is the same for the forward and the backward replay, so
+  // intermediates whose intervals do not overlap share one derivative slot.
+  // The amplitudes, the residual and h (whose derivatives are the passes'
+  // inputs and outputs) keep derivative buffers of their own.
   struct Buf {
     const double* base;
     int64_t elems;
     bool constant;
+    bool intermediate = false;   // derivative lives inside a pass only
+    int first = -1, last = -1;   // operation interval (intermediates)
+    int slot = -1;               // shared derivative slot (intermediates)
     std::string name;
-    DeviceBuffer d;  // derivative
+    DeviceBuffer d;  // derivative (the passes' inputs and outputs)
   };
   int find(const double* p) const;
   void tangent_gemm(const Gemm& g, Prec prec, int64_t nb);
   void adjoint_gemm(const Gemm& g, Prec prec, int64_t nb);
+  void plan_derivatives();
+  void zero_starting(const std::vector<int>& bufs, int64_t nb);
   std::vector<Op> ops_;
   std::vector<std::unique_ptr<Buf>> bufs_;
   std::map<std::string, DeviceBuffer> named_;
+  size_t planned_ops_ = size_t(-1);     // ops_.size() when the slots were planned
+  std::vector<DeviceBuffer> slots_;
+  std::vector<int64_t> slot_elems_;
+  std::vector<std::vector<int>> starts_fwd_, starts_bwd_;   // per operation: lifetimes starting there
 };
 
 }  // namespace etg

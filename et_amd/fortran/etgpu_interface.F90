@@ -197,10 +197,12 @@ module etgpu_interface
       end function etg_construct_eta
 !
 !     construct_multiplier_equation (multiplier_equation_ccs.F90:58-94)
-      integer(c_int) function etg_construct_multiplier_equation(handle, x) bind(C, name='etg_construct_multiplier_equation')
+      integer(c_int) function etg_construct_multiplier_equation(handle, x, precision) &
+                                                   bind(C, name='etg_construct_multiplier_equation')
          import :: c_int, c_ptr, c_double
          type(c_ptr), value :: handle
          real(c_double), dimension(*), intent(out) :: x
+         integer(c_int), value :: precision
       end function etg_construct_multiplier_equation
 !
 !     get_l2_norm of the multiplier residual (diis_cc_multipliers_class.F90:249)
@@ -247,6 +249,15 @@ module etgpu_interface
          type(c_ptr), value :: handle
          integer(c_int64_t), value :: max_vectors
       end function etg_set_transform_batch
+!
+!     diagnostic: bytes of the Jacobian record and per trial vector of a pass
+      integer(c_int) function etg_jacobian_memory(handle, record_bytes, bytes_per_vector, &
+                                                  unshared_bytes_per_vector) &
+                                                  bind(C, name='etg_jacobian_memory')
+         import :: c_int, c_ptr, c_int64_t
+         type(c_ptr), value :: handle
+         integer(c_int64_t), intent(out) :: record_bytes, bytes_per_vector, unshared_bytes_per_vector
+      end function etg_jacobian_memory
 !
 !     construct_omega with the residual left in HBM (omega_ccsd.F90:39-110)
       integer(c_int) function etg_construct_omega_resident(handle, precision) &

@@ -5,6 +5,8 @@ quasi-Newton step and DIIS extrapolation on the device in FP64.
 MixedPrecisionGroundState: the same iteration on split-bf16, FP32, then FP64
 residuals.
 DiisCCMultipliers: run_diis_cc_multipliers (diis_cc_multipliers_class.F90:195-299).
+MixedPrecisionMultipliers: the same iteration on split-bf16, FP32, then FP64 left
+transformations.
 EigenDavidson: run_eigen_davidson_solver (eigen_davidson_solver_class.F90:122-318),
 with the whole Davidson space on the device.
 MixedPrecisionDavidson: the same eigensolver on split-bf16 transformations,
@@ -165,6 +167,66 @@ class DiisCCMultipliers:
                 return wf.get_multipliers()
             wf.multiplier_quasi_newton_update()
             wf.multiplier_diis_update()
+        raise _ccsd.ETGError('was not able to converge the equations in the given number of '
+                             'maximum iterations.')
+
+
+class MixedPrecisionMultipliers:
+    """Multiplier iterations on reduced-precision left transformations, finished on FP64 ones.
+
+    The iteration of DiisCCMultipliers with R = A^T t-bar + eta formed with the left
+    transformation's bulk products in the precisions of `schedule`, then in FP64.
+    Each reduced form's noise floor ||R_p - R_FP64|| is measured once at the
+    starting multipliers; a phase ends when ||R_p|| falls below `margin` times its
+    floor or stops halving within four iterations.  The DIIS space carries over,
+    and the last phase is run_diis_cc_multipliers unchanged, so convergence is
+    decided on the FP64 residual against `residual_threshold`.
+    """
+
+    def __init__(self, wf, residual_threshold=1e-5, max_iterations=100, diis_dimension=8,
+                 schedule=(_ccsd.BF16X3, _ccsd.FP32), margin=10.0, log=None):
+        self.wf = wf
+        self.residual_threshold = residual_threshold
+        self.max_iterations = max_iterations
+        self.diis_dimension = diis_dimension
+        self.schedule = tuple(schedule)
+        self.margin = margin
+        self.log = log
+        self.history = []          # (precision, norm)
+        self.noise = {}
+
+    def run(self, restart=False):
+        wf = self.wf
+        wf.prepare_for_multiplier_equation()
+        wf.diis_initialize(self.diis_dimension)
+        wf.set_initial_multipliers_guess(restart)
+        ref = wf.construct_multiplier_equation(precision=_ccsd.FP64)
+        for p in self.schedule:
+            self.noise[p] = float(np.linalg.norm(wf.construct_multiplier_equation(precision=p) - ref))
+        phases = [(p, self.margin * self.noise[p]) for p in self.schedule]
+        phases.append((_ccsd.FP64, self.residual_threshold))
+        it, phase = 0, 0
+        best, since_best = np.inf, 0
+        while it < self.max_iterations:
+            it += 1
+            precision, threshold = phases[phase]
+            wf.construct_multiplier_equation(copy_out=False, precision=precision)
+            norm = wf.multiplier_residual_norm()
+            self.history.append((precision, norm))
+            if self.log:
+                self.log('%3d  %-6s %11.4e' % (it, _PREC_NAMES[precision], norm))
+            if norm < threshold and precision == _ccsd.FP64:
+                return wf.get_multipliers()
+            if norm < 0.5 * best:
+                best, since_best = norm, 0
+            else:
+                since_best += 1
+            advance = precision != _ccsd.FP64 and (norm < threshold or since_best >= 4)
+            wf.multiplier_quasi_newton_update()
+            wf.multiplier_diis_update()
+            if advance:
+                phase += 1
+                best, since_best = np.inf, 0
         raise _ccsd.ETGError('was not able to converge the equations in the given number of '
                              'maximum iterations.')
 

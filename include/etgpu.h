@@ -197,9 +197,11 @@ int etg_get_multipliers(etg_handle* handle, double* tbar);
 int etg_construct_eta(etg_handle* handle, double* eta);
 
 /* R = A^T t-bar + eta on the device (needs etg_prepare_for_multiplier_equation
- * at the current amplitudes).  residual may be NULL (kept in HBM).
+ * at the current amplitudes), the left transformation's bulk products in
+ * `precision` (etg_fp64 for eT's equation; the reduced forms for the early
+ * iterations of a mixed-precision solve).  residual may be NULL (kept in HBM).
  * Replaces: construct_multiplier_equation (multiplier_equation_ccs.F90:58-94). */
-int etg_construct_multiplier_equation(etg_handle* handle, double* residual);
+int etg_construct_multiplier_equation(etg_handle* handle, double* residual, int precision);
 
 /* ||R||_2 (FP64 reduction on the device) of the last multiplier residual.
  * Replaces: get_l2_norm(residual) (diis_cc_multipliers_class.F90:249). */
@@ -263,6 +265,13 @@ int etg_set_transform_graphs(etg_handle* handle, int on);
  * every t-dependent operand are read once per group.  Results agree with
  * one-vector passes to the products' rounding. */
 int etg_set_transform_batch(etg_handle* handle, int64_t max_vectors);
+
+/* Diagnostic, after etg_prepare_for_jacobian (each may be NULL): bytes of the
+ * recorded intermediates; bytes one trial vector of a transformation pass
+ * needs (derivatives of intermediates with disjoint lifetimes share memory);
+ * the same with one derivative buffer per intermediate. */
+int etg_jacobian_memory(etg_handle* handle, int64_t* record_bytes, int64_t* bytes_per_vector,
+                        int64_t* unshared_bytes_per_vector);
 
 /* C = alpha op(A) op(B) + beta C on host arrays (copied to the device and
  * back), column-major, op = 'N' or 'T', with the given precision. */

@@ -60,6 +60,17 @@ def test_electron_repulsion_matrix_matches_oracle():
     assert np.array_equal(G, G.T)                  # (pq|rs) = (rs|pq) written exactly
 
 
+def test_chunked_integral_launches_write_the_same_matrix(monkeypatch):
+    """Large molecules launch the quartets in chunks (a dispatch counts its work-items
+    in 32 bits); chunks of 997 quartets give the one-launch matrix bit for bit."""
+    with molecule() as m:
+        G = m.ao_eri()
+    monkeypatch.setenv('ETG_ERI_LAUNCH_GROUPS', '997')
+    with molecule() as m:
+        Gc = m.ao_eri()
+    assert np.array_equal(G, Gc)
+
+
 def test_producer_reproduces_et_hf_energy_and_cholesky_count():
     f = fixture()
     g = f['golden']

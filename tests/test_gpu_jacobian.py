@@ -233,3 +233,26 @@ def test_left_transformation_against_central_differences_on_water_dimer():
             an = AtB[:, k] @ C[:, k]
             worst = max(worst, abs(fd - an) / max(abs(an), 1e-12))
         assert worst < 1e-6, worst
+
+
+def test_derivative_memory_is_shared_between_lifetimes():
+    """Derivatives of intermediates whose lifetimes do not overlap share memory: a
+    trial vector of a pass needs less than one derivative buffer per intermediate,
+    and the passes still equal the complex-step / dense-transpose results above."""
+    s = make(6, 17, 40, 11)
+    with load(s, 5) as wf:
+        wf.prepare_for_jacobian()
+        m = wf.jacobian_memory()
+        print('record %d B, per vector %d B (unshared %d B)' % (m['record'], m['per_vector'],
+                                                              m['per_vector_unshared']))
+        assert 0 < m['per_vector'] < 0.6 * m['per_vector_unshared']
+        assert m['record'] > 8 * wf.n_gs_amplitudes
+        rng = np.random.default_rng(0)
+        c, b = rng.normal(size=(2, wf.n_gs_amplitudes))
+        ref = O.jacobian_transform(s['L'], s['h'], s['t1'], s['t2'], 6, c)
+        # passes in a row reuse the slots: each starts from zeroed derivatives
+        for _ in range(2):
+            rho = wf.jacobian_transformation(c)
+            sigma = wf.jacobian_transpose_transformation(b)
+            assert rel(rho, ref) < 1e-11
+            assert abs(b @ rho - sigma @ c) < 1e-11 * max(abs(b @ rho), 1.0)

@@ -89,3 +89,30 @@ def test_multipliers_reproduce_et_converged_multipliers():
     s = signs.orbital_signs(v['t'], t, n_o, n_v)
     assert np.abs(v['t'] - signs.apply_signs(t, s, n_o, n_v)).max() < 1e-10
     assert np.abs(v['tbar'] - signs.apply_signs(tbar, s, n_o, n_v)).max() < 1e-9
+
+
+def test_mixed_precision_multipliers_reproduce_et():
+    """Split-bf16, FP32, then FP64 left transformations: the FP64 phase decides
+    convergence, and the multipliers equal eT's converged ones (1e-9 after orbital
+    signs) and the FP64 solver's (1e-10)."""
+    v = np.load(V10)
+    m = workloads.h2o_he_inputs()
+    n_o = m['n_o']
+    n_v = m['L_mo'].shape[1] - n_o
+    with ccsd.CCSD(n_o, n_v, m['L_mo'].shape[0]) as wf:
+        wf.set_cholesky(m['L_mo'])
+        wf.set_oei(m['h_mo'])
+        wf.set_orbital_energies(m['eps'])
+        solvers.DiisCCGroundState(wf, residual_threshold=1e-11).run()
+        t = wf.get_amplitudes()
+        ref = solvers.DiisCCMultipliers(wf, residual_threshold=1e-11).run()
+        mixed = solvers.MixedPrecisionMultipliers(wf, residual_threshold=1e-11)
+        tbar = mixed.run()
+    precisions = [p for p, _ in mixed.history]
+    print('phases', [(p, precisions.count(p)) for p in (ccsd.BF16X3, ccsd.FP32, ccsd.FP64)], mixed.noise)
+    assert precisions[0] == ccsd.BF16X3 and precisions[-1] == ccsd.FP64
+    assert mixed.noise[ccsd.BF16X3] > mixed.noise[ccsd.FP32] > 0
+    assert mixed.history[-1][1] < 1e-11
+    assert np.abs(tbar - ref).max() < 1e-10
+    s = signs.orbital_signs(v['t'], t, n_o, n_v)
+    assert np.abs(v['tbar'] - signs.apply_signs(tbar, s, n_o, n_v)).max() < 1e-9
