@@ -76,6 +76,7 @@ _SIGNATURES = {
     'etg_multiplier_diis_update': [ctypes.c_void_p],
     'etg_set_transform_graphs': [ctypes.c_void_p, ctypes.c_int],
     'etg_one_electron_expectation': [ctypes.c_void_p, _c_dbl_p, _c_dbl_p],
+    'etg_one_electron_density': [ctypes.c_void_p, _c_dbl_p],
     'etg_molecule_dipole_integrals': [ctypes.c_void_p, _c_dbl_p, _c_dbl_p],
     'etg_set_transform_batch': [ctypes.c_void_p, _i64],
     'etg_set_kernel_ledger': [ctypes.c_void_p, ctypes.c_int],
@@ -527,6 +528,14 @@ class CCSD:
         v = ctypes.c_double()
         _check(self._lib.etg_one_electron_expectation(self._h, _ptr(X), ctypes.byref(v)))
         return v.value
+
+    def one_electron_density(self):
+        """CC one-electron density D (n_mo x n_mo, MO basis): <X> = sum(D * X) for any
+        one-electron X (one backward replay; same preconditions as
+        one_electron_expectation)."""
+        D = np.empty((self.n_mo, self.n_mo), order='F')
+        _check(self._lib.etg_one_electron_density(self._h, _ptr(D)))
+        return D
 
     def set_transform_graphs(self, on):
         """Replay captured HIP graphs of the transformation passes (default) or run them eagerly."""

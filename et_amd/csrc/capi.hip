@@ -251,6 +251,10 @@ int etg_multiplier_diis_update(etg_handle* h) {
   return guarded([&] { wf_of(h).multiplier_diis_update(); });
 }
 
+int etg_one_electron_density(etg_handle* h, double* D) {
+  return guarded([&] { wf_of(h).one_electron_density(D); });
+}
+
 int etg_one_electron_expectation(etg_handle* h, const double* X_mo, double* value) {
   return guarded([&] {
     if (value == nullptr) etg::fail("etg_one_electron_expectation: null result");

@@ -66,6 +66,8 @@ class CCSD {
   void multiplier_diis_update();                  // records (e, t-bar); t-bar extrapolated
   // CC mean value <X> of a one-electron operator X (host, MO basis) with the
   // Lagrangian density at the current amplitudes, multipliers and record
+  // one-electron density D (n_mo x n_mo, host) with <X> = sum D_pq X_pq
+  void one_electron_density(double* D);
   double one_electron_expectation(const double* X_host);
 
   // Jacobian transformations at the current amplitudes (A = dOmega/dt D2)

@@ -189,4 +189,27 @@ double CCSD::one_electron_expectation(const double* X) {
   return e + lag;
 }
 
+// D_pq = dL/dh_pq, so that <X> = sum_pq D_pq X_pq for every one-electron X: the
+// backward replay from t-bar on the residual gives d(t-bar . Omega)/dh in h's
+// adjoint; the energy adds 2 on the occupied diagonal and 2 t_ai at (i, a).
+void CCSD::one_electron_density(double* D) {
+  require(D != nullptr, "one_electron_density: null output");
+  require(tape_ != nullptr, "one_electron_density: prepare_for_jacobian has not been called at these amplitudes");
+  require(have_tbar_, "one_electron_density: multipliers required");
+  const int64_t n = n_t1 + n_t2, nm = n_mo;
+  std::vector<double> t1(size_t(std::max<int64_t>(1, n_t1)));
+  ETG_HIP(hipMemcpyAsync(t1.data(), t_.as<double>(), sizeof(double) * n_t1, hipMemcpyDeviceToHost, stream_));
+  tape_->zero_derivatives(1);
+  double* hb = tape_->deriv(h_.as<double>());
+  require(hb != nullptr, "one_electron_density: the record holds h as a constant");
+  ETG_HIP(hipMemcpyAsync(tape_->deriv(omega_.as<double>()), tbar_.as<double>(), sizeof(double) * n,
+                         hipMemcpyDeviceToDevice, stream_));
+  tape_->adjoint(Prec::F64, 1);
+  ETG_HIP(hipMemcpyAsync(D, hb, sizeof(double) * nm * nm, hipMemcpyDeviceToHost, stream_));
+  synchronize();
+  for (int64_t i = 0; i < n_o; ++i) D[i + nm * i] += 2.0;
+  for (int64_t i = 0; i < n_o; ++i)
+    for (int64_t a = 0; a < n_v; ++a) D[i + nm * (n_o + a)] += 2.0 * t1[size_t(a + n_v * i)];
+}
+
 }  // namespace etg

@@ -277,6 +277,13 @@ module etgpu_interface
          real(c_double), intent(out) :: value
       end function etg_one_electron_expectation
 !
+!     CC one-electron density, <X> = sum D_pq X_pq (mean_value_ccs.F90:94-122)
+      integer(c_int) function etg_one_electron_density(handle, D) bind(C, name='etg_one_electron_density')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: handle
+         real(c_double), dimension(*), intent(out) :: D
+      end function etg_one_electron_density
+!
 !     molecular producer: integrals -> Cholesky -> RHF -> MO vectors
 !     (src/libint/eri.cpp:33-89, eri_cd_class.F90:241,3132, scf_solver_class.F90)
       integer(c_int) function etg_molecule_create(n_atoms, charges, xyz_bohr, n_shells, shell_atom, shell_l, &

@@ -13,10 +13,12 @@ AU_TO_DEBYE = 2.541746473
 def dipole_moment(wf, molecule, C, origin=(0.0, 0.0, 0.0)):
     """wf: converged ground state with its multipliers and the record of the residual
     program at those amplitudes (prepare_for_jacobian or prepare_for_multiplier_equation);
-    molecule: the et_amd.molecule.Molecule it came from; C: its MO coefficients."""
+    molecule: the et_amd.molecule.Molecule it came from; C: its MO coefficients.
+    The three components come from one CC density (one backward replay)."""
     o = np.asarray(origin, dtype=float)
     D = molecule.dipole_integrals(o)
-    electronic = np.array([-wf.one_electron_expectation(C.T @ D[c] @ C) for c in range(3)])
+    rho = wf.one_electron_density()
+    electronic = np.array([-np.sum(rho * (C.T @ D[c] @ C)) for c in range(3)])
     nuclear = (molecule.charges[:, None] * (molecule.xyz_bohr - o[None, :])).sum(axis=0)
     total = electronic + nuclear
     return dict(electronic=electronic, nuclear=nuclear, total=total, electronic_debye=electronic * AU_TO_DEBYE,

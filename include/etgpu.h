@@ -302,6 +302,17 @@ int etg_bench_gemm(char ta, char tb, int64_t m, int64_t n, int64_t k, int precis
  * (src/engines/cc/tasks/cc_mean_values_task_class.F90:88-92). */
 int etg_one_electron_expectation(etg_handle* handle, const double* X_mo, double* value);
 
+/* The CC one-electron (Lambda) density D (n_mo x n_mo, MO basis, host) with
+ * <X> = sum_pq D_pq X_pq for every one-electron operator X: D = dL/dh, from
+ * one backward replay of the recorded residual program seeded with the
+ * multipliers, plus the reference and t1 terms of the energy.  One replay serves
+ * every operator (the three dipole components at once).  Same preconditions as
+ * etg_one_electron_expectation.
+ * Replaces: construct_gs_density (src/wavefunctions/ccs/mean_value_ccs.F90:94-122,
+ * src/wavefunctions/doubles/mean_value_doubles.F90:78), back-transformed from
+ * the T1 basis to the MO basis. */
+int etg_one_electron_density(etg_handle* handle, double* D);
+
 /* ---- molecular producer of the inputs: AO integrals -> RHF -> Cholesky -> MO -- */
 
 typedef struct etg_molecule etg_molecule;

@@ -2,7 +2,8 @@
 
 * <X> with the Lagrangian density through the C ABI equals the oracle's
   complex-step derivative of L = E + t-bar . Omega along h -> h + e X, at the
-  device's own amplitudes and multipliers (synthetic system, random X);
+  device's own amplitudes and multipliers (synthetic system, random X), and the
+  CC density from one backward replay gives sum_pq D_pq X_pq equal to it (1e-12);
 * the whole chain -- producer (STO-3G), ground state, multipliers, dipole
   integrals, mean value -- reproduces eT's tests/ccsd_dipole: E_HF, n_J, E_CCSD
   and the electronic, nuclear and total dipole in Debye.
@@ -43,6 +44,7 @@ def test_expectation_value_matches_oracle_complex_step():
         tbar = rng.normal(size=wf.n_gs_amplitudes) * 0.05
         wf.set_multipliers(tbar)
         t = wf.get_amplitudes()
+        D = wf.one_electron_density()
         for k in range(3):
             X = rng.normal(size=(n, n))
             X = X + X.T
@@ -50,6 +52,8 @@ def test_expectation_value_matches_oracle_complex_step():
             ref = O.one_electron_expectation(s['L'], s['h'], t[:n_o * n_v].reshape(n_v, n_o, order='F'),
                                              t[n_o * n_v:], tbar, n_o, X)
             assert abs(got - ref) < 1e-11 * max(1.0, abs(ref)), k
+            # the density from one backward replay gives the same value for every X
+            assert abs(np.sum(D * X) - got) < 1e-12 * max(1.0, abs(got)), k
 
 
 def test_dipole_chain_reproduces_et():
