@@ -95,7 +95,10 @@ __device__ inline f4 mfma(bf8 a, bf8 b, f4 c) {
 
 // 4 waves in 2 x 2, each a 64 x 64 sub-tile of 4 x 4 MFMA blocks.  One stage
 // holds 32 k of both splits of a 128-row slab of each operand: 4 x 128 x 32
-// bf16; stages are double buffered with one barrier per stage.
+// bf16; stages are double buffered in LDS with one barrier per stage.  A
+// stage's matrix-core work (48 MFMAs per wave) is short next to a load that
+// misses L2, so global loads run two stages ahead through two register sets:
+// the loads of step s + 2 are issued at step s.
 __global__ __launch_bounds__(kThreads, 2) void gemm_bf16x3_kernel(Params p) {
   __shared__ __bf16 As[2][2][kTile * kLd];
   __shared__ __bf16 Bs[2][2][kTile * kLd];
@@ -113,30 +116,29 @@ __global__ __launch_bounds__(kThreads, 2) void gemm_bf16x3_kernel(Params p) {
 
   // global -> LDS: 1024 chunks of 16 B per operand and stage (2 splits x 128
   // rows x 4 chunks); thread tid moves chunks tid + 256 i, i = 0..3.
-  const __bf16* srcA[4];
-  const __bf16* srcB[4];
-  int dst[4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int q = tid + kThreads * i;
-    const int part = q >> 9, rem = q & 511, row = rem >> 2, kc = rem & 3;
-    srcA[i] = (part ? p.Alo : p.Ahi) + bz * p.strideAp + (m0 + row) * p.Kp + 8 * kc;
-    srcB[i] = (part ? p.Blo : p.Bhi) + bz * p.strideBp + (n0 + row) * p.Kp + 8 * kc;
-    dst[i] = part * (kTile * kLd) + row * kLd + 8 * kc;
-  }
-  u4 ra[4], rb[4];
-  auto load = [&](int64_t k0) {
+  // chunk i = 0..3 of this thread: split i / 2 (hi, lo), row (tid / 4) + 64 (i % 2),
+  // 8 k from 8 (tid % 4): one per-thread offset, the rest uniform
+  const int trow = tid >> 2, tk = 8 * (tid & 3);
+  const int64_t toff = int64_t(trow) * p.Kp + tk;
+  const __bf16* baseA[2] = {p.Ahi + bz * p.strideAp + m0 * p.Kp, p.Alo + bz * p.strideAp + m0 * p.Kp};
+  const __bf16* baseB[2] = {p.Bhi + bz * p.strideBp + n0 * p.Kp, p.Blo + bz * p.strideBp + n0 * p.Kp};
+  const int64_t half = 64 * p.Kp;
+  const int tdst = trow * kLd + tk;
+  u4 ra[2][4], rb[2][4];
+  auto load = [&](int set, int64_t k0) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      ra[i] = *reinterpret_cast<const u4*>(srcA[i] + k0);
-      rb[i] = *reinterpret_cast<const u4*>(srcB[i] + k0);
+      const int64_t o = toff + (i & 1) * half + k0;
+      ra[set][i] = *reinterpret_cast<const u4*>(baseA[i >> 1] + o);
+      rb[set][i] = *reinterpret_cast<const u4*>(baseB[i >> 1] + o);
     }
   };
-  auto store = [&](int buf) {
+  auto store = [&](int set, int buf) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      *reinterpret_cast<u4*>(&As[buf][0][0] + dst[i]) = ra[i];
-      *reinterpret_cast<u4*>(&Bs[buf][0][0] + dst[i]) = rb[i];
+      const int d = (i >> 1) * (kTile * kLd) + (i & 1) * (64 * kLd) + tdst;
+      *reinterpret_cast<u4*>(&As[buf][0][0] + d) = ra[set][i];
+      *reinterpret_cast<u4*>(&Bs[buf][0][0] + d) = rb[set][i];
     }
   };
 
@@ -146,16 +148,8 @@ __global__ __launch_bounds__(kThreads, 2) void gemm_bf16x3_kernel(Params p) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
 
-  if (nsteps > 0) {
-    load(kbeg);
-    store(0);
-  }
-  __syncthreads();
   const int frow = lane & 15, fk = 8 * (lane >> 4);
-  for (int64_t s = 0; s < nsteps; ++s) {
-    const int buf = int(s & 1);
-    // the last step reloads its own slab into the idle buffer: no branch
-    load(kbeg + min(s + 1, nsteps - 1) * kDepth);
+  auto compute = [&](int buf) {
     bf8 ah[4], al[4], bh[4], bl[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -182,7 +176,27 @@ __global__ __launch_bounds__(kThreads, 2) void gemm_bf16x3_kernel(Params p) {
     for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int j = 0; j < 4; ++j) acc[i][j] = mfma(al[i], bh[j], acc[i][j]);
-    store(buf ^ 1);
+  };
+  // Step s computes on LDS buffer s % 2, issues the loads of step s + 2 into
+  // register set s % 2 (whose slab is already in LDS) and stores set (s + 1) % 2,
+  // loaded one step earlier, into the other buffer.  Loads past the last step
+  // reload its slab: no branch on the loads.
+  auto slab = [&](int64_t s) { return kbeg + min(s, nsteps - 1) * kDepth; };
+  if (nsteps > 0) {
+    load(0, slab(0));
+    load(1, slab(1));
+    store(0, 0);
+  }
+  __syncthreads();
+  for (int64_t s = 0; s < nsteps; s += 2) {
+    load(0, slab(s + 2));
+    compute(0);
+    store(1, 1);
+    __syncthreads();
+    if (s + 1 >= nsteps) break;
+    load(1, slab(s + 3));
+    compute(1);
+    store(0, 0);
     __syncthreads();
   }
 
