@@ -1,10 +1,13 @@
-"""Writes cc-pvdz.json and sto-3g.json (H-Ne) from the reference's Gaussian-94 basis files.
+"""Writes cc-pvdz.json, aug-cc-pvdz.json and sto-3g.json (H-Ne) from the reference's
+Gaussian-94 basis files.
 
 Run in a checkout that has the reference next to it; the JSON files are
-committed and are what the package reads.  Data sources: ao_basis/cc-pvdz.g94
-and ao_basis/sto-3g.g94 (eT's basis library).  Each element maps to a list of
-shells {"l": 0|1|2, "exps": [...], "coefs": [...]} in file order; an "SP" shell
-(shared exponents) becomes an s shell followed by a p shell.
+committed and are what the package reads.  Data sources: ao_basis/cc-pvdz.g94,
+ao_basis/augmentation-cc-pvdz.g94 and ao_basis/sto-3g.g94 (eT's basis library).
+Each element maps to a list of shells {"l": 0|1|2, "exps": [...], "coefs": [...]}
+in file order; an "SP" shell (shared exponents) becomes an s shell followed by a
+p shell.  aug-cc-pVDZ is cc-pVDZ followed by the augmentation shells, as eT reads
+an "aug" basis (src/molecule/atomic_center_class.F90:460-497).
 """
 import json
 import os
@@ -35,8 +38,16 @@ def parse(path):
     return out
 
 
+def write(name, data):
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), name + '.json'), 'w') as f:
+        json.dump(data, f, indent=0)
+
+
 if __name__ == '__main__':
     for name in ('cc-pvdz', 'sto-3g'):
-        data = {'source': 'ao_basis/%s.g94' % name, 'basis': parse(os.path.join(REF, 'ao_basis', name + '.g94'))}
-        with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), name + '.json'), 'w') as f:
-            json.dump(data, f, indent=0)
+        write(name, {'source': 'ao_basis/%s.g94' % name,
+                     'basis': parse(os.path.join(REF, 'ao_basis', name + '.g94'))})
+    base = parse(os.path.join(REF, 'ao_basis', 'cc-pvdz.g94'))
+    aug = parse(os.path.join(REF, 'ao_basis', 'augmentation-cc-pvdz.g94'))
+    write('aug-cc-pvdz', {'source': 'ao_basis/cc-pvdz.g94 + ao_basis/augmentation-cc-pvdz.g94',
+                          'basis': {e: base[e] + aug.get(e, []) for e in base}})
