@@ -212,3 +212,26 @@ def test_gpu_mixed_precision_eom_reproduces_et():
             again, hist = wf.davidson(3, side, 1e-8, 10, 0, ccsd.FP64, start=sv.eigenvectors)
             assert len(hist) == 1 and not wf.davidson_stalled
             np.testing.assert_allclose(again, w_ref, rtol=0, atol=1e-10)
+
+
+@pytest.mark.gpu
+def test_gpu_eom_with_reduced_space_replacement_reproduces_et():
+    """eT's tests/ccsd_right_es_replace_red_space: two right roots to residual 1e-10 with the
+    reduced space capped at 20 (restarts on the Ritz vectors) give eT's energies
+    (result/ccsd_right_es_replace_red_space.out:858-859) to 1e-10 Eh."""
+    from et_amd import ccsd, solvers
+    g = fixture()['golden']
+    m = inputs()
+    n_o = m['n_o']
+    n_v = m['L_mo'].shape[1] - n_o
+    with ccsd.CCSD(n_o, n_v, m['n_J']) as wf:
+        wf.set_cholesky(m['L_mo'])
+        wf.set_oei(m['h_mo'])
+        wf.set_orbital_energies(m['eps'])
+        solvers.DiisCCGroundState(wf, residual_threshold=1e-11).run()
+        dav = solvers.EigenDavidson(wf, 2, 'right', residual_threshold=1e-10, max_dim=20)
+        w = dav.run()
+    print('iterations', len(dav.history))
+    assert len(dav.history) > 9                    # more trial vectors than the capped space holds
+    np.testing.assert_allclose(w, [0.247194255570, 0.312716698776], rtol=0, atol=1e-10)
+    np.testing.assert_allclose(w, g['right_excitation_energies'], rtol=0, atol=1e-10)
