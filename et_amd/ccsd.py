@@ -61,6 +61,9 @@ _SIGNATURES = {
                                      ctypes.c_int],
     'etg_davidson': [ctypes.c_void_p, ctypes.c_int, _i64, ctypes.c_double, _i64, _i64, ctypes.c_int,
                      _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, ctypes.POINTER(_i64)],
+    'etg_diis_excited_states': [ctypes.c_void_p, ctypes.c_int, _i64, ctypes.c_double, ctypes.c_double, _i64,
+                                _i64, ctypes.c_int, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p,
+                                ctypes.POINTER(_i64)],
     'etg_test_reduced_eigen': [_i64, _c_dbl_p, _i64, _c_dbl_p, _c_dbl_p, _c_dbl_p],
     'etg_diis_initialize': [ctypes.c_void_p, _i64],
     'etg_quasi_newton_update': [ctypes.c_void_p],
@@ -445,6 +448,29 @@ class CCSD:
         self.davidson_stalled = its.value < 0
         hist = hist[:abs(its.value)]
         return (e, hist, X) if vectors else (e, hist)
+
+    def diis_excited_states(self, n_states, side='right', residual_threshold=1e-6, energy_threshold=0.0,
+                            max_iterations=100, diis_dimension=20, precision=FP64, start=None):
+        """eT's DIIS excited-state solver on the device (diis_cc_es_class.F90:260-492).
+
+        Returns (energies sorted ascending, residual-norm history per iteration,
+        eigenvectors in the same order); the per-iteration energies (start order) are
+        left in diis_energy_history."""
+        e = np.empty(n_states)
+        hist = np.full((max_iterations, n_states), np.nan)
+        ehist = np.full((max_iterations, n_states), np.nan)
+        X = np.empty((self.n_gs_amplitudes, n_states), order='F')
+        S = None if start is None else np.asfortranarray(start, dtype=np.float64)
+        if S is not None and S.shape != (self.n_gs_amplitudes, n_states):
+            raise ETGError('start vectors must be (n_t1 + n_t2) x n_states')
+        its = _i64()
+        _check(self._lib.etg_diis_excited_states(self._h, 0 if side == 'right' else 1, n_states,
+                                                 residual_threshold, energy_threshold, max_iterations,
+                                                 diis_dimension, int(precision),
+                                                 _ptr(S) if S is not None else None, _ptr(e), _ptr(X),
+                                                 _ptr(hist), _ptr(ehist), ctypes.byref(its)))
+        self.diis_energy_history = ehist[:its.value]
+        return e, hist[:its.value], X
 
     def diis_initialize(self, dimension=8):
         """HBM records for DIIS (diis_tool_class.F90:249-277)."""

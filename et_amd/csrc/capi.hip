@@ -200,6 +200,19 @@ int etg_davidson(etg_handle* h, int side, int64_t n_states, double residual_thre
   });
 }
 
+int etg_diis_excited_states(etg_handle* h, int side, int64_t n_states, double residual_threshold,
+                            double energy_threshold, int64_t max_iterations, int64_t diis_dimension,
+                            int precision, const double* start_vectors, double* energies, double* vectors,
+                            double* residual_norms, double* energy_history, int64_t* iterations) {
+  return guarded([&] {
+    const int64_t it = wf_of(h).diis_excited_states(side, n_states, residual_threshold, energy_threshold,
+                                                    max_iterations, diis_dimension, prec_of(precision),
+                                                    start_vectors, energies, vectors, residual_norms,
+                                                    energy_history);
+    if (iterations) *iterations = it;
+  });
+}
+
 int etg_diis_initialize(etg_handle* h, int64_t dimension) {
   return guarded([&] { wf_of(h).diis_initialize(dimension); });
 }

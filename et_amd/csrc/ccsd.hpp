@@ -20,6 +20,15 @@ void reduced_eigen_solve(hipStream_t stream, DeviceBuffer& scratch, const double
                          int64_t ldg, int64_t n_want, double* wr, double* wi, double* X_dev,
                          int64_t ldx);
 
+// One DIIS space in HBM (diis_tool_class.F90): up to `dim` records of an error
+// vector and a parameter vector of length n, the matrix of error overlaps and
+// the weights of the last solve.
+struct DiisSpace {
+  int dim = 0;
+  int64_t count = 0, n = 0;
+  DeviceBuffer e, x, G, w, info;
+};
+
 class CCSD {
  public:
   CCSD(int64_t n_o, int64_t n_v, int64_t n_J);
@@ -94,6 +103,17 @@ class CCSD {
   int64_t davidson(int side, int64_t n_states, double threshold, int64_t max_iterations,
                    int64_t max_dim, Prec prec, const double* start, double* energies,
                    double* vectors, double* residual_norms);
+  // eT's DIIS excited-state solver (run_diis_cc_es, diis_cc_es_class.F90:260-492):
+  // each state iterates X += -(A X - E X)/(eps - E), DIIS over its own records,
+  // X normalised, until ||A X - E X|| < residual_threshold and |dE| <
+  // energy_threshold (energy_threshold <= 0: residual only).  Start vectors as
+  // davidson's.  energies sorted ascending on return, vectors in the same order;
+  // residual_norms and energy_history (max_iterations x n_states, iteration-major,
+  // may be null) per iteration.  Returns the number of iterations.
+  int64_t diis_excited_states(int side, int64_t n_states, double residual_threshold,
+                              double energy_threshold, int64_t max_iterations, int64_t diis_dimension,
+                              Prec prec, const double* start, double* energies, double* vectors,
+                              double* residual_norms, double* energy_history);
   // lowest n_want eigenpairs of the k x k matrix G (device, ld ldg): wr, wi (host,
   // k, ascending real part), X (device, k x n_want, ld ldx)
   void reduced_eigen(const double* G_dev, int64_t k, int64_t ldg, int64_t n_want, double* wr,
@@ -134,7 +154,7 @@ class CCSD {
  private:
   void require(bool ok, const char* what) const;
   void build_ladder_plan();
-  void diis_step(const double* e, double* x);
+  void diis_step(DiisSpace& d, const double* e, double* x);
   // the residual pipeline, written once against Ops
   void program_t1_cholesky(Ops& X);
   void program_fock(Ops& X);
@@ -172,9 +192,9 @@ class CCSD {
   void build_ladder_t1_intermediate();
 
   // DIIS records: errors and parameters, dim x (n_t1 + n_t2) each
-  int diis_dim_ = 0;
-  int64_t diis_count_ = 0;
-  DeviceBuffer diis_e_, diis_x_, diis_G_, diis_w_, diis_info_, update_;
+  DiisSpace diis_;                                // ground state and multipliers
+  std::vector<DiisSpace> es_diis_;                // one per excited state (DIIS excited-state solver)
+  DeviceBuffer update_, es_X_, es_R_;
   bool have_update_ = false, have_omega_ = false;
 
   // Davidson space (V, AV), Ritz block, reduced matrices
@@ -224,5 +244,8 @@ class CCSD {
 
   bool have_L_ = false, have_h_ = false, have_eps_ = false, have_t_ = false, have_F_ = false;
 };
+
+// Empties a DIIS space and sizes it for `dimension` records of length n.
+void diis_space_initialize(hipStream_t s, DiisSpace& d, int64_t n, int64_t dimension);
 
 }  // namespace etg

@@ -130,17 +130,21 @@ void quasi_newton_step(hipStream_t s, const double* eps, const double* residual,
   ETG_LAUNCH("quasi_newton");
 }
 
+void diis_space_initialize(hipStream_t s, DiisSpace& d, int64_t n, int64_t dimension) {
+  if (dimension < 1 || dimension > CCSD::kMaxDiis) fail("diis_initialize: dimension must be in [1, 32]");
+  d.dim = int(dimension);
+  d.count = 0;
+  d.n = n;
+  d.e.resize(sizeof(double) * size_t(n) * size_t(dimension));
+  d.x.resize(sizeof(double) * size_t(n) * size_t(dimension));
+  d.G.resize(sizeof(double) * CCSD::kMaxDiis * CCSD::kMaxDiis);
+  d.w.resize(sizeof(double) * CCSD::kMaxDiis);
+  d.info.resize(sizeof(int));
+  fill(s, d.G.as<double>(), CCSD::kMaxDiis * CCSD::kMaxDiis, 0.0);
+}
+
 void CCSD::diis_initialize(int64_t dimension) {
-  if (dimension < 1 || dimension > kMaxDiis) fail("diis_initialize: dimension must be in [1, 32]");
-  const int64_t n = n_t1 + n_t2;
-  diis_dim_ = int(dimension);
-  diis_count_ = 0;
-  diis_e_.resize(sizeof(double) * size_t(n) * size_t(dimension));
-  diis_x_.resize(sizeof(double) * size_t(n) * size_t(dimension));
-  diis_G_.resize(sizeof(double) * kMaxDiis * kMaxDiis);
-  diis_w_.resize(sizeof(double) * kMaxDiis);
-  diis_info_.resize(sizeof(int));
-  fill(stream_, diis_G_.as<double>(), kMaxDiis * kMaxDiis, 0.0);
+  diis_space_initialize(stream_, diis_, n_t1 + n_t2, dimension);
 }
 
 void CCSD::quasi_newton_update() {
@@ -159,21 +163,21 @@ void CCSD::quasi_newton_update() {
 
 void CCSD::diis_update() {
   require(have_update_, "diis_update: no update vector (call quasi_newton_update first)");
-  diis_step(update_.as<double>(), t_.as<double>());
+  diis_step(diis_, update_.as<double>(), t_.as<double>());
   have_update_ = false;
   reset_record();
 }
 
-void CCSD::diis_step(const double* e, double* x) {
-  require(diis_dim_ > 0, "diis_update: DIIS has not been initialised");
-  const int64_t n = n_t1 + n_t2;
+void CCSD::diis_step(DiisSpace& d, const double* e, double* x) {
+  require(d.dim > 0, "diis_update: DIIS has not been initialised");
+  const int64_t n = d.n;
   // record slot: the oldest record is replaced once the space is full
   // (diis_tool_class.F90 save_errors_and_parameters / modify_mappings_to_remove_oldest_record)
-  const int slot = int(diis_count_ % diis_dim_);
-  const int dim = int(std::min<int64_t>(diis_count_ + 1, diis_dim_));
-  double* E = diis_e_.as<double>();
-  double* X = diis_x_.as<double>();
-  double* G = diis_G_.as<double>();
+  const int slot = int(d.count % d.dim);
+  const int dim = int(std::min<int64_t>(d.count + 1, d.dim));
+  double* E = d.e.as<double>();
+  double* X = d.x.as<double>();
+  double* G = d.G.as<double>();
   ETG_HIP(hipMemcpyAsync(E + int64_t(slot) * n, e, sizeof(double) * n, hipMemcpyDeviceToDevice,
                          stream_));
   ETG_HIP(hipMemcpyAsync(X + int64_t(slot) * n, x, sizeof(double) * n, hipMemcpyDeviceToDevice,
@@ -184,16 +188,16 @@ void CCSD::diis_step(const double* e, double* x) {
         G + slot + int64_t(i) * kMaxDiis);
   mirror_row_kernel<<<1, kMaxDiis, 0, stream_>>>(G, kMaxDiis, slot, dim);
   ETG_LAUNCH("diis_mirror_row");
-  diis_solve_kernel<<<1, 1, 0, stream_>>>(G, kMaxDiis, dim, diis_w_.as<double>(), diis_info_.as<int>());
+  diis_solve_kernel<<<1, 1, 0, stream_>>>(G, kMaxDiis, dim, d.w.as<double>(), d.info.as<int>());
   ETG_LAUNCH("diis_solve");
-  extrapolate_kernel<<<grid_for_n(n), kBlock, 0, stream_>>>(X, n, dim, diis_w_.as<double>(),
-                                                            diis_info_.as<int>(), x);
+  extrapolate_kernel<<<grid_for_n(n), kBlock, 0, stream_>>>(X, n, dim, d.w.as<double>(),
+                                                            d.info.as<int>(), x);
   ETG_LAUNCH("diis_extrapolate");
   int info = 0;
-  ETG_HIP(hipMemcpyAsync(&info, diis_info_.as<int>(), sizeof(int), hipMemcpyDeviceToHost, stream_));
+  ETG_HIP(hipMemcpyAsync(&info, d.info.as<int>(), sizeof(int), hipMemcpyDeviceToHost, stream_));
   synchronize();
   if (info != 0) fail("diis_update: could not solve the DIIS matrix equation (zero pivot)");
-  ++diis_count_;
+  ++d.count;
 }
 
 }  // namespace etg

@@ -154,7 +154,7 @@ void CCSD::multiplier_quasi_newton_update() {
 
 void CCSD::multiplier_diis_update() {
   require(have_mupdate_, "multiplier_diis_update: no update vector (call the quasi-Newton step first)");
-  diis_step(mupdate_.as<double>(), tbar_.as<double>());
+  diis_step(diis_, mupdate_.as<double>(), tbar_.as<double>());
   have_mupdate_ = false;
 }
 

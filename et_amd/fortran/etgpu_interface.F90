@@ -133,6 +133,21 @@ module etgpu_interface
       end function etg_jacobian_transpose_transformation
 !
 !     diis%initialize (diis_tool_class.F90:249-277)
+      integer(c_int) function etg_diis_excited_states(handle, side, n_states, residual_threshold, &
+                                                      energy_threshold, max_iterations, diis_dimension, &
+                                                      precision, start_vectors, energies, vectors, &
+                                                      residual_norms, energy_history, iterations) &
+                                                      bind(C, name='etg_diis_excited_states')
+         import :: c_int, c_ptr, c_double, c_int64_t
+         type(c_ptr), value :: handle
+         integer(c_int), value :: side, precision
+         integer(c_int64_t), value :: n_states, max_iterations, diis_dimension
+         real(c_double), value :: residual_threshold, energy_threshold
+         type(c_ptr), value :: start_vectors, vectors, residual_norms, energy_history
+         real(c_double), dimension(*), intent(out) :: energies
+         integer(c_int64_t), intent(out) :: iterations
+      end function etg_diis_excited_states
+!
       integer(c_int) function etg_diis_initialize(handle, dimension) bind(C, name='etg_diis_initialize')
          import :: c_int, c_ptr, c_int64_t
          type(c_ptr), value :: handle

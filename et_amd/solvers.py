@@ -9,6 +9,8 @@ MixedPrecisionMultipliers: the same iteration on split-bf16, FP32, then FP64 lef
 transformations.
 EigenDavidson: run_eigen_davidson_solver (eigen_davidson_solver_class.F90:122-318),
 with the whole Davidson space on the device.
+DiisCCExcitedStates: run_diis_cc_es (diis_cc_es_class.F90:260-492), one DIIS space
+per state on the device.
 MixedPrecisionDavidson: the same eigensolver on split-bf16 transformations,
 finished by an FP64 solve started from the reduced-precision eigenvectors.
 Only scalars (energies, norms) reach the host during the iterations.
@@ -269,6 +271,50 @@ class EigenDavidson:
         w, hist, X = wf.davidson(self.n_states, self.side, self.residual_threshold, self.max_iterations,
                                  self.max_dim, self.precision, vectors=True, start=start)
         self.history = hist
+        self.eigenvectors = X
+        if self.log:
+            for it, row in enumerate(hist, 1):
+                self.log('%3d  max|res| %.3e' % (it, row.max()))
+        return w
+
+
+class DiisCCExcitedStates:
+    """eT's DIIS excited-state solver (run_diis_cc_es, diis_cc_es_class.F90:260-492).
+
+    Each state's vector X follows X += -(A X - E X)/(eps - E) with E = X . A X and
+    DIIS over its own records, normalised every iteration, until its residual (and,
+    with energy_threshold, its energy change) is converged; unconverged states share
+    a transformation pass.  Start vectors on the lowest orbital differences (or the
+    restart files, `restart`); the roots found are those the start vectors lead to.
+    Energies are returned sorted.  Vectors, records and the DIIS algebra stay in HBM.
+    """
+
+    def __init__(self, wf, n_states, side='right', residual_threshold=1e-6, energy_threshold=None,
+                 max_iterations=100, diis_dimension=20, precision=_ccsd.FP64, start=None, restart=False,
+                 log=None):
+        self.wf = wf
+        self.n_states = n_states
+        self.side = side
+        self.residual_threshold = residual_threshold
+        self.energy_threshold = energy_threshold
+        self.max_iterations = max_iterations
+        self.diis_dimension = diis_dimension
+        self.precision = precision
+        self.start = start
+        self.restart = restart
+        self.log = log
+
+    def run(self):
+        wf = self.wf
+        wf.prepare_for_jacobian(self.side)
+        start = self.start
+        if start is None and self.restart:
+            start = wf.restart_start_vectors(self.n_states, self.side)[0]
+        w, hist, X = wf.diis_excited_states(self.n_states, self.side, self.residual_threshold,
+                                            self.energy_threshold or 0.0, self.max_iterations,
+                                            self.diis_dimension, self.precision, start)
+        self.history = hist
+        self.energy_history = wf.diis_energy_history
         self.eigenvectors = X
         if self.log:
             for it, row in enumerate(hist, 1):

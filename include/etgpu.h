@@ -174,6 +174,26 @@ int etg_davidson(etg_handle* handle, int side, int64_t n_states, double residual
                  const double* start_vectors, double* energies, double* vectors,
                  double* residual_norms, int64_t* iterations);
 
+/* The n_states eigenpairs of A (side 0) or A^T (side 1) by eT's DIIS
+ * excited-state solver, on the device: each state's vector X follows
+ * X += -(A X - E X) / (eps - E), E = X . A X, with DIIS over that state's own
+ * records (dimension diis_dimension, eT's default 20) and X normalised; a state
+ * is converged when ||A X - E X|| < residual_threshold and |dE| <
+ * energy_threshold (energy_threshold <= 0: residual only; the first iteration
+ * tests the residual only), and is not transformed again.  The unconverged
+ * states share one transformation pass per iteration.  Start vectors as in
+ * etg_davidson (the roots reached are those the start vectors lead to, as in
+ * eT).  On return energies (n_states) are sorted ascending and vectors
+ * ((n_t1 + n_t2) x n_states, host or device, may be NULL) follow them;
+ * residual_norms and energy_history (max_iterations x n_states, in the start
+ * order, may be NULL) hold each iteration's norms and energies; fails after
+ * max_iterations.
+ * Replaces: run_diis_cc_es (src/solvers/cc/diis_cc_es_class.F90:260-492). */
+int etg_diis_excited_states(etg_handle* handle, int side, int64_t n_states, double residual_threshold,
+                            double energy_threshold, int64_t max_iterations, int64_t diis_dimension,
+                            int precision, const double* start_vectors, double* energies, double* vectors,
+                            double* residual_norms, double* energy_history, int64_t* iterations);
+
 /* ---- multiplier (Lambda) equations  t-bar^T A + eta = 0 ---------------- */
 
 /* Records the residual program at the current amplitudes for the left

@@ -235,3 +235,34 @@ def test_gpu_eom_with_reduced_space_replacement_reproduces_et():
     assert len(dav.history) > 9                    # more trial vectors than the capped space holds
     np.testing.assert_allclose(w, [0.247194255570, 0.312716698776], rtol=0, atol=1e-10)
     np.testing.assert_allclose(w, g['right_excitation_energies'], rtol=0, atol=1e-10)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('side', ['right', 'left'])
+def test_gpu_diis_excited_states_reproduce_et(side):
+    """eT's DIIS excited-state solver (tests/ccsd_right_es_diis, tests/ccsd_left_es_diis: two
+    states, residual and energy thresholds 1e-10): the same number of iterations, every
+    iteration's energies (1e-8 Eh; the first ones are diagonal elements of A, 1e-10) and
+    residual norms (as printed, 4 digits, where above 1e-6), and the final roots -- the
+    first and the third root, where the start vectors lead -- to 1e-10 Eh."""
+    from et_amd import ccsd, solvers
+    ref = json.load(open(os.path.join(os.path.dirname(FIXTURE), 'h2o_he_es_diis.json')))[side]
+    m = inputs()
+    n_o = m['n_o']
+    n_v = m['L_mo'].shape[1] - n_o
+    with ccsd.CCSD(n_o, n_v, m['n_J']) as wf:
+        wf.set_cholesky(m['L_mo'])
+        wf.set_oei(m['h_mo'])
+        wf.set_orbital_energies(m['eps'])
+        solvers.DiisCCGroundState(wf, residual_threshold=1e-11).run()
+        es = solvers.DiisCCExcitedStates(wf, 2, side, residual_threshold=1e-10, energy_threshold=1e-10)
+        w = es.run()
+    print(side, 'iterations', len(es.history), 'eT', ref['iterations'], w)
+    np.testing.assert_allclose(w, ref['excitation_energies'], rtol=0, atol=1e-10)
+    assert len(es.history) == ref['iterations']
+    for it, state, e_ref, r_ref in ref['table']:
+        e = es.energy_history[it - 1][state - 1]
+        r = es.history[it - 1][state - 1]
+        assert abs(e - e_ref) < (1e-10 if it == 1 else 1e-8), (it, state, e, e_ref)
+        if r_ref > 1e-6:
+            assert abs(r - r_ref) <= 6e-4 * r_ref, (it, state, r, r_ref)
