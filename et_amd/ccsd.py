@@ -61,6 +61,8 @@ _SIGNATURES = {
                                      ctypes.c_int],
     'etg_davidson': [ctypes.c_void_p, ctypes.c_int, _i64, ctypes.c_double, _i64, _i64, ctypes.c_int,
                      _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, ctypes.POINTER(_i64)],
+    'etg_linear_davidson': [ctypes.c_void_p, ctypes.c_int, _c_dbl_p, _c_dbl_p, ctypes.c_double, _i64, _i64,
+                            ctypes.c_int, _c_dbl_p, _c_dbl_p, ctypes.POINTER(_i64)],
     'etg_diis_excited_states': [ctypes.c_void_p, ctypes.c_int, _i64, ctypes.c_double, ctypes.c_double, _i64,
                                 _i64, ctypes.c_int, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p,
                                 ctypes.POINTER(_i64)],
@@ -448,6 +450,25 @@ class CCSD:
         self.davidson_stalled = its.value < 0
         hist = hist[:abs(its.value)]
         return (e, hist, X) if vectors else (e, hist)
+
+    def linear_davidson(self, rhs, side='right', start=None, residual_threshold=1e-6, max_iterations=100,
+                        max_dim=50, precision=FP64):
+        """Solves A X = rhs (right) or A^T X = rhs (left) with eT's linear Davidson solver on
+        the device; returns (X, residual-norm history)."""
+        F = np.ascontiguousarray(rhs, dtype=np.float64)
+        if F.shape != (self.n_gs_amplitudes,):
+            raise ETGError('rhs must have length n_t1 + n_t2')
+        S = None if start is None else np.ascontiguousarray(start, dtype=np.float64)
+        if S is not None and S.shape != F.shape:
+            raise ETGError('start vector must have length n_t1 + n_t2')
+        X = np.empty(self.n_gs_amplitudes)
+        hist = np.full(max_iterations, np.nan)
+        its = _i64()
+        _check(self._lib.etg_linear_davidson(self._h, 0 if side == 'right' else 1, _ptr(F),
+                                             _ptr(S) if S is not None else None, residual_threshold,
+                                             max_iterations, max_dim, int(precision), _ptr(X), _ptr(hist),
+                                             ctypes.byref(its)))
+        return X, hist[:its.value]
 
     def diis_excited_states(self, n_states, side='right', residual_threshold=1e-6, energy_threshold=0.0,
                             max_iterations=100, diis_dimension=20, precision=FP64, start=None):

@@ -200,6 +200,16 @@ int etg_davidson(etg_handle* h, int side, int64_t n_states, double residual_thre
   });
 }
 
+int etg_linear_davidson(etg_handle* h, int side, const double* rhs, const double* start_vector,
+                        double residual_threshold, int64_t max_iterations, int64_t max_dim, int precision,
+                        double* solution, double* residual_norms, int64_t* iterations) {
+  return guarded([&] {
+    const int64_t it = wf_of(h).linear_davidson(side, rhs, start_vector, residual_threshold, max_iterations,
+                                                max_dim, prec_of(precision), solution, residual_norms);
+    if (iterations) *iterations = it;
+  });
+}
+
 int etg_diis_excited_states(etg_handle* h, int side, int64_t n_states, double residual_threshold,
                             double energy_threshold, int64_t max_iterations, int64_t diis_dimension,
                             int precision, const double* start_vectors, double* energies, double* vectors,

@@ -103,6 +103,15 @@ class CCSD {
   int64_t davidson(int side, int64_t n_states, double threshold, int64_t max_iterations,
                    int64_t max_dim, Prec prec, const double* start, double* energies,
                    double* vectors, double* residual_norms);
+  // eT's linear Davidson solver (run_linear_davidson_solver,
+  // linear_davidson_solver_class.F90:122-270) for A X = F (side 0) or A^T X = F
+  // (side 1): start from `start` or F / eps, restart on the solution at max_dim
+  // (<= 0: 50), new trials -R / eps, converged when ||A X - F|| <= threshold.  F,
+  // start, X: host or device (n_t1 + n_t2); residual_norms (max_iterations, may be
+  // null).  Returns the number of iterations.
+  int64_t linear_davidson(int side, const double* F, const double* start, double residual_threshold,
+                          int64_t max_iterations, int64_t max_dim, Prec prec, double* X_out,
+                          double* residual_norms);
   // eT's DIIS excited-state solver (run_diis_cc_es, diis_cc_es_class.F90:260-492):
   // each state iterates X += -(A X - E X)/(eps - E), DIIS over its own records,
   // X normalised, until ||A X - E X|| < residual_threshold and |dE| <

@@ -133,6 +133,20 @@ module etgpu_interface
       end function etg_jacobian_transpose_transformation
 !
 !     diis%initialize (diis_tool_class.F90:249-277)
+      integer(c_int) function etg_linear_davidson(handle, side, rhs, start_vector, residual_threshold, &
+                                                  max_iterations, max_dim, precision, solution, &
+                                                  residual_norms, iterations) bind(C, name='etg_linear_davidson')
+         import :: c_int, c_ptr, c_double, c_int64_t
+         type(c_ptr), value :: handle
+         integer(c_int), value :: side, precision
+         real(c_double), dimension(*), intent(in) :: rhs
+         type(c_ptr), value :: start_vector, residual_norms
+         real(c_double), value :: residual_threshold
+         integer(c_int64_t), value :: max_iterations, max_dim
+         real(c_double), dimension(*), intent(out) :: solution
+         integer(c_int64_t), intent(out) :: iterations
+      end function etg_linear_davidson
+!
       integer(c_int) function etg_diis_excited_states(handle, side, n_states, residual_threshold, &
                                                       energy_threshold, max_iterations, diis_dimension, &
                                                       precision, start_vectors, energies, vectors, &

@@ -7,6 +7,8 @@ residuals.
 DiisCCMultipliers: run_diis_cc_multipliers (diis_cc_multipliers_class.F90:195-299).
 MixedPrecisionMultipliers: the same iteration on split-bf16, FP32, then FP64 left
 transformations.
+DavidsonCCMultipliers: eT's default CCSD multiplier solver, the linear Davidson solver
+on A^T t-bar = -eta (cc_multipliers_solver_factory_class.F90:137-230).
 EigenDavidson: run_eigen_davidson_solver (eigen_davidson_solver_class.F90:122-318),
 with the whole Davidson space on the device.
 DiisCCExcitedStates: run_diis_cc_es (diis_cc_es_class.F90:260-492), one DIIS space
@@ -171,6 +173,39 @@ class DiisCCMultipliers:
             wf.multiplier_diis_update()
         raise _ccsd.ETGError('was not able to converge the equations in the given number of '
                              'maximum iterations.')
+
+
+class DavidsonCCMultipliers:
+    """eT's default CCSD multiplier solver (cc_multipliers_solver_factory_class.F90:137-230):
+    the linear Davidson solver (linear_davidson_solver_class.F90:122-270) on
+    A^T t-bar = -eta, started from the initial multipliers (t1, 2 t_aibj - t_ajbi, or the
+    tbar file with `restart`), reduced space at most `max_dim` (eT's default 50); the
+    trial space, its transforms and the solution stay on the device."""
+
+    def __init__(self, wf, residual_threshold=1e-5, max_iterations=100, max_dim=50, precision=_ccsd.FP64,
+                 log=None):
+        self.wf = wf
+        self.residual_threshold = residual_threshold
+        self.max_iterations = max_iterations
+        self.max_dim = max_dim
+        self.precision = precision
+        self.log = log
+        self.history = []
+
+    def run(self, restart=False):
+        wf = self.wf
+        wf.prepare_for_multiplier_equation()
+        wf.set_initial_multipliers_guess(restart)
+        start = wf.get_multipliers()
+        rhs = -wf.construct_eta()
+        tbar, hist = wf.linear_davidson(rhs, 'left', start, self.residual_threshold, self.max_iterations,
+                                        self.max_dim, self.precision)
+        self.history = list(hist)
+        if self.log:
+            for it, r in enumerate(hist, 1):
+                self.log('%3d %11.4e' % (it, r))
+        wf.set_multipliers(tbar)
+        return tbar
 
 
 class MixedPrecisionMultipliers:

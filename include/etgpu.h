@@ -174,6 +174,22 @@ int etg_davidson(etg_handle* handle, int side, int64_t n_states, double residual
                  const double* start_vectors, double* energies, double* vectors,
                  double* residual_norms, int64_t* iterations);
 
+/* A X = F (side 0) or A^T X = F (side 1) by eT's linear Davidson solver, on
+ * the device: start from start_vector (may be NULL: the preconditioner guess
+ * F / eps), one new trial per iteration (-R / eps of the residual R = A X - F,
+ * orthonormalised against the space), the reduced equation solved exactly,
+ * restart on the current solution at max_dim (<= 0: 50; at most 128),
+ * converged when ||R|| <= residual_threshold.  rhs, start_vector, solution:
+ * n_t1 + n_t2, host or device; residual_norms (max_iterations, may be NULL);
+ * fails after max_iterations.  Needs etg_prepare_for_jacobian.  With side 1 and
+ * rhs = -eta this is eT's default CCSD multiplier solver.
+ * Replaces: run_linear_davidson_solver (src/solvers/linear_davidson_solver_class.F90:122-270)
+ * with the linear Davidson tool (src/solver_tools/davidson/linear_davidson_tool_class.F90),
+ * as cc_multipliers_solver_factory_class.F90:137-230 builds it. */
+int etg_linear_davidson(etg_handle* handle, int side, const double* rhs, const double* start_vector,
+                        double residual_threshold, int64_t max_iterations, int64_t max_dim, int precision,
+                        double* solution, double* residual_norms, int64_t* iterations);
+
 /* The n_states eigenpairs of A (side 0) or A^T (side 1) by eT's DIIS
  * excited-state solver, on the device: each state's vector X follows
  * X += -(A X - E X) / (eps - E), E = X . A X, with DIIS over that state's own

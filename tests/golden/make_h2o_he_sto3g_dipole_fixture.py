@@ -59,6 +59,9 @@ def main():
     table = out.split('Dipole moment in [Debye]:')[1]
     comps = {c: [float(x) for x in re.search(r'^\s+%s\s+(-?\d+\.\d+)\s+(-?\d+\.\d+)\s+(-?\d+\.\d+)\s*$' % c,
                                              table, flags=re.M).groups()] for c in 'xyz'}
+    # eT's default CCSD multiplier solver (linear Davidson): its residual-norm table
+    mult = out.split('Determining CC multipliers')[1].split('Convergence criterion met')[0]
+    mult_rows = [float(m) for m in re.findall(r'^\s+\d+\s+(\d\.\d+E[-+]\d+)\s*$', mult, flags=re.M)]
     fixture = {
         'source': 'eT test data: tests/ccsd_dipole (ccsd_dipole.inp, result/ccsd_dipole.out), '
                   'ao_basis/sto-3g.g94, src/various/parameters.F90',
@@ -75,6 +78,7 @@ def main():
             'debye_to_au': float(grab(table, r'Conversion factor from Debye a\.u\.:\s+(\d+\.\d+)')),
             'au_to_debye': float(grab(params, r'au_to_debye\s*=\s*([0-9.]+)D0')),
             'dipole_debye': {c: dict(zip(('electronic', 'nuclear', 'total'), comps[c])) for c in 'xyz'},
+            'multiplier_davidson_residuals': mult_rows,
         },
         'citations': {
             'nuclear_repulsion': 'tests/ccsd_dipole/result/ccsd_dipole.out:186',
@@ -82,6 +86,7 @@ def main():
             'n_cholesky': 'tests/ccsd_dipole/result/ccsd_dipole.out:252',
             'ccsd_energy': 'tests/ccsd_dipole/result/ccsd_dipole.out:318',
             'dipole_debye': 'tests/ccsd_dipole/result/ccsd_dipole.out:434-445',
+            'multiplier_davidson_residuals': 'tests/ccsd_dipole/result/ccsd_dipole.out:360-395',
             'au_to_debye': 'src/various/parameters.F90:110',
         },
     }

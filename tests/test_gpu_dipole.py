@@ -5,8 +5,10 @@
   device's own amplitudes and multipliers (synthetic system, random X), and the
   CC density from one backward replay gives sum_pq D_pq X_pq equal to it (1e-12);
 * the whole chain -- producer (STO-3G), ground state, multipliers, dipole
-  integrals, mean value -- reproduces eT's tests/ccsd_dipole: E_HF, n_J, E_CCSD
-  and the electronic, nuclear and total dipole in Debye.
+  integrals, mean value -- reproduces eT's tests/ccsd_dipole: E_HF, n_J, E_CCSD,
+  the multiplier solver's run (eT's default linear Davidson: 14 iterations, every
+  residual norm to the printed digits; its multipliers equal the DIIS solver's), and
+  the electronic, nuclear and total dipole in Debye.
 """
 import functools
 import json
@@ -74,8 +76,17 @@ def test_dipole_chain_reproduces_et():
             wf.set_orbital_energies(r['eps'])
             solvers.DiisCCGroundState(wf, residual_threshold=1e-11).run()
             assert abs(r['e_hf'] + wf.correlation_energy - g['ccsd_energy']) < 1e-10
-            solvers.DiisCCMultipliers(wf, residual_threshold=1e-11).run()
+            # eT's default CCSD multiplier solver, as in the eT run: linear Davidson
+            dav = solvers.DavidsonCCMultipliers(wf, residual_threshold=1e-11)
+            tbar = dav.run()
             mu = properties.dipole_moment(wf, mol, r['C'])
+            ref_tbar = solvers.DiisCCMultipliers(wf, residual_threshold=1e-12, max_iterations=200).run()
+    ref_norms = g['multiplier_davidson_residuals']
+    print('multiplier Davidson', len(dav.history), 'iterations (eT %d)' % len(ref_norms))
+    assert len(dav.history) == len(ref_norms)
+    for it, (r, r_ref) in enumerate(zip(dav.history, ref_norms), 1):
+        assert abs(r - r_ref) <= 6e-4 * r_ref + 1e-13, (it, r, r_ref)
+    assert np.abs(tbar - ref_tbar).max() < 1e-10
     for c, comp in enumerate('xyz'):
         ref = g['dipole_debye'][comp]
         assert abs(mu['nuclear_debye'][c] - ref['nuclear']) < 1e-9

@@ -96,6 +96,11 @@ def test_ground_state_and_multipliers_restart_from_ets_files(et_dir, inputs, tmp
         print('restarted multipliers: %d iterations, |R| %.2e' % (len(mult.history), mult.history[0]))
         assert len(mult.history) == 1
         assert np.abs(tbar - v['tbar']).max() < 1e-9
+        # eT's default multiplier solver (linear Davidson), as eT's v1.0 test ran it
+        dav = solvers.DavidsonCCMultipliers(wf, residual_threshold=1e-9)
+        tbar = dav.run(restart=True)
+        assert len(dav.history) == 1
+        assert np.abs(tbar - v['tbar']).max() < 1e-9
 
         out = wf.initialize_files(str(tmp_path / 'out'))
         wf.save_amplitudes()
