@@ -63,6 +63,8 @@ _SIGNATURES = {
                      _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, ctypes.POINTER(_i64)],
     'etg_linear_davidson': [ctypes.c_void_p, ctypes.c_int, _c_dbl_p, _c_dbl_p, ctypes.c_double, _i64, _i64,
                             ctypes.c_int, _c_dbl_p, _c_dbl_p, ctypes.POINTER(_i64)],
+    'etg_newton_raphson_update': [ctypes.c_void_p, ctypes.c_double, _i64, ctypes.c_int, _c_dbl_p,
+                                  ctypes.POINTER(_i64)],
     'etg_diis_excited_states': [ctypes.c_void_p, ctypes.c_int, _i64, ctypes.c_double, ctypes.c_double, _i64,
                                 _i64, ctypes.c_int, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p,
                                 ctypes.POINTER(_i64)],
@@ -500,6 +502,15 @@ class CCSD:
     def quasi_newton_update(self):
         """Device quasi-Newton step t += -omega/eps (quasi_newton_updater_class.F90:81-118)."""
         _check(self._lib.etg_quasi_newton_update(self._h))
+
+    def newton_raphson_update(self, rel_threshold=1e-2, max_iterations=100, precision=FP64):
+        """eT's Newton-Raphson step t += u with A u = -Omega (linear Davidson); the next
+        diis_update records (Omega, t).  Returns the linear iterations' residual norms."""
+        hist = np.full(max_iterations, np.nan)
+        its = _i64()
+        _check(self._lib.etg_newton_raphson_update(self._h, rel_threshold, max_iterations, int(precision),
+                                                   _ptr(hist), ctypes.byref(its)))
+        return hist[:its.value]
 
     def diis_update(self):
         """Device DIIS extrapolation of the amplitudes (diis_tool_class.F90:302-347)."""

@@ -210,6 +210,14 @@ int etg_linear_davidson(etg_handle* h, int side, const double* rhs, const double
   });
 }
 
+int etg_newton_raphson_update(etg_handle* h, double rel_threshold, int64_t max_iterations, int precision,
+                              double* micro_norms, int64_t* iterations) {
+  return guarded([&] {
+    const int64_t it = wf_of(h).newton_raphson_update(rel_threshold, max_iterations, prec_of(precision), micro_norms);
+    if (iterations) *iterations = it;
+  });
+}
+
 int etg_diis_excited_states(etg_handle* h, int side, int64_t n_states, double residual_threshold,
                             double energy_threshold, int64_t max_iterations, int64_t diis_dimension,
                             int precision, const double* start_vectors, double* energies, double* vectors,

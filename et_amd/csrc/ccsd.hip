@@ -914,6 +914,35 @@ void CCSD::transform_body(int side, Prec prec, int64_t nb) {
   }
 }
 
+int64_t CCSD::newton_raphson_update(double rel_threshold, int64_t max_iterations, Prec prec,
+                                    double* micro_norms) {
+  require(have_omega_, "newton_raphson_update: the residual has not been constructed");
+  require(tape_ != nullptr, "newton_raphson_update: prepare_for_jacobian has not been called at these amplitudes");
+  require(rel_threshold > 0.0, "newton_raphson_update: the relative threshold must be positive");
+  const int64_t n = n_t1 + n_t2;
+  const double norm = omega_norm();
+  nr_rhs_.resize(sizeof(double) * size_t(n));
+  update_.resize(sizeof(double) * size_t(n));
+  ETG_HIP(hipMemcpyAsync(nr_rhs_.as<double>(), omega_.as<double>(), sizeof(double) * n, hipMemcpyDeviceToDevice,
+                         stream_));
+  scale(stream_, nr_rhs_.as<double>(), n, -1.0);
+  const double threshold = rel_threshold * norm;
+  const int64_t its = linear_davidson(0, nr_rhs_.as<double>(), nullptr, threshold, max_iterations, 50, prec,
+                                      update_.as<double>(), micro_norms, std::min(1e-11, threshold));
+  const dim3 g(unsigned(ceil_div(n_t1, 256)), 1);
+  scale_doubles_diagonal_kernel<<<g, 256, 0, stream_>>>(update_.as<double>(), n_t1, n, 2.0);
+  ETG_LAUNCH("scale_doubles_diagonal");
+  axpy(stream_, n, 1.0, update_.as<double>(), t_.as<double>());
+  // the DIIS step that follows records (Omega, t)
+  ETG_HIP(hipMemcpyAsync(update_.as<double>(), omega_.as<double>(), sizeof(double) * n, hipMemcpyDeviceToDevice,
+                         stream_));
+  synchronize();
+  have_update_ = true;
+  have_omega_ = false;
+  reset_record();
+  return its;
+}
+
 // The record and everything captured from it are dropped together.
 void CCSD::reset_record() {
   drop_transform_graphs();

@@ -111,7 +111,16 @@ class CCSD {
   // null).  Returns the number of iterations.
   int64_t linear_davidson(int side, const double* F, const double* start, double residual_threshold,
                           int64_t max_iterations, int64_t max_dim, Prec prec, double* X_out,
-                          double* residual_norms);
+                          double* residual_norms, double lindep = 0.0);
+  // eT's Newton-Raphson amplitude update (newton_raphson_updater_class.F90:118-210,
+  // as cc_amplitudes_solver_factory_class.F90 sets it up for CCSD): solves A u = -Omega
+  // with the linear Davidson solver to ||A u + Omega|| < rel_threshold ||Omega||
+  // (start -Omega / eps, reduced space 50), doubles the packed doubles diagonal of u
+  // (scale_amplitudes), t += u, and leaves Omega as the error of the next DIIS step
+  // (scale_residual off).  Needs the residual of the current amplitudes and
+  // prepare_for_jacobian at them.  micro_norms: max_iterations, may be null.
+  // Returns the number of linear iterations.
+  int64_t newton_raphson_update(double rel_threshold, int64_t max_iterations, Prec prec, double* micro_norms);
   // eT's DIIS excited-state solver (run_diis_cc_es, diis_cc_es_class.F90:260-492):
   // each state iterates X += -(A X - E X)/(eps - E), DIIS over its own records,
   // X normalised, until ||A X - E X|| < residual_threshold and |dE| <
@@ -203,7 +212,7 @@ class CCSD {
   // DIIS records: errors and parameters, dim x (n_t1 + n_t2) each
   DiisSpace diis_;                                // ground state and multipliers
   std::vector<DiisSpace> es_diis_;                // one per excited state (DIIS excited-state solver)
-  DeviceBuffer update_, es_X_, es_R_;
+  DeviceBuffer update_, es_X_, es_R_, nr_rhs_;
   bool have_update_ = false, have_omega_ = false;
 
   // Davidson space (V, AV), Ritz block, reduced matrices

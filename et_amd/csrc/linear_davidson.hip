@@ -9,7 +9,8 @@
 // below the linear-dependence threshold); its transform A c (right) or A^T c
 // (left) is added; the reduced equation (V^T A V) y = V^T F is solved; the
 // residual R = A V y - F is tested against the threshold, and otherwise
-// -R / (eps - 0), normalised, is the next trial.  V, A V and every vector stay
+// -R / (eps - 0), normalised, is the next trial.  lindep: the linear-dependence
+// threshold on a normalised trial's projected norm (<= 0: min(1e-11, 0.1 threshold)).  V, A V and every vector stay
 // in HBM; the reduced matrices (max_dim <= 128) go to the host for the solve.
 #include <algorithm>
 #include <cmath>
@@ -61,7 +62,7 @@ bool solve_dense(std::vector<double> A, std::vector<double>& b, int64_t k) {
 
 int64_t CCSD::linear_davidson(int side, const double* F, const double* start, double residual_threshold,
                               int64_t max_iterations, int64_t max_dim, Prec prec, double* X_out,
-                              double* residual_norms) {
+                              double* residual_norms, double lindep) {
   require(tape_ != nullptr, "linear_davidson: prepare_for_jacobian has not been called");
   require(have_eps_, "linear_davidson: orbital energies required");
   require(side == 0 || side == 1, "linear_davidson: side must be 0 (right) or 1 (left)");
@@ -71,7 +72,7 @@ int64_t CCSD::linear_davidson(int side, const double* F, const double* start, do
   if (max_dim <= 0) max_dim = 50;
   max_dim = std::min<int64_t>(std::min<int64_t>(max_dim, kMaxLinearDim), n);
   require(max_dim >= 2, "linear_davidson: maximum reduced dimension below 2");
-  const double lindep = std::min(1e-11, 0.1 * residual_threshold);
+  if (!(lindep > 0.0)) lindep = std::min(1e-11, 0.1 * residual_threshold);
 
   // HBM: V, AV (n x max_dim), the right-hand side, the trial and residual vectors
   dav_V_.resize(sizeof(double) * size_t(n) * size_t(max_dim));
@@ -161,10 +162,12 @@ int64_t CCSD::linear_davidson(int side, const double* F, const double* start, do
       synchronize();
       return it;
     }
-    // next trial: -R / (eps - 0), normalised in the orthonormalisation
+    // next trial: -R / (eps - 0), normalised before the orthonormalisation, so that
+    // the linear-dependence test compares the fraction left after projection
     ETG_HIP(hipMemcpyAsync(c, R, sizeof(double) * n, hipMemcpyDeviceToDevice, stream_));
     precondition_kernel<<<grid_ld(n), kBlock, 0, stream_>>>(c, d, n, -1.0);
     ETG_LAUNCH("precondition");
+    scale(stream_, c, n, 1.0 / std::sqrt(dot_host(c, c)));
   }
   fail("linear_davidson: was not able to converge the equations in the given number of maximum iterations.");
 }

@@ -147,6 +147,18 @@ module etgpu_interface
          integer(c_int64_t), intent(out) :: iterations
       end function etg_linear_davidson
 !
+      integer(c_int) function etg_newton_raphson_update(handle, rel_threshold, max_iterations, precision, &
+                                                        micro_norms, iterations) &
+                                                        bind(C, name='etg_newton_raphson_update')
+         import :: c_int, c_ptr, c_double, c_int64_t
+         type(c_ptr), value :: handle
+         real(c_double), value :: rel_threshold
+         integer(c_int64_t), value :: max_iterations
+         integer(c_int), value :: precision
+         type(c_ptr), value :: micro_norms
+         integer(c_int64_t), intent(out) :: iterations
+      end function etg_newton_raphson_update
+!
       integer(c_int) function etg_diis_excited_states(handle, side, n_states, residual_threshold, &
                                                       energy_threshold, max_iterations, diis_dimension, &
                                                       precision, start_vectors, energies, vectors, &

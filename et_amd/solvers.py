@@ -4,6 +4,8 @@ DiisCCGroundState: run_diis_cc_gs (src/solvers/cc/diis_cc_gs_class.F90:209-315);
 quasi-Newton step and DIIS extrapolation on the device in FP64.
 MixedPrecisionGroundState: the same iteration on split-bf16, FP32, then FP64
 residuals.
+NewtonRaphsonCCGroundState: the same loop with eT's Newton-Raphson amplitude updater
+(A u = -Omega by the linear Davidson solver).
 DiisCCMultipliers: run_diis_cc_multipliers (diis_cc_multipliers_class.F90:195-299).
 MixedPrecisionMultipliers: the same iteration on split-bf16, FP32, then FP64 left
 transformations.
@@ -135,6 +137,56 @@ class MixedPrecisionGroundState:
                 best, since_best = np.inf, 0
                 if self.restart_diis:
                     wf.diis_initialize(self.diis_dimension)
+        raise _ccsd.ETGError('Did not converge in the max number of iterations.')
+
+
+class NewtonRaphsonCCGroundState:
+    """run_diis_cc_gs with eT's Newton-Raphson amplitude updater (algorithm: newton-raphson,
+    cc_amplitudes_solver_factory_class.F90): each iteration forms the T1 Fock matrix, the
+    energy and the FP64 residual, tests ||Omega|| (and |dE| when energy_threshold is set;
+    the first iteration tests the residual only), records the residual program at the
+    current amplitudes, solves A u = -Omega by the linear Davidson solver to
+    rel_threshold ||Omega|| (`precision` for its transformations), t += u, then the DIIS
+    extrapolation over (Omega, t) and the T1 dressing, all on the device.
+    """
+
+    def __init__(self, wf, residual_threshold=1e-5, energy_threshold=None, max_iterations=100,
+                 diis_dimension=8, rel_threshold=1e-2, max_micro_iterations=100, precision=_ccsd.FP64, log=None):
+        self.wf = wf
+        self.residual_threshold = residual_threshold
+        self.energy_threshold = energy_threshold
+        self.max_iterations = max_iterations
+        self.diis_dimension = diis_dimension
+        self.rel_threshold = rel_threshold
+        self.max_micro_iterations = max_micro_iterations
+        self.precision = precision
+        self.log = log
+        self.history = []          # (energy, ||Omega||, |dE|)
+        self.micro = []            # residual norms of each update's linear iterations
+
+    def run(self, restart=False):
+        wf = self.wf
+        wf.set_initial_amplitudes_guess(restart)
+        wf.diis_initialize(self.diis_dimension)
+        previous = 0.0
+        for it in range(1, self.max_iterations + 1):
+            wf.construct_fock('gs')
+            wf.calculate_energy()
+            wf.construct_omega(_ccsd.FP64, copy_out=False)
+            norm = wf.omega_norm()
+            de = abs(wf.energy - previous)
+            self.history.append((wf.energy, norm, de))
+            if self.log:
+                self.log('%3d %25.12f %11.4e %11.4e' % (it, wf.energy, norm, de))
+            energy_ok = self.energy_threshold is None or de < self.energy_threshold or it == 1
+            if norm < self.residual_threshold and energy_ok:
+                return wf.energy
+            wf.prepare_for_jacobian()
+            self.micro.append(list(wf.newton_raphson_update(self.rel_threshold, self.max_micro_iterations,
+                                                             self.precision)))
+            wf.diis_update()
+            wf.construct_t1_cholesky()
+            previous = wf.energy
         raise _ccsd.ETGError('Did not converge in the max number of iterations.')
 
 

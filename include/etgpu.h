@@ -190,6 +190,19 @@ int etg_linear_davidson(etg_handle* handle, int side, const double* rhs, const d
                         double residual_threshold, int64_t max_iterations, int64_t max_dim, int precision,
                         double* solution, double* residual_norms, int64_t* iterations);
 
+/* eT's Newton-Raphson amplitude update: A u = -Omega by the linear Davidson
+ * solver (as etg_linear_davidson, start -Omega / eps, reduced space 50) to
+ * ||A u + Omega|| < rel_threshold ||Omega||, the packed doubles diagonal of u
+ * doubled, t += u; the following etg_diis_update records (Omega, t).  Needs the
+ * residual of the current amplitudes (etg_construct_omega) and
+ * etg_prepare_for_jacobian at them.  micro_norms (max_iterations, may be NULL):
+ * the linear iterations' residual norms; iterations: their number.
+ * Replaces: newton_raphson_updater%update
+ * (src/solver_tools/amplitude_updaters/newton_raphson_updater_class.F90:118-210,
+ * amplitude_updater_class.F90:86-121) in run_diis_cc_gs. */
+int etg_newton_raphson_update(etg_handle* handle, double rel_threshold, int64_t max_iterations, int precision,
+                              double* micro_norms, int64_t* iterations);
+
 /* The n_states eigenpairs of A (side 0) or A^T (side 1) by eT's DIIS
  * excited-state solver, on the device: each state's vector X follows
  * X += -(A X - E X) / (eps - E), E = X . A X, with DIIS over that state's own
