@@ -72,6 +72,7 @@ _SIGNATURES = {
     'etg_diis_initialize': [ctypes.c_void_p, _i64],
     'etg_quasi_newton_update': [ctypes.c_void_p],
     'etg_diis_update': [ctypes.c_void_p],
+    'etg_diis_initialize_crop': [ctypes.c_void_p, _i64],
     'etg_prepare_for_multiplier_equation': [ctypes.c_void_p],
     'etg_set_initial_multipliers_guess': [ctypes.c_void_p],
     'etg_set_multipliers': [ctypes.c_void_p, _c_dbl_p],
@@ -495,9 +496,13 @@ class CCSD:
         self.diis_energy_history = ehist[:its.value]
         return e, hist[:its.value], X
 
-    def diis_initialize(self, dimension=8):
-        """HBM records for DIIS (diis_tool_class.F90:249-277)."""
-        _check(self._lib.etg_diis_initialize(self._h, int(dimension)))
+    def diis_initialize(self, dimension=8, crop=False):
+        """HBM records for DIIS (diis_tool_class.F90:249-277); crop: eT's CROP variant
+        (diis_tool_class.F90:763-805)."""
+        if crop:
+            _check(self._lib.etg_diis_initialize_crop(self._h, int(dimension)))
+        else:
+            _check(self._lib.etg_diis_initialize(self._h, int(dimension)))
 
     def quasi_newton_update(self):
         """Device quasi-Newton step t += -omega/eps (quasi_newton_updater_class.F90:81-118)."""

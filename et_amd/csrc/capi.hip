@@ -239,6 +239,10 @@ int etg_quasi_newton_update(etg_handle* h) {
   return guarded([&] { wf_of(h).quasi_newton_update(); });
 }
 
+int etg_diis_initialize_crop(etg_handle* h, int64_t dimension) {
+  return guarded([&] { wf_of(h).diis_initialize(dimension, true); });
+}
+
 int etg_diis_update(etg_handle* h) {
   return guarded([&] { wf_of(h).diis_update(); });
 }

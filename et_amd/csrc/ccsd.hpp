@@ -25,6 +25,7 @@ void reduced_eigen_solve(hipStream_t stream, DeviceBuffer& scratch, const double
 // the weights of the last solve.
 struct DiisSpace {
   int dim = 0;
+  bool crop = false;          // eT's CROP variant (conjugate residual with optimal trial vectors)
   int64_t count = 0, n = 0;
   DeviceBuffer e, x, G, w, info;
 };
@@ -58,7 +59,7 @@ class CCSD {
   // ground-state amplitude update on the device (FP64): quasi-Newton step and
   // DIIS extrapolation over records kept in HBM
   static constexpr int kMaxDiis = 32;
-  void diis_initialize(int64_t dimension);
+  void diis_initialize(int64_t dimension, bool crop = false);
   void quasi_newton_update();                     // e = -omega/eps (doubles diagonal x2); t += e
   void diis_update();                             // records (e, t); t = DIIS-extrapolated t
 
@@ -264,6 +265,6 @@ class CCSD {
 };
 
 // Empties a DIIS space and sizes it for `dimension` records of length n.
-void diis_space_initialize(hipStream_t s, DiisSpace& d, int64_t n, int64_t dimension);
+void diis_space_initialize(hipStream_t s, DiisSpace& d, int64_t n, int64_t dimension, bool crop = false);
 
 }  // namespace etg

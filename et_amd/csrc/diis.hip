@@ -118,6 +118,20 @@ __global__ void extrapolate_kernel(const double* __restrict__ X, int64_t n, int 
   }
 }
 
+// CROP (diis_tool_class.F90:763-805): the newest error record E_slot becomes the
+// extrapolated error <e> = sum_r w_r E_r.  Each element is read and written by one
+// thread, so the record is updated in place.
+__global__ void crop_error_kernel(double* E, int64_t n, int dim, int slot, const double* __restrict__ w,
+                                  const int* __restrict__ info) {
+  if (*info != 0) return;
+  for (int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; k < n;
+       k += int64_t(gridDim.x) * blockDim.x) {
+    double s = 0.0;
+    for (int r = 0; r < dim; ++r) s += w[r] * E[k + int64_t(r) * n];
+    E[k + int64_t(slot) * n] = s;
+  }
+}
+
 unsigned grid_for_n(int64_t n) {
   return unsigned(std::min<int64_t>((n + kBlock - 1) / kBlock, 1 << 16));
 }
@@ -130,9 +144,10 @@ void quasi_newton_step(hipStream_t s, const double* eps, const double* residual,
   ETG_LAUNCH("quasi_newton");
 }
 
-void diis_space_initialize(hipStream_t s, DiisSpace& d, int64_t n, int64_t dimension) {
+void diis_space_initialize(hipStream_t s, DiisSpace& d, int64_t n, int64_t dimension, bool crop) {
   if (dimension < 1 || dimension > CCSD::kMaxDiis) fail("diis_initialize: dimension must be in [1, 32]");
   d.dim = int(dimension);
+  d.crop = crop;
   d.count = 0;
   d.n = n;
   d.e.resize(sizeof(double) * size_t(n) * size_t(dimension));
@@ -143,8 +158,8 @@ void diis_space_initialize(hipStream_t s, DiisSpace& d, int64_t n, int64_t dimen
   fill(s, d.G.as<double>(), CCSD::kMaxDiis * CCSD::kMaxDiis, 0.0);
 }
 
-void CCSD::diis_initialize(int64_t dimension) {
-  diis_space_initialize(stream_, diis_, n_t1 + n_t2, dimension);
+void CCSD::diis_initialize(int64_t dimension, bool crop) {
+  diis_space_initialize(stream_, diis_, n_t1 + n_t2, dimension, crop);
 }
 
 void CCSD::quasi_newton_update() {
@@ -183,16 +198,26 @@ void CCSD::diis_step(DiisSpace& d, const double* e, double* x) {
   ETG_HIP(hipMemcpyAsync(X + int64_t(slot) * n, x, sizeof(double) * n, hipMemcpyDeviceToDevice,
                          stream_));
   // new row of G (construct_new_G_elements, diis_tool_class.F90:503-546)
-  for (int i = 0; i < dim; ++i)
-    dot(stream_, E + int64_t(slot) * n, E + int64_t(i) * n, n, red_partial_.as<double>(),
-        G + slot + int64_t(i) * kMaxDiis);
-  mirror_row_kernel<<<1, kMaxDiis, 0, stream_>>>(G, kMaxDiis, slot, dim);
-  ETG_LAUNCH("diis_mirror_row");
+  auto new_row = [&] {
+    for (int i = 0; i < dim; ++i)
+      dot(stream_, E + int64_t(slot) * n, E + int64_t(i) * n, n, red_partial_.as<double>(),
+          G + slot + int64_t(i) * kMaxDiis);
+    mirror_row_kernel<<<1, kMaxDiis, 0, stream_>>>(G, kMaxDiis, slot, dim);
+    ETG_LAUNCH("diis_mirror_row");
+  };
+  new_row();
   diis_solve_kernel<<<1, 1, 0, stream_>>>(G, kMaxDiis, dim, d.w.as<double>(), d.info.as<int>());
   ETG_LAUNCH("diis_solve");
   extrapolate_kernel<<<grid_for_n(n), kBlock, 0, stream_>>>(X, n, dim, d.w.as<double>(),
                                                             d.info.as<int>(), x);
   ETG_LAUNCH("diis_extrapolate");
+  if (d.crop) {
+    // the newest records become <x> and <e>, and G's row follows <e>
+    ETG_HIP(hipMemcpyAsync(X + int64_t(slot) * n, x, sizeof(double) * n, hipMemcpyDeviceToDevice, stream_));
+    crop_error_kernel<<<grid_for_n(n), kBlock, 0, stream_>>>(E, n, dim, slot, d.w.as<double>(), d.info.as<int>());
+    ETG_LAUNCH("diis_crop_error");
+    new_row();
+  }
   int info = 0;
   ETG_HIP(hipMemcpyAsync(&info, d.info.as<int>(), sizeof(int), hipMemcpyDeviceToHost, stream_));
   synchronize();

@@ -180,6 +180,12 @@ module etgpu_interface
          integer(c_int64_t), value :: dimension
       end function etg_diis_initialize
 !
+      integer(c_int) function etg_diis_initialize_crop(handle, dimension) bind(C, name='etg_diis_initialize_crop')
+         import :: c_int, c_ptr, c_int64_t
+         type(c_ptr), value :: handle
+         integer(c_int64_t), value :: dimension
+      end function etg_diis_initialize_crop
+!
 !     t_updater%update (quasi_newton_updater_class.F90:81-118), on the device
       integer(c_int) function etg_quasi_newton_update(handle) bind(C, name='etg_quasi_newton_update')
          import :: c_int, c_ptr

@@ -34,20 +34,21 @@ class DiisCCGroundState:
     """
 
     def __init__(self, wf, residual_threshold=1e-5, max_iterations=100, diis_dimension=8,
-                 precision=_ccsd.FP64, log=None):
+                 precision=_ccsd.FP64, log=None, crop=False):
         self.wf = wf
         self.residual_threshold = residual_threshold
         self.max_iterations = max_iterations
         self.diis_dimension = diis_dimension
         self.precision = precision
         self.log = log
+        self.crop = crop
         self.history = []
 
     def run(self, restart=False):
         """restart: start from the wavefunction's t file (initialize_files) where it has one."""
         wf = self.wf
         wf.set_initial_amplitudes_guess(restart)
-        wf.diis_initialize(self.diis_dimension)
+        wf.diis_initialize(self.diis_dimension, self.crop)
         previous = 0.0
         for it in range(1, self.max_iterations + 1):
             wf.construct_fock('gs')
@@ -151,7 +152,8 @@ class NewtonRaphsonCCGroundState:
     """
 
     def __init__(self, wf, residual_threshold=1e-5, energy_threshold=None, max_iterations=100,
-                 diis_dimension=8, rel_threshold=1e-2, max_micro_iterations=100, precision=_ccsd.FP64, log=None):
+                 diis_dimension=8, rel_threshold=1e-2, max_micro_iterations=100, precision=_ccsd.FP64, log=None,
+                 crop=False):
         self.wf = wf
         self.residual_threshold = residual_threshold
         self.energy_threshold = energy_threshold
@@ -161,13 +163,14 @@ class NewtonRaphsonCCGroundState:
         self.max_micro_iterations = max_micro_iterations
         self.precision = precision
         self.log = log
+        self.crop = crop
         self.history = []          # (energy, ||Omega||, |dE|)
         self.micro = []            # residual norms of each update's linear iterations
 
     def run(self, restart=False):
         wf = self.wf
         wf.set_initial_amplitudes_guess(restart)
-        wf.diis_initialize(self.diis_dimension)
+        wf.diis_initialize(self.diis_dimension, self.crop)
         previous = 0.0
         for it in range(1, self.max_iterations + 1):
             wf.construct_fock('gs')
@@ -199,19 +202,20 @@ class DiisCCMultipliers:
     the device.
     """
 
-    def __init__(self, wf, residual_threshold=1e-5, max_iterations=100, diis_dimension=8, log=None):
+    def __init__(self, wf, residual_threshold=1e-5, max_iterations=100, diis_dimension=8, log=None, crop=False):
         self.wf = wf
         self.residual_threshold = residual_threshold
         self.max_iterations = max_iterations
         self.diis_dimension = diis_dimension
         self.log = log
+        self.crop = crop
         self.history = []
 
     def run(self, restart=False):
         """restart: start from the wavefunction's tbar file (initialize_files) where it has one."""
         wf = self.wf
         wf.prepare_for_multiplier_equation()
-        wf.diis_initialize(self.diis_dimension)
+        wf.diis_initialize(self.diis_dimension, self.crop)
         wf.set_initial_multipliers_guess(restart)
         for it in range(1, self.max_iterations + 1):
             wf.construct_multiplier_equation(copy_out=False)

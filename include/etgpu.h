@@ -134,6 +134,13 @@ int etg_construct_omega_resident(etg_handle* handle, int precision);
  * as run_diis_cc_gs calls it (src/solvers/cc/diis_cc_gs_class.F90:230). */
 int etg_diis_initialize(etg_handle* handle, int64_t dimension);
 
+/* etg_diis_initialize with eT's CROP variant: after each extrapolation the newest
+ * records become the extrapolated parameters <x> and error <e> = sum_i w_i e_i.
+ * Replaces: diis_tool(..., crop=.true.) (diis_tool_class.F90:150-210, 763-805) as
+ * the keyword crop of the solver cc gs / solver cc multipliers sections sets it
+ * (diis_cc_gs_class.F90:393, diis_cc_multipliers_class.F90:370). */
+int etg_diis_initialize_crop(etg_handle* handle, int64_t dimension);
+
 /* Quasi-Newton step on the device, FP64: e = -Omega / eps (orbital
  * differences; doubles diagonal doubled), amplitudes += e.  Needs a residual
  * constructed at the current amplitudes; e is kept for etg_diis_update.

@@ -1,16 +1,18 @@
 #!/usr/bin/env python3
-"""Writes tests/golden/h2o_he_newton_raphson.json from eT's tests/ccsd_gs_energy_nr.
-
-H2O-He / cc-pVDZ, Cholesky threshold 1e-10, the CCSD ground state by eT's
-Newton-Raphson solver (relative micro threshold 1e-2, residual and energy thresholds
-1e-10).  Read as text from the input and result/ccsd_gs_energy_nr.out:
+"""Writes the Newton-Raphson fixtures from eT's tests:
+  tests/golden/h2o_he_newton_raphson.json  tests/ccsd_gs_energy_nr: H2O-He / cc-pVDZ,
+      Cholesky threshold 1e-10, relative micro threshold 1e-2, residual and energy
+      thresholds 1e-10, DIIS dimension 8;
+  tests/golden/o3_newton_raphson_crop.json  tests/ccsd_gs_energy_nr_crop: ozone / cc-pVDZ,
+      Cholesky threshold 1e-10, residual threshold 1e-8, CROP with DIIS dimension 3.
+Read as text from each input and its result/*.out:
   geometry (Angstrom), thresholds;
   E_nuc, E_HF, number of Cholesky vectors, number of occupied orbitals;
   the macro-iteration table (energy, |omega|, delta E) and, after each macro iteration,
   the residual norms of its linear (micro) iterations;
   the final ground-state energy.
 
-Usage: python tests/golden/make_h2o_he_newton_raphson_fixture.py [/root/reference]
+Usage: python tests/golden/make_newton_raphson_fixtures.py [/root/reference]
 """
 import json
 import os
@@ -18,12 +20,20 @@ import re
 import sys
 
 REF = sys.argv[1] if len(sys.argv) > 1 else '/root/reference'
-STEM = 'tests/ccsd_gs_energy_nr/ccsd_gs_energy_nr'
-REL = 'tests/ccsd_gs_energy_nr/result/ccsd_gs_energy_nr.out'
-OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'h2o_he_newton_raphson.json')
+HERE = os.path.dirname(os.path.abspath(__file__))
+CASES = [
+    ('ccsd_gs_energy_nr', 'h2o_he_newton_raphson.json',
+     {'cholesky_threshold': 1e-10, 'relative_micro_threshold': 1e-2, 'residual_threshold': 1e-10,
+      'energy_threshold': 1e-10, 'diis_dimension': 8, 'crop': False}),
+    ('ccsd_gs_energy_nr_crop', 'o3_newton_raphson_crop.json',
+     {'cholesky_threshold': 1e-10, 'relative_micro_threshold': 1e-2, 'residual_threshold': 1e-8,
+      'energy_threshold': None, 'diis_dimension': 3, 'crop': True}),
+]
 
 
-def main():
+def write(name, out_name, settings):
+    STEM = 'tests/%s/%s' % (name, name)
+    REL = 'tests/%s/result/%s.out' % (name, name)
     inp = open(os.path.join(REF, STEM + '.inp')).read()
     lines = open(os.path.join(REF, REL)).read().splitlines()
     geom = [[s[0], float(s[1]), float(s[2]), float(s[3])]
@@ -60,10 +70,15 @@ def main():
     g['micro_residual_norms'] = micro         # one list per Newton-Raphson update
     cite['macro_iterations'] = '%s:%d-%d' % (REL, head + 1, end + 1)
     out = {'source': STEM + '.inp', 'basis': 'cc-pVDZ', 'bohr_to_angstrom': 0.52917721092,
-           'geometry_angstrom': geom, 'cholesky_threshold': 1e-10, 'relative_micro_threshold': 1e-2,
-           'residual_threshold': 1e-10, 'energy_threshold': 1e-10, 'golden': g, 'citations': cite}
-    json.dump(out, open(OUT, 'w'), indent=1)
-    print(json.dumps(g, indent=1))
+           'geometry_angstrom': geom, **settings, 'golden': g, 'citations': cite}
+    json.dump(out, open(os.path.join(HERE, out_name), 'w'), indent=1)
+    print(out_name, len(g['macro_iterations']), 'macro iterations,',
+          [len(m) for m in g['micro_residual_norms']], 'linear iterations')
+
+
+def main():
+    for case in CASES:
+        write(*case)
 
 
 if __name__ == '__main__':
