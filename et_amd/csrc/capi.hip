@@ -453,6 +453,10 @@ int etg_molecule_set_orbitals(etg_molecule* molecule, int64_t n_ao, int64_t n_mo
   });
 }
 
+int etg_molecule_set_frozen_core(etg_molecule* molecule, int64_t n_frozen) {
+  return guarded([&] { mol_of(molecule).set_frozen_core(n_frozen); });
+}
+
 int etg_molecule_dims(etg_molecule* molecule, int64_t* n_ao, int64_t* n_occ, int64_t* n_J,
                       int64_t* scf_iterations) {
   return guarded([&] {

@@ -955,6 +955,11 @@ void Molecule::set_orbitals(const double* C, const double* eps) {
   given_orbitals_ = true;
 }
 
+void Molecule::set_frozen_core(int64_t n_frozen) {
+  if (n_frozen < 0 || n_frozen >= n_occ_) fail("set_frozen_core: the number of frozen orbitals must be in [0, n_occ)");
+  n_frozen_core_ = n_frozen;
+}
+
 void Molecule::mo_transform() {
   using namespace fe;
   const int64_t n = n_, nn = n * n, J = n_J_;
@@ -975,6 +980,24 @@ void Molecule::mo_transform() {
   // h_mo = C^T h C
   DeviceBuffer h(sizeof(double) * nn), t(sizeof(double) * nn), hm(sizeof(double) * nn);
   ETG_HIP(hipMemcpyAsync(h.as<double>(), h_ao_.data(), sizeof(double) * nn, hipMemcpyHostToDevice, stream_));
+  if (n_frozen_core_ > 0) {
+    // h += 2 J[D_c] - K[D_c],  D_c = C_c C_c^T over the frozen MOs (construct_mo_fock_fc_term_hf,
+    // frozen_orbital_hf.F90:547-600), from the AO Cholesky vectors as in the RHF Fock build
+    const int64_t c = n_frozen_core_;
+    DeviceBuffer D(sizeof(double) * nn), u(sizeof(double) * std::max<int64_t>(1, J)),
+        Y(sizeof(double) * n * c * std::max<int64_t>(1, J));
+    etg::gemm(stream_, 'N', 'T', n, n, c, 1.0, C.as<double>(), n, C.as<double>(), n, 0.0, D.as<double>(), n,
+              Prec::F64, &ws);
+    etg::gemm(stream_, 'T', 'N', J, 1, nn, 1.0, Lao_.as<double>(), nn, D.as<double>(), nn, 0.0, u.as<double>(), J,
+              Prec::F64, &ws);
+    etg::gemm(stream_, 'N', 'N', nn, 1, J, 2.0, Lao_.as<double>(), nn, u.as<double>(), J, 1.0, h.as<double>(), nn,
+              Prec::F64, &ws);
+    Gemm gy = make_gemm('N', 'N', n, c, n, 1.0, Lao_.as<double>(), n, C.as<double>(), n, 0.0, Y.as<double>(), n);
+    gy.batch = J; gy.strideA = nn; gy.strideB = 0; gy.strideC = n * c;
+    etg::gemm(stream_, gy, Prec::F64, &ws);
+    etg::gemm(stream_, 'N', 'T', n, n, c * J, -1.0, Y.as<double>(), n, Y.as<double>(), n, 1.0, h.as<double>(), n,
+              Prec::F64, &ws);
+  }
   etg::gemm(stream_, 'N', 'N', n, n, n, 1.0, h.as<double>(), n, C.as<double>(), n, 0.0, t.as<double>(), n, Prec::F64, &ws);
   etg::gemm(stream_, 'T', 'N', n, n, n, 1.0, C.as<double>(), n, t.as<double>(), n, 0.0, hm.as<double>(), n, Prec::F64, &ws);
   h_mo_.assign(size_t(nn), 0.0);

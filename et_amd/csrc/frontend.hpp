@@ -43,6 +43,10 @@ class Molecule {
   // the RHF iterations are skipped, one Fock build gives E_HF and checks that
   // max|FDS - SDF| < scf_threshold (eT's `skip` of a restarted SCF)
   void set_orbitals(const double* C, const double* eps);
+  // frozen core for the next run: h_mo then carries the Coulomb and exchange of the
+  // n_frozen lowest (doubly occupied) MOs, F^fc_pq = sum_c 2 (pq|cc) - (pc|cq), so that
+  // the MOs above them describe the correlated problem with E_HF unchanged
+  void set_frozen_core(int64_t n_frozen);
   // after run: E_nuc, E_HF, eps (n), h_mo (n x n), L_mo (n_J x n x n, J fastest), C (n x n); any may be null
   void results(double* e_nuc, double* e_hf, double* eps, double* h_mo, double* L_mo, double* C) const;
 
@@ -60,7 +64,7 @@ class Molecule {
 
   std::vector<fe::HostShell> shells_;
   std::vector<double> Z_, xyz_;
-  int64_t n_ = 0, n_occ_ = 0, n_J_ = 0, scf_iterations_ = 0;
+  int64_t n_ = 0, n_occ_ = 0, n_J_ = 0, scf_iterations_ = 0, n_frozen_core_ = 0;
   double e_nuc_ = 0.0, e_hf_ = 0.0;
   bool done_ = false, given_orbitals_ = false;
   hipStream_t stream_ = nullptr;

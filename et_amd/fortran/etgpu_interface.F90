@@ -359,6 +359,13 @@ module etgpu_interface
          real(c_double), dimension(*), intent(in) :: C, eps
       end function etg_molecule_set_orbitals
 !
+      integer(c_int) function etg_molecule_set_frozen_core(molecule, n_frozen) &
+                                                           bind(C, name='etg_molecule_set_frozen_core')
+         import :: c_int, c_ptr, c_int64_t
+         type(c_ptr), value :: molecule
+         integer(c_int64_t), value :: n_frozen
+      end function etg_molecule_set_frozen_core
+!
       integer(c_int) function etg_molecule_dims(molecule, n_ao, n_occ, n_J, scf_iterations) &
                                                 bind(C, name='etg_molecule_dims')
          import :: c_int, c_ptr, c_int64_t

@@ -93,6 +93,17 @@ def water_lattice(shape=(2, 2, 2), spacing=2.9, seed=0, min_distance=1.5, max_dr
     return atoms
 
 
+def frozen_core_orbitals(Z):
+    """eT's frozen-core count per atom (get_frozen_cores_and_n_frozen_orbitals,
+    ao_tool_class.F90:2425-2507): 1 for 5 <= Z <= 12, 5 for 13..30, 9 for 31..38, 14 for
+    39..48, 18 for 49..70, 23 for 71..80, 34 for 81..103, 50 for 104..105."""
+    for lo, hi, k in ((1, 4, 0), (5, 12, 1), (13, 30, 5), (31, 38, 9), (39, 48, 14), (49, 70, 18), (71, 80, 23),
+                      (81, 103, 34), (104, 105, 50)):
+        if lo <= Z <= hi:
+            return k
+    raise _ccsd.ETGError('No support for frozen core for Z > 105.')
+
+
 class Molecule:
     """A closed-shell molecule in a Gaussian basis on the device (libetgpu.so)."""
 
@@ -110,6 +121,7 @@ class Molecule:
                 exps.extend(sh['exps'])
                 coefs.extend(sh['coefs'])
         self.charges = np.array(Z)
+        self._shell_atom, self._shell_l = list(sh_atom), list(sh_l)
         self.xyz_bohr = np.array(xyz).reshape(-1, 3)
         n_el = int(round(sum(Z))) - int(charge)
         arr = lambda v, t: np.ascontiguousarray(v, dtype=t)
@@ -180,8 +192,17 @@ class Molecule:
         C, eps = _restart.OrbitalFile(path).read(n_ao=n, n_mo=n)
         self.set_orbitals(C, eps)
 
-    def run(self, cholesky_threshold=1e-8, scf_threshold=1e-10, max_scf_iterations=200):
-        """Integrals, decomposition, RHF, MO vectors; returns the CCSD inputs."""
+    def run(self, cholesky_threshold=1e-8, scf_threshold=1e-10, max_scf_iterations=200, frozen_core=False):
+        """Integrals, decomposition, RHF, MO vectors; returns the CCSD inputs.
+
+        frozen_core: eT's `frozen orbitals: core` (remove_core_orbitals_hf,
+        frozen_orbital_hf.F90:81-205): the lowest MOs (frozen_core_orbitals per atom) leave
+        the correlated space and their Coulomb and exchange go into h_mo.  The returned
+        n_o, eps, h_mo, L_mo and C then cover the active MOs only ('n_frozen_core' says how
+        many were removed; E_HF is that of all electrons).
+        """
+        n_fc = int(sum(frozen_core_orbitals(int(round(z))) for z in self.charges)) if frozen_core else 0
+        _ccsd._check(self._lib.etg_molecule_set_frozen_core(self._h, n_fc))
         _ccsd._check(self._lib.etg_molecule_run(self._h, cholesky_threshold, scf_threshold, max_scf_iterations))
         d = self.dims()
         n, nJ = d['n_ao'], d['n_J']
@@ -192,5 +213,24 @@ class Molecule:
         C = np.empty((n, n), order='F')
         _ccsd._check(self._lib.etg_molecule_results(self._h, ctypes.byref(e_nuc), ctypes.byref(e_hf), _ccsd._ptr(eps),
                                                     _ccsd._ptr(h_mo), _ccsd._ptr(L_mo), _ccsd._ptr(C)))
-        return dict(n_o=d['n_o'], n_J=nJ, e_nuc=e_nuc.value, e_hf=e_hf.value, eps=eps, h_mo=h_mo, L_mo=L_mo, C=C,
-                    scf_iterations=d['scf_iterations'])
+        if n_fc:
+            self._check_frozen_core(C, n_fc)
+            a = slice(n_fc, n)
+            eps, h_mo, C = eps[a].copy(), np.asfortranarray(h_mo[a, a]), np.asfortranarray(C[:, a])
+            L_mo = np.asfortranarray(L_mo[:, a, a])
+        return dict(n_o=d['n_o'] - n_fc, n_J=nJ, e_nuc=e_nuc.value, e_hf=e_hf.value, eps=eps, h_mo=h_mo, L_mo=L_mo,
+                    C=C, scf_iterations=d['scf_iterations'], n_frozen_core=n_fc)
+
+    def _check_frozen_core(self, C, n_fc):
+        """eT's crossover check (frozen_orbital_hf.F90:150-165): the largest AO weight of each
+        frozen MO must sit on an atom that is frozen."""
+        n = C.shape[0]
+        ao_atom = []
+        for size in (lambda l: 2 * l + 1, lambda l: (l + 1) * (l + 2) // 2):
+            ao_atom = [a for a, l in zip(self._shell_atom, self._shell_l) for _ in range(size(l))]
+            if len(ao_atom) == n:
+                break
+        frozen = [frozen_core_orbitals(int(round(z))) > 0 for z in self.charges]
+        for k in range(n_fc):
+            if not frozen[ao_atom[int(np.argmax(np.abs(C[:, k])))]]:
+                raise _ccsd.ETGError('Frozen MO has large contribution from a non-frozen atom.')

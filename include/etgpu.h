@@ -405,6 +405,15 @@ int etg_molecule_run(etg_molecule* molecule, double cholesky_threshold, double s
 int etg_molecule_set_orbitals(etg_molecule* molecule, int64_t n_ao, int64_t n_mo, const double* C,
                               const double* eps);
 
+/* Frozen core for the next etg_molecule_run: h_mo then includes the Coulomb and
+ * exchange of the n_frozen lowest MOs, F^fc_pq = sum_c 2 (pq|cc) - (pc|cq), and the
+ * caller passes the MOs above them to the CCSD handle (n_o - n_frozen occupied;
+ * E_HF is unchanged).  eT freezes one orbital per atom with 5 <= Z <= 12, five for
+ * 13..30, nine for 31..38 (ao_tool_class.F90:2425-2507).
+ * Replaces: remove_core_orbitals_hf and construct_mo_fock_fc_term_hf
+ * (src/wavefunctions/hf/frozen_orbital_hf.F90:81-205, 547-600). */
+int etg_molecule_set_frozen_core(etg_molecule* molecule, int64_t n_frozen);
+
 /* Sizes: AO (= MO) count, occupied orbitals, Cholesky vectors, RHF iterations. */
 int etg_molecule_dims(etg_molecule* molecule, int64_t* n_ao, int64_t* n_occ, int64_t* n_J,
                       int64_t* scf_iterations);

@@ -1,0 +1,67 @@
+"""Frozen core: eT's tests/ccsd_frozen_core.
+
+HOF-He / cc-pVDZ with eT's `frozen orbitals: core` (the O and F 1s MOs), Cholesky
+threshold 1e-11.  The producer folds the frozen MOs' Coulomb and exchange into h_mo and
+returns the active MOs; then:
+  * the Fock matrix of the active MOs built from that h_mo and the active occupied MOs
+    alone is diag(eps_active) (the canonical RHF Fock matrix, so the frozen-core term
+    F^fc_pq = sum_c 2 (pq|cc) - (pc|cq) is complete, with its factor and sign);
+  * the FP64 DIIS ground state follows eT's 23-row table (1e-9; eT's RHF and ours agree
+    to ~1e-10) to E_CCSD, and the two right EOM-CCSD roots equal eT's (1e-9).
+"""
+import functools
+import json
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+FIXTURE = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden', 'hof_he_frozen_core.json')
+
+
+@functools.lru_cache(maxsize=1)
+def fixture():
+    return json.load(open(FIXTURE))
+
+
+def test_frozen_core_reproduces_et():
+    from et_amd import ccsd, molecule as M, solvers
+    f = fixture()
+    g = f['golden']
+    atoms = [(s, (x, y, z)) for s, x, y, z in f['geometry_angstrom']]
+    with M.Molecule(atoms, None, f['bohr_to_angstrom']) as mol:
+        r = mol.run(cholesky_threshold=f['cholesky_threshold'], scf_threshold=1e-10, frozen_core=True)
+    print('E_nuc %.12f E_HF %.12f n_J %d (eT %d), frozen %d, n_o %d' % (
+        r['e_nuc'], r['e_hf'], r['n_J'], g['n_cholesky'], r['n_frozen_core'], r['n_o']), flush=True)
+    assert r['n_frozen_core'] == g['n_frozen_core']
+    assert r['n_o'] == g['n_o_all_electron'] - g['n_frozen_core']
+    assert r['n_J'] == g['n_cholesky']
+    assert abs(r['e_nuc'] - g['nuclear_repulsion']) < 1e-9
+    assert abs(r['e_hf'] - g['hf_energy']) < 1e-8
+    o, L, h = r['n_o'], r['L_mo'], r['h_mo']
+    n = h.shape[0]
+    assert L.shape[1:] == (n, n) and r['eps'].shape == (n,)
+    Lo = L[:, :o, :o]
+    F = h + 2 * np.einsum('Jpq,J->pq', L, np.einsum('Jii->J', Lo)) - np.einsum('Jpi,Jiq->pq', L[:, :, :o], L[:, :o, :])
+    assert np.abs(F - np.diag(r['eps'])).max() < 1e-8
+    n_v = n - o
+    with ccsd.CCSD(o, n_v, r['n_J']) as wf:
+        wf.set_cholesky(L)
+        wf.set_oei(h)
+        wf.set_orbital_energies(r['eps'])
+        wf.hf_energy = r['e_hf']
+        gs = solvers.DiisCCGroundState(wf, residual_threshold=f['residual_threshold'], max_iterations=60)
+        e = gs.run()
+        print('E_CCSD %.12f (eT %.12f), %d iterations (eT %d)' % (e, g['ccsd_energy'], len(gs.history),
+                                                                 len(g['ccsd_iterations'])))
+        assert abs(e - g['ccsd_energy']) < 1e-9
+        ref = np.array([row[1] for row in g['ccsd_iterations']])
+        ours = np.array([row[0] for row in gs.history])
+        k = min(len(ref), len(ours))
+        assert abs(len(ours) - len(ref)) <= 1
+        assert np.abs(ours[:k] - ref[:k]).max() < 1e-9
+        w = solvers.EigenDavidson(wf, f['n_states'], 'right', residual_threshold=1e-9).run()
+    print('roots', w, 'eT', g['excitation_energies'])
+    np.testing.assert_allclose(w, g['excitation_energies'], rtol=0, atol=1e-9)
