@@ -199,7 +199,7 @@ class Molecule:
         frozen_orbital_hf.F90:81-205): the lowest MOs (frozen_core_orbitals per atom) leave
         the correlated space and their Coulomb and exchange go into h_mo.  The returned
         n_o, eps, h_mo, L_mo and C then cover the active MOs only ('n_frozen_core' says how
-        many were removed; E_HF is that of all electrons).
+        many were removed, 'C_frozen' holds them; E_HF is that of all electrons).
         """
         n_fc = int(sum(frozen_core_orbitals(int(round(z))) for z in self.charges)) if frozen_core else 0
         _ccsd._check(self._lib.etg_molecule_set_frozen_core(self._h, n_fc))
@@ -213,13 +213,14 @@ class Molecule:
         C = np.empty((n, n), order='F')
         _ccsd._check(self._lib.etg_molecule_results(self._h, ctypes.byref(e_nuc), ctypes.byref(e_hf), _ccsd._ptr(eps),
                                                     _ccsd._ptr(h_mo), _ccsd._ptr(L_mo), _ccsd._ptr(C)))
+        C_frozen = np.asfortranarray(C[:, :n_fc])
         if n_fc:
             self._check_frozen_core(C, n_fc)
             a = slice(n_fc, n)
             eps, h_mo, C = eps[a].copy(), np.asfortranarray(h_mo[a, a]), np.asfortranarray(C[:, a])
             L_mo = np.asfortranarray(L_mo[:, a, a])
         return dict(n_o=d['n_o'] - n_fc, n_J=nJ, e_nuc=e_nuc.value, e_hf=e_hf.value, eps=eps, h_mo=h_mo, L_mo=L_mo,
-                    C=C, scf_iterations=d['scf_iterations'], n_frozen_core=n_fc)
+                    C=C, scf_iterations=d['scf_iterations'], n_frozen_core=n_fc, C_frozen=C_frozen)
 
     def _check_frozen_core(self, C, n_fc):
         """eT's crossover check (frozen_orbital_hf.F90:150-165): the largest AO weight of each

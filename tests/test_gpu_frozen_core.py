@@ -65,3 +65,38 @@ def test_frozen_core_reproduces_et():
         w = solvers.EigenDavidson(wf, f['n_states'], 'right', residual_threshold=1e-9).run()
     print('roots', w, 'eT', g['excitation_energies'])
     np.testing.assert_allclose(w, g['excitation_energies'], rtol=0, atol=1e-9)
+
+
+def test_frozen_core_dipole_reproduces_et():
+    """eT's tests/ccsd_dipole_fc: H2O / 6-31G, O 1s frozen, Cholesky 1e-11; ground state and
+    multipliers to 1e-11, then the CC mean-value dipole with the frozen core's density
+    (eT's table in Debye, 1e-8)."""
+    from et_amd import ccsd, molecule as M, properties, solvers
+    f = json.load(open(os.path.join(os.path.dirname(FIXTURE), 'h2o_631g_dipole_fc.json')))
+    g = f['golden']
+    atoms = [(s, (x, y, z)) for s, x, y, z in f['geometry_bohr']]
+    with M.Molecule(atoms, M.load_basis('6-31g'), 1.0) as mol:
+        r = mol.run(cholesky_threshold=f['cholesky_threshold'], scf_threshold=1e-11, frozen_core=True)
+        print('E_nuc %.12f E_HF %.12f n_J %d (eT %d)' % (r['e_nuc'], r['e_hf'], r['n_J'], g['n_cholesky']),
+              flush=True)
+        assert r['n_frozen_core'] == g['n_frozen_core']
+        assert r['n_J'] == g['n_cholesky']
+        assert abs(r['e_nuc'] - g['nuclear_repulsion']) < 1e-10
+        assert abs(r['e_hf'] - g['hf_energy']) < 1e-9
+        n_o = r['n_o']
+        with ccsd.CCSD(n_o, r['L_mo'].shape[1] - n_o, r['n_J']) as wf:
+            wf.set_cholesky(r['L_mo'])
+            wf.set_oei(r['h_mo'])
+            wf.set_orbital_energies(r['eps'])
+            wf.hf_energy = r['e_hf']
+            e = solvers.DiisCCGroundState(wf, residual_threshold=f['residual_threshold']).run()
+            print('E_CCSD %.12f (eT %.12f)' % (e, g['ccsd_energy']))
+            assert abs(e - g['ccsd_energy']) < 1e-9
+            solvers.DavidsonCCMultipliers(wf, residual_threshold=f['residual_threshold']).run()
+            mu = properties.dipole_moment(wf, mol, r['C'], C_frozen=r['C_frozen'])
+    print('dipole (D)', mu['electronic_debye'], mu['nuclear_debye'], mu['total_debye'])
+    for c, comp in enumerate('xyz'):
+        ref = g['dipole_debye'][comp]
+        assert abs(mu['nuclear_debye'][c] - ref['nuclear']) < 1e-9
+        assert abs(mu['electronic_debye'][c] - ref['electronic']) < 1e-8, comp
+        assert abs(mu['total_debye'][c] - ref['total']) < 1e-8, comp
