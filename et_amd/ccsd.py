@@ -63,6 +63,7 @@ _SIGNATURES = {
                      _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, ctypes.POINTER(_i64)],
     'etg_linear_davidson': [ctypes.c_void_p, ctypes.c_int, _c_dbl_p, _c_dbl_p, ctypes.c_double, _i64, _i64,
                             ctypes.c_int, _c_dbl_p, _c_dbl_p, ctypes.POINTER(_i64)],
+    'etg_set_cvs': [ctypes.c_void_p, _i64, ctypes.POINTER(_i64)],
     'etg_newton_raphson_update': [ctypes.c_void_p, ctypes.c_double, _i64, ctypes.c_int, _c_dbl_p,
                                   ctypes.POINTER(_i64)],
     'etg_diis_excited_states': [ctypes.c_void_p, ctypes.c_int, _i64, ctypes.c_double, ctypes.c_double, _i64,
@@ -508,6 +509,22 @@ class CCSD:
     def quasi_newton_update(self):
         """Device quasi-Newton step t += -omega/eps (quasi_newton_updater_class.F90:81-118)."""
         _check(self._lib.etg_quasi_newton_update(self._h))
+
+    def set_cvs(self, core_mos=()):
+        """Core-valence separation for the excited-state solvers: project every transformed
+        vector onto excitations out of the occupied MOs core_mos (0-based); () switches it off."""
+        core = np.ascontiguousarray(core_mos, dtype=np.int64)
+        _check(self._lib.etg_set_cvs(self._h, len(core), core.ctypes.data_as(ctypes.POINTER(_i64))))
+
+    def cvs_start_vectors(self, n_states, core_mos):
+        """eT's CVS start vectors (set_cvs_start_indices_ccs, ccs_class.F90:1393-1438): unit
+        singles (a, i) for a = 0, 1, ... and, for each a, the core MOs in the order given."""
+        n = self.n_t1 + self.n_t2
+        X = np.zeros((n, n_states), order='F')
+        for k in range(n_states):
+            a, c = divmod(k, len(core_mos))
+            X[a + self.n_v * int(core_mos[c]), k] = 1.0
+        return X
 
     def newton_raphson_update(self, rel_threshold=1e-2, max_iterations=100, precision=FP64):
         """eT's Newton-Raphson step t += u with A u = -Omega (linear Davidson); the next

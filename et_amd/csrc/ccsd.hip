@@ -914,6 +914,27 @@ void CCSD::transform_body(int side, Prec prec, int64_t nb) {
   }
 }
 
+void CCSD::set_cvs(int64_t n_core, const int64_t* core_mos) {
+  if (n_core == 0) {
+    cvs_ = false;
+    return;
+  }
+  require(n_core > 0 && core_mos != nullptr, "set_cvs: core MOs required");
+  std::vector<unsigned char> flags(size_t(n_o), 0);
+  for (int64_t c = 0; c < n_core; ++c) {
+    require(core_mos[c] >= 0 && core_mos[c] < n_o, "set_cvs: a core MO must be an occupied orbital");
+    flags[size_t(core_mos[c])] = 1;
+  }
+  cvs_core_.resize(size_t(n_o));
+  ETG_HIP(hipMemcpyAsync(cvs_core_.as<unsigned char>(), flags.data(), size_t(n_o), hipMemcpyHostToDevice, stream_));
+  synchronize();
+  cvs_ = true;
+}
+
+void CCSD::project_cvs(double* x, int64_t n_vec) {
+  if (cvs_) cvs_project(stream_, x, n_vec, n_t1, n_t1 + n_t2, n_v, cvs_core_.as<unsigned char>());
+}
+
 int64_t CCSD::newton_raphson_update(double rel_threshold, int64_t max_iterations, Prec prec,
                                     double* micro_norms) {
   require(have_omega_, "newton_raphson_update: the residual has not been constructed");

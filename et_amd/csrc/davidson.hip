@@ -503,6 +503,7 @@ int64_t CCSD::davidson(int side, int64_t n_states, double threshold, int64_t max
   for (int64_t it = 1; it <= max_iterations; ++it) {
     // transforms of the new trial vectors
     transform_batch(side, m, V + k * n, AV + k * n, prec);
+    project_cvs(AV + k * n, m);              // eigen_davidson_solver_class.F90:187
     // G = V^T AV: new columns for all rows, new rows for the old columns
     const int64_t kn = k + m;
     run_gemm(make_gemm('T', 'N', kn, m, n, 1.0, V, n, AV + k * n, n, 0.0, G + k * max_dim, max_dim),

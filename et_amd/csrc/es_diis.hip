@@ -103,6 +103,7 @@ int64_t CCSD::diis_excited_states(int side, int64_t n_states, double residual_th
       ETG_HIP(hipMemcpyAsync(R + q * n, X + active[size_t(q)] * n, sizeof(double) * n, hipMemcpyDeviceToDevice,
                              stream_));
     transform_batch(side, na, R, R, prec);       // columns are read before they are written
+    project_cvs(R, na);                          // diis_cc_es_class.F90 projector
     for (int64_t q = 0; q < na; ++q) {
       const int64_t j = active[size_t(q)];
       double* Rj = R + q * n;

@@ -68,6 +68,10 @@ void mp2_guess(hipStream_t s, const double* g_vovo, const double* eps, double* t
 // orbital differences for singles (v,o) and packed doubles
 void orbital_differences(hipStream_t s, const double* eps, double* x, int64_t n_o, int64_t n_v,
                          int64_t n);
+// core-valence separation projection of n_vec vectors of length n (eT layout):
+// elements with no core occupied index are zeroed; core: n_o flags on the device
+void cvs_project(hipStream_t s, double* x, int64_t n_vec, int64_t n_t1, int64_t n, int64_t n_v,
+                 const unsigned char* core);
 // x (+)= alpha * y
 void axpy(hipStream_t s, int64_t n, double alpha, const double* x, double* y);
 void fill(hipStream_t s, double* x, int64_t n, double value);

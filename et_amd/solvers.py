@@ -337,12 +337,14 @@ class EigenDavidson:
     matrix X^T (A X) and its non-symmetric eigenproblem, residuals preconditioned
     by the orbital differences and orthonormalised against the space.  Trial
     vectors, transforms and the reduced-space algebra stay on the device (FP64);
-    the transformations run in `precision`.
+    the transformations run in `precision`.  core_mos (0-based occupied MOs): eT's
+    `core excitation`, core-valence separation with eT's CVS start vectors.
     """
 
     def __init__(self, wf, n_states, side='right', residual_threshold=1e-6, max_iterations=100,
-                 max_dim=0, precision=_ccsd.FP64, log=None, start=None, restart=False):
+                 max_dim=0, precision=_ccsd.FP64, log=None, start=None, restart=False, core_mos=None):
         self.wf = wf
+        self.core_mos = core_mos
         self.restart = restart
         self.n_states = n_states
         self.side = side
@@ -359,8 +361,14 @@ class EigenDavidson:
         start = self.start
         if start is None and self.restart:
             start = wf.restart_start_vectors(self.n_states, self.side)[0]
-        w, hist, X = wf.davidson(self.n_states, self.side, self.residual_threshold, self.max_iterations,
-                                 self.max_dim, self.precision, vectors=True, start=start)
+        if self.core_mos is not None and start is None:
+            start = wf.cvs_start_vectors(self.n_states, self.core_mos)
+        wf.set_cvs(self.core_mos if self.core_mos is not None else ())
+        try:
+            w, hist, X = wf.davidson(self.n_states, self.side, self.residual_threshold, self.max_iterations,
+                                     self.max_dim, self.precision, vectors=True, start=start)
+        finally:
+            wf.set_cvs(())
         self.history = hist
         self.eigenvectors = X
         if self.log:
@@ -378,12 +386,14 @@ class DiisCCExcitedStates:
     a transformation pass.  Start vectors on the lowest orbital differences (or the
     restart files, `restart`); the roots found are those the start vectors lead to.
     Energies are returned sorted.  Vectors, records and the DIIS algebra stay in HBM.
+    core_mos: core-valence separation as in EigenDavidson.
     """
 
     def __init__(self, wf, n_states, side='right', residual_threshold=1e-6, energy_threshold=None,
                  max_iterations=100, diis_dimension=20, precision=_ccsd.FP64, start=None, restart=False,
-                 log=None):
+                 log=None, core_mos=None):
         self.wf = wf
+        self.core_mos = core_mos
         self.n_states = n_states
         self.side = side
         self.residual_threshold = residual_threshold
@@ -401,9 +411,15 @@ class DiisCCExcitedStates:
         start = self.start
         if start is None and self.restart:
             start = wf.restart_start_vectors(self.n_states, self.side)[0]
-        w, hist, X = wf.diis_excited_states(self.n_states, self.side, self.residual_threshold,
-                                            self.energy_threshold or 0.0, self.max_iterations,
-                                            self.diis_dimension, self.precision, start)
+        if self.core_mos is not None and start is None:
+            start = wf.cvs_start_vectors(self.n_states, self.core_mos)
+        wf.set_cvs(self.core_mos if self.core_mos is not None else ())
+        try:
+            w, hist, X = wf.diis_excited_states(self.n_states, self.side, self.residual_threshold,
+                                                self.energy_threshold or 0.0, self.max_iterations,
+                                                self.diis_dimension, self.precision, start)
+        finally:
+            wf.set_cvs(())
         self.history = hist
         self.energy_history = wf.diis_energy_history
         self.eigenvectors = X

@@ -197,6 +197,18 @@ int etg_linear_davidson(etg_handle* handle, int side, const double* rhs, const d
                         double residual_threshold, int64_t max_iterations, int64_t max_dim, int precision,
                         double* solution, double* residual_norms, int64_t* iterations);
 
+/* Core-valence separation for etg_davidson and etg_diis_excited_states (eT's
+ * `core excitation` keyword): each transformed vector is projected onto
+ * excitations with at least one occupied index among the n_core core_mos (0-based
+ * occupied MOs): singles (a, i) with i a core MO, doubles (ai, bj) with i or j one.
+ * n_core = 0 switches it off.  Start vectors are the caller's: eT starts on the
+ * unit singles (a, i) with a = 1, 2, ... and, for each a, the core MOs in order.
+ * Replaces: es_cvs_projection_tool%project (es_cvs_projection_tool_class.F90:73-85;
+ * cvs_projection_ccs, ccs_class.F90:1219-1250; cvs_projection_doubles,
+ * doubles_class.F90:287-328) in eigen_davidson_solver_class.F90:187 and
+ * diis_cc_es_class.F90:359; start vectors set_cvs_start_indices_ccs (ccs_class.F90:1393-1438). */
+int etg_set_cvs(etg_handle* handle, int64_t n_core, const int64_t* core_mos);
+
 /* eT's Newton-Raphson amplitude update: A u = -Omega by the linear Davidson
  * solver (as etg_linear_davidson, start -Omega / eps, reduced space 50) to
  * ||A u + Omega|| < rel_threshold ||Omega||, the packed doubles diagonal of u
