@@ -64,6 +64,7 @@ _SIGNATURES = {
     'etg_linear_davidson': [ctypes.c_void_p, ctypes.c_int, _c_dbl_p, _c_dbl_p, ctypes.c_double, _i64, _i64,
                             ctypes.c_int, _c_dbl_p, _c_dbl_p, ctypes.POINTER(_i64)],
     'etg_set_cvs': [ctypes.c_void_p, _i64, ctypes.POINTER(_i64)],
+    'etg_set_remove_core': [ctypes.c_void_p, _i64, ctypes.POINTER(_i64)],
     'etg_newton_raphson_update': [ctypes.c_void_p, ctypes.c_double, _i64, ctypes.c_int, _c_dbl_p,
                                   ctypes.POINTER(_i64)],
     'etg_diis_excited_states': [ctypes.c_void_p, ctypes.c_int, _i64, ctypes.c_double, ctypes.c_double, _i64,
@@ -515,6 +516,12 @@ class CCSD:
         vector onto excitations out of the occupied MOs core_mos (0-based); () switches it off."""
         core = np.ascontiguousarray(core_mos, dtype=np.int64)
         _check(self._lib.etg_set_cvs(self._h, len(core), core.ctypes.data_as(ctypes.POINTER(_i64))))
+
+    def set_remove_core(self, core_mos=()):
+        """eT's `remove core`: project every transformed vector of the excited-state solvers
+        onto excitations with no occupied index among core_mos (0-based); () switches it off."""
+        core = np.ascontiguousarray(core_mos, dtype=np.int64)
+        _check(self._lib.etg_set_remove_core(self._h, len(core), core.ctypes.data_as(ctypes.POINTER(_i64))))
 
     def cvs_start_vectors(self, n_states, core_mos):
         """eT's CVS start vectors (set_cvs_start_indices_ccs, ccs_class.F90:1393-1438): unit

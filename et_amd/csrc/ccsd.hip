@@ -914,7 +914,7 @@ void CCSD::transform_body(int side, Prec prec, int64_t nb) {
   }
 }
 
-void CCSD::set_cvs(int64_t n_core, const int64_t* core_mos) {
+void CCSD::set_cvs(int64_t n_core, const int64_t* core_mos, bool remove_core) {
   if (n_core == 0) {
     cvs_ = false;
     return;
@@ -929,10 +929,11 @@ void CCSD::set_cvs(int64_t n_core, const int64_t* core_mos) {
   ETG_HIP(hipMemcpyAsync(cvs_core_.as<unsigned char>(), flags.data(), size_t(n_o), hipMemcpyHostToDevice, stream_));
   synchronize();
   cvs_ = true;
+  remove_core_ = remove_core;
 }
 
 void CCSD::project_cvs(double* x, int64_t n_vec) {
-  if (cvs_) cvs_project(stream_, x, n_vec, n_t1, n_t1 + n_t2, n_v, cvs_core_.as<unsigned char>());
+  if (cvs_) cvs_project(stream_, x, n_vec, n_t1, n_t1 + n_t2, n_v, cvs_core_.as<unsigned char>(), remove_core_ ? 1 : 0);
 }
 
 int64_t CCSD::newton_raphson_update(double rel_threshold, int64_t max_iterations, Prec prec,

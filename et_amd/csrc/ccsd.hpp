@@ -92,8 +92,10 @@ class CCSD {
   void set_transform_batch(int64_t nb);
   // core-valence separation for the excited-state solvers (eT's `core excitation`):
   // the Davidson and DIIS excited-state solvers project every transformed vector onto
-  // excitations out of these occupied MOs (0-based); n_core = 0 switches it off
-  void set_cvs(int64_t n_core, const int64_t* core_mos);
+  // excitations out of these occupied MOs (0-based); n_core = 0 switches it off.
+  // remove_core: the opposite projection (eT's `remove core`), onto excitations with
+  // no occupied index among them
+  void set_cvs(int64_t n_core, const int64_t* core_mos, bool remove_core = false);
   void project_cvs(double* x, int64_t n_vec);
   int64_t transform_batch() const { return nb_cap_; }
   // bytes of the record's intermediates, and of one trial vector's derivatives
@@ -224,7 +226,7 @@ class CCSD {
   // Davidson space (V, AV), Ritz block, reduced matrices
   DeviceBuffer dav_V_, dav_AV_, dav_Y_, dav_small_, red_eig_;
   DeviceBuffer cvs_core_;                         // n_o flags: core occupied MOs
-  bool cvs_ = false;
+  bool cvs_ = false, remove_core_ = false;
 
   // multipliers: t-bar, eta, the multiplier residual and its update vector
   DeviceBuffer tbar_, eta_, mres_, mupdate_;

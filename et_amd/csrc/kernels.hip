@@ -275,12 +275,14 @@ __global__ void mp2_guess_kernel(const double* __restrict__ g, const double* __r
   }
 }
 
-// Core-valence separation (cvs_projection_ccs, ccs_class.F90:1219-1250, and
+// Core projections; core[i] != 0 marks core occupied MOs; blockIdx.y is the vector.
+// remove_core = 0, core-valence separation (cvs_projection_ccs, ccs_class.F90:1219-1250,
 // cvs_projection_doubles, doubles_class.F90:287-328): zero every singles element whose
-// occupied index is not a core MO and every packed doubles element (ai, bj) with
-// neither i nor j a core MO.  core[i] != 0 marks core occupied MOs; blockIdx.y is the vector.
+// occupied index is not a core MO and every packed doubles element (ai, bj) with neither
+// i nor j a core MO.  remove_core = 1 (rm_core_projection_ccs, ccs_class.F90:1251-1280,
+// rm_core_projection_doubles, doubles_class.F90:330-370): zero exactly the others.
 __global__ void cvs_project_kernel(double* __restrict__ x, int64_t n_t1, int64_t n, int64_t n_v,
-                                   const unsigned char* __restrict__ core) {
+                                   const unsigned char* __restrict__ core, int remove_core) {
   double* xv = x + int64_t(blockIdx.y) * n;
   for (int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; k < n;
        k += int64_t(gridDim.x) * blockDim.x) {
@@ -292,7 +294,7 @@ __global__ void cvs_project_kernel(double* __restrict__ x, int64_t n_t1, int64_t
       const int64_t X = tri_root(q), Y = q - X * (X + 1) / 2;
       keep = core[X / n_v] != 0 || core[Y / n_v] != 0;
     }
-    if (!keep) xv[k] = 0.0;
+    if (keep == (remove_core != 0)) xv[k] = 0.0;
   }
 }
 
@@ -454,10 +456,10 @@ void orbital_differences(hipStream_t s, const double* eps, double* x, int64_t n_
 }
 
 void cvs_project(hipStream_t s, double* x, int64_t n_vec, int64_t n_t1, int64_t n, int64_t n_v,
-                 const unsigned char* core) {
+                 const unsigned char* core, int remove_core) {
   for (int64_t v0 = 0; v0 < n_vec; v0 += 65535) {
     const dim3 g(grid_for(n, kBlock, 1 << 14), unsigned(std::min<int64_t>(n_vec - v0, 65535)));
-    cvs_project_kernel<<<g, kBlock, 0, s>>>(x + v0 * n, n_t1, n, n_v, core);
+    cvs_project_kernel<<<g, kBlock, 0, s>>>(x + v0 * n, n_t1, n, n_v, core, remove_core);
     ETG_LAUNCH("cvs_project");
   }
 }

@@ -154,6 +154,13 @@ module etgpu_interface
          type(c_ptr), value :: core_mos
       end function etg_set_cvs
 !
+      integer(c_int) function etg_set_remove_core(handle, n_core, core_mos) bind(C, name='etg_set_remove_core')
+         import :: c_int, c_ptr, c_int64_t
+         type(c_ptr), value :: handle
+         integer(c_int64_t), value :: n_core
+         type(c_ptr), value :: core_mos
+      end function etg_set_remove_core
+!
       integer(c_int) function etg_newton_raphson_update(handle, rel_threshold, max_iterations, precision, &
                                                         micro_norms, iterations) &
                                                         bind(C, name='etg_newton_raphson_update')

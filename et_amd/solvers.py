@@ -338,13 +338,16 @@ class EigenDavidson:
     by the orbital differences and orthonormalised against the space.  Trial
     vectors, transforms and the reduced-space algebra stay on the device (FP64);
     the transformations run in `precision`.  core_mos (0-based occupied MOs): eT's
-    `core excitation`, core-valence separation with eT's CVS start vectors.
+    `core excitation`, core-valence separation with eT's CVS start vectors; remove_core:
+    eT's `remove core`, the excitations out of those MOs projected away.
     """
 
     def __init__(self, wf, n_states, side='right', residual_threshold=1e-6, max_iterations=100,
-                 max_dim=0, precision=_ccsd.FP64, log=None, start=None, restart=False, core_mos=None):
+                 max_dim=0, precision=_ccsd.FP64, log=None, start=None, restart=False, core_mos=None,
+                 remove_core=None):
         self.wf = wf
         self.core_mos = core_mos
+        self.remove_core = remove_core
         self.restart = restart
         self.n_states = n_states
         self.side = side
@@ -364,6 +367,8 @@ class EigenDavidson:
         if self.core_mos is not None and start is None:
             start = wf.cvs_start_vectors(self.n_states, self.core_mos)
         wf.set_cvs(self.core_mos if self.core_mos is not None else ())
+        if self.remove_core is not None:
+            wf.set_remove_core(self.remove_core)
         try:
             w, hist, X = wf.davidson(self.n_states, self.side, self.residual_threshold, self.max_iterations,
                                      self.max_dim, self.precision, vectors=True, start=start)
@@ -386,14 +391,15 @@ class DiisCCExcitedStates:
     a transformation pass.  Start vectors on the lowest orbital differences (or the
     restart files, `restart`); the roots found are those the start vectors lead to.
     Energies are returned sorted.  Vectors, records and the DIIS algebra stay in HBM.
-    core_mos: core-valence separation as in EigenDavidson.
+    core_mos, remove_core: core projections as in EigenDavidson.
     """
 
     def __init__(self, wf, n_states, side='right', residual_threshold=1e-6, energy_threshold=None,
                  max_iterations=100, diis_dimension=20, precision=_ccsd.FP64, start=None, restart=False,
-                 log=None, core_mos=None):
+                 log=None, core_mos=None, remove_core=None):
         self.wf = wf
         self.core_mos = core_mos
+        self.remove_core = remove_core
         self.n_states = n_states
         self.side = side
         self.residual_threshold = residual_threshold
@@ -414,6 +420,8 @@ class DiisCCExcitedStates:
         if self.core_mos is not None and start is None:
             start = wf.cvs_start_vectors(self.n_states, self.core_mos)
         wf.set_cvs(self.core_mos if self.core_mos is not None else ())
+        if self.remove_core is not None:
+            wf.set_remove_core(self.remove_core)
         try:
             w, hist, X = wf.diis_excited_states(self.n_states, self.side, self.residual_threshold,
                                                 self.energy_threshold or 0.0, self.max_iterations,

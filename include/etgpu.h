@@ -209,6 +209,14 @@ int etg_linear_davidson(etg_handle* handle, int side, const double* rhs, const d
  * diis_cc_es_class.F90:359; start vectors set_cvs_start_indices_ccs (ccs_class.F90:1393-1438). */
 int etg_set_cvs(etg_handle* handle, int64_t n_core, const int64_t* core_mos);
 
+/* The opposite projection (eT's `remove core`): onto excitations with no occupied
+ * index among the core_mos, for etg_davidson and etg_diis_excited_states (start
+ * vectors: eT's usual ones, the lowest orbital differences).  n_core = 0 switches
+ * it off, as does etg_set_cvs(handle, 0, NULL).
+ * Replaces: es_rm_core_projection_tool (rm_core_projection_ccs, ccs_class.F90:1251-1280;
+ * rm_core_projection_doubles, doubles_class.F90:330-370; cc_es_projector_factory_class.F90:100-102). */
+int etg_set_remove_core(etg_handle* handle, int64_t n_core, const int64_t* core_mos);
+
 /* eT's Newton-Raphson amplitude update: A u = -Omega by the linear Davidson
  * solver (as etg_linear_davidson, start -Omega / eps, reduced space 50) to
  * ||A u + Omega|| < rel_threshold ||Omega||, the packed doubles diagonal of u

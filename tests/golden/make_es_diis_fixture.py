@@ -4,7 +4,8 @@ tests/ccsd_left_es_diis (H2O-He / cc-pVDZ, the system of h2o_he_ccpvdz.json).
 
 Both run eT's DIIS excited-state solver (diis_cc_es_class.F90) for two states at residual
 and energy thresholds 1e-10.  Read as text from result/*.out: every iteration's table
-(root, energy, residual norm), the number of iterations and the final excitation energies.
+(root, energy, residual norm), the number of iterations and the final excitation energies; likewise from tests/ccsd_es_remove_core (eigen
+Davidson with `remove core: {1}`, key right_remove_core_davidson).
 The same for tests/ccsd_right_cvs_es_energies_diis (key right_cvs: core-valence separation,
 `core excitation: {1}`, the oxygen 1s), and from tests/ccsd_right_cvs_es_energies (the
 eigen Davidson solver with the same CVS settings, key right_cvs_davidson) the number of
@@ -48,12 +49,14 @@ def main():
         k, final = final_energies(lines)
         out[side] = {'iterations': n_it, 'table': rows, 'excitation_energies': final,
                      'citation': '%s:%d-%d' % (rel, start, k + 4)}
-    rel = 'tests/ccsd_right_cvs_es_energies/result/ccsd_right_cvs_es_energies.out'
-    lines = open(os.path.join(REF, rel)).read().splitlines()
-    k, final = final_energies(lines)
-    conv = [ln for ln in lines if 'Convergence criterion met' in ln][-1]     # the ground state's comes first
-    out['right_cvs_davidson'] = {'iterations': int(re.search(r'in (\d+) iterations', conv).group(1)),
-                                 'excitation_energies': final, 'citation': '%s:%d-%d' % (rel, k + 3, k + 4)}
+    for key, test in (('right_cvs_davidson', 'ccsd_right_cvs_es_energies'),
+                      ('right_remove_core_davidson', 'ccsd_es_remove_core')):
+        rel = 'tests/%s/result/%s.out' % (test, test)
+        lines = open(os.path.join(REF, rel)).read().splitlines()
+        k, final = final_energies(lines)
+        conv = [ln for ln in lines if 'Convergence criterion met' in ln][-1]     # the ground state's comes first
+        out[key] = {'iterations': int(re.search(r'in (\d+) iterations', conv).group(1)),
+                    'excitation_energies': final, 'citation': '%s:%d-%d' % (rel, k + 3, k + 4)}
     json.dump(out, open(OUT, 'w'), indent=0)
     for side, v in out.items():
         print(side, v['iterations'], v['excitation_energies'], len(v.get('table', [])), v.get('table', [])[:2])

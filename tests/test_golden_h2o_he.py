@@ -316,3 +316,33 @@ def test_gpu_cvs_excited_states_reproduce_et():
     outside = (occ_of(Xi) != 0) & (occ_of(Yi) != 0)
     assert np.abs(X[n_t1:][outside]).max() == 0.0
     assert np.abs(X[n_t1:][~outside]).max() > 0.0
+
+
+@pytest.mark.gpu
+def test_gpu_remove_core_reproduces_et():
+    """eT's `remove core: {1}` (tests/ccsd_es_remove_core): the eigen Davidson solver with every
+    transformed vector projected away from excitations out of the oxygen 1s, from the usual
+    start vectors: eT's iteration count and roots (1e-10), no eigenvector element with the
+    core MO among its occupied indices."""
+    from et_amd import ccsd, solvers
+    ref = json.load(open(os.path.join(os.path.dirname(FIXTURE), 'h2o_he_es_diis.json')))['right_remove_core_davidson']
+    m = inputs()
+    n_o = m['n_o']
+    n_v = m['L_mo'].shape[1] - n_o
+    with ccsd.CCSD(n_o, n_v, m['n_J']) as wf:
+        wf.set_cholesky(m['L_mo'])
+        wf.set_oei(m['h_mo'])
+        wf.set_orbital_energies(m['eps'])
+        solvers.DiisCCGroundState(wf, residual_threshold=1e-11).run()
+        dav = solvers.EigenDavidson(wf, 2, 'right', residual_threshold=1e-10, remove_core=[0])
+        w = dav.run()
+        X = dav.eigenvectors
+    print('remove-core Davidson iterations', len(dav.history), 'eT', ref['iterations'], w)
+    np.testing.assert_allclose(w, ref['excitation_energies'], rtol=0, atol=1e-10)
+    assert len(dav.history) == ref['iterations']
+    n_t1 = n_o * n_v
+    assert np.abs(X[:n_t1][np.arange(n_t1) // n_v == 0]).max() == 0.0
+    q = np.arange(X.shape[0] - n_t1)
+    Xi = ((np.sqrt(8.0 * q + 1) - 1) // 2).astype(np.int64)
+    Yi = q - Xi * (Xi + 1) // 2
+    assert np.abs(X[n_t1:][(Xi // n_v == 0) | (Yi // n_v == 0)]).max() == 0.0
