@@ -65,6 +65,7 @@ _SIGNATURES = {
                             ctypes.c_int, _c_dbl_p, _c_dbl_p, ctypes.POINTER(_i64)],
     'etg_set_cvs': [ctypes.c_void_p, _i64, ctypes.POINTER(_i64)],
     'etg_set_remove_core': [ctypes.c_void_p, _i64, ctypes.POINTER(_i64)],
+    'etg_set_ionization': [ctypes.c_void_p, ctypes.c_int],
     'etg_newton_raphson_update': [ctypes.c_void_p, ctypes.c_double, _i64, ctypes.c_int, _c_dbl_p,
                                   ctypes.POINTER(_i64)],
     'etg_diis_excited_states': [ctypes.c_void_p, ctypes.c_int, _i64, ctypes.c_double, ctypes.c_double, _i64,
@@ -522,6 +523,20 @@ class CCSD:
         onto excitations with no occupied index among core_mos (0-based); () switches it off."""
         core = np.ascontiguousarray(core_mos, dtype=np.int64)
         _check(self._lib.etg_set_remove_core(self._h, len(core), core.ctypes.data_as(ctypes.POINTER(_i64))))
+
+    def set_ionization(self, on=True):
+        """eT's `ionization`: the excited-state solvers keep only excitations into the last
+        virtual, the bath orbital (et_amd.molecule.with_bath_orbital appends it)."""
+        _check(self._lib.etg_set_ionization(self._h, 1 if on else 0))
+
+    def ip_start_vectors(self, n_states):
+        """eT's ionisation start vectors (set_ip_start_indices_ccs, ccs_class.F90:1585-1608):
+        unit singles (bath, i) for i from the highest occupied down."""
+        n = self.n_t1 + self.n_t2
+        X = np.zeros((n, n_states), order='F')
+        for k in range(n_states):
+            X[(self.n_o - 1 - k) * self.n_v + self.n_v - 1, k] = 1.0
+        return X
 
     def cvs_start_vectors(self, n_states, core_mos):
         """eT's CVS start vectors (set_cvs_start_indices_ccs, ccs_class.F90:1393-1438): unit

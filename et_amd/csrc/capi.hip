@@ -218,6 +218,10 @@ int etg_set_remove_core(etg_handle* h, int64_t n_core, const int64_t* core_mos) 
   return guarded([&] { wf_of(h).set_cvs(n_core, core_mos, true); });
 }
 
+int etg_set_ionization(etg_handle* h, int on) {
+  return guarded([&] { wf_of(h).set_ionization(on != 0); });
+}
+
 int etg_newton_raphson_update(etg_handle* h, double rel_threshold, int64_t max_iterations, int precision,
                               double* micro_norms, int64_t* iterations) {
   return guarded([&] {

@@ -919,6 +919,7 @@ void CCSD::set_cvs(int64_t n_core, const int64_t* core_mos, bool remove_core) {
     cvs_ = false;
     return;
   }
+  ionization_ = false;
   require(n_core > 0 && core_mos != nullptr, "set_cvs: core MOs required");
   std::vector<unsigned char> flags(size_t(n_o), 0);
   for (int64_t c = 0; c < n_core; ++c) {
@@ -932,8 +933,16 @@ void CCSD::set_cvs(int64_t n_core, const int64_t* core_mos, bool remove_core) {
   remove_core_ = remove_core;
 }
 
+void CCSD::set_ionization(bool on) {
+  ionization_ = on;
+  if (on) cvs_ = false;
+}
+
 void CCSD::project_cvs(double* x, int64_t n_vec) {
-  if (cvs_) cvs_project(stream_, x, n_vec, n_t1, n_t1 + n_t2, n_v, cvs_core_.as<unsigned char>(), remove_core_ ? 1 : 0);
+  if (ionization_)
+    cvs_project(stream_, x, n_vec, n_t1, n_t1 + n_t2, n_v, nullptr, 2);
+  else if (cvs_)
+    cvs_project(stream_, x, n_vec, n_t1, n_t1 + n_t2, n_v, cvs_core_.as<unsigned char>(), remove_core_ ? 1 : 0);
 }
 
 int64_t CCSD::newton_raphson_update(double rel_threshold, int64_t max_iterations, Prec prec,

@@ -96,6 +96,9 @@ class CCSD {
   // remove_core: the opposite projection (eT's `remove core`), onto excitations with
   // no occupied index among them
   void set_cvs(int64_t n_core, const int64_t* core_mos, bool remove_core = false);
+  // ionisation (eT's `ionization` with a bath orbital, the last virtual): the
+  // excited-state solvers keep only excitations into the bath orbital
+  void set_ionization(bool on);
   void project_cvs(double* x, int64_t n_vec);
   int64_t transform_batch() const { return nb_cap_; }
   // bytes of the record's intermediates, and of one trial vector's derivatives
@@ -226,7 +229,7 @@ class CCSD {
   // Davidson space (V, AV), Ritz block, reduced matrices
   DeviceBuffer dav_V_, dav_AV_, dav_Y_, dav_small_, red_eig_;
   DeviceBuffer cvs_core_;                         // n_o flags: core occupied MOs
-  bool cvs_ = false, remove_core_ = false;
+  bool cvs_ = false, remove_core_ = false, ionization_ = false;
 
   // multipliers: t-bar, eta, the multiplier residual and its update vector
   DeviceBuffer tbar_, eta_, mres_, mupdate_;

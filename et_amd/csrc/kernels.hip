@@ -281,20 +281,23 @@ __global__ void mp2_guess_kernel(const double* __restrict__ g, const double* __r
 // occupied index is not a core MO and every packed doubles element (ai, bj) with neither
 // i nor j a core MO.  remove_core = 1 (rm_core_projection_ccs, ccs_class.F90:1251-1280,
 // rm_core_projection_doubles, doubles_class.F90:330-370): zero exactly the others.
+// mode 2, ionisation (ip_projection_ccs, ccs_class.F90:1611-1640, ip_projection_doubles,
+// doubles_class.F90:376-418): keep only singles into the last virtual (the bath orbital)
+// and doubles with a or b the bath orbital; core is not read.
 __global__ void cvs_project_kernel(double* __restrict__ x, int64_t n_t1, int64_t n, int64_t n_v,
-                                   const unsigned char* __restrict__ core, int remove_core) {
+                                   const unsigned char* __restrict__ core, int mode) {
   double* xv = x + int64_t(blockIdx.y) * n;
   for (int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; k < n;
        k += int64_t(gridDim.x) * blockDim.x) {
-    bool keep;
+    bool hit;                                // an occupied core index (modes 0, 1) or the bath (2)
     if (k < n_t1) {
-      keep = core[k / n_v] != 0;
+      hit = mode == 2 ? k % n_v == n_v - 1 : core[k / n_v] != 0;
     } else {
       const int64_t q = k - n_t1;
       const int64_t X = tri_root(q), Y = q - X * (X + 1) / 2;
-      keep = core[X / n_v] != 0 || core[Y / n_v] != 0;
+      hit = mode == 2 ? (X % n_v == n_v - 1 || Y % n_v == n_v - 1) : (core[X / n_v] != 0 || core[Y / n_v] != 0);
     }
-    if (keep == (remove_core != 0)) xv[k] = 0.0;
+    if (hit == (mode == 1)) xv[k] = 0.0;
   }
 }
 
@@ -456,10 +459,10 @@ void orbital_differences(hipStream_t s, const double* eps, double* x, int64_t n_
 }
 
 void cvs_project(hipStream_t s, double* x, int64_t n_vec, int64_t n_t1, int64_t n, int64_t n_v,
-                 const unsigned char* core, int remove_core) {
+                 const unsigned char* core, int mode) {
   for (int64_t v0 = 0; v0 < n_vec; v0 += 65535) {
     const dim3 g(grid_for(n, kBlock, 1 << 14), unsigned(std::min<int64_t>(n_vec - v0, 65535)));
-    cvs_project_kernel<<<g, kBlock, 0, s>>>(x + v0 * n, n_t1, n, n_v, core, remove_core);
+    cvs_project_kernel<<<g, kBlock, 0, s>>>(x + v0 * n, n_t1, n, n_v, core, mode);
     ETG_LAUNCH("cvs_project");
   }
 }

@@ -68,11 +68,13 @@ void mp2_guess(hipStream_t s, const double* g_vovo, const double* eps, double* t
 // orbital differences for singles (v,o) and packed doubles
 void orbital_differences(hipStream_t s, const double* eps, double* x, int64_t n_o, int64_t n_v,
                          int64_t n);
-// core projections of n_vec vectors of length n (eT layout); core: n_o flags on the
-// device.  remove_core = 0 (core-valence separation): elements with no core occupied
-// index are zeroed; remove_core = 1: elements with a core occupied index are zeroed
+// excited-state projections of n_vec vectors of length n (eT layout); core: n_o flags
+// on the device.  mode 0 (core-valence separation): elements with no core occupied
+// index are zeroed; mode 1 (remove core): elements with a core occupied index are
+// zeroed; mode 2 (ionisation): elements with no index on the last virtual (the bath
+// orbital) are zeroed, core unused
 void cvs_project(hipStream_t s, double* x, int64_t n_vec, int64_t n_t1, int64_t n, int64_t n_v,
-                 const unsigned char* core, int remove_core);
+                 const unsigned char* core, int mode);
 // x (+)= alpha * y
 void axpy(hipStream_t s, int64_t n, double alpha, const double* x, double* y);
 void fill(hipStream_t s, double* x, int64_t n, double value);

@@ -161,6 +161,12 @@ module etgpu_interface
          type(c_ptr), value :: core_mos
       end function etg_set_remove_core
 !
+      integer(c_int) function etg_set_ionization(handle, on) bind(C, name='etg_set_ionization')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: handle
+         integer(c_int), value :: on
+      end function etg_set_ionization
+!
       integer(c_int) function etg_newton_raphson_update(handle, rel_threshold, max_iterations, precision, &
                                                         micro_norms, iterations) &
                                                         bind(C, name='etg_newton_raphson_update')

@@ -93,6 +93,23 @@ def water_lattice(shape=(2, 2, 2), spacing=2.9, seed=0, min_distance=1.5, max_dr
     return atoms
 
 
+def with_bath_orbital(r):
+    """The CCSD inputs of Molecule.run with eT's bath orbital appended as the last virtual
+    (make_bath_orbital_ccs, ccs_class.F90:1494-1582): zero MO coefficients, so zero
+    Cholesky vectors and h, and orbital energy 0.  For ionisation energies."""
+    L, h, eps, C = r['L_mo'], r['h_mo'], r['eps'], r['C']
+    n = h.shape[0]
+    out = dict(r)
+    out['L_mo'] = np.zeros((L.shape[0], n + 1, n + 1), order='F')
+    out['L_mo'][:, :n, :n] = L
+    out['h_mo'] = np.zeros((n + 1, n + 1), order='F')
+    out['h_mo'][:n, :n] = h
+    out['eps'] = np.append(eps, 0.0)
+    out['C'] = np.zeros((C.shape[0], n + 1), order='F')
+    out['C'][:, :n] = C
+    return out
+
+
 def frozen_core_orbitals(Z):
     """eT's frozen-core count per atom (get_frozen_cores_and_n_frozen_orbitals,
     ao_tool_class.F90:2425-2507): 1 for 5 <= Z <= 12, 5 for 13..30, 9 for 31..38, 14 for

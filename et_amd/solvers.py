@@ -339,13 +339,16 @@ class EigenDavidson:
     vectors, transforms and the reduced-space algebra stay on the device (FP64);
     the transformations run in `precision`.  core_mos (0-based occupied MOs): eT's
     `core excitation`, core-valence separation with eT's CVS start vectors; remove_core:
-    eT's `remove core`, the excitations out of those MOs projected away.
+    eT's `remove core`, the excitations out of those MOs projected away; ionization: eT's
+    `ionization` (the last virtual is the bath orbital, et_amd.molecule.with_bath_orbital),
+    with eT's start vectors.
     """
 
     def __init__(self, wf, n_states, side='right', residual_threshold=1e-6, max_iterations=100,
                  max_dim=0, precision=_ccsd.FP64, log=None, start=None, restart=False, core_mos=None,
-                 remove_core=None):
+                 remove_core=None, ionization=False):
         self.wf = wf
+        self.ionization = ionization
         self.core_mos = core_mos
         self.remove_core = remove_core
         self.restart = restart
@@ -369,11 +372,16 @@ class EigenDavidson:
         wf.set_cvs(self.core_mos if self.core_mos is not None else ())
         if self.remove_core is not None:
             wf.set_remove_core(self.remove_core)
+        if self.ionization:
+            if start is None:
+                start = wf.ip_start_vectors(self.n_states)
+            wf.set_ionization(True)
         try:
             w, hist, X = wf.davidson(self.n_states, self.side, self.residual_threshold, self.max_iterations,
                                      self.max_dim, self.precision, vectors=True, start=start)
         finally:
             wf.set_cvs(())
+            wf.set_ionization(False)
         self.history = hist
         self.eigenvectors = X
         if self.log:
@@ -396,8 +404,9 @@ class DiisCCExcitedStates:
 
     def __init__(self, wf, n_states, side='right', residual_threshold=1e-6, energy_threshold=None,
                  max_iterations=100, diis_dimension=20, precision=_ccsd.FP64, start=None, restart=False,
-                 log=None, core_mos=None, remove_core=None):
+                 log=None, core_mos=None, remove_core=None, ionization=False):
         self.wf = wf
+        self.ionization = ionization
         self.core_mos = core_mos
         self.remove_core = remove_core
         self.n_states = n_states
@@ -422,12 +431,17 @@ class DiisCCExcitedStates:
         wf.set_cvs(self.core_mos if self.core_mos is not None else ())
         if self.remove_core is not None:
             wf.set_remove_core(self.remove_core)
+        if self.ionization:
+            if start is None:
+                start = wf.ip_start_vectors(self.n_states)
+            wf.set_ionization(True)
         try:
             w, hist, X = wf.diis_excited_states(self.n_states, self.side, self.residual_threshold,
                                                 self.energy_threshold or 0.0, self.max_iterations,
                                                 self.diis_dimension, self.precision, start)
         finally:
             wf.set_cvs(())
+            wf.set_ionization(False)
         self.history = hist
         self.energy_history = wf.diis_energy_history
         self.eigenvectors = X

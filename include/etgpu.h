@@ -217,6 +217,18 @@ int etg_set_cvs(etg_handle* handle, int64_t n_core, const int64_t* core_mos);
  * rm_core_projection_doubles, doubles_class.F90:330-370; cc_es_projector_factory_class.F90:100-102). */
 int etg_set_remove_core(etg_handle* handle, int64_t n_core, const int64_t* core_mos);
 
+/* Ionisation (eT's `ionization` with `bath orbital`): the caller appends one virtual
+ * orbital with zero Cholesky vectors, zero h and orbital energy 0 (eT's
+ * make_bath_orbital_ccs, ccs_class.F90:1494-1582); with on != 0 etg_davidson and
+ * etg_diis_excited_states keep only excitations into it (singles (bath, i),
+ * doubles with a or b the bath), and their roots are ionisation energies.  eT
+ * starts on the singles (bath, i) from the highest occupied down
+ * (set_ip_start_indices_ccs, ccs_class.F90:1585-1608).  Switches off the core
+ * projections; etg_set_cvs / etg_set_remove_core switch it off.
+ * Replaces: es_ip_projection_tool (ip_projection_ccs, ccs_class.F90:1611-1640;
+ * ip_projection_doubles, doubles_class.F90:376-418). */
+int etg_set_ionization(etg_handle* handle, int on);
+
 /* eT's Newton-Raphson amplitude update: A u = -Omega by the linear Davidson
  * solver (as etg_linear_davidson, start -Omega / eps, reduced space 50) to
  * ||A u + Omega|| < rel_threshold ||Omega||, the packed doubles diagonal of u
