@@ -102,6 +102,7 @@ _SIGNATURES = {
     'etg_molecule_run': [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, _i64],
     'etg_molecule_set_orbitals': [ctypes.c_void_p, _i64, _i64, _c_dbl_p, _c_dbl_p],
     'etg_molecule_set_frozen_core': [ctypes.c_void_p, _i64],
+    'etg_molecule_set_cartesian': [ctypes.c_void_p, ctypes.c_int],
     'etg_molecule_dims': [ctypes.c_void_p, ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_i64),
                           ctypes.POINTER(_i64)],
     'etg_molecule_results': [ctypes.c_void_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p],

@@ -465,6 +465,10 @@ int etg_molecule_set_orbitals(etg_molecule* molecule, int64_t n_ao, int64_t n_mo
   });
 }
 
+int etg_molecule_set_cartesian(etg_molecule* molecule, int cartesian) {
+  return guarded([&] { mol_of(molecule).set_cartesian(cartesian != 0); });
+}
+
 int etg_molecule_set_frozen_core(etg_molecule* molecule, int64_t n_frozen) {
   return guarded([&] { mol_of(molecule).set_frozen_core(n_frozen); });
 }

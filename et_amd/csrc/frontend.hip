@@ -78,6 +78,13 @@ __host__ __device__ inline double sph_coef(int l, int i, int c) {
   }
 }
 
+// A shell's functions and its Cartesian -> basis coefficients: spherical d (above) or,
+// for a Cartesian shell, the individually normalised Cartesian components themselves.
+__host__ __device__ inline int nfun(int l, bool cartesian) { return cartesian ? ncart(l) : nsph(l); }
+__host__ __device__ inline double tcoef(int l, bool cartesian, int i, int c) {
+  return cartesian ? (i == c ? 1.0 : 0.0) : sph_coef(l, i, c);
+}
+
 // E^{ij}_t for i <= la, j <= lb, t <= i + j:  E[(i (lb+1) + j) (la+lb+1) + t]
 __host__ __device__ inline void hermite_e(int la, int lb, double a, double b, double Ax, double Bx,
                                           double* E) {
@@ -170,6 +177,7 @@ void hermite_r_host(int L, double alpha, double X, double Y, double Z, double* R
 struct DevShell {
   double x, y, z;
   int l, nprim, prim0, f0;
+  bool cart;
 };
 
 double dfact(int n) {
@@ -272,14 +280,14 @@ void one_electron(const std::vector<HostShell>& sh, const std::vector<double>& Z
           }
         }
       // Cartesian -> spherical: out(i, j) = sum_ab T_a(i, a) x(a, b) T_b(j, b), T = sph * comp_scale
-      for (int i = 0; i < nsph(la); ++i)
-        for (int j = 0; j < nsph(lb); ++j) {
+      for (int i = 0; i < nfun(la, A.cart); ++i)
+        for (int j = 0; j < nfun(lb, B.cart); ++j) {
           double s = 0.0, t = 0.0, v = 0.0;
           for (int ca = 0; ca < na; ++ca) {
-            const double ta = sph_coef(la, i, ca) * comp_scale(la, ca);
+            const double ta = tcoef(la, A.cart, i, ca) * comp_scale(la, ca);
             if (ta == 0.0) continue;
             for (int cb = 0; cb < nb; ++cb) {
-              const double tbv = sph_coef(lb, j, cb) * comp_scale(lb, cb);
+              const double tbv = tcoef(lb, B.cart, j, cb) * comp_scale(lb, cb);
               if (tbv == 0.0) continue;
               s += ta * tbv * sb[ca][cb];
               t += ta * tbv * tb[ca][cb];
@@ -326,13 +334,13 @@ void dipole(const std::vector<HostShell>& sh, const double* O, int64_t n, double
             }
         }
       for (int c = 0; c < 3; ++c)
-        for (int i = 0; i < nsph(la); ++i)
-          for (int j = 0; j < nsph(lb); ++j) {
+        for (int i = 0; i < nfun(la, A.cart); ++i)
+          for (int j = 0; j < nfun(lb, B.cart); ++j) {
             double v = 0.0;
             for (int ca = 0; ca < na; ++ca) {
-              const double ta = sph_coef(la, i, ca) * comp_scale(la, ca);
+              const double ta = tcoef(la, A.cart, i, ca) * comp_scale(la, ca);
               if (ta == 0.0) continue;
-              for (int cb = 0; cb < nb; ++cb) v += ta * sph_coef(lb, j, cb) * comp_scale(lb, cb) * blk[c][ca][cb];
+              for (int cb = 0; cb < nb; ++cb) v += ta * tcoef(lb, B.cart, j, cb) * comp_scale(lb, cb) * blk[c][ca][cb];
             }
             D[int64_t(c) * n * n + (A.f0 + i) + n * (B.f0 + j)] = v;
           }
@@ -485,12 +493,12 @@ __global__ __launch_bounds__(kEriThreads) void eri_kernel(const DevShell* __rest
     }
   __syncthreads();
   // Cartesian -> spherical, one index at a time (acc -> buf -> acc -> buf -> acc)
-  const int sa = nsph(la), sb = nsph(lb), scn = nsph(lc), sd = nsph(ld);
+  const int sa = nfun(la, A.cart), sb = nfun(lb, B.cart), scn = nfun(lc, C.cart), sd = nfun(ld, D.cart);
   // index d
   for (int k = tid; k < na * nb * nc * sd; k += kEriThreads) {
     const int l = k % sd, rest = k / sd;             // rest = (a, b, c)
     double v = 0.0;
-    for (int c = 0; c < nd; ++c) v += sph_coef(ld, l, c) * comp_scale(ld, c) * acc[rest * nd + c];
+    for (int c = 0; c < nd; ++c) v += tcoef(ld, D.cart, l, c) * comp_scale(ld, c) * acc[rest * nd + c];
     buf[k] = v;
   }
   __syncthreads();
@@ -498,7 +506,7 @@ __global__ __launch_bounds__(kEriThreads) void eri_kernel(const DevShell* __rest
   for (int k = tid; k < na * nb * scn * sd; k += kEriThreads) {
     const int l = k % sd, kk = (k / sd) % scn, ab = k / (sd * scn);
     double v = 0.0;
-    for (int c = 0; c < nc; ++c) v += sph_coef(lc, kk, c) * comp_scale(lc, c) * buf[(ab * nc + c) * sd + l];
+    for (int c = 0; c < nc; ++c) v += tcoef(lc, C.cart, kk, c) * comp_scale(lc, c) * buf[(ab * nc + c) * sd + l];
     acc[k] = v;
   }
   __syncthreads();
@@ -506,7 +514,7 @@ __global__ __launch_bounds__(kEriThreads) void eri_kernel(const DevShell* __rest
   for (int k = tid; k < na * sb * scn * sd; k += kEriThreads) {
     const int kl = k % (scn * sd), j = (k / (scn * sd)) % sb, a = k / (scn * sd * sb);
     double v = 0.0;
-    for (int b = 0; b < nb; ++b) v += sph_coef(lb, j, b) * comp_scale(lb, b) * acc[(a * nb + b) * scn * sd + kl];
+    for (int b = 0; b < nb; ++b) v += tcoef(lb, B.cart, j, b) * comp_scale(lb, b) * acc[(a * nb + b) * scn * sd + kl];
     buf[k] = v;
   }
   __syncthreads();
@@ -514,7 +522,7 @@ __global__ __launch_bounds__(kEriThreads) void eri_kernel(const DevShell* __rest
   for (int k = tid; k < sa * sb * scn * sd; k += kEriThreads) {
     const int jkl = k % (sb * scn * sd), i = k / (sb * scn * sd);
     double v = 0.0;
-    for (int a = 0; a < na; ++a) v += sph_coef(la, i, a) * comp_scale(la, a) * buf[a * sb * scn * sd + jkl];
+    for (int a = 0; a < na; ++a) v += tcoef(la, A.cart, i, a) * comp_scale(la, a) * buf[a * sb * scn * sd + jkl];
     acc[k] = v;
   }
   __syncthreads();
@@ -684,7 +692,7 @@ void Molecule::build_eri() {
   std::vector<double> ex, co;
   for (int64_t s = 0; s < ns; ++s) {
     const HostShell& h = shells_[size_t(s)];
-    ds[size_t(s)] = DevShell{h.c[0], h.c[1], h.c[2], h.l, int(h.exps.size()), int(ex.size()), h.f0};
+    ds[size_t(s)] = DevShell{h.c[0], h.c[1], h.c[2], h.l, int(h.exps.size()), int(ex.size()), h.f0, h.cart};
     ex.insert(ex.end(), h.exps.begin(), h.exps.end());
     co.insert(co.end(), h.coefs.begin(), h.coefs.end());
   }
@@ -953,6 +961,18 @@ void Molecule::set_orbitals(const double* C, const double* eps) {
   C_.assign(C, C + nn);
   eps_.assign(eps, eps + size_t(n_));
   given_orbitals_ = true;
+}
+
+void Molecule::set_cartesian(bool cartesian) {
+  if (done_ || G_.bytes() != 0 || n_J_ != 0) fail("set_cartesian: the integrals have already been built");
+  int64_t f = 0;
+  for (fe::HostShell& h : shells_) {
+    h.cart = cartesian;
+    h.f0 = int(f);
+    f += fe::nfun(h.l, cartesian);
+  }
+  n_ = f;
+  if (n_occ_ >= n_) fail("molecule: no virtual orbitals");
 }
 
 void Molecule::set_frozen_core(int64_t n_frozen) {

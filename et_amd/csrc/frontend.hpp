@@ -17,6 +17,7 @@ struct HostShell {
   int l;
   std::vector<double> exps, coefs;    // coefs normalised for the x^l component
   int f0;                             // first AO function
+  bool cart = false;                  // Cartesian functions (else spherical d)
 };
 }  // namespace fe
 
@@ -47,6 +48,9 @@ class Molecule {
   // n_frozen lowest (doubly occupied) MOs, F^fc_pq = sum_c 2 (pq|cc) - (pc|cq), so that
   // the MOs above them describe the correlated problem with E_HF unchanged
   void set_frozen_core(int64_t n_frozen);
+  // Cartesian Gaussians (eT's `cartesian gaussians`): every d shell keeps its six
+  // Cartesian components instead of the five spherical ones; before any integrals
+  void set_cartesian(bool cartesian);
   // after run: E_nuc, E_HF, eps (n), h_mo (n x n), L_mo (n_J x n x n, J fastest), C (n x n); any may be null
   void results(double* e_nuc, double* e_hf, double* eps, double* h_mo, double* L_mo, double* C) const;
 

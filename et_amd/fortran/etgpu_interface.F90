@@ -386,6 +386,13 @@ module etgpu_interface
          integer(c_int64_t), value :: n_frozen
       end function etg_molecule_set_frozen_core
 !
+      integer(c_int) function etg_molecule_set_cartesian(molecule, cartesian) &
+                                                         bind(C, name='etg_molecule_set_cartesian')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: molecule
+         integer(c_int), value :: cartesian
+      end function etg_molecule_set_cartesian
+!
       integer(c_int) function etg_molecule_dims(molecule, n_ao, n_occ, n_J, scf_iterations) &
                                                 bind(C, name='etg_molecule_dims')
          import :: c_int, c_ptr, c_int64_t

@@ -124,7 +124,8 @@ def frozen_core_orbitals(Z):
 class Molecule:
     """A closed-shell molecule in a Gaussian basis on the device (libetgpu.so)."""
 
-    def __init__(self, atoms, basis=None, bohr_to_angstrom=BOHR_TO_ANGSTROM, charge=0):
+    def __init__(self, atoms, basis=None, bohr_to_angstrom=BOHR_TO_ANGSTROM, charge=0, cartesian=False):
+        """cartesian: eT's `cartesian gaussians` (six Cartesian d functions per d shell)."""
         basis = load_basis() if basis is None else basis
         self._lib = _ccsd.load_library()
         Z, xyz, sh_atom, sh_l, sh_np, exps, coefs = [], [], [], [], [], [], []
@@ -138,7 +139,7 @@ class Molecule:
                 exps.extend(sh['exps'])
                 coefs.extend(sh['coefs'])
         self.charges = np.array(Z)
-        self._shell_atom, self._shell_l = list(sh_atom), list(sh_l)
+        self._shell_atom, self._shell_l, self._cartesian = list(sh_atom), list(sh_l), bool(cartesian)
         self.xyz_bohr = np.array(xyz).reshape(-1, 3)
         n_el = int(round(sum(Z))) - int(charge)
         arr = lambda v, t: np.ascontiguousarray(v, dtype=t)
@@ -151,6 +152,8 @@ class Molecule:
                                                    i64p(k[2]), i64p(k[3]), i64p(k[4]), _ccsd._ptr(k[5]),
                                                    _ccsd._ptr(k[6]), n_el, ctypes.byref(h)))
         self._h = h
+        if cartesian:
+            _ccsd._check(self._lib.etg_molecule_set_cartesian(self._h, 1))
 
     def close(self):
         if getattr(self, '_h', None) is not None and self._h.value:
@@ -244,7 +247,8 @@ class Molecule:
         frozen MO must sit on an atom that is frozen."""
         n = C.shape[0]
         ao_atom = []
-        for size in (lambda l: 2 * l + 1, lambda l: (l + 1) * (l + 2) // 2):
+        for size in (lambda l: (l + 1) * (l + 2) // 2 if l > 1 and self._cartesian else 2 * l + 1,
+                     lambda l: (l + 1) * (l + 2) // 2):
             ao_atom = [a for a, l in zip(self._shell_atom, self._shell_l) for _ in range(size(l))]
             if len(ao_atom) == n:
                 break

@@ -437,6 +437,14 @@ int etg_molecule_run(etg_molecule* molecule, double cholesky_threshold, double s
 int etg_molecule_set_orbitals(etg_molecule* molecule, int64_t n_ao, int64_t n_mo, const double* C,
                               const double* eps);
 
+/* Cartesian Gaussians (eT's `cartesian gaussians` in `system`): every d shell
+ * keeps its six Cartesian components (each normalised) instead of the five real
+ * solid harmonics; the AO count grows accordingly (etg_molecule_dims).  Call before
+ * any integrals are built.
+ * Replaces: the Cartesian basis of eT's AO tool (ao_tool_class.F90, Libint's
+ * Cartesian shells, src/libint/eri.cpp). */
+int etg_molecule_set_cartesian(etg_molecule* molecule, int cartesian);
+
 /* Frozen core for the next etg_molecule_run: h_mo then includes the Coulomb and
  * exchange of the n_frozen lowest MOs, F^fc_pq = sum_c 2 (pq|cc) - (pc|cq), and the
  * caller passes the MOs above them to the CCSD handle (n_o - n_frozen occupied;

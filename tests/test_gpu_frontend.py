@@ -101,3 +101,37 @@ def test_producer_cholesky_vectors_reproduce_et_ccsd_energy():
         solvers.DiisCCGroundState(wf, residual_threshold=1e-11).run()
         e = r['e_hf'] + wf.correlation_energy
     assert abs(e - g['ccsd_energy']) < 1e-10
+
+
+@pytest.mark.gpu
+def test_cartesian_gaussians_reproduce_et():
+    """eT's tests/ccsd_right_es_energies_cartgauss: H2O-He / cc-pVDZ with Cartesian d functions
+    (30 AOs), Cholesky threshold 1e-12: the AO count, E_nuc, E_HF (1e-9), eT's 362 vectors,
+    E_CCSD (1e-10) and the two right EOM-CCSD roots (1e-9), all different from the spherical
+    basis's; the overlap matrix is positive definite with unit diagonal."""
+    import json
+    from et_amd import ccsd, molecule as M, solvers
+    f = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden', 'h2o_he_cartesian.json')))
+    g = f['golden']
+    atoms = [(s, (x, y, z)) for s, x, y, z in f['geometry_angstrom']]
+    with M.Molecule(atoms, None, f['bohr_to_angstrom'], cartesian=True) as mol:
+        assert mol.dims()['n_ao'] == g['n_ao']
+        S = mol.one_electron()[0]
+        assert np.abs(np.diag(S) - 1.0).max() < 1e-12
+        assert np.linalg.eigvalsh(S).min() > 0.0
+        r = mol.run(cholesky_threshold=f['cholesky_threshold'], scf_threshold=1e-10)
+    print('E_nuc %.12f E_HF %.12f n_J %d (eT %d)' % (r['e_nuc'], r['e_hf'], r['n_J'], g['n_cholesky']), flush=True)
+    assert abs(r['e_nuc'] - g['nuclear_repulsion']) < 1e-9
+    assert abs(r['e_hf'] - g['hf_energy']) < 1e-9
+    assert r['n_J'] == g['n_cholesky']
+    n_o = r['n_o']
+    with ccsd.CCSD(n_o, r['L_mo'].shape[1] - n_o, r['n_J']) as wf:
+        wf.set_cholesky(r['L_mo'])
+        wf.set_oei(r['h_mo'])
+        wf.set_orbital_energies(r['eps'])
+        wf.hf_energy = r['e_hf']
+        e = solvers.DiisCCGroundState(wf, residual_threshold=1e-10).run()
+        w = solvers.EigenDavidson(wf, 2, 'right', residual_threshold=1e-9).run()
+    print('E_CCSD %.12f (eT %.12f); roots %s (eT %s)' % (e, g['ccsd_energy'], w, g['excitation_energies']))
+    assert abs(e - g['ccsd_energy']) < 1e-10
+    np.testing.assert_allclose(w, g['excitation_energies'], rtol=0, atol=1e-9)
