@@ -455,7 +455,8 @@ class MixedPrecisionDavidson:
     """EOM roots from reduced-precision transformations, finished in FP64.
 
     Each precision of `schedule` (split-bf16, then FP32) runs the device Davidson with
-    the transformations in that precision down to `margin` times its noise floor
+    the transformations in that precision down to `margin` (one factor, or a dict of
+    factors per precision) times its noise floor
     ||A_p x - A_FP64 x|| (measured once on a normalised random probe), or until
     its residuals stop decreasing; its Ritz vectors start the next phase.  The
     last phase is the FP64 eigen Davidson solver, so convergence is decided on
@@ -496,7 +497,8 @@ class MixedPrecisionDavidson:
         start = self.start
         if start is None and self.restart:
             start = wf.restart_start_vectors(self.n_states, self.side)[0]
-        phases = [(p, max(self.residual_threshold, self.margin * self.noise[p])) for p in self.schedule]
+        margin = self.margin if isinstance(self.margin, dict) else {p: self.margin for p in self.schedule}
+        phases = [(p, max(self.residual_threshold, margin[p] * self.noise[p])) for p in self.schedule]
         phases.append((_ccsd.FP64, self.residual_threshold))
         for p, threshold in phases:
             w, hist, start = wf.davidson(self.n_states, self.side, threshold, self.max_iterations,
