@@ -61,3 +61,36 @@ def test_complex_pair_of_right_roots_matches_et():
     assert abs(lam.real.mean() - g['last_iteration_re'][2]) < 1e-5
     assert abs(abs(lam.imag).max() - abs(g['last_iteration_im'][2])) < 2e-5
     assert abs(lam.imag).min() > 1e-4                   # a genuine complex pair
+
+
+def test_root_next_to_a_complex_pair_follows_et():
+    """eT's tests/ccsd_es_false_positive_defect: HOF / aug-cc-pVDZ at Cholesky threshold 1e-6,
+    three right roots at residual 1e-4, the third next to a complex pair (eT warns about the
+    pair (3, 4)).  At this Cholesky threshold the producer's E_HF is 1.9e-6 from eT's (the
+    same 420 vectors), so the comparison is at that level: E_CCSD (2e-6), the roots (2e-6),
+    eT's 25 Davidson iterations and the last iteration's residual norms (8 %)."""
+    from et_amd import ccsd, molecule as M, solvers
+    f = json.load(open(os.path.join(os.path.dirname(FIXTURE), 'hof_false_positive.json')))
+    g = f['golden']
+    atoms = [(s, (x, y, z)) for s, x, y, z in f['geometry_angstrom']]
+    with M.Molecule(atoms, M.load_basis('aug-cc-pvdz'), f['bohr_to_angstrom']) as mol:
+        r = mol.run(cholesky_threshold=f['cholesky_threshold'], scf_threshold=1e-9)
+    print('E_HF %.12f (eT %.12f) n_J %d (eT %d)' % (r['e_hf'], g['hf_energy'], r['n_J'], g['n_cholesky']),
+          flush=True)
+    assert r['n_J'] == g['n_cholesky']
+    assert abs(r['e_hf'] - g['hf_energy']) < 5e-6
+    n_o = r['n_o']
+    with ccsd.CCSD(n_o, r['L_mo'].shape[1] - n_o, r['n_J']) as wf:
+        wf.set_cholesky(r['L_mo'])
+        wf.set_oei(r['h_mo'])
+        wf.set_orbital_energies(r['eps'])
+        wf.hf_energy = r['e_hf']
+        e = solvers.DiisCCGroundState(wf, residual_threshold=1e-9).run()
+        sv = solvers.EigenDavidson(wf, f['n_states'], 'right', residual_threshold=f['residual_threshold'])
+        w = sv.run()
+    print('E_CCSD %.12f (eT %.12f); roots %s (eT %s), %d iterations, last residuals %s' % (
+        e, g['ccsd_energy'], w, g['excitation_energies'], len(sv.history), sv.history[-1]))
+    assert abs(e - g['ccsd_energy']) < 2e-6
+    np.testing.assert_allclose(w, g['excitation_energies'], rtol=0, atol=2e-6)
+    assert len(sv.history) == g['iterations']
+    np.testing.assert_allclose(sv.history[-1], g['last_iteration_residual'], rtol=0.08)
