@@ -1,9 +1,11 @@
-"""Writes cc-pvdz.json, aug-cc-pvdz.json, sto-3g.json and 6-31g.json (H-Ne) from the reference's
+"""Writes cc-pvdz.json, aug-cc-pvdz.json, sto-3g.json, 6-31g.json (H-Ne) and 3-21g.json (H-Ne, Xe)
+from the reference's
 Gaussian-94 basis files.
 
 Run in a checkout that has the reference next to it; the JSON files are
 committed and are what the package reads.  Data sources: ao_basis/cc-pvdz.g94,
-ao_basis/augmentation-cc-pvdz.g94, ao_basis/sto-3g.g94 and ao_basis/6-31g.g94 (eT's basis library).
+ao_basis/augmentation-cc-pvdz.g94, ao_basis/sto-3g.g94, ao_basis/6-31g.g94 and ao_basis/3-21g.g94 (eT's
+basis library).
 Each element maps to a list of shells {"l": 0|1|2, "exps": [...], "coefs": [...]}
 in file order; an "SP" shell (shared exponents) becomes an s shell followed by a
 p shell.  aug-cc-pVDZ is cc-pVDZ followed by the augmentation shells, as eT reads
@@ -17,11 +19,11 @@ REF = sys.argv[1] if len(sys.argv) > 1 else '/root/reference'
 ELEMENTS = ('H', 'He', 'Li', 'Be', 'B', 'C', 'N', 'O', 'F', 'Ne')
 
 
-def parse(path):
+def parse(path, elements=ELEMENTS):
     out = {}
     for block in open(path).read().split('****'):
         rows = [r for r in block.splitlines() if r.strip() and not r.lstrip().startswith('!')]
-        if not rows or rows[0].split()[0] not in ELEMENTS or len(rows[0].split()) != 2:
+        if not rows or rows[0].split()[0] not in elements or len(rows[0].split()) != 2:
             continue
         element, shells, i = rows[0].split()[0], [], 1
         while i < len(rows):
@@ -47,6 +49,8 @@ if __name__ == '__main__':
     for name in ('cc-pvdz', 'sto-3g', '6-31g'):
         write(name, {'source': 'ao_basis/%s.g94' % name,
                      'basis': parse(os.path.join(REF, 'ao_basis', name + '.g94'))})
+    write('3-21g', {'source': 'ao_basis/3-21g.g94',
+                    'basis': parse(os.path.join(REF, 'ao_basis', '3-21g.g94'), ELEMENTS + ('Xe',))})
     base = parse(os.path.join(REF, 'ao_basis', 'cc-pvdz.g94'))
     aug = parse(os.path.join(REF, 'ao_basis', 'augmentation-cc-pvdz.g94'))
     write('aug-cc-pvdz', {'source': 'ao_basis/cc-pvdz.g94 + ao_basis/augmentation-cc-pvdz.g94',

@@ -24,7 +24,7 @@ from . import restart as _restart
 
 # eT's Bohr radius in Angstrom (CODATA 2014, src/various/parameters.F90:105)
 BOHR_TO_ANGSTROM = 0.52917721067
-Z_OF = {'H': 1, 'He': 2, 'Li': 3, 'Be': 4, 'B': 5, 'C': 6, 'N': 7, 'O': 8, 'F': 9, 'Ne': 10}
+Z_OF = {'H': 1, 'He': 2, 'Li': 3, 'Be': 4, 'B': 5, 'C': 6, 'N': 7, 'O': 8, 'F': 9, 'Ne': 10, 'Xe': 54}
 _BASIS_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'basis')
 
 # the water molecule of the (H2O)_n systems (Angstrom): O on the z axis, H in the yz plane
@@ -91,6 +91,18 @@ def water_lattice(shape=(2, 2, 2), spacing=2.9, seed=0, min_distance=1.5, max_dr
                 placed.extend(p for _, p in mol)
                 atoms.extend((s, tuple(p)) for s, p in mol)
     return atoms
+
+
+# basis sets eT uses with Cartesian d functions unless told otherwise
+# (set_cartesian_atomic_center, atomic_center_class.F90:706-756)
+CARTESIAN_BY_DEFAULT = ('3-21g', '6-31g', '6-31+g', '6-31++g', '6-31g*', '6-31g**', '6-31+g*', '6-31+g**',
+                        '6-31++g**', '6-31g(d,p)', '6-31g(2df,p)', '6-31g(3df,3pd)')
+
+
+def cartesian_by_default(basis_name):
+    """Whether eT builds this basis from Cartesian Gaussians when the input says neither
+    `cartesian gaussians` nor `pure gaussians` (the Pople sets)."""
+    return basis_name.lower() in CARTESIAN_BY_DEFAULT
 
 
 def with_bath_orbital(r):

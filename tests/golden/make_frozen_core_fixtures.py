@@ -14,6 +14,11 @@ bohr), frozen core (O 1s), Cholesky threshold 1e-11, ground state and multiplier
 the CC mean-value dipole: E_nuc, E_HF, the number of Cholesky vectors, the number frozen,
 E_CCSD and the dipole table in Debye (electronic, nuclear, total per component).
 
+tests/golden/xe_321g_frozen_core.json, from tests/ccsd_frozen_core_xenon: Xe / 3-21G (Cartesian d,
+eT's default for Pople sets: 33 AOs), 18 frozen core orbitals, Cholesky threshold 1e-12, ground
+state to 1e-10, three right roots (a degenerate triple) to 1e-9: the AO count, E_HF, the number
+of Cholesky vectors, the number frozen, E_CCSD and the roots.
+
 Each value keeps the output line it came from.
 
 Usage: python tests/golden/make_frozen_core_fixtures.py [/root/reference]
@@ -68,7 +73,33 @@ def dipole_fc():
     print(json.dumps(g, indent=1))
 
 
+def xenon():
+    stem = 'tests/ccsd_frozen_core_xenon/ccsd_frozen_core_xenon'
+    rel = 'tests/ccsd_frozen_core_xenon/result/ccsd_frozen_core_xenon.out'
+    inp = open(os.path.join(REF, stem + '.inp')).read()
+    lines = open(os.path.join(REF, rel)).read().splitlines()
+    geom = [[s[0], float(s[1]), float(s[2]), float(s[3])]
+            for s in (ln.split() for ln in inp.split('basis: 3-21G')[1].split('end geometry')[0].splitlines())
+            if len(s) == 4]
+    g, cite = {}, {}
+    grab = grabber(lines, rel, g, cite)
+    grab('n_ao', r'Number of atomic orbitals:\s+([0-9]+)', int)
+    grab('hf_energy', r'Total energy:\s+(-?[0-9.]+)', float)
+    grab('n_cholesky', r'Final number of Cholesky vectors:\s+([0-9]+)', int)
+    grab('ccsd_energy', r'Final ground state energy \(a\.u\.\):\s*(-?[0-9.]+)', float)
+    grab('n_frozen_core', r'There are ([0-9]+) frozen core orbitals', int)
+    k = next(i for i, ln in enumerate(lines) if 'State                (Hartree)             (eV)' in ln)
+    g['excitation_energies'] = [float(lines[k + 2 + q].split()[1]) for q in range(3)]
+    cite['excitation_energies'] = '%s:%d-%d' % (rel, k + 3, k + 5)
+    out = {'source': stem + '.inp', 'basis': '3-21G', 'cartesian': True, 'bohr_to_angstrom': 0.52917721092,
+           'geometry_angstrom': geom, 'cholesky_threshold': 1e-12, 'residual_threshold': 1e-10,
+           'es_residual_threshold': 1e-9, 'n_states': 3, 'golden': g, 'citations': cite}
+    json.dump(out, open(os.path.join(os.path.dirname(OUT), 'xe_321g_frozen_core.json'), 'w'), indent=1)
+    print(json.dumps(g, indent=1))
+
+
 def main():
+    xenon()
     dipole_fc()
     inp = open(os.path.join(REF, STEM + '.inp')).read()
     lines = open(os.path.join(REF, REL)).read().splitlines()

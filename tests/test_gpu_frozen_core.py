@@ -100,3 +100,35 @@ def test_frozen_core_dipole_reproduces_et():
         assert abs(mu['nuclear_debye'][c] - ref['nuclear']) < 1e-9
         assert abs(mu['electronic_debye'][c] - ref['electronic']) < 1e-8, comp
         assert abs(mu['total_debye'][c] - ref['total']) < 1e-8, comp
+
+
+def test_frozen_core_xenon_reproduces_et():
+    """eT's tests/ccsd_frozen_core_xenon: Xe / 3-21G with Cartesian d functions (eT's default for
+    Pople sets, 33 AOs), eT's 18 frozen core orbitals for Z = 54, Cholesky threshold 1e-12:
+    E_HF and E_CCSD (5e-8 at 7200 Eh) and the degenerate triple of right roots (1e-8).  The
+    decomposition keeps 321 vectors to eT's 319 for this one heavy atom (both to the 1e-12
+    threshold; E_HF agrees to 1e-10), so the count is checked to within 2."""
+    from et_amd import ccsd, molecule as M, solvers
+    f = json.load(open(os.path.join(os.path.dirname(FIXTURE), 'xe_321g_frozen_core.json')))
+    g = f['golden']
+    atoms = [(s, (x, y, z)) for s, x, y, z in f['geometry_angstrom']]
+    assert M.cartesian_by_default(f['basis'])
+    with M.Molecule(atoms, M.load_basis('3-21g'), f['bohr_to_angstrom'], cartesian=True) as mol:
+        assert mol.dims()['n_ao'] == g['n_ao']
+        r = mol.run(cholesky_threshold=f['cholesky_threshold'], scf_threshold=1e-10, frozen_core=True)
+    print('E_HF %.10f (eT %.10f) n_J %d (eT %d), frozen %d' % (r['e_hf'], g['hf_energy'], r['n_J'],
+                                                              g['n_cholesky'], r['n_frozen_core']), flush=True)
+    assert r['n_frozen_core'] == g['n_frozen_core']
+    assert abs(r['n_J'] - g['n_cholesky']) <= 2
+    assert abs(r['e_hf'] - g['hf_energy']) < 5e-8
+    n_o = r['n_o']
+    with ccsd.CCSD(n_o, r['L_mo'].shape[1] - n_o, r['n_J']) as wf:
+        wf.set_cholesky(r['L_mo'])
+        wf.set_oei(r['h_mo'])
+        wf.set_orbital_energies(r['eps'])
+        wf.hf_energy = r['e_hf']
+        e = solvers.DiisCCGroundState(wf, residual_threshold=f['residual_threshold']).run()
+        w = solvers.EigenDavidson(wf, f['n_states'], 'right', residual_threshold=f['es_residual_threshold']).run()
+    print('E_CCSD %.10f (eT %.10f); roots %s (eT %s)' % (e, g['ccsd_energy'], w, g['excitation_energies']))
+    assert abs(e - g['ccsd_energy']) < 5e-8
+    np.testing.assert_allclose(w, g['excitation_energies'], rtol=0, atol=1e-8)
