@@ -1,11 +1,12 @@
-"""Writes cc-pvdz.json, aug-cc-pvdz.json, sto-3g.json, 6-31g.json (H-Ne) and 3-21g.json (H-Ne, Xe)
+"""Writes cc-pvdz.json, aug-cc-pvdz.json, sto-3g.json, 6-31g.json, ccx-dz.json (H-Ne) and 3-21g.json
+(H-Ne, Xe)
 from the reference's
 Gaussian-94 basis files.
 
 Run in a checkout that has the reference next to it; the JSON files are
 committed and are what the package reads.  Data sources: ao_basis/cc-pvdz.g94,
-ao_basis/augmentation-cc-pvdz.g94, ao_basis/sto-3g.g94, ao_basis/6-31g.g94 and ao_basis/3-21g.g94 (eT's
-basis library).
+ao_basis/augmentation-cc-pvdz.g94, ao_basis/sto-3g.g94, ao_basis/6-31g.g94 and ao_basis/3-21g.g94 and ao_basis/ccx-dz.g94
+(eT's basis library).
 Each element maps to a list of shells {"l": 0|1|2, "exps": [...], "coefs": [...]}
 in file order; an "SP" shell (shared exponents) becomes an s shell followed by a
 p shell.  aug-cc-pVDZ is cc-pVDZ followed by the augmentation shells, as eT reads
@@ -46,7 +47,7 @@ def write(name, data):
 
 
 if __name__ == '__main__':
-    for name in ('cc-pvdz', 'sto-3g', '6-31g'):
+    for name in ('cc-pvdz', 'sto-3g', '6-31g', 'ccx-dz'):
         write(name, {'source': 'ao_basis/%s.g94' % name,
                      'basis': parse(os.path.join(REF, 'ao_basis', name + '.g94'))})
     write('3-21g', {'source': 'ao_basis/3-21g.g94',

@@ -42,3 +42,34 @@ def test_degenerate_cvs_roots_reproduce_et():
             print(side, w, len(sv.history), 'iterations; eT', g[side]['excitation_energies'], g[side]['iterations'],
                   flush=True)
             np.testing.assert_allclose(w, g[side]['excitation_energies'], rtol=0, atol=1e-9)
+
+
+def test_cvs_with_core_valence_basis_reproduces_et():
+    """eT's tests/ccsd_right_cvs_ccX-dz: H2O with ccX-DZ on O and cc-pVDZ on H (one basis per
+    element, 48 AOs), Cholesky threshold 1e-10 (eT: 546 vectors), the ground state, and the
+    right CVS root out of the O 1s: E_HF (1e-9), E_CCSD (1e-8; eT stops at a residual of 1e-7)
+    and the root (1e-6; eT stops at a residual of 1e-5)."""
+    from et_amd import ccsd, molecule as M, solvers
+    f = json.load(open(os.path.join(os.path.dirname(FIXTURE), 'h2o_ccx_cvs.json')))
+    g = f['golden']
+    basis = {}
+    for sym, _, _, _, name in f['geometry_angstrom_basis']:
+        basis[sym] = M.load_basis(name)[sym]
+    atoms = [(s, (x, y, z)) for s, x, y, z, _ in f['geometry_angstrom_basis']]
+    with M.Molecule(atoms, basis, f['bohr_to_angstrom']) as mol:
+        assert mol.dims()['n_ao'] == g['n_ao']
+        r = mol.run(cholesky_threshold=f['cholesky_threshold'], scf_threshold=1e-10)
+    print('E_HF %.12f (eT %.12f) n_J %d (eT %d)' % (r['e_hf'], g['hf_energy'], r['n_J'], g['n_cholesky']), flush=True)
+    assert r['n_J'] == g['n_cholesky']
+    assert abs(r['e_hf'] - g['hf_energy']) < 1e-9
+    n_o = r['n_o']
+    with ccsd.CCSD(n_o, r['L_mo'].shape[1] - n_o, r['n_J']) as wf:
+        wf.set_cholesky(r['L_mo'])
+        wf.set_oei(r['h_mo'])
+        wf.set_orbital_energies(r['eps'])
+        wf.hf_energy = r['e_hf']
+        e = solvers.DiisCCGroundState(wf, residual_threshold=1e-9).run()
+        w = solvers.EigenDavidson(wf, 1, 'right', residual_threshold=1e-7, core_mos=f['core_mos']).run()
+    print('E_CCSD %.12f (eT %.12f); root %s (eT %s)' % (e, g['ccsd_energy'], w, g['excitation_energies']))
+    assert abs(e - g['ccsd_energy']) < 1e-8
+    np.testing.assert_allclose(w, g['excitation_energies'], rtol=0, atol=1e-6)
