@@ -378,9 +378,9 @@ def main():
         print(json.dumps({'lambda_dipole': lambda_dipole(fp64=args.fp64_reference)}), flush=True)
         return
     if args.gpus != 1:
-        print('bench.py measures one GPU: every box in this setup has a single MI355X, so '
-              '--gpus %d is not supported (the multi-GPU sharding is described in DESIGN.md).'
-              % args.gpus, file=sys.stderr)
+        print('bench.py measures one GPU (a single MI355X); --gpus %d is not supported: the '
+              'ladder and ring products would shard over GPUs by output blocks, which this '
+              'library does not implement.' % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     w = WORKLOAD
