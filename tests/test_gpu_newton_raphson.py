@@ -70,3 +70,32 @@ def test_newton_raphson_ground_state_follows_et(name):
     for m, ref in zip(nr.micro, g['micro_residual_norms']):
         m, ref = np.array(m), np.array(ref)
         np.testing.assert_allclose(m[ref > 1e-9], ref[ref > 1e-9], rtol=1e-3)
+
+
+def test_newton_raphson_with_split_bf16_linear_solves():
+    """The linear solves of a Newton-Raphson step need only ||A u + Omega|| < 1e-2 ||Omega||, far
+    above the split-bf16 transformations' relative error (~2e-5), so they can run in split-bf16
+    while the residual and the convergence test stay FP64: H2O-He (eT's ccsd_gs_energy_nr
+    settings) reaches the FP64 solver's energy (1e-10) in as many macro iterations, +-1."""
+    from et_amd import ccsd, molecule as M, solvers
+    f = fixture('h2o_he')
+    g = f['golden']
+    atoms = [(s, (x, y, z)) for s, x, y, z in f['geometry_angstrom']]
+    with M.Molecule(atoms, None, f['bohr_to_angstrom']) as mol:
+        r = mol.run(cholesky_threshold=f['cholesky_threshold'], scf_threshold=1e-10)
+    n_v = r['L_mo'].shape[1] - r['n_o']
+    runs = {}
+    for name, prec in (('fp64', ccsd.FP64), ('bf16x3', ccsd.BF16X3)):
+        with ccsd.CCSD(r['n_o'], n_v, r['n_J']) as wf:
+            wf.set_cholesky(r['L_mo'])
+            wf.set_oei(r['h_mo'])
+            wf.set_orbital_energies(r['eps'])
+            wf.hf_energy = r['e_hf']
+            nr = solvers.NewtonRaphsonCCGroundState(wf, residual_threshold=f['residual_threshold'],
+                                                    energy_threshold=f['energy_threshold'],
+                                                    rel_threshold=f['relative_micro_threshold'], precision=prec)
+            runs[name] = (nr.run(), len(nr.history), [len(m) for m in nr.micro])
+        print(name, runs[name], flush=True)
+    assert abs(runs['bf16x3'][0] - runs['fp64'][0]) < 1e-10
+    assert abs(runs['bf16x3'][0] - g['ccsd_energy']) < 1e-10
+    assert abs(runs['bf16x3'][1] - runs['fp64'][1]) <= 1
