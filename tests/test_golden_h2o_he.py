@@ -346,3 +346,29 @@ def test_gpu_remove_core_reproduces_et():
     Xi = ((np.sqrt(8.0 * q + 1) - 1) // 2).astype(np.int64)
     Yi = q - Xi * (Xi + 1) // 2
     assert np.abs(X[n_t1:][(Xi // n_v == 0) | (Yi // n_v == 0)]).max() == 0.0
+
+
+@pytest.mark.gpu
+def test_gpu_crop_multipliers_and_ground_state_agree_with_diis():
+    """eT's CROP variant of DIIS (`crop` in `solver cc gs` / `solver cc multipliers`) changes the
+    path to the solution, not the solution: on H2O-He the CROP ground state (DIIS dimension 3,
+    as eT's ccsd_gs_energy_nr_crop) and CROP multipliers reach the plain DIIS solvers' energy
+    (1e-10) and multipliers (1e-9)."""
+    from et_amd import ccsd, solvers
+    m = inputs()
+    n_o = m['n_o']
+    n_v = m['L_mo'].shape[1] - n_o
+    out = {}
+    for crop, dim in ((False, 8), (True, 3)):
+        with ccsd.CCSD(n_o, n_v, m['n_J']) as wf:
+            wf.set_cholesky(m['L_mo'])
+            wf.set_oei(m['h_mo'])
+            wf.set_orbital_energies(m['eps'])
+            gs = solvers.DiisCCGroundState(wf, residual_threshold=1e-11, diis_dimension=dim, crop=crop)
+            gs.run()
+            mu = solvers.DiisCCMultipliers(wf, residual_threshold=1e-11, diis_dimension=dim, crop=crop)
+            tbar = mu.run()
+            out[crop] = (wf.correlation_energy, tbar, len(gs.history), len(mu.history))
+        print('crop' if crop else 'diis', 'E_corr %.12f, %d + %d iterations' % (out[crop][0], out[crop][2], out[crop][3]))
+    assert abs(out[True][0] - out[False][0]) < 1e-10
+    assert np.abs(out[True][1] - out[False][1]).max() < 1e-9
