@@ -14,17 +14,17 @@
 // min(1e-11, 0.1 threshold) (:315).
 //
 // The reduced eigenproblem (non-symmetric, dimension <= 128) is solved by one
-// wavefront: Householder reduction to upper Hessenberg form and the Francis
-// double-shift QR iteration for the eigenvalues (the published EISPACK
-// orthes / hqr algorithms), then inverse iteration on G - w I with a partially
-// pivoted LU for the wanted eigenvectors (for a complex pair a +- ib, the real
-// 2k system of u + i v).  Complex pairs are carried as (u, v) column pairs with
+// wavefront: Householder reduction to upper Hessenberg form, the implicitly
+// double-shifted QR iteration for the eigenvalues (francis.hpp), then inverse
+// iteration on G - w I with a partially pivoted LU for the wanted eigenvectors
+// (for a complex pair a +- ib, the real 2k system of u + i v).  Complex pairs are carried as (u, v) column pairs with
 // real and imaginary residuals, as eT's Davidson tool does.
 #include <algorithm>
 #include <cmath>
 #include <vector>
 
 #include "ccsd.hpp"
+#include "francis.hpp"
 #include "kernels.hpp"
 
 namespace etg {
@@ -34,6 +34,13 @@ constexpr int kMaxRed = 128;
 constexpr int kLanes = 64;
 
 __device__ inline double sgn(double a, double b) { return b >= 0.0 ? fabs(a) : -fabs(a); }
+
+// The reduced eigensolver's team: one wavefront.
+struct WaveTeam {
+  __device__ int rank() const { return int(threadIdx.x); }
+  __device__ int size() const { return kLanes; }
+  __device__ void sync() const { __syncthreads(); }
+};
 
 // H(i, j) of the k x k working matrix in LDS, column-major with ld kMaxRed.
 #define HH(i, j) H[(i) + (j) * kMaxRed]
@@ -155,146 +162,11 @@ __global__ __launch_bounds__(kLanes) void reduced_eigen_kernel(const double* __r
     __syncthreads();
   }
 
-  // ---- eigenvalues of the Hessenberg matrix (Francis double-shift QR, hqr) ----
-  // Every lane runs the scalar control on the same LDS values; the row and
-  // column sweeps of a double-shift step are split across lanes.
+  // ---- eigenvalues of the Hessenberg matrix (implicit double-shift QR) ----
   double anorm = 0.0;
-  for (int i = 0; i < k; ++i)
-    for (int j = (i > 0 ? i - 1 : 0); j < k; ++j) anorm += fabs(HH(i, j));
-  int nn = k - 1;
-  double t = 0.0;
-  while (nn >= 0 && status == 0) {
-    int its = 0, l;
-    do {
-      for (l = nn; l >= 1; --l) {
-        double s = fabs(HH(l - 1, l - 1)) + fabs(HH(l, l));
-        if (s == 0.0) s = anorm;
-        if (fabs(HH(l, l - 1)) + s == s) break;
-      }
-      __syncthreads();
-      if (l >= 1 && lane == 0) HH(l, l - 1) = 0.0;
-      __syncthreads();
-      double x = HH(nn, nn);
-      if (l == nn) {
-        if (lane == 0) { wr[nn] = x + t; wi[nn] = 0.0; }
-        --nn;
-      } else {
-        double yv = HH(nn - 1, nn - 1);
-        double w = HH(nn, nn - 1) * HH(nn - 1, nn);
-        if (l == nn - 1) {
-          const double p = 0.5 * (yv - x);
-          const double q = p * p + w;
-          double z = sqrt(fabs(q));
-          x += t;
-          if (lane == 0) {
-            if (q >= 0.0) {
-              z = p + sgn(z, p);
-              wr[nn - 1] = wr[nn] = x + z;
-              if (z != 0.0) wr[nn] = x - w / z;
-              wi[nn - 1] = wi[nn] = 0.0;
-            } else {
-              wr[nn - 1] = wr[nn] = x + p;
-              wi[nn - 1] = -z;
-              wi[nn] = z;
-            }
-          }
-          nn -= 2;
-        } else {
-          if (its == 60) {
-            if (lane == 0) status = 1;
-            __syncthreads();
-            break;
-          }
-          if (its == 10 || its == 20 || its == 40) {        // exceptional shift
-            t += x;
-            __syncthreads();
-            for (int i = lane; i <= nn; i += kLanes) HH(i, i) -= x;
-            __syncthreads();
-            const double s = fabs(HH(nn, nn - 1)) + fabs(HH(nn - 1, nn - 2));
-            yv = x = 0.75 * s;
-            w = -0.4375 * s * s;
-          }
-          ++its;
-          int m;
-          double p = 0.0, q = 0.0, r = 0.0, z;
-          for (m = nn - 2; m >= l; --m) {
-            z = HH(m, m);
-            r = x - z;
-            const double s0 = yv - z;
-            p = (r * s0 - w) / HH(m + 1, m) + HH(m, m + 1);
-            q = HH(m + 1, m + 1) - z - r - s0;
-            r = HH(m + 2, m + 1);
-            const double s = fabs(p) + fabs(q) + fabs(r);
-            p /= s;
-            q /= s;
-            r /= s;
-            if (m == l) break;
-            const double u = fabs(HH(m, m - 1)) * (fabs(q) + fabs(r));
-            const double v = fabs(p) * (fabs(HH(m - 1, m - 1)) + fabs(z) + fabs(HH(m + 1, m + 1)));
-            if (u + v == v) break;
-          }
-          __syncthreads();
-          for (int i = m + 2 + lane; i <= nn; i += kLanes) {
-            HH(i, i - 2) = 0.0;
-            if (i != m + 2) HH(i, i - 3) = 0.0;
-          }
-          __syncthreads();
-          for (int kk = m; kk <= nn - 1; ++kk) {
-            if (kk != m) {
-              p = HH(kk, kk - 1);
-              q = HH(kk + 1, kk - 1);
-              r = (kk != nn - 1) ? HH(kk + 2, kk - 1) : 0.0;
-              x = fabs(p) + fabs(q) + fabs(r);
-              if (x != 0.0) {
-                p /= x;
-                q /= x;
-                r /= x;
-              }
-            }
-            const double s = sgn(sqrt(p * p + q * q + r * r), p);
-            __syncthreads();
-            if (s != 0.0) {
-              if (lane == 0) {
-                if (kk == m) {
-                  if (l != m) HH(kk, kk - 1) = -HH(kk, kk - 1);
-                } else {
-                  HH(kk, kk - 1) = -s * x;
-                }
-              }
-              p += s;
-              x = p / s;
-              yv = q / s;
-              z = r / s;
-              q /= p;
-              r /= p;
-              __syncthreads();
-              for (int j = kk + lane; j <= nn; j += kLanes) {      // row modification
-                double pp = HH(kk, j) + q * HH(kk + 1, j);
-                if (kk != nn - 1) {
-                  pp += r * HH(kk + 2, j);
-                  HH(kk + 2, j) -= pp * z;
-                }
-                HH(kk + 1, j) -= pp * yv;
-                HH(kk, j) -= pp * x;
-              }
-              __syncthreads();
-              const int mmin = nn < kk + 3 ? nn : kk + 3;
-              for (int i = l + lane; i <= mmin; i += kLanes) {    // column modification
-                double pp = x * HH(i, kk) + yv * HH(i, kk + 1);
-                if (kk != nn - 1) {
-                  pp += z * HH(i, kk + 2);
-                  HH(i, kk + 2) -= pp * r;
-                }
-                HH(i, kk + 1) -= pp * q;
-                HH(i, kk) -= pp;
-              }
-              __syncthreads();
-            }
-          }
-        }
-      }
-      __syncthreads();
-    } while (l < nn - 1 && status == 0);
+  {
+    const int st = francis::hessenberg_eigenvalues(WaveTeam(), H, kMaxRed, k, wr, wi, &anorm);
+    if (lane == 0 && st != 0) status = st;
   }
   __syncthreads();
 

@@ -11,8 +11,8 @@ Formats (unformatted stream files in native byte order, no record markers):
 * ``orbital_coefficients`` (src/wavefunctions/hf/file_handling_hf.F90:36-81): n_ao and
   n_mo (int64), the orbital energies (n_mo), then C (n_ao, n_mo) column-major.
 * eT v1.0 files are Fortran sequential records (4-byte length markers around each
-  record); ``convert_v1_0_files`` rewrites a v1.0 restart directory in the formats above
-  as eT's dev_tools/convert_v1_0_restart_files.py:83-372 does.
+  record); ``convert_v1_0_directory`` rewrites a v1.0 restart directory in the formats above
+  (the directory layout is described where the converter is defined).
 
 Vectors are in eT's CC layout: singles (v, o) column-major, then the doubles packed over
 the compound index ai = a + n_v i (element (ai, bj), ai <= bj, at bj (bj + 1) / 2 + ai).
@@ -21,7 +21,6 @@ reverse (get_restart_vector_doubles, src/wavefunctions/doubles/file_handling_dou
 are the permutation ai,bj -> bi,aj applied to that packed storage.
 """
 import os
-import sys
 
 import numpy as np
 
@@ -242,145 +241,140 @@ class CCSDFiles:
 
 
 # ---- eT v1.0 restart directories ----------------------------------------------------------
+#
+# Layout of a v1.0 directory (every file Fortran sequential unformatted: each record is
+# a 4-byte native-order length L, L bytes, then L again):
+#
+#   cc_restart_file      four records of one int64: n_o, n_v, and the lengths of the
+#                        ground-state and excited-state vectors (n_t1 only for CCS / CC2
+#                        ground states, n_t1 + n_t2 for CCSD)
+#   t, tbar              the ground-state vector, one record per block (singles, doubles)
+#   r_00k, l_00k         the k-th excited state, the whole vector in one record
+#   excitation_energies  a record with the int64 state count, then one with the energies
+#   orbital_energies     one record of n_mo energies per spin (one for RHF)
+#   orbital_coefficients one record of the n_ao x n_mo coefficients per spin
+#   scf_restart_file     present when the Hartree-Fock files are
+#
+# The conversion re-reads every vector as a list of records, checks its length against
+# the dimensions of cc_restart_file, and writes it through AmplitudeFile / the orbital
+# layout above.  n_mo is the length of an orbital-energy record and n_ao that of a
+# coefficient record divided by n_mo.  v1.0 ground-state files carry no energy: 0 is
+# written; excited states take theirs from excitation_energies.
 
-def read_record(f, endian=sys.byteorder):
-    """One Fortran sequential record: 4-byte length, data, 4-byte length
-    (convert_v1_0_restart_files.py:83-97)."""
-    n = int.from_bytes(_read_exact(f, 4, 'a record marker'), endian)
-    data = _read_exact(f, n, 'a record')
-    if int.from_bytes(_read_exact(f, 4, 'a record marker'), endian) != n:
-        raise RestartError('%s: record markers do not match' % f.name)
-    return data
-
-
-def v1_0_dimensions(path, endian=sys.byteorder):
-    """(n1, n2, n_gs) from a v1.0 cc_restart_file (convert_v1_0_restart_files.py:100-116)."""
+def fortran_records(path):
+    """Every record of a Fortran sequential unformatted file, as bytes."""
+    out = []
     with open(path, 'rb') as f:
-        n_o, n_v, n_gs, n_es = (int.from_bytes(read_record(f, endian), endian) for _ in range(4))
-    n1 = n_o * n_v
-    n2 = 0 if n_es == n1 else n1 * (n1 + 1) // 2
-    return n1, n2, n_gs
+        data = f.read()
+    pos = 0
+    while pos < len(data):
+        if pos + 4 > len(data):
+            raise RestartError('%s: %d stray bytes after the last record' % (path, len(data) - pos))
+        n = int(np.frombuffer(data, dtype='=i4', count=1, offset=pos)[0])
+        end = pos + 4 + n
+        if n < 0 or end + 4 > len(data) or int(np.frombuffer(data, dtype='=i4', count=1, offset=end)[0]) != n:
+            raise RestartError('%s: not a Fortran sequential file (record at byte %d)' % (path, pos))
+        out.append(data[pos + 4:end])
+        pos = end + 4
+    return out
 
 
-def _convert_gs_file(old, new, energy, n1, n2, endian):
-    """make_new_gs_file (convert_v1_0_restart_files.py:147-184): one record per block."""
-    n_records = 2 if n2 > 0 else 1
-    size = os.path.getsize(old)
-    if size != 8 * (n1 + n2) + 8 * n_records:
-        raise RestartError('Expected file size %d found %d. Number of amplitudes does not match '
-                           'in file %s.' % (8 * (n1 + n2) + 8 * n_records, size, old))
-    with open(old, 'rb') as src, open(new, 'wb') as dst:
-        dst.write(energy)
-        dst.write(n1.to_bytes(8, endian))
-        dst.write(read_record(src, endian))
-        if n2 > 0:
-            dst.write(n2.to_bytes(8, endian))
-            dst.write(read_record(src, endian))
+def _int_records(path):
+    recs = fortran_records(path)
+    for r in recs:
+        if len(r) != 8:
+            raise RestartError('%s: expected records of one 8-byte integer' % path)
+    return [int(np.frombuffer(r, dtype='=i8')[0]) for r in recs]
 
 
-def _convert_es_file(old, new, energy, n1, n2, endian):
-    """make_new_es_file (convert_v1_0_restart_files.py:187-215): the whole vector in one record."""
-    size = os.path.getsize(old)
-    if size != 8 * (n1 + n2) + 8:
-        raise RestartError('Expected file size %d found %d. Number of amplitudes does not match '
-                           'in file %s.' % (8 * (n1 + n2) + 8, size, old))
-    with open(old, 'rb') as src, open(new, 'wb') as dst:
-        src.read(4)
-        dst.write(energy)
-        dst.write(n1.to_bytes(8, endian))
-        dst.write(_read_exact(src, 8 * n1, 'the singles'))
-        if n2 > 0:
-            dst.write(n2.to_bytes(8, endian))
-            dst.write(_read_exact(src, 8 * n2, 'the doubles'))
+def _doubles(record):
+    if len(record) % 8:
+        raise RestartError('a record of %d bytes does not hold float64 values' % len(record))
+    return np.frombuffer(record, dtype='=f8')
 
 
-def _v1_0_excitation_energies(path, endian):
-    """read_excitation_energies (convert_v1_0_restart_files.py:130-144): raw 8-byte energies."""
-    with open(path, 'rb') as f:
-        n_states = int.from_bytes(read_record(f, endian), endian)
-        f.read(4)
-        return [_read_exact(f, 8, 'an excitation energy') for _ in range(n_states)]
+def _vector_records(path, lengths):
+    """The float64 records of a v1.0 vector file; their lengths must be `lengths`."""
+    recs = [_doubles(r) for r in fortran_records(path)]
+    found = [r.size for r in recs]
+    if found != list(lengths):
+        raise RestartError('%s: records of %s amplitudes, the dimensions in cc_restart_file give %s'
+                           % (path, found, list(lengths)))
+    return recs
 
 
-def _convert_orbitals(old_dir, new_dir, n_ao, endian):
-    """make_new_orbital_file (convert_v1_0_restart_files.py:262-318)."""
-    coefficients = os.path.join(old_dir, 'orbital_coefficients')
-    energies = os.path.join(old_dir, 'orbital_energies')
-    for p in (coefficients, energies):
-        if not os.path.isfile(p):
-            raise FileNotFoundError(p)
-    size = os.path.getsize(coefficients)
-    if size - 8 <= 8 * n_ao ** 2:
-        n_records = 1
-    elif size - 16 <= 2 * 8 * n_ao ** 2:
-        n_records = 2
-    else:
-        raise RestartError('File size of file %s does not match n_ao = %d' % (coefficients, n_ao))
-    n_mo = (size - 8 * n_records) // (n_ao * 8 * n_records)
-    with open(os.path.join(new_dir, 'orbital_coefficients'), 'wb') as dst, \
-            open(energies, 'rb') as e, open(coefficients, 'rb') as c:
-        dst.write(n_ao.to_bytes(8, endian))
-        dst.write(n_mo.to_bytes(8, endian))
-        for _ in range(n_records):
-            dst.write(read_record(e, endian))
-            dst.write(read_record(c, endian))
+def _convert_orbital_files(old_dir, new_dir):
+    energies = [_doubles(r) for r in fortran_records(os.path.join(old_dir, 'orbital_energies'))]
+    coefficients = [_doubles(r) for r in fortran_records(os.path.join(old_dir, 'orbital_coefficients'))]
+    if not energies or len(energies) != len(coefficients):
+        raise RestartError('%s: orbital_energies and orbital_coefficients hold %d and %d records'
+                           % (old_dir, len(energies), len(coefficients)))
+    n_mo = energies[0].size
+    n_ao = coefficients[0].size // max(n_mo, 1)
+    for e, c in zip(energies, coefficients):
+        if e.size != n_mo or c.size != n_ao * n_mo or n_ao * n_mo == 0:
+            raise RestartError('%s: orbital records of inconsistent lengths' % old_dir)
+    with open(os.path.join(new_dir, 'orbital_coefficients'), 'wb') as f:
+        f.write(np.array([n_ao, n_mo], dtype=np.int64).tobytes())
+        for e, c in zip(energies, coefficients):
+            f.write(e.astype(np.float64).tobytes())
+            f.write(c.astype(np.float64).tobytes())
+    return n_ao, n_mo
 
 
-def convert_v1_0_files(old_dir, new_dir, verbose=False):
-    """convert_v1_0_files (convert_v1_0_restart_files.py:321-365): rewrite a v1.0 restart
-    directory (scf_restart_file, orbital files, cc_restart_file, t, tbar, r_00k, l_00k,
-    excitation_energies) in the current formats.  The ground-state energy is written as zero
-    (v1.0 files do not hold it); excited states carry their excitation energies."""
-    endian = sys.byteorder
-    old_dir, new_dir = os.path.abspath(old_dir), os.path.abspath(new_dir)
+def convert_v1_0_directory(old_dir, new_dir):
+    """Rewrites an eT v1.0 restart directory (layout above) in the current formats:
+    amplitude files t, tbar, r_00k, l_00k and the orbital file.  Returns a dict of
+    what was written ('orbitals': (n_ao, n_mo) or None, 'ground_state': names,
+    'excited_states': number of states)."""
+    old_dir, new_dir = os.path.realpath(old_dir), os.path.realpath(new_dir)
     if not os.path.isdir(old_dir):
         raise NotADirectoryError(old_dir)
     if old_dir == new_dir:
-        raise RestartError('v1.0 directory cannot be the same as v1.x directory.')
+        raise RestartError('the converted files would overwrite the v1.0 files in %s' % old_dir)
     os.makedirs(new_dir, exist_ok=True)
-    log = print if verbose else (lambda *a: None)
-
-    scf = os.path.join(old_dir, 'scf_restart_file')
-    if os.path.isfile(scf):
-        log('Creating new Hartree-Fock files.')
-        with open(scf, 'rb') as f:
-            n_ao = int.from_bytes(read_record(f, endian), endian)
-        _convert_orbitals(old_dir, new_dir, n_ao, endian)
-    else:
-        log('scf_restart_file not found in %s. Did not update Hartree-Fock restart files.' % old_dir)
-
-    cc = os.path.join(old_dir, 'cc_restart_file')
-    if not os.path.isfile(cc):
-        log('cc_restart_file not found in %s. Did not update coupled cluster restart files.' % old_dir)
-        return
-    n1, n2, n_gs = v1_0_dimensions(cc, endian)
-    log('Creating new CC ground state files.')
+    done = {'orbitals': None, 'ground_state': [], 'excited_states': 0}
+    have = lambda name: os.path.isfile(os.path.join(old_dir, name))
+    if have('scf_restart_file') and have('orbital_energies') and have('orbital_coefficients'):
+        done['orbitals'] = _convert_orbital_files(old_dir, new_dir)
+    if not have('cc_restart_file'):
+        return done
+    n_o, n_v, n_gs, n_es = _int_records(os.path.join(old_dir, 'cc_restart_file'))[:4]
+    n_t1 = n_o * n_v
+    n_t2 = n_t1 * (n_t1 + 1) // 2
+    gs_blocks = [n_t1] if n_gs == n_t1 else [n_t1, n_t2]
+    es_blocks = [n_t1] if n_es == n_t1 else [n_t1, n_t2]
     for name in ('t', 'tbar'):
-        old = os.path.join(old_dir, name)
-        if os.path.isfile(old):        # CC2 ground-state files hold singles only (n_gs = n1)
-            _convert_gs_file(old, os.path.join(new_dir, name), bytes(8), n1,
-                             0 if n_gs == n1 else n2, endian)
-    ee_path = os.path.join(old_dir, 'excitation_energies')
-    if os.path.isfile(ee_path):
-        log('Creating new CC excited state files.')
-        for k, energy in enumerate(_v1_0_excitation_energies(ee_path, endian), 1):
-            for side in 'rl':
-                old = os.path.join(old_dir, '%s_%03d' % (side, k))
-                if os.path.isfile(old):
-                    _convert_es_file(old, os.path.join(new_dir, '%s_%03d' % (side, k)), energy, n1, n2,
-                                     endian)
+        if have(name):
+            x = np.concatenate(_vector_records(os.path.join(old_dir, name), gs_blocks))
+            AmplitudeFile(os.path.join(new_dir, name), gs_blocks).save(x, 0.0)
+            done['ground_state'].append(name)
+    if have('excitation_energies'):
+        recs = fortran_records(os.path.join(old_dir, 'excitation_energies'))
+        if len(recs) < 2 or len(recs[0]) != 8:
+            raise RestartError('%s: expected the state count and the energies' % old_dir)
+        n_states = int(np.frombuffer(recs[0], dtype='=i8')[0])
+        omega = _doubles(recs[1])
+        if omega.size < n_states:
+            raise RestartError('%s: %d excitation energies for %d states' % (old_dir, omega.size, n_states))
+        for k in range(1, n_states + 1):
+            for prefix in ('r', 'l'):
+                name = '%s_%03d' % (prefix, k)
+                if have(name):
+                    x = _vector_records(os.path.join(old_dir, name), [sum(es_blocks)])[0]
+                    AmplitudeFile(os.path.join(new_dir, name), es_blocks).save(x, float(omega[k - 1]))
+        done['excited_states'] = n_states
+    return done
 
 
 def main(argv=None):
-    """python -m et_amd.restart OLD_V1_0_DIR [NEW_DIR] [-v]: the v1.0 converter's command line."""
+    """python -m et_amd.restart V1_0_DIR NEW_DIR: converts an eT v1.0 restart directory."""
     import argparse
-    p = argparse.ArgumentParser(description='Convert an eT v1.0 restart directory.')
-    p.add_argument('old_restart_dir')
-    p.add_argument('new_restart_dir', nargs='?')
-    p.add_argument('-v', '--verbose', action='store_true')
+    p = argparse.ArgumentParser(description='Rewrite an eT v1.0 restart directory in the current formats.')
+    p.add_argument('v1_0_dir')
+    p.add_argument('new_dir')
     a = p.parse_args(argv)
-    new = a.new_restart_dir or os.path.join(os.path.abspath(a.old_restart_dir), 'v_1_x_restart_files')
-    convert_v1_0_files(a.old_restart_dir, new, a.verbose)
+    print(convert_v1_0_directory(a.v1_0_dir, a.new_dir))
 
 
 if __name__ == '__main__':

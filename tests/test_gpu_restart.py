@@ -42,7 +42,7 @@ def et_dir(tmp_path_factory):
     from et_amd import restart as R
     base = tmp_path_factory.mktemp('v1_0')
     new = str(base / 'new')
-    R.convert_v1_0_files(V.build(str(base / 'old')), new)
+    R.convert_v1_0_directory(V.build(str(base / 'old')), new)
     return new
 
 
@@ -152,7 +152,7 @@ def test_singles_only_files(inputs, tmp_path):
         n1 = wf.n_t1
         old = V.build(str(tmp_path / 'cc2_old'), 'cc2')
         new = str(tmp_path / 'cc2_new')
-        R.convert_v1_0_files(old, new)
+        R.convert_v1_0_directory(old, new)
         wf.set_initial_amplitudes_guess()
         cold = wf.get_amplitudes()
         wf.set_amplitudes(np.full(wf.n_gs_amplitudes, 0.01))

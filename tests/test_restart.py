@@ -1,9 +1,9 @@
 """eT restart files on the host (et_amd.restart), no GPU.
 
 * eT's own v1.0 restart directories (tests/restart_ccsd_v1.0, restart_ccsd_from_cc2_v1.0),
-  rebuilt byte for byte from the fixtures, convert to the current format with the
-  layout of dev_tools/convert_v1_0_restart_files.py: zero energy, int64 block
-  lengths, the records' doubles; excited states carry eT's excitation energies;
+  rebuilt byte for byte from the fixtures, convert to the current amplitude-file
+  format: zero energy, int64 block lengths, the records' doubles; excited states
+  carry eT's excitation energies;
   the CC2 t file (singles only) reads back as a partial vector;
 * amplitude files round-trip and reject blocks of the wrong length as eT's
   read_ does; orbital files round-trip and check their dimensions;
@@ -30,7 +30,7 @@ def test_rebuilt_v1_0_directories_are_ets_files(tmp_path):
 def test_v1_0_ccsd_directory_converts_to_amplitude_files(tmp_path):
     old = V.build(str(tmp_path / 'old'))
     new = str(tmp_path / 'new')
-    R.convert_v1_0_files(old, new)
+    R.convert_v1_0_directory(old, new)
     v = V.vectors()
     n_t2 = V.N_T1 * (V.N_T1 + 1) // 2
     raw = open(os.path.join(new, 't'), 'rb').read()
@@ -61,7 +61,7 @@ def test_v1_0_cc2_ground_state_is_singles_only(tmp_path):
     """CC2 v1.0 ground-state files hold singles only: CCSD reads n_t1 amplitudes."""
     old = V.build(str(tmp_path / 'old'), 'cc2')
     new = str(tmp_path / 'new')
-    R.convert_v1_0_files(old, new)
+    R.convert_v1_0_directory(old, new)
     assert os.path.getsize(os.path.join(new, 't')) == 16 + 8 * V.N_T1
     x, e, n_read = R.CCSDFiles(new, V.N_O, V.N_V).t.read()
     assert n_read == V.N_T1
@@ -73,10 +73,14 @@ def test_converter_rejects_wrong_sizes(tmp_path):
     old = V.build(str(tmp_path / 'old'))
     with open(os.path.join(old, 'r_002'), 'ab') as f:
         f.write(bytes(8))
-    with pytest.raises(R.RestartError, match='Number of amplitudes does not match'):
-        R.convert_v1_0_files(old, str(tmp_path / 'new'))
-    with pytest.raises(R.RestartError, match='cannot be the same'):
-        R.convert_v1_0_files(old, old)
+    with pytest.raises(R.RestartError, match='dimensions in cc_restart_file'):
+        R.convert_v1_0_directory(old, str(tmp_path / 'new'))
+    with open(os.path.join(old, 'r_002'), 'r+b') as f:          # a record cut short
+        f.truncate(os.path.getsize(os.path.join(old, 'r_002')) - 12)
+    with pytest.raises(R.RestartError, match='not a Fortran sequential file'):
+        R.convert_v1_0_directory(old, str(tmp_path / 'new'))
+    with pytest.raises(R.RestartError, match='would overwrite'):
+        R.convert_v1_0_directory(old, old)
 
 
 def test_amplitude_file_round_trip_and_errors(tmp_path):
