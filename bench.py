@@ -40,6 +40,9 @@ METRIC = 'CCSD omega evals/s & time-to-solution, mixed vs FP64 GPU at |ΔE|≤1e
 # dense matrix-core peaks; split-bf16 counts its three bf16 products per term
 PEAKS = {'fp64': 78.6e12, 'fp32': 157.3e12, 'bf16x3': 2.5e15 / 3}
 WORKLOAD = dict(n_o=48, n_v=400, n_J=1600, seed=0)
+# the 1-thread CPU residual: the dimensions of (H2O)_8 / cc-pVDZ at Cholesky 1e-8
+# (o = 40, v = 152, n_J = 2541), synthetic inputs of that shape
+CPU_SINGLE_THREAD = dict(n_o=40, n_v=152, n_J=2541, seed=0)
 HARTREE_TO_EV = 27.21138602          # CODATA 2014, the factor eT prints
 _PN = {ccsd.BF16X3: 'bf16x3', ccsd.FP32: 'fp32', ccsd.FP64: 'fp64'}
 WORKLOAD_NAME = 'omega_o48_v400_J1600_seed0'
@@ -65,37 +68,82 @@ def build_system(n_o, n_v, n_J, seed):
     return L, h, eps, t2
 
 
-def cpu_baseline(n_o, n_v, n_J, cols=4800):
-    """Oracle FP64 product of the C2/D2 (ov) x (ov) shape on a column slice, host BLAS.
-
-    CPU omega/s = measured FLOP rate / F_omega: every eT term is credited with
-    the rate of its largest DGEMM, so this bounds eT's CPU throughput from above.
-    """
+def cpu_omega_seconds(n_o, n_v, n_J, seed, threads, ladder_batch=80):
+    """One full residual of the oracle (eT's factorisation restated on the host:
+    numpy products on the host BLAS, eT's batched ladder with OpenMP gathers) on the
+    synthetic workload, `threads` BLAS/OpenMP threads, after a small warm-up."""
+    from threadpoolctl import threadpool_limits
     from oracle import ccsd as O
-    ov = n_o * n_v
-    rng = np.random.default_rng(1)
-    A = rng.standard_normal((ov, ov))
-    B = rng.standard_normal((ov, cols))
-    O.symmetric_product(1.0, A[:512], B[:, :64])          # warm the BLAS thread pool
-    t0 = time.perf_counter()
-    O.symmetric_product(-0.5, A, B)
-    dt = time.perf_counter() - t0
-    rate = 2.0 * ov * ov * cols / dt
-    try:
-        from threadpoolctl import threadpool_info
-        threads = max(int(p.get('num_threads', 1)) for p in threadpool_info()) if threadpool_info() else 1
-    except Exception:
-        threads = int(os.environ.get('OMP_NUM_THREADS', os.cpu_count() or 1))
-    f_omega = flopcount.omega_flops(n_o, n_v, n_J)
-    return {
-        'value': rate / f_omega,
+    with threadpool_limits(limits=threads):
+        w = syn.system(1, 6, 30, 60, t2_sigma=1e-3, t1_sigma=0.0, J_decay=20.0, L_scale=0.5)
+        Lt = O.t1_cholesky(w['L'], w['t1'], 6)
+        O.omega(w['L'], Lt, O.fock_t1(w['h'], Lt, w['t1'], 6), w['t1'], w['t2'], 6, ladder_batch=16)
+        L = syn.cholesky_vectors(seed, n_o + n_v, n_J)
+        eps = syn.orbital_energies(seed, n_o, n_v)
+        h = syn.canonical_core_hamiltonian(L, eps, n_o)
+        t2 = syn.doubles_amplitudes(seed, n_o, n_v, sigma=1e-3)
+        t1 = np.zeros((n_v, n_o))
+        t0 = time.perf_counter()
+        Lt = O.t1_cholesky(L, t1, n_o)
+        F = O.fock_t1(h, Lt, t1, n_o)
+        om = O.omega(L, Lt, F, t1, t2, n_o, ladder_batch=ladder_batch)
+        dt = time.perf_counter() - t0
+    return dt, float(np.linalg.norm(om))
+
+
+def cpu_baseline_worker(args):
+    """Child process of the bench (no GPU): the timed CPU residuals, as one JSON line."""
+    import resource
+    w = WORKLOAD
+    threads = int(args.cpu_threads)
+    dt, norm = cpu_omega_seconds(w['n_o'], w['n_v'], w['n_J'], w['seed'], threads)
+    out = {'seconds': dt, 'omega_norm': norm, 'threads': threads}
+    if not args.no_cpu_single:
+        c2 = CPU_SINGLE_THREAD
+        dt1, _ = cpu_omega_seconds(c2['n_o'], c2['n_v'], c2['n_J'], c2['seed'], 1)
+        out['single_thread'] = dict(c2, seconds=dt1, omega_per_s=1.0 / dt1)
+    out['peak_rss_gb'] = resource.getrusage(resource.RUSAGE_SELF).ru_maxrss / 1e6
+    print(json.dumps(out), flush=True)
+
+
+def start_cpu_baseline(args):
+    """Starts the CPU residual timing in a child process before the GPU is touched
+    (it runs beside the GPU legs on all host cores but one)."""
+    import subprocess
+    cores = min(os.cpu_count() or 1, int(os.environ.get('OMP_NUM_THREADS', os.cpu_count() or 1)))
+    threads = max(1, cores - 1)
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OPENBLAS_NUM_THREADS=str(threads))
+    cmd = [sys.executable, os.path.abspath(__file__), '--cpu-baseline-worker', '--cpu-threads', str(threads)]
+    if args.no_cpu_single:
+        cmd.append('--no-cpu-single')
+    return subprocess.Popen(cmd, stdout=subprocess.PIPE, env=env, text=True), threads
+
+
+def finish_cpu_baseline(proc, threads):
+    outs, _ = proc.communicate()
+    if proc.returncode != 0:
+        log('CPU baseline failed with status %d' % proc.returncode)
+        return None
+    r = json.loads(outs.strip().splitlines()[-1])
+    w = WORKLOAD
+    f_omega = flopcount.omega_flops(w['n_o'], w['n_v'], w['n_J'])
+    out = {
+        'value': 1.0 / r['seconds'],
         'unit': 'omega evals/s',
         'cores': threads,
         'kind': 'port',
-        'sample': ('oracle FP64 C2/D2 product (ov=%d)x(ov) times (ov)x%d on host BLAS, '
-                   '%.2e FLOP in %.2f s (%.1f GFLOP/s); omega/s = rate / F_omega(%.3e)'
-                   % (ov, cols, 2.0 * ov * ov * cols, dt, rate / 1e9, f_omega)),
+        'sample': ('one full oracle residual (eT factorisation, batched ladder) at o=%d v=%d J=%d seed %d '
+                   'after a warm-up: %.1f s on %d threads (host BLAS + OpenMP), %.0f GFLOP/s over F_omega; '
+                   'ran beside the GPU legs' % (w['n_o'], w['n_v'], w['n_J'], w['seed'], r['seconds'],
+                                                threads, f_omega / r['seconds'] / 1e9)),
+        'peak_rss_gb': r.get('peak_rss_gb'),
     }
+    if 'single_thread' in r:
+        s1 = r['single_thread']
+        out['single_thread'] = {'workload': 'omega_o%d_v%d_J%d_seed%d' % (s1['n_o'], s1['n_v'], s1['n_J'], s1['seed']),
+                                'omega_evals_per_s': s1['omega_per_s'], 'seconds': s1['seconds'], 'cores': 1}
+    log('CPU baseline: %s' % json.dumps(out))
+    return out
 
 
 def jacobian_throughput(wf):
@@ -213,8 +261,10 @@ def eom_time_to_solution(r, n_states=12, threshold=1e-5):
                     sv = solvers.MixedPrecisionDavidson(wf, n_states, side, residual_threshold=threshold)
                     w = sv.run()
                     names = {ccsd.FP64: 'fp64', ccsd.BF16X3: 'bf16x3', ccsd.FP32: 'fp32'}
-                    iters = {names[p]: k for p, k, _, _ in sv.phases}
-                    phases = [{'precision': names[p], 'iterations': k, 'stalled': st, 'max_residual': rr}
+                    iters = {}
+                    for p, k, _, _ in sv.phases:
+                        iters[names.get(p, p)] = iters.get(names.get(p, p), 0) + k
+                    phases = [{'precision': names.get(p, p), 'iterations': k, 'stalled': st, 'max_residual': rr}
                               for p, k, st, rr in sv.phases]
                 wf.synchronize()
                 dt = time.perf_counter() - t0
@@ -222,6 +272,7 @@ def eom_time_to_solution(r, n_states=12, threshold=1e-5):
                 if name == 'mixed':
                     res[name]['phases'] = phases
                     res[name]['noise'] = {names[p]: v for p, v in sv.noise.items()}
+                    res[name]['fp64_checks_max_residual'] = [float(c.max()) for c in sv.checks]
                 log('EOM %s %s: %.2f s, %s' % (side, name, dt, iters))
             res['mixed']['max_abs_err_vs_fp64_gpu_ev'] = float(
                 np.abs(np.array(res['mixed']['energies_ev']) - np.array(res['fp64']['energies_ev'])).max())
@@ -365,9 +416,16 @@ def main():
                          '(mixed precision; a few minutes)')
     ap.add_argument('--fp64-reference', action='store_true',
                     help='with --lambda-dipole: also the FP64 ground state and Lambda, and the differences')
+    ap.add_argument('--cpu-baseline-worker', action='store_true', help=argparse.SUPPRESS)
+    ap.add_argument('--cpu-threads', type=int, default=1, help=argparse.SUPPRESS)
+    ap.add_argument('--no-cpu-single', action='store_true',
+                    help='skip the 1-thread CPU residual at the (H2O)8 dimensions')
     ap.add_argument('--copy-calibration', action='store_true',
                     help='only run three 1 GiB device-to-device copies (HBM counter calibration)')
     args = ap.parse_args()
+    if args.cpu_baseline_worker:
+        cpu_baseline_worker(args)
+        return
     if args.copy_calibration:
         copy_calibration()
         return
@@ -383,6 +441,7 @@ def main():
               'library does not implement.' % args.gpus, file=sys.stderr)
         sys.exit(2)
 
+    cpu_job = None if args.no_cpu else start_cpu_baseline(args)
     w = WORKLOAD
     n_o, n_v, n_J = w['n_o'], w['n_v'], w['n_J']
     L, h, eps, t2 = build_system(n_o, n_v, n_J, w['seed'])
@@ -469,9 +528,7 @@ def main():
         extra['eom_time_to_solution'] = eom_time_to_solution(octamer)
         del octamer
 
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(n_o, n_v, n_J)
+    cpu = finish_cpu_baseline(*cpu_job) if cpu_job else None
 
     out = {
         'metric': METRIC,

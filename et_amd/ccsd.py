@@ -72,6 +72,7 @@ _SIGNATURES = {
                                 _i64, ctypes.c_int, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p,
                                 ctypes.POINTER(_i64)],
     'etg_test_reduced_eigen': [_i64, _c_dbl_p, _i64, _c_dbl_p, _c_dbl_p, _c_dbl_p],
+    'etg_set_davidson_start_precision': [ctypes.c_void_p, ctypes.c_int],
     'etg_diis_initialize': [ctypes.c_void_p, _i64],
     'etg_quasi_newton_update': [ctypes.c_void_p],
     'etg_diis_update': [ctypes.c_void_p],
@@ -458,6 +459,10 @@ class CCSD:
         self.davidson_stalled = its.value < 0
         hist = hist[:abs(its.value)]
         return (e, hist, X) if vectors else (e, hist)
+
+    def set_davidson_start_precision(self, precision=None):
+        """Precision of the start block's transformation in davidson() (None: the solve's)."""
+        _check(self._lib.etg_set_davidson_start_precision(self._h, -1 if precision is None else int(precision)))
 
     def linear_davidson(self, rhs, side='right', start=None, residual_threshold=1e-6, max_iterations=100,
                         max_dim=50, precision=FP64):

@@ -200,6 +200,13 @@ int etg_davidson(etg_handle* h, int side, int64_t n_states, double residual_thre
   });
 }
 
+int etg_set_davidson_start_precision(etg_handle* h, int precision) {
+  return guarded([&] {
+    if (precision >= 0) (void)prec_of(precision);
+    wf_of(h).set_davidson_start_precision(precision < 0 ? -1 : precision);
+  });
+}
+
 int etg_linear_davidson(etg_handle* h, int side, const double* rhs, const double* start_vector,
                         double residual_threshold, int64_t max_iterations, int64_t max_dim, int precision,
                         double* solution, double* residual_norms, int64_t* iterations) {

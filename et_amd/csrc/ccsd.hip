@@ -833,6 +833,8 @@ void CCSD::prepare_for_jacobian() {
   program_omega(*tape, Prec::F64);
   build_ladder_t1_intermediate();       // t and L_t1 of this record
   synchronize();
+  // keep only the primal values the passes read and cannot rebuild cheaply
+  tape->finalize();
   // the record holds its own intermediates: the direct program's (ov)^2 slots
   // are given back (and taken again by the next direct residual)
   for (DeviceBuffer& w : work_) w.release();

@@ -143,6 +143,12 @@ class CCSD {
                               double energy_threshold, int64_t max_iterations, int64_t diis_dimension,
                               Prec prec, const double* start, double* energies, double* vectors,
                               double* residual_norms, double* energy_history);
+  // precision of the first transformation of a Davidson solve (its start vectors);
+  // the expansion vectors use the solve's own precision.  With FP64 here and a
+  // reduced precision for the solve, every Ritz vector is its FP64-transformed start
+  // block plus reduced-precision corrections of the size of the start residuals.
+  // start_precision < 0: the solve's precision.
+  void set_davidson_start_precision(int start_precision) { dav_start_prec_ = start_precision; }
   // lowest n_want eigenpairs of the k x k matrix G (device, ld ldg): wr, wi (host,
   // k, ascending real part), X (device, k x n_want, ld ldx)
   void reduced_eigen(const double* G_dev, int64_t k, int64_t ldg, int64_t n_want, double* wr,
@@ -228,6 +234,7 @@ class CCSD {
 
   // Davidson space (V, AV), Ritz block, reduced matrices
   DeviceBuffer dav_V_, dav_AV_, dav_Y_, dav_small_, red_eig_;
+  int dav_start_prec_ = -1;                       // < 0: the solve's precision
   DeviceBuffer cvs_core_;                         // n_o flags: core occupied MOs
   bool cvs_ = false, remove_core_ = false, ionization_ = false;
 

@@ -372,9 +372,10 @@ int64_t CCSD::davidson(int side, int64_t n_states, double threshold, int64_t max
   std::vector<double> wr(max_dim), wi(max_dim), norms(s1), hcoef(3 * s1);
   double best = HUGE_VAL;                 // reduced precision: smallest largest-residual so far
   int64_t since_best = 0;
+  const Prec start_prec = dav_start_prec_ < 0 ? prec : static_cast<Prec>(dav_start_prec_);
   for (int64_t it = 1; it <= max_iterations; ++it) {
-    // transforms of the new trial vectors
-    transform_batch(side, m, V + k * n, AV + k * n, prec);
+    // transforms of the new trial vectors (the start block in its own precision)
+    transform_batch(side, m, V + k * n, AV + k * n, it == 1 ? start_prec : prec);
     project_cvs(AV + k * n, m);              // eigen_davidson_solver_class.F90:187
     // G = V^T AV: new columns for all rows, new rows for the old columns
     const int64_t kn = k + m;

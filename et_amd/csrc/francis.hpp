@@ -146,20 +146,24 @@ ETG_HD int hessenberg_eigenvalues(const Team& team, double* H, int ldh, int k, d
     // trailing entries instead.  They break the cycles of the standard shift
     // (a cyclic permutation matrix never deflates with it), and being unequal
     // they do not tie on eigenvalues placed symmetrically about h.
-    double s, p;
+    // First column of (H - s1 I)(H - s2 I) on the block, written with differences
+    // of diagonal entries so that it does not cancel when the block is close to
+    // a multiple of I (a converging multiple eigenvalue): with the trailing
+    // block [[a, b], [c, d]], (h00 - s1)(h00 - s2) = (h00 - a)(h00 - d) - b c.
+    const double h00 = ETG_H(lo, lo), h10 = ETG_H(lo + 1, lo), h01 = ETG_H(lo, lo + 1);
+    const double h11 = ETG_H(lo + 1, lo + 1);
+    double x, y;
     if (sweeps % 10 == 0) {
       const double h = ETG_H(hi, hi);
       const double xi = std::fabs(ETG_H(hi, hi - 1)) + std::fabs(ETG_H(hi - 1, hi - 2));
-      s = 2.0 * h + 0.5 * xi;
-      p = (h + xi) * (h - 0.5 * xi);
+      const double e = h00 - h;                           // shifts h + xi, h - xi / 2
+      x = (e - xi) * (e + 0.5 * xi) + h01 * h10;
+      y = h10 * (e + (h11 - h) - 0.5 * xi);
     } else {
-      s = ETG_H(hi - 1, hi - 1) + ETG_H(hi, hi);
-      p = ETG_H(hi - 1, hi - 1) * ETG_H(hi, hi) - ETG_H(hi - 1, hi) * ETG_H(hi, hi - 1);
+      const double a = ETG_H(hi - 1, hi - 1), b = ETG_H(hi - 1, hi), c = ETG_H(hi, hi - 1), d = ETG_H(hi, hi);
+      x = (h00 - a) * (h00 - d) - b * c + h01 * h10;
+      y = h10 * ((h00 - a) + (h11 - d));
     }
-    // first column of (H - s1 I)(H - s2 I) = H^2 - s H + p I on the block
-    const double h00 = ETG_H(lo, lo), h10 = ETG_H(lo + 1, lo), h01 = ETG_H(lo, lo + 1);
-    double x = h00 * h00 + h01 * h10 - s * h00 + p;
-    double y = h10 * (h00 + ETG_H(lo + 1, lo + 1) - s);
     double z = h10 * ETG_H(lo + 2, lo + 1);
     const double sz = std::fabs(x) + std::fabs(y) + std::fabs(z);
     if (sz != 0.0) {

@@ -20,8 +20,8 @@
 // The dense AO matrix (pq|rs) (n^2 x n^2) is built in HBM by one wavefront per
 // unique shell quartet; the pivoted Cholesky decomposition runs on it with the
 // largest remaining diagonal as pivot (ties to the lowest index), each new
-// vector one matrix-vector product against the earlier ones; RHF uses the
-// Cholesky vectors for J and K.
+// vector one matrix-vector product against the earlier ones; RHF builds J and
+// K from the dense integrals, as eT's Hartree-Fock does from its AO integrals.
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
@@ -621,6 +621,50 @@ __global__ void symmetrise_kernel(double* __restrict__ L, int64_t n, int64_t nve
 
 unsigned grid_of(int64_t n) { return unsigned(std::min<int64_t>((n + kBlk - 1) / kBlk, 1 << 16)); }
 
+// ---------------------------------------------------------------------------
+// Coulomb and exchange matrices from the dense AO integrals (eT's Hartree-Fock
+// builds its Fock matrix from the AO integrals, not from Cholesky vectors).
+// One workgroup per output element, a fixed summation order per thread and a
+// fixed tree across the workgroup: reproducible bit for bit.  Both read the
+// n^4 integrals once per build (HBM-bound).
+// ---------------------------------------------------------------------------
+
+__device__ double workgroup_sum(double v) {
+  __shared__ double part[kBlk];
+  part[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = kBlk / 2; s > 0; s >>= 1) {
+    if (int(threadIdx.x) < s) part[threadIdx.x] += part[threadIdx.x + s];
+    __syncthreads();
+  }
+  return part[0];
+}
+
+// J_pq = sum_rs (pq|rs) D_rs: column pq of the symmetric n^2 x n^2 matrix against D
+__global__ __launch_bounds__(kBlk) void coulomb_kernel(const double* __restrict__ G, const double* __restrict__ D,
+                                                       int64_t nn, double alpha, double* __restrict__ out) {
+  const int64_t pq = blockIdx.x;
+  const double* col = G + pq * nn;
+  double acc = 0.0;
+  for (int64_t rs = threadIdx.x; rs < nn; rs += kBlk) acc += col[rs] * D[rs];
+  const double sum = workgroup_sum(acc);
+  if (threadIdx.x == 0) out[pq] += alpha * sum;
+}
+
+// K_pq = sum_rs (pr|qs) D_rs = sum_{r,s} (rp|sq) D_rs: for each s the n integrals
+// (rp|sq), r = 0..n-1, are contiguous in the matrix
+__global__ __launch_bounds__(kBlk) void exchange_kernel(const double* __restrict__ G, const double* __restrict__ D,
+                                                        int64_t n, double alpha, double* __restrict__ out) {
+  const int64_t pq = blockIdx.x, p = pq % n, q = pq / n, nn = n * n;
+  double acc = 0.0;
+  for (int64_t rs = threadIdx.x; rs < nn; rs += kBlk) {
+    const int64_t r = rs % n, s = rs / n;
+    acc += G[r + n * p + nn * (s + n * q)] * D[rs];
+  }
+  const double sum = workgroup_sum(acc);
+  if (threadIdx.x == 0) out[pq] += alpha * sum;
+}
+
 }  // namespace fe
 
 // ---------------------------------------------------------------------------
@@ -804,6 +848,7 @@ __global__ void fock_kernel(const double* H, const double* J, const double* K, d
 void Molecule::scf(const double* S_host, const double* H_host, double conv, int64_t max_iterations) {
   using namespace fe;
   const int64_t n = n_, nn = n * n, o = n_occ_, J = n_J_;
+  const bool exact_jk = G_.bytes() != 0;
   rocblas_handle rh;
   check_rocblas(rocblas_create_handle(&rh), "rocblas_create_handle");
   struct Guard {
@@ -844,11 +889,19 @@ void Molecule::scf(const double* S_host, const double* H_host, double conv, int6
     // D = C_occ C_occ^T
     etg::gemm(stream_, 'N', 'T', n, n, o, 1.0, Cd.as<double>(), n, Cd.as<double>(), n, 0.0, D.as<double>(), n,
               Prec::F64, &ws);
-    // J = sum_J L^J (L^J . D)
-    etg::gemm(stream_, 'T', 'N', J, 1, nn, 1.0, Lao, nn, D.as<double>(), nn, 0.0, u.as<double>(), J, Prec::F64, &ws);
-    etg::gemm(stream_, 'N', 'N', nn, 1, J, 1.0, Lao, nn, u.as<double>(), J, 0.0, Jm.as<double>(), nn, Prec::F64, &ws);
-    // K = sum_J (L^J C_o)(L^J C_o)^T
-    {
+    if (exact_jk) {
+      // J_pq = sum_rs (pq|rs) D_rs,  K_pq = sum_rs (pr|qs) D_rs  from the AO integrals
+      fill(stream_, Jm.as<double>(), nn, 0.0);
+      fill(stream_, Km.as<double>(), nn, 0.0);
+      coulomb_kernel<<<unsigned(nn), kBlk, 0, stream_>>>(G_.as<double>(), D.as<double>(), nn, 1.0, Jm.as<double>());
+      ETG_LAUNCH("scf_coulomb");
+      exchange_kernel<<<unsigned(nn), kBlk, 0, stream_>>>(G_.as<double>(), D.as<double>(), n, 1.0, Km.as<double>());
+      ETG_LAUNCH("scf_exchange");
+    } else {
+      // J = sum_J L^J (L^J . D)
+      etg::gemm(stream_, 'T', 'N', J, 1, nn, 1.0, Lao, nn, D.as<double>(), nn, 0.0, u.as<double>(), J, Prec::F64, &ws);
+      etg::gemm(stream_, 'N', 'N', nn, 1, J, 1.0, Lao, nn, u.as<double>(), J, 0.0, Jm.as<double>(), nn, Prec::F64, &ws);
+      // K = sum_J (L^J C_o)(L^J C_o)^T
       Gemm g = make_gemm('N', 'N', n, o, n, 1.0, Lao, n, Cd.as<double>(), n, 0.0, Y.as<double>(), n);
       g.batch = J;
       g.strideA = nn;
@@ -949,9 +1002,9 @@ void Molecule::run(double cholesky_threshold, double scf_threshold, int64_t max_
   for (int64_t k = 0; k < nn; ++k) H[size_t(k)] = T[size_t(k)] + V[size_t(k)];
   build_eri();
   decompose(cholesky_threshold);
-  G_.release();                         // the Cholesky vectors replace the dense matrix
-  scf(S.data(), H.data(), scf_threshold, max_scf_iterations);
+  scf(S.data(), H.data(), scf_threshold, max_scf_iterations);   // J and K from the dense integrals
   mo_transform();
+  G_.release();                         // the MO Cholesky vectors replace the dense matrix
   done_ = true;
 }
 
@@ -1002,21 +1055,16 @@ void Molecule::mo_transform() {
   ETG_HIP(hipMemcpyAsync(h.as<double>(), h_ao_.data(), sizeof(double) * nn, hipMemcpyHostToDevice, stream_));
   if (n_frozen_core_ > 0) {
     // h += 2 J[D_c] - K[D_c],  D_c = C_c C_c^T over the frozen MOs (construct_mo_fock_fc_term_hf,
-    // frozen_orbital_hf.F90:547-600), from the AO Cholesky vectors as in the RHF Fock build
+    // frozen_orbital_hf.F90:547-600), from the AO integrals as in the RHF Fock build
     const int64_t c = n_frozen_core_;
-    DeviceBuffer D(sizeof(double) * nn), u(sizeof(double) * std::max<int64_t>(1, J)),
-        Y(sizeof(double) * n * c * std::max<int64_t>(1, J));
+    DeviceBuffer D(sizeof(double) * nn);
     etg::gemm(stream_, 'N', 'T', n, n, c, 1.0, C.as<double>(), n, C.as<double>(), n, 0.0, D.as<double>(), n,
               Prec::F64, &ws);
-    etg::gemm(stream_, 'T', 'N', J, 1, nn, 1.0, Lao_.as<double>(), nn, D.as<double>(), nn, 0.0, u.as<double>(), J,
-              Prec::F64, &ws);
-    etg::gemm(stream_, 'N', 'N', nn, 1, J, 2.0, Lao_.as<double>(), nn, u.as<double>(), J, 1.0, h.as<double>(), nn,
-              Prec::F64, &ws);
-    Gemm gy = make_gemm('N', 'N', n, c, n, 1.0, Lao_.as<double>(), n, C.as<double>(), n, 0.0, Y.as<double>(), n);
-    gy.batch = J; gy.strideA = nn; gy.strideB = 0; gy.strideC = n * c;
-    etg::gemm(stream_, gy, Prec::F64, &ws);
-    etg::gemm(stream_, 'N', 'T', n, n, c * J, -1.0, Y.as<double>(), n, Y.as<double>(), n, 1.0, h.as<double>(), n,
-              Prec::F64, &ws);
+    if (G_.bytes() == 0) fail("frozen core: the AO integrals are no longer held");
+    coulomb_kernel<<<unsigned(nn), kBlk, 0, stream_>>>(G_.as<double>(), D.as<double>(), nn, 2.0, h.as<double>());
+    ETG_LAUNCH("frozen_core_coulomb");
+    exchange_kernel<<<unsigned(nn), kBlk, 0, stream_>>>(G_.as<double>(), D.as<double>(), n, -1.0, h.as<double>());
+    ETG_LAUNCH("frozen_core_exchange");
   }
   etg::gemm(stream_, 'N', 'N', n, n, n, 1.0, h.as<double>(), n, C.as<double>(), n, 0.0, t.as<double>(), n, Prec::F64, &ws);
   etg::gemm(stream_, 'T', 'N', n, n, n, 1.0, C.as<double>(), n, t.as<double>(), n, 0.0, hm.as<double>(), n, Prec::F64, &ws);

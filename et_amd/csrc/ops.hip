@@ -102,6 +102,9 @@ void Tape::clear() {
   named_.clear();
   slots_.clear();
   slot_elems_.clear();
+  rslots_.clear();
+  op_active_.clear();
+  finalized_ = false;
   planned_ops_ = size_t(-1);
 }
 
@@ -115,7 +118,7 @@ int Tape::find(const double* p) const {
 
 double* Tape::deriv(const double* p) {
   Buf& b = *bufs_[size_t(find(p))];
-  if (b.constant) return nullptr;
+  if (b.constant || !b.active) return nullptr;
   const size_t need = size_t(std::max<int64_t>(b.elems, 1)) * sizeof(double);
   if (b.intermediate) {
     if (b.slot < 0 || slots_[size_t(b.slot)].bytes() < need) fail("tape: derivatives not initialised");
@@ -138,7 +141,7 @@ void Tape::plan_derivatives() {
   auto touch = [&](const double* p, int i) {
     if (p == nullptr) return;
     Buf& b = *bufs_[size_t(find(p))];
-    if (b.constant || !b.intermediate) return;
+    if (b.constant || !b.intermediate || !b.active) return;
     if (b.first < 0) b.first = i;
     b.last = i;
   };
@@ -157,7 +160,7 @@ void Tape::plan_derivatives() {
   }
   std::vector<int> order;
   for (size_t k = 0; k < bufs_.size(); ++k)
-    if (bufs_[k]->intermediate && bufs_[k]->first >= 0) order.push_back(int(k));
+    if (bufs_[k]->intermediate && bufs_[k]->active && bufs_[k]->first >= 0) order.push_back(int(k));
   std::sort(order.begin(), order.end(), [&](int a, int b) { return bufs_[size_t(a)]->first < bufs_[size_t(b)]->first; });
   slots_.clear();
   slot_elems_.clear();
@@ -197,7 +200,7 @@ int64_t Tape::derivative_elems() {
   plan_derivatives();
   int64_t e = 0;
   for (const auto& bp : bufs_)
-    if (!bp->constant && !bp->intermediate) e += std::max<int64_t>(bp->elems, 1);
+    if (!bp->constant && !bp->intermediate && bp->active) e += std::max<int64_t>(bp->elems, 1);
   for (int64_t s : slot_elems_) e += s;
   return e;
 }
@@ -205,13 +208,14 @@ int64_t Tape::derivative_elems() {
 int64_t Tape::unshared_derivative_elems() const {
   int64_t e = 0;
   for (const auto& bp : bufs_)
-    if (!bp->constant) e += std::max<int64_t>(bp->elems, 1);
+    if (!bp->constant && bp->active) e += std::max<int64_t>(bp->elems, 1);
   return e;
 }
 
 int64_t Tape::record_elems() const {
   int64_t e = 0;
   for (const auto& kv : named_) e += int64_t(kv.second.bytes() / sizeof(double));
+  for (const auto& r : rslots_) e += int64_t(r.mem.bytes() / sizeof(double));
   return e;
 }
 
@@ -220,7 +224,7 @@ bool Tape::reserve_derivatives(int64_t nb) {
   bool moved = false;
   for (auto& bp : bufs_) {
     Buf& b = *bp;
-    if (b.constant || b.intermediate) continue;
+    if (b.constant || b.intermediate || !b.active) continue;
     moved |= b.d.resize(size_t(nb) * size_t(std::max<int64_t>(b.elems, 1)) * sizeof(double));
   }
   for (size_t s = 0; s < slots_.size(); ++s)
@@ -232,7 +236,7 @@ void Tape::zero_derivatives(int64_t nb) {
   reserve_derivatives(nb);
   for (auto& bp : bufs_) {
     Buf& b = *bp;
-    if (b.constant || b.intermediate) continue;
+    if (b.constant || b.intermediate || !b.active) continue;
     fill(wf_.stream(), b.d.as<double>(), nb * std::max<int64_t>(b.elems, 1), 0.0);
   }
 }
@@ -382,6 +386,246 @@ void Tape::ladder(const double* t2, const double* LvvT, double* om2, Prec prec) 
 }
 
 // ---------------------------------------------------------------------------
+// Tape: what the passes need of the record
+// ---------------------------------------------------------------------------
+
+void Tape::op_io(const Op& op, const double* in[3], const double* out[2]) {
+  in[0] = in[1] = in[2] = nullptr;
+  out[0] = out[1] = nullptr;
+  switch (op.kind) {
+    case GEMM:
+      in[0] = op.g.A;
+      in[1] = op.g.B;
+      if (op.g.beta != 0.0) in[2] = op.g.C;
+      out[0] = op.g.C;
+      break;
+    case SORT4:
+    case PACKIN:
+    case COPY_SUB:
+      in[0] = op.x;
+      if (op.beta != 0.0) in[1] = op.y;
+      out[0] = op.y;
+      break;
+    case SQUAREUP:
+    case SYMSUM:
+    case TRACE_OO:
+    case FOCK_OV_VEC:
+      in[0] = op.x;
+      out[0] = op.y;
+      break;
+    case SYMPACK:                       // accumulates into the packed output
+      in[0] = op.x;
+      in[1] = op.y;
+      out[0] = op.y;
+      break;
+    case FILL0:
+      out[0] = op.y;
+      break;
+    case T1_XY:
+      in[0] = op.x;
+      out[0] = op.y;
+      out[1] = op.y2;
+      break;
+    case LADDER:
+      in[0] = op.x;
+      in[1] = op.x2;
+      in[2] = op.y;
+      out[0] = op.y;
+      break;
+  }
+}
+
+void Tape::finalize(int rebuild_slots) {
+  const int nops = int(ops_.size());
+  const int nbuf = int(bufs_.size());
+  // 1. derivatives: the passes' inputs (amplitudes, h) carry one; an output
+  //    carries one if any operation that writes it reads one that does
+  std::vector<char> active(size_t(nbuf), 0);
+  for (int k = 0; k < nbuf; ++k) {
+    const Buf& b = *bufs_[size_t(k)];
+    active[size_t(k)] = !b.constant && !b.intermediate;
+  }
+  for (auto& bp : bufs_) {
+    bp->recipe.clear();
+    bp->last_write = -1;
+    bp->rebuild = false;
+  }
+  op_active_.assign(size_t(nops), 0);
+  std::vector<std::vector<int>> reads(static_cast<size_t>(nops)), writes(static_cast<size_t>(nops));
+  for (int i = 0; i < nops; ++i) {
+    const double* in[3];
+    const double* out[2];
+    op_io(ops_[size_t(i)], in, out);
+    bool live = false;
+    for (const double* p : in)
+      if (p) {
+        const int k = find(p);
+        reads[size_t(i)].push_back(k);
+        live |= bool(active[size_t(k)]);
+      }
+    for (const double* p : out)
+      if (p) {
+        const int k = find(p);
+        writes[size_t(i)].push_back(k);
+        if (live) active[size_t(k)] = 1;
+        bufs_[size_t(k)]->recipe.push_back(i);
+        bufs_[size_t(k)]->last_write = i;
+      }
+  }
+  for (int k = 0; k < nbuf; ++k) bufs_[size_t(k)]->active = bufs_[size_t(k)]->constant ? false : bool(active[size_t(k)]);
+  for (int i = 0; i < nops; ++i)
+    for (int k : writes[size_t(i)]) op_active_[size_t(i)] |= char(active[size_t(k)]);
+  // 2. primal values the passes read: a product's operand pairs with the other
+  //    operand's derivative (tangent and adjoint alike); the ladder reads its
+  //    amplitudes and integrals
+  std::vector<char> needed(size_t(nbuf), 0);
+  std::vector<int> first_use(size_t(nbuf), nops);
+  for (int i = 0; i < nops; ++i) {
+    const Op& op = ops_[size_t(i)];
+    if (!op_active_[size_t(i)]) continue;
+    if (op.kind == GEMM) {
+      const int a = find(op.g.A), b = find(op.g.B);
+      if (bufs_[size_t(b)]->active) { needed[size_t(a)] = 1; first_use[size_t(a)] = std::min(first_use[size_t(a)], i); }
+      if (bufs_[size_t(a)]->active) { needed[size_t(b)] = 1; first_use[size_t(b)] = std::min(first_use[size_t(b)], i); }
+    } else if (op.kind == LADDER) {
+      for (const double* p : {op.x, op.x2}) {
+        const int k = find(p);
+        needed[size_t(k)] = 1;
+        first_use[size_t(k)] = std::min(first_use[size_t(k)], i);
+      }
+    }
+  }
+  // the passes read each primal operand as its last write left it
+  for (int k = 0; k < nbuf; ++k)
+    if (needed[size_t(k)] && bufs_[size_t(k)]->last_write > first_use[size_t(k)])
+      fail("tape: a product reads a buffer that is written again later: " + bufs_[size_t(k)]->name);
+  // 3. rebuilt instead of stored: an intermediate written only by bandwidth
+  //    operations whose inputs stay in HBM or are rebuilt themselves, each input
+  //    final when it is read (taken in the order of the buffers' first writes)
+  auto bandwidth_op = [](Kind kd) {
+    return kd == SORT4 || kd == SQUAREUP || kd == COPY_SUB || kd == FILL0 || kd == SYMSUM || kd == PACKIN ||
+           kd == SYMPACK;
+  };
+  std::vector<int> order;
+  for (int k = 0; k < nbuf; ++k)
+    if (bufs_[size_t(k)]->intermediate && !bufs_[size_t(k)]->recipe.empty() && named_.count(bufs_[size_t(k)]->name))
+      order.push_back(k);
+  std::sort(order.begin(), order.end(),
+            [&](int a, int b) { return bufs_[size_t(a)]->recipe[0] < bufs_[size_t(b)]->recipe[0]; });
+  int64_t rebuild_elems = 0;
+  for (int k : order) {
+    Buf& b = *bufs_[size_t(k)];
+    bool ok = true;
+    for (int i : b.recipe) {
+      if (!bandwidth_op(ops_[size_t(i)].kind)) { ok = false; break; }
+      for (int q : reads[size_t(i)]) {
+        if (q == k) continue;
+        const Buf& in = *bufs_[size_t(q)];
+        const bool stays = !named_.count(in.name) || in.rebuild;   // registered outside the tape, or rebuilt
+        if (!stays || in.last_write >= i) { ok = false; break; }
+      }
+      if (!ok) break;
+    }
+    b.rebuild = ok;
+    if (ok) rebuild_elems = std::max(rebuild_elems, std::max<int64_t>(b.elems, 1));
+  }
+  // 4. keep what is read and cannot be rebuilt; release the rest
+  for (int k : order) {
+    Buf& b = *bufs_[size_t(k)];
+    if (needed[size_t(k)] && !b.rebuild) continue;
+    named_[b.name].release();
+  }
+  for (int k = 0; k < nbuf; ++k) {           // intermediates with no write (never read as primal)
+    Buf& b = *bufs_[size_t(k)];
+    if (b.intermediate && b.recipe.empty() && named_.count(b.name) && !needed[size_t(k)]) named_[b.name].release();
+  }
+  rslots_.clear();
+  rslots_.resize(rebuild_elems > 0 ? size_t(std::max(2, rebuild_slots)) : 0);
+  for (auto& r : rslots_) r.mem.resize(sizeof(double) * size_t(rebuild_elems));
+  planned_ops_ = size_t(-1);
+  finalized_ = true;
+}
+
+void Tape::reset_rebuilt() {
+  for (auto& r : rslots_) {
+    r.owner = -1;
+    r.pins = 0;
+    r.stamp = 0;
+  }
+  rclock_ = 0;
+}
+
+void Tape::unpin_all() {
+  for (auto& r : rslots_) r.pins = 0;
+}
+
+const double* Tape::primal(const double* p) {
+  if (!finalized_) return p;
+  const int k = find(p);
+  const Buf& b = *bufs_[size_t(k)];
+  if (!b.rebuild) {
+    if (b.intermediate && named_.count(b.name) && named_[b.name].bytes() == 0)
+      fail("tape: primal values of " + b.name + " were released");
+    return p;
+  }
+  const int s = ensure(k);
+  return rslots_[size_t(s)].mem.as<double>() + (p - b.base);
+}
+
+// Slot holding buffer k's primal values: the slot it is in, else the least
+// recently used unpinned slot, filled by replaying k's operations (their kernel
+// launches follow every earlier read of the slot on the stream).
+int Tape::ensure(int k) {
+  for (size_t s = 0; s < rslots_.size(); ++s)
+    if (rslots_[s].owner == k) {
+      rslots_[s].stamp = ++rclock_;
+      ++rslots_[s].pins;
+      return int(s);
+    }
+  int pick = -1;
+  for (size_t s = 0; s < rslots_.size(); ++s) {
+    if (rslots_[s].pins > 0) continue;
+    if (pick < 0 || rslots_[s].stamp < rslots_[size_t(pick)].stamp) pick = int(s);
+  }
+  if (pick < 0) fail("tape: every rebuild slot is in use");
+  RebuildSlot& r = rslots_[size_t(pick)];
+  r.owner = -1;
+  ++r.pins;
+  const Buf& b = *bufs_[size_t(k)];
+  if (size_t(std::max<int64_t>(b.elems, 1)) * sizeof(double) > r.mem.bytes()) fail("tape: rebuild slot too small for " + b.name);
+  for (int i : b.recipe) run_recipe_op(ops_[size_t(i)], k, r.mem.as<double>());
+  r.owner = k;
+  r.stamp = ++rclock_;
+  return pick;
+}
+
+void Tape::run_recipe_op(const Op& op, int k, double* base) {
+  hipStream_t s = wf_.stream();
+  const Buf& b = *bufs_[size_t(k)];
+  auto out = [&](const double* p) { return base + (p - b.base); };
+  std::vector<int> pinned;
+  auto in = [&](const double* p) -> const double* {
+    if (p >= b.base && p < b.base + std::max<int64_t>(b.elems, 1)) return out(p);
+    const int q = find(p);
+    if (!bufs_[size_t(q)]->rebuild) return p;
+    const int sl = ensure(q);
+    pinned.push_back(sl);
+    return rslots_[size_t(sl)].mem.as<double>() + (p - bufs_[size_t(q)]->base);
+  };
+  switch (op.kind) {
+    case SORT4: etg::sort4(s, in(op.x), op.d, op.perm, out(op.y), op.alpha, op.beta); break;
+    case SQUAREUP: etg::squareup(s, in(op.x), out(op.y), op.N); break;
+    case COPY_SUB: etg::copy_sub(s, in(op.x), op.lds, out(op.y), op.ldd, op.m, op.n, op.alpha, op.beta); break;
+    case FILL0: fill(s, out(op.y), op.n, 0.0); break;
+    case SYMSUM: etg::symmetric_sum(s, in(op.x), out(op.y), op.N); break;
+    case PACKIN: etg::packin(s, in(op.x), out(op.y), op.N, op.alpha, op.beta, op.diag); break;
+    case SYMPACK: etg::symmetrize_and_pack(s, in(op.x), out(op.y), op.nv, op.no, op.alpha, op.f1, op.f2); break;
+    default: fail("tape: not a bandwidth operation in a rebuild recipe");
+  }
+  for (int sl : pinned) --rslots_[size_t(sl)].pins;    // queued reads precede later writes on the stream
+}
+
+// ---------------------------------------------------------------------------
 // Tape: replays
 // ---------------------------------------------------------------------------
 
@@ -400,9 +644,14 @@ void Tape::tangent_gemm(const Gemm& g, Prec prec, int64_t nb) {
   const int64_t sC = dstride(g.C), sA = dA ? dstride(g.A) : 0, sB = dB ? dstride(g.B) : 0;
   const bool vec_batch = g.batch == 1 && nb > 1;
   const int64_t reps = vec_batch ? 1 : nb;
+  // primal operands: B pairs with dA, A with dB
+  const double* pA = dB ? primal(g.A) : g.A;
+  const double* pB = dA ? primal(g.B) : g.B;
   for (int64_t k = 0; k < reps; ++k) {
     Gemm base = g;
     base.a_tag = 0;
+    base.A = pA;
+    base.B = pB;
     base.C = dC + k * sC;
     if (vec_batch) {
       base.batch = nb;
@@ -435,11 +684,16 @@ void Tape::tangent_gemm(const Gemm& g, Prec prec, int64_t nb) {
 
 // A_bar_k += alpha C_bar_k op(B)^T (or its transpose), B_bar_k likewise, then
 // C_bar_k *= beta; batched over the vectors as in tangent_gemm.
-void Tape::adjoint_gemm(const Gemm& g, Prec prec, int64_t nb) {
-  double* Cb0 = deriv(g.C);
-  double* Ab0 = deriv(g.A);
-  double* Bb0 = deriv(g.B);
-  const int64_t sC = dstride(g.C), sA = Ab0 ? dstride(g.A) : 0, sB = Bb0 ? dstride(g.B) : 0;
+void Tape::adjoint_gemm(const Gemm& g0, Prec prec, int64_t nb) {
+  double* Cb0 = deriv(g0.C);
+  double* Ab0 = deriv(g0.A);
+  double* Bb0 = deriv(g0.B);
+  // primal operands: A_bar reads B, B_bar reads A
+  Gemm g = g0;
+  g.a_tag = 0;
+  if (Bb0) g.A = primal(g0.A);
+  if (Ab0) g.B = primal(g0.B);
+  const int64_t sC = dstride(g0.C), sA = Ab0 ? dstride(g0.A) : 0, sB = Bb0 ? dstride(g0.B) : 0;
   const bool vec_batch = g.batch == 1 && nb > 1;
   const int64_t reps = vec_batch ? 1 : nb;
   for (int64_t v = 0; v < reps; ++v) {
@@ -542,9 +796,12 @@ void Tape::tangent(Prec prec, int64_t nb) {
     return d ? d + k * dstride(p) : nullptr;
   };
   plan_derivatives();
+  reset_rebuilt();
   for (size_t i = 0; i < ops_.size(); ++i) {
     const Op& op = ops_[i];
     zero_starting(starts_fwd_[i], nb);
+    if (!op_active_.empty() && !op_active_[i]) continue;   // no output carries a derivative
+    unpin_all();
     if (op.kind == GEMM) {
       tangent_gemm(op.g, prec, nb);
       continue;
@@ -626,9 +883,12 @@ void Tape::adjoint(Prec prec, int64_t nb) {
     return d ? d + k * dstride(p) : nullptr;
   };
   plan_derivatives();
+  reset_rebuilt();
   for (size_t i = ops_.size(); i-- > 0;) {
     const Op& op = ops_[i];
     zero_starting(starts_bwd_[i], nb);
+    if (!op_active_.empty() && !op_active_[i]) continue;   // no output carries a derivative
+    unpin_all();
     if (op.kind == GEMM) {
       adjoint_gemm(op.g, prec, nb);
       continue;

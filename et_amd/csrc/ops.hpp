@@ -147,6 +147,16 @@ class Tape : public DirectOps {
   void adjoint(Prec prec, int64_t nb = 1);
   // Room for nb derivative vectors per buffer; true when a buffer moved.
   bool reserve_derivatives(int64_t nb);
+  // After recording: which buffers carry a derivative (an intermediate written
+  // only from constant data carries none, and the passes skip its terms), which
+  // primal values the passes read (operands of products whose other operand
+  // carries a derivative), and which of those are rebuilt on demand by their
+  // bandwidth operations from data that stays in HBM instead of being stored.
+  // Every other intermediate's memory is released.  `rebuild_slots` buffers of
+  // the largest rebuilt size hold the rebuilt values during a pass.
+  void finalize(int rebuild_slots = 4);
+  // Address of the primal values at p during a pass (p itself unless p's buffer is rebuilt).
+  const double* primal(const double* p);
   // Zeroes the first nb derivative vectors of the buffers that exist outside
   // the tape (allocating on first use); the passes zero each intermediate's
   // derivative where its lifetime starts.
@@ -191,12 +201,33 @@ class Tape : public DirectOps {
     int64_t elems;
     bool constant;
     bool intermediate = false;   // derivative lives inside a pass only
+    bool active = true;          // carries a derivative (set by finalize)
+    bool rebuild = false;        // primal values rebuilt on demand by `recipe`
+    std::vector<int> recipe;     // the operations that write the buffer, in order
+    int last_write = -1;
     int first = -1, last = -1;   // operation interval (intermediates)
     int slot = -1;               // shared derivative slot (intermediates)
     std::string name;
     DeviceBuffer d;  // derivative (the passes' inputs and outputs)
   };
   int find(const double* p) const;
+  // operands an operation reads (up to 3) and writes (up to 2), null-padded
+  static void op_io(const Op& op, const double* in[3], const double* out[2]);
+  // rebuilt primal values: slot of buffer k (rebuilt if not current), pinned for the current operation
+  int ensure(int k);
+  void run_recipe_op(const Op& op, int k, double* base);
+  void reset_rebuilt();
+  void unpin_all();
+  std::vector<char> op_active_;          // some output of the operation carries a derivative
+  struct RebuildSlot {
+    DeviceBuffer mem;
+    int owner = -1;
+    uint64_t stamp = 0;
+    int pins = 0;
+  };
+  std::vector<RebuildSlot> rslots_;
+  uint64_t rclock_ = 0;
+  bool finalized_ = false;
   void tangent_gemm(const Gemm& g, Prec prec, int64_t nb);
   void adjoint_gemm(const Gemm& g, Prec prec, int64_t nb);
   void plan_derivatives();

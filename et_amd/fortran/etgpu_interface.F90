@@ -413,6 +413,119 @@ module etgpu_interface
          type(c_ptr), value :: molecule
       end function etg_molecule_destroy
 !
+!     AO dipole integrals <mu| r_c - origin_c |nu>, c = x, y, z (three n x n matrices)
+      integer(c_int) function etg_molecule_dipole_integrals(molecule, origin, D) &
+                                                   bind(C, name='etg_molecule_dipole_integrals')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: molecule
+         real(c_double), dimension(3), intent(in) :: origin
+         real(c_double), dimension(*), intent(out) :: D
+      end function etg_molecule_dipole_integrals
+!
+!     checks: S, T, V and the dense AO integral matrix (small molecules)
+      integer(c_int) function etg_molecule_one_electron(molecule, S, T, V) &
+                                                   bind(C, name='etg_molecule_one_electron')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: molecule
+         real(c_double), dimension(*), intent(out) :: S, T, V
+      end function etg_molecule_one_electron
+!
+      integer(c_int) function etg_molecule_ao_eri(molecule, G) bind(C, name='etg_molecule_ao_eri')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: molecule
+         real(c_double), dimension(*), intent(out) :: G
+      end function etg_molecule_ao_eri
+!
+!     the last residual, copied to the host
+      integer(c_int) function etg_get_omega(handle, omega) bind(C, name='etg_get_omega')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: handle
+         real(c_double), dimension(*), intent(out) :: omega
+      end function etg_get_omega
+!
+!     precision of the start block's transformation in etg_davidson (-1: the solve's)
+      integer(c_int) function etg_set_davidson_start_precision(handle, precision) &
+                                                   bind(C, name='etg_set_davidson_start_precision')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: handle
+         integer(c_int), value :: precision
+      end function etg_set_davidson_start_precision
+!
+!     virtual batch length of the ladder term's g_acbd blocks
+      integer(c_int) function etg_set_ladder_batch(handle, batch) bind(C, name='etg_set_ladder_batch')
+         import :: c_int, c_ptr, c_int64_t
+         type(c_ptr), value :: handle
+         integer(c_int64_t), value :: batch
+      end function etg_set_ladder_batch
+!
+      integer(c_int) function etg_synchronize(handle) bind(C, name='etg_synchronize')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: handle
+      end function etg_synchronize
+!
+!     diagnostics and tuning: the product ledger, graphs on or off, the measured tile choice
+      integer(c_int) function etg_last_omega_flops(handle, flops) bind(C, name='etg_last_omega_flops')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: handle
+         real(c_double), intent(out) :: flops
+      end function etg_last_omega_flops
+!
+      integer(c_int) function etg_set_kernel_ledger(handle, on) bind(C, name='etg_set_kernel_ledger')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: handle
+         integer(c_int), value :: on
+      end function etg_set_kernel_ledger
+!
+      integer(c_int) function etg_kernel_ledger(handle, index, name, name_len, launches, seconds, flops, &
+                                                bytes, count) bind(C, name='etg_kernel_ledger')
+         import :: c_int, c_ptr, c_int64_t, c_double, c_char
+         type(c_ptr), value :: handle
+         integer(c_int64_t), value :: index, name_len
+         character(kind=c_char), dimension(*), intent(out) :: name
+         integer(c_int64_t), intent(out) :: launches, count
+         real(c_double), intent(out) :: seconds, flops, bytes
+      end function etg_kernel_ledger
+!
+      integer(c_int) function etg_set_transform_graphs(handle, on) bind(C, name='etg_set_transform_graphs')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: handle
+         integer(c_int), value :: on
+      end function etg_set_transform_graphs
+!
+      integer(c_int) function etg_set_gemm_autotune(on, tuned) bind(C, name='etg_set_gemm_autotune')
+         import :: c_int, c_int64_t
+         integer(c_int), value :: on
+         integer(c_int64_t), intent(out) :: tuned
+      end function etg_set_gemm_autotune
+!
+      integer(c_int) function etg_test_gemm(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, precision) &
+                                            bind(C, name='etg_test_gemm')
+         import :: c_int, c_int64_t, c_double, c_char
+         character(kind=c_char), value :: ta, tb
+         integer(c_int64_t), value :: m, n, k, lda, ldb, ldc
+         real(c_double), value :: alpha, beta
+         real(c_double), dimension(*), intent(in) :: A, B
+         real(c_double), dimension(*), intent(inout) :: C
+         integer(c_int), value :: precision
+      end function etg_test_gemm
+!
+      integer(c_int) function etg_bench_gemm(ta, tb, m, n, k, precision, reps, seconds) &
+                                             bind(C, name='etg_bench_gemm')
+         import :: c_int, c_int64_t, c_double, c_char
+         character(kind=c_char), value :: ta, tb
+         integer(c_int64_t), value :: m, n, k
+         integer(c_int), value :: precision, reps
+         real(c_double), intent(out) :: seconds
+      end function etg_bench_gemm
+!
+      integer(c_int) function etg_test_reduced_eigen(k, G, n_want, wr, wi, X) &
+                                                     bind(C, name='etg_test_reduced_eigen')
+         import :: c_int, c_int64_t, c_double
+         integer(c_int64_t), value :: k, n_want
+         real(c_double), dimension(*), intent(in) :: G
+         real(c_double), dimension(*), intent(out) :: wr, wi, X
+      end function etg_test_reduced_eigen
+!
    end interface
 !
 contains

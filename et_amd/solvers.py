@@ -452,21 +452,32 @@ class DiisCCExcitedStates:
 
 
 class MixedPrecisionDavidson:
-    """EOM roots from reduced-precision transformations, finished in FP64.
+    """EOM roots from reduced-precision transformations, finished against FP64.
 
-    Each precision of `schedule` (split-bf16, then FP32) runs the device Davidson with
-    the transformations in that precision down to `margin` (one factor, or a dict of
-    factors per precision) times its noise floor
-    ||A_p x - A_FP64 x|| (measured once on a normalised random probe), or until
-    its residuals stop decreasing; its Ritz vectors start the next phase.  The
-    last phase is the FP64 eigen Davidson solver, so convergence is decided on
-    FP64 transformations against `residual_threshold`.  The reduced-space
-    algebra is FP64 throughout.
+    Each precision of `schedule` (split-bf16, optionally then FP32) runs the device
+    Davidson with the transformations in that precision down to `margin` (one factor,
+    or a dict of factors per precision) times its noise floor ||A_p x - A_FP64 x||
+    (measured once on a normalised random probe), or until its residuals stop
+    decreasing; its Ritz vectors start the next phase.
+
+    refine=True (the default) then runs the refinement phase: the device Davidson
+    started from those Ritz vectors with the start block transformed in FP64 and
+    every expansion vector in the last reduced precision
+    (etg_set_davidson_start_precision).  Each Ritz vector is then the FP64 image of
+    the start block plus reduced-precision corrections whose weights are of the size
+    of the start residuals, so the residual falls far below the reduced precision's
+    floor.  The result is accepted when an FP64 transformation of every returned
+    vector, outside the solver, gives ||A x - w x|| / ||x|| below
+    `residual_threshold`; otherwise the phase repeats from the new vectors (at most
+    `refine_rounds` times) and an FP64 Davidson solve finishes as without refinement.
+    refine=False: the last phase is the FP64 eigen Davidson solver.  Either way
+    convergence is decided on FP64 transformations; the reduced-space algebra is
+    FP64 throughout.
     """
 
     def __init__(self, wf, n_states, side='right', residual_threshold=1e-6, max_iterations=100,
-                 max_dim=0, schedule=(_ccsd.BF16X3, _ccsd.FP32), margin=10.0, log=None, seed=7, start=None,
-                 restart=False):
+                 max_dim=0, schedule=(_ccsd.BF16X3,), margin=10.0, log=None, seed=7, start=None,
+                 restart=False, refine=True, refine_rounds=2):
         self.wf = wf
         self.restart = restart
         self.n_states = n_states
@@ -474,13 +485,16 @@ class MixedPrecisionDavidson:
         self.residual_threshold = residual_threshold
         self.max_iterations = max_iterations
         self.max_dim = max_dim
-        self.schedule = tuple(schedule)
+        self.schedule = tuple(schedule) if refine or len(schedule) != 1 else tuple(schedule)
         self.margin = margin
         self.log = log
         self.seed = seed
         self.start = start
+        self.refine = refine
+        self.refine_rounds = refine_rounds
         self.noise = {}
         self.phases = []          # (precision, iterations, stalled, final max residual)
+        self.checks = []          # FP64 residual norms of the refined vectors, per round
 
     def _measure_noise(self):
         wf = self.wf
@@ -489,6 +503,11 @@ class MixedPrecisionDavidson:
         ref = wf.jacobian_transformations(x, self.side, _ccsd.FP64)
         for p in self.schedule:
             self.noise[p] = float(np.linalg.norm(wf.jacobian_transformations(x, self.side, p) - ref))
+
+    def _say(self, p, hist, stalled, tag=''):
+        if self.log:
+            self.log('%-6s%s %3d iterations%s, max|res| %.3e' % (_PREC_NAMES[p], tag, len(hist),
+                                                                 ' (stalled)' if stalled else '', hist[-1].max()))
 
     def run(self):
         wf = self.wf
@@ -499,15 +518,37 @@ class MixedPrecisionDavidson:
             start = wf.restart_start_vectors(self.n_states, self.side)[0]
         margin = self.margin if isinstance(self.margin, dict) else {p: self.margin for p in self.schedule}
         phases = [(p, max(self.residual_threshold, margin[p] * self.noise[p])) for p in self.schedule]
-        phases.append((_ccsd.FP64, self.residual_threshold))
+        w = None
         for p, threshold in phases:
             w, hist, start = wf.davidson(self.n_states, self.side, threshold, self.max_iterations,
                                          self.max_dim, p, vectors=True, start=start)
-            stalled = p != _ccsd.FP64 and wf.davidson_stalled
+            stalled = wf.davidson_stalled
             self.phases.append((p, len(hist), stalled, float(hist[-1].max())))
-            if self.log:
-                self.log('%-6s %3d iterations%s, max|res| %.3e' % (_PREC_NAMES[p], len(hist),
-                                                                  ' (stalled)' if stalled else '',
-                                                                  hist[-1].max()))
+            self._say(p, hist, stalled)
+        if self.refine:
+            p = self.schedule[-1]
+            for _ in range(self.refine_rounds):
+                wf.set_davidson_start_precision(_ccsd.FP64)
+                try:
+                    w, hist, X = wf.davidson(self.n_states, self.side, self.residual_threshold,
+                                             self.max_iterations, self.max_dim, p, vectors=True, start=start)
+                finally:
+                    wf.set_davidson_start_precision(None)
+                stalled = wf.davidson_stalled
+                self.phases.append(('refined-' + _PREC_NAMES[p], len(hist), stalled, float(hist[-1].max())))
+                self._say(p, hist, stalled, ' refined')
+                AX = wf.jacobian_transformations(X, self.side, _ccsd.FP64)
+                res = np.linalg.norm(AX - X * np.asarray(w)[None, :], axis=0) / np.linalg.norm(X, axis=0)
+                self.checks.append(res)
+                start = X
+                if self.log:
+                    self.log('FP64 check of the refined vectors: max|A x - w x| %.3e' % res.max())
+                if res.max() < self.residual_threshold:
+                    self.eigenvectors = X
+                    return w
+        w, hist, start = wf.davidson(self.n_states, self.side, self.residual_threshold, self.max_iterations,
+                                     self.max_dim, _ccsd.FP64, vectors=True, start=start)
+        self.phases.append((_ccsd.FP64, len(hist), False, float(hist[-1].max())))
+        self._say(_ccsd.FP64, hist, False)
         self.eigenvectors = start
         return w

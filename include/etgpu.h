@@ -181,6 +181,18 @@ int etg_davidson(etg_handle* handle, int side, int64_t n_states, double residual
                  const double* start_vectors, double* energies, double* vectors,
                  double* residual_norms, int64_t* iterations);
 
+/* Precision of the first transformation pass of etg_davidson (the start
+ * vectors); the expansion vectors of later iterations use the solve's own
+ * precision.  -1 (the default): the solve's precision.  With ETG_FP64 here and
+ * a reduced-precision solve started from converged reduced-precision Ritz
+ * vectors, every Ritz vector is an FP64-transformed start block plus
+ * reduced-precision corrections whose weights are of the size of the start
+ * residuals, so the solve reaches residuals far below the reduced precision's
+ * rounding floor while transforming only the start block in FP64.
+ * Replaces: no eT procedure (eT's Davidson transforms in FP64 throughout); the
+ * precision schedule of the excited-state solve, set before etg_davidson. */
+int etg_set_davidson_start_precision(etg_handle* handle, int precision);
+
 /* A X = F (side 0) or A^T X = F (side 1) by eT's linear Davidson solver, on
  * the device: start from start_vector (may be NULL: the preconditioner guess
  * F / eps), one new trial per iteration (-R / eps of the residual R = A X - F,

@@ -15,6 +15,8 @@ Pinning (how each function is checked independently of retyping its formula):
   closed forms Omega(t=0) = (F_ai, g_aibj with 1/2 on the diagonal) and
   E(MP2 guess) = E_MP2.
 * ``omega_a2_plus_minus``: against the naive ladder sum_cd g_acbd t_cidj.
+* ``omega_a2_batched`` (eT's batched ladder with the C gathers and scatter of
+  ``ladder_pm.c``): equal to ``omega_a2_plus_minus`` for every batch length.
 * ``t1_cholesky``: against the matrix identity L_t1 = X L Y^T.
 * ``diis_extrapolate``: against the closed-form two-vector DIIS solution.
 """

@@ -30,17 +30,53 @@ def packed_index(n):
     return hi * (hi + 1) // 2 + lo
 
 
+def _fast(*arrays):
+    """Real FP64 arrays take the OpenMP helpers of ladder_pm.c (same values as the
+    numpy forms below, which complex-step derivatives use)."""
+    return all(a.dtype == np.float64 for a in arrays)
+
+
 def squareup(packed, n):
     """squareup_real (reordering.F90:4174): packed -> full symmetric (n, n)."""
+    if _fast(packed) and n > 64:
+        import ctypes
+        full = np.empty((n, n))
+        p = np.ascontiguousarray(packed)
+        dp = ctypes.POINTER(ctypes.c_double)
+        _ladder_c().sym_unpack(p.ctypes.data_as(dp), n, full.ctypes.data_as(dp))
+        return full
     return packed[packed_index(n)]
 
 
 def packin(full):
     """packin_real (reordering.F90:4267): lower triangle full[i, j], i >= j -> packed."""
     n = full.shape[0]
+    if _fast(full) and n > 64:
+        import ctypes
+        f = np.ascontiguousarray(full)
+        out = np.empty(n * (n + 1) // 2)
+        dp = ctypes.POINTER(ctypes.c_double)
+        _ladder_c().sym_pack_lower(f.ctypes.data_as(dp), n, out.ctypes.data_as(dp))
+        return out
     i, j = np.tril_indices(n)
     out = np.empty(n * (n + 1) // 2, dtype=full.dtype)
     out[i * (i + 1) // 2 + j] = full[i, j]
+    return out
+
+
+def contiguous(x):
+    """C-ordered copy of a (strided) 4-index array."""
+    if (x.ndim != 4 or not _fast(x) or x.size < (1 << 16) or x.flags.c_contiguous
+            or any(st <= 0 or st % 8 for st in x.strides)):
+        return np.ascontiguousarray(x)
+    import ctypes
+    out = np.empty(x.shape)
+    shape = np.array(x.shape, dtype=np.int64)
+    strides = np.array([st // 8 for st in x.strides], dtype=np.int64)
+    ip, dp = ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double)
+    base = x.ctypes.data
+    _ladder_c().permute4_copy(ctypes.cast(base, dp), shape.ctypes.data_as(ip), strides.ctypes.data_as(ip),
+                              out.ctypes.data_as(dp))
     return out
 
 
@@ -52,7 +88,7 @@ def mat_to_aibj(m, n_v, n_o):
 def aibj_to_mat(x):
     """tensor [a, i, b, j] -> (ov, ov) matrix with row ai = a + n_v*i."""
     n_v, n_o = x.shape[0], x.shape[1]
-    return np.ascontiguousarray(x.transpose(1, 0, 3, 2)).reshape(n_o * n_v, n_o * n_v)
+    return contiguous(x.transpose(1, 0, 3, 2)).reshape(n_o * n_v, n_o * n_v)
 
 
 def unpack_t2(t2, n_v, n_o):
@@ -277,21 +313,30 @@ def omega_c2_d2(L_t1, t2, n_o, n_v):
     t = unpack_t2(t2, n_v, n_o)
     T = aibj_to_mat(t.transpose(0, 3, 2, 1))
     U = aibj_to_mat(construct_u(t))
+    del t
     # C_term[(c,k),(a,i)] = g_acki ; D_term = 2 g_aikc - g_acki   (:632-640)
     g_vvoo = eri(L_t1, v, v, o, o)                             # g_abij
     C = aibj_to_mat(g_vvoo.transpose(1, 2, 0, 3))              # C(b,i,a,j) = g_abij
+    del g_vvoo
     g_voov = eri(L_t1, v, o, o, v)                             # g_aijb
     D = -C + 2.0 * aibj_to_mat(g_voov.transpose(3, 2, 0, 1))   # D(c,k,a,i) += 2 g_aikc
+    del g_voov
     # G[(c,k),(d,l)] = g_lckd  (sort_1234_to_2341 of g_ovov)   (:643-644)
     g_ovov = eri(L_t1, o, v, o, v)
     G = aibj_to_mat(g_ovov.transpose(1, 2, 3, 0))              # G(c,l,d,k) = g_kcld
+    del g_ovov
     C += symmetric_product(-0.5, G, T)                         # dsymm R,U (:648)
     x = mat_to_aibj(G, n_v, n_o)
     Gc = aibj_to_mat(2.0 * x.transpose(0, 3, 2, 1) - x)       # contra, psrq (:661)
+    del x, G
     D += symmetric_product(0.5, Gc, U)                         # dsymm R,L (:663)
+    del Gc
     W = symmetric_product(-1.5, T, C)                          # dsymm L,U (:675)
+    del C, T
     out = symmetrize_and_pack(W, n_v, n_o, contra=True, psrq=True)
+    del W
     V = symmetric_product(0.5, U, D)                           # dsymm L,L (:689)
+    del U, D
     out += symmetrize_and_pack(V, n_v, n_o)
     return out
 
@@ -360,6 +405,110 @@ def omega_a2_plus_minus(L_t1, t2, n_o, n_v):
     return out
 
 
+_ladder_lib = None
+
+
+def _ladder_c():
+    """The C helper of the batched ladder (oracle/ladder_pm.c, built by oracle/build.py)."""
+    global _ladder_lib
+    if _ladder_lib is None:
+        import ctypes
+        import os
+        from . import build as _build
+        if not os.path.exists(_build.LIB):
+            _build.build()
+        lib = ctypes.CDLL(_build.LIB)
+        i64, dp = ctypes.c_int64, ctypes.POINTER(ctypes.c_double)
+        ip = ctypes.POINTER(ctypes.c_int64)
+        lib.ladder_gather_pm.argtypes = [dp, i64, i64, i64, i64, i64, i64, ip, ip, dp, dp]
+        lib.ladder_gather_pm.restype = None
+        lib.ladder_scatter_pm.argtypes = [dp, dp, i64, i64, dp]
+        lib.ladder_scatter_pm.restype = None
+        lib.ladder_scatter_block.argtypes = [dp, dp, i64, ip, ip, i64, i64, dp]
+        lib.ladder_scatter_block.restype = None
+        lib.permute4_copy.argtypes = [dp, ip, ip, dp]
+        lib.permute4_copy.restype = None
+        lib.sym_unpack.argtypes = [dp, i64, dp]
+        lib.sym_unpack.restype = None
+        lib.sym_pack_lower.argtypes = [dp, i64, dp]
+        lib.sym_pack_lower.restype = None
+        _ladder_lib = lib
+    return _ladder_lib
+
+
+def ladder_t_plus_minus(t2, n_o, n_v):
+    """construct_plus_minus_2413_from_packed + scale_double_packed_blocks
+    (packed_array_utilities_r.F90:140-332): t+ (o+ x v+) and t- (o- x v-) with rows
+    ij (i <= j, resp. i < j) and columns cd (c <= d, resp. c < d), each in the packed
+    order j(j+1)/2 + i (resp. j(j-1)/2 + i)."""
+    t = unpack_t2(t2, n_v, n_o)                                # t[c,i,d,j]
+    oi, oj = np.triu_indices(n_o)
+    order = np.argsort(oj * (oj + 1) // 2 + oi)
+    i_p, j_p = oi[order], oj[order]
+    ci, cj = np.triu_indices(n_v)
+    order = np.argsort(cj * (cj + 1) // 2 + ci)
+    c_p, d_p = ci[order], cj[order]
+    im, jm = np.triu_indices(n_o, 1)
+    order = np.argsort(jm * (jm - 1) // 2 + im)
+    i_m, j_m = im[order], jm[order]
+    cm, dm = np.triu_indices(n_v, 1)
+    order = np.argsort(dm * (dm - 1) // 2 + cm)
+    c_m, d_m = cm[order], dm[order]
+    tp = np.empty((len(i_p), len(c_p)))
+    for r, (i, j) in enumerate(zip(i_p, j_p)):           # one row of t+ per occupied pair
+        tp[r] = t[c_p, i, d_p, j] + t[c_p, j, d_p, i]
+    tp *= np.where(c_p == d_p, 0.5, 1.0)[None, :]
+    tp *= np.where(i_p == j_p, 0.5, 1.0)[:, None]
+    tm = np.empty((len(i_m), len(c_m)))
+    for r, (i, j) in enumerate(zip(i_m, j_m)):
+        tm[r] = t[c_m, i, d_m, j] - t[c_m, j, d_m, i]
+    return tp, tm
+
+
+def omega_a2_batched(L_t1, t2, n_o, n_v, batch):
+    """omega_ccsd_a2 (omega_ccsd.F90:113-480) with eT's batching over virtual blocks:
+    for each (a-batch A, b-batch B >= A) the block g_acbd (a in A, b in B) is built from
+    the dressed vv Cholesky block (one product, K = n_J), its +/- combinations for the
+    pairs a <= b of the block are gathered (ladder_pm.c), multiplied with t+/- (two
+    products) and the results scattered into the packed residual.  Equals
+    omega_a2_plus_minus for any batch length.  Returns the packed contribution."""
+    lib = _ladder_c()
+    import ctypes
+    dp = ctypes.POINTER(ctypes.c_double)
+    ip = ctypes.POINTER(ctypes.c_int64)
+    v = slice(n_o, n_o + n_v)
+    n_J = L_t1.shape[0]
+    vp, vm = n_v * (n_v + 1) // 2, n_v * (n_v - 1) // 2
+    tp, tm = ladder_t_plus_minus(t2, n_o, n_v)
+    out = np.zeros(n_t2(n_o, n_v))
+    Lvv = np.ascontiguousarray(L_t1[:, v, v])                  # Lvv[J, a, c]
+    starts = list(range(0, n_v, batch))
+    for a0 in starts:
+        la = min(batch, n_v - a0)
+        LA = np.ascontiguousarray(Lvv[:, a0:a0 + la, :]).reshape(n_J, la * n_v)
+        for b0 in starts:
+            if b0 < a0:
+                continue
+            lb = min(batch, n_v - b0)
+            LB = np.ascontiguousarray(Lvv[:, b0:b0 + lb, :]).reshape(n_J, lb * n_v)
+            G = LA.T @ LB                                      # G[(a, c), (b, d)] = g_acbd
+            aa, bb = np.meshgrid(np.arange(a0, a0 + la), np.arange(b0, b0 + lb), indexing='ij')
+            keep = aa <= bb
+            pa = np.ascontiguousarray(aa[keep], dtype=np.int64)
+            pb = np.ascontiguousarray(bb[keep], dtype=np.int64)
+            npairs = pa.size
+            gp = np.empty((npairs, vp))
+            gm = np.empty((npairs, max(vm, 1)))
+            lib.ladder_gather_pm(G.ctypes.data_as(dp), n_v, a0, la, b0, lb, npairs, pa.ctypes.data_as(ip),
+                                 pb.ctypes.data_as(ip), gp.ctypes.data_as(dp), gm.ctypes.data_as(dp))
+            del G
+            wp = np.ascontiguousarray(0.5 * (tp @ gp.T))       # (o+ x pairs)
+            wm = np.ascontiguousarray(0.5 * (tm @ gm.T)) if vm and tm.shape[0] else np.zeros((1, npairs))
+            lib.ladder_scatter_block(wp.ctypes.data_as(dp), wm.ctypes.data_as(dp), npairs, pa.ctypes.data_as(ip),
+                                     pb.ctypes.data_as(ip), n_o, n_v, out.ctypes.data_as(dp))
+    return out
+
+
 def omega_a2_naive(L_t1, t2, n_o, n_v):
     """Ladder sum_cd g_acbd t_cidj with the (1 - delta_{ai,bj}/2) packing factor (test helper)."""
     v = slice(n_o, n_o + n_v)
@@ -372,8 +521,11 @@ def omega_a2_naive(L_t1, t2, n_o, n_v):
     return packin(Wm)
 
 
-def omega(L_mo, L_t1, F, t1, t2, n_o):
-    """construct_omega_ccsd (omega_ccsd.F90:39-110).  Returns omega(n_t1 + n_t2)."""
+def omega(L_mo, L_t1, F, t1, t2, n_o, ladder_batch=None):
+    """construct_omega_ccsd (omega_ccsd.F90:39-110).  Returns omega(n_t1 + n_t2).
+
+    ladder_batch: virtual batch length of the ladder term (eT's batched form,
+    omega_a2_batched); None forms g_vvvv whole (omega_a2_plus_minus)."""
     n_v = L_mo.shape[1] - n_o
     ov = n_o * n_v
     t = unpack_t2(t2, n_v, n_o)
@@ -388,7 +540,10 @@ def omega(L_mo, L_t1, F, t1, t2, n_o):
     m[d, d] *= 0.5
     om2 = packin(m)
     om2 += omega_c2_d2(L_t1, t2, n_o, n_v)                     # (:105)
-    om2 += omega_a2_plus_minus(L_t1, t2, n_o, n_v)             # (:106)
+    if ladder_batch is None:
+        om2 += omega_a2_plus_minus(L_t1, t2, n_o, n_v)         # (:106)
+    else:
+        om2 += omega_a2_batched(L_t1, t2, n_o, n_v, ladder_batch)
     return np.concatenate([om1.ravel(order='F'), om2])
 
 

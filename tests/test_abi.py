@@ -45,3 +45,12 @@ def test_python_signatures_cover_the_header():
 def test_last_error_is_empty_initially():
     lib = ccsd.load_library()
     assert lib.etg_last_error() == b''
+
+
+def test_fortran_module_binds_every_declared_function():
+    """et_amd/fortran/etgpu_interface.F90 has a bind(C) interface for every function
+    include/etgpu.h declares (the Fortran caller reaches the whole boundary)."""
+    text = open(os.path.join(ROOT, 'et_amd', 'fortran', 'etgpu_interface.F90')).read()
+    bound = set(re.findall(r"name='(etg_\w+)'", text))
+    missing = sorted(set(declared_functions()) - bound)
+    assert not missing, missing

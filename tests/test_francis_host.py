@@ -115,3 +115,26 @@ def test_unit_circle_spectra(lib):
             info, w = eig_hessenberg(lib, A)
             assert info == 0, k
             assert match(w, np.linalg.eigvals(A), 1e-11 * k)
+
+
+def test_multiplets_close_to_scalar_blocks(lib):
+    """Exactly degenerate multiplets of Davidson-like reduced matrices: the block
+    of a converging multiple root is close to a multiple of I, where the first
+    column of the double-shift polynomial must be formed without cancellation
+    (a degenerate triple of EOM roots stalled the iteration before)."""
+    rng = np.random.default_rng(21)
+    for trial in range(300):
+        k = int(rng.integers(4, 100))
+        base = np.sort(rng.uniform(0.2, 2.0, k))
+        A = np.diag(base) + 1e-8 * rng.standard_normal((k, k))
+        for _ in range(2):
+            i = int(rng.integers(0, k - 3))
+            A[i:i + 3, i:i + 3] = base[i] * np.eye(3) + 1e-12 * rng.standard_normal((3, 3))
+        if trial % 2:
+            S = np.eye(k) + 0.1 * rng.standard_normal((k, k))
+            A = S @ A @ np.linalg.inv(S)
+        info, w = eig_hessenberg(lib, A)
+        assert info == 0, trial
+        ref = np.linalg.eigvals(A)
+        # multiple roots are determined to ~ sqrt(eps) of the coupling inside the block
+        assert match(w, ref, 1e-9 * np.abs(A).max() * k), trial

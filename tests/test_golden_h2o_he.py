@@ -189,10 +189,12 @@ def v1_0_excitation_energies():
 
 @pytest.mark.gpu
 def test_gpu_mixed_precision_eom_reproduces_et():
-    """Split-bf16 Davidson phase, finished by FP64 Davidson from its vectors (residuals
-    <= 1e-9): eT's three lowest excitation energies (tests/restart_ccsd_v1.0) to 1e-10 Eh,
-    right and left.
-    A second FP64 solve started from the converged vectors converges at once."""
+    """Split-bf16 and FP32 Davidson phases, finished by FP64 Davidson from their vectors
+    (residuals <= 1e-9): eT's three lowest excitation energies (tests/restart_ccsd_v1.0) to
+    1e-10 Eh, right and left; a second FP64 solve started from the converged vectors
+    converges at once.  The split-bf16 phase followed by the refined phase (FP64 start
+    block, split-bf16 expansions) gives the same roots, and an FP64 transformation of its
+    vectors confirms residuals below 1e-9."""
     from et_amd import ccsd, solvers
     w_ref = v1_0_excitation_energies()
     m = inputs()
@@ -204,7 +206,8 @@ def test_gpu_mixed_precision_eom_reproduces_et():
         wf.set_orbital_energies(m['eps'])
         solvers.DiisCCGroundState(wf, residual_threshold=1e-11).run()
         for side in ('right', 'left'):
-            sv = solvers.MixedPrecisionDavidson(wf, 3, side, residual_threshold=1e-9)
+            sv = solvers.MixedPrecisionDavidson(wf, 3, side, residual_threshold=1e-9,
+                                                schedule=(ccsd.BF16X3, ccsd.FP32), refine=False)
             w = sv.run()
             np.testing.assert_allclose(w, w_ref, rtol=0, atol=1e-10)
             assert [p for p, *_ in sv.phases] == [ccsd.BF16X3, ccsd.FP32, ccsd.FP64]
@@ -212,6 +215,14 @@ def test_gpu_mixed_precision_eom_reproduces_et():
             again, hist = wf.davidson(3, side, 1e-8, 10, 0, ccsd.FP64, start=sv.eigenvectors)
             assert len(hist) == 1 and not wf.davidson_stalled
             np.testing.assert_allclose(again, w_ref, rtol=0, atol=1e-10)
+            # split-bf16, then the refined phase (FP64 start block, split-bf16 expansions):
+            # eT's roots, and an FP64 transformation of the vectors confirms the residual
+            sv = solvers.MixedPrecisionDavidson(wf, 3, side, residual_threshold=1e-9)
+            w = sv.run()
+            print(side, 'refined phases', sv.phases, [c.max() for c in sv.checks], flush=True)
+            np.testing.assert_allclose(w, w_ref, rtol=0, atol=1e-10)
+            assert sv.checks and sv.checks[-1].max() < 1e-9
+            assert str(sv.phases[-1][0]).startswith('refined')
 
 
 @pytest.mark.gpu

@@ -1,0 +1,213 @@
+"""The five benchmark workloads as GPU tests, each against an independent check.
+
+1. H2O / cc-pVDZ at the stated geometry, Cholesky 1e-8, DIIS to 1e-10: the
+   device producer and ground state against the oracle's front end and DIIS
+   (oracle/frontend.py, oracle/ccsd.py) -- E_HF, n_J and E_CCSD to 1e-10 Eh.
+2. (H2O)_8 / cc-pVDZ (seeded 2x2x2 lattice, Cholesky 1e-8), FP64 ground state to
+   1e-10: the oracle's residual and energy at the GPU's converged amplitudes on
+   the same L_mo, h, eps -- |dE_corr| <= 1e-10 Eh, ||Omega_oracle(t_GPU)|| <=
+   1e-9; the GPU residual at the MP2 guess equals the oracle's elementwise.
+3. o = 48, v = 400, n_J = 1600 synthetic (seed 0), at full size: Omega(t = 0)
+   equals its closed form (F_ai; g_aibj with 1/2 on the packed diagonal from a
+   host product), two ladder batchings agree to 1e-13, FP32 and split-bf16 stay
+   within their rounding bounds of FP64, every element finite.
+4. 12-root EOM on (H2O)_8, right and left, split-bf16 Davidson then the refined
+   phase (FP64 start block, split-bf16 expansions): every root re-checked with a
+   fresh FP64 transformation of its vector (||A x - w x|| / ||x|| below the
+   threshold), right and left energies equal to 1e-5 eV.
+5. (H2O)_16 / cc-pVDZ (4x2x2 lattice) mixed-precision ground state, multipliers
+   and dipole: ||Omega_FP64|| <= 1e-10, ||A^T t-bar + eta|| <= 1e-8, and the
+   dipole from the density of one backward replay equals <r_c> from forward
+   replays along each dipole operator.
+"""
+import numpy as np
+import pytest
+
+from et_amd import ccsd
+from et_amd import molecule as M
+from et_amd import solvers
+from et_amd import synthetic as syn
+from oracle import ccsd as O
+from oracle import frontend as fe
+
+pytestmark = pytest.mark.gpu
+
+HARTREE_TO_EV = 27.21138602
+
+
+def load(wf, r):
+    wf.set_cholesky(r['L_mo'])
+    wf.set_oei(r['h_mo'])
+    wf.set_orbital_energies(r['eps'])
+    wf.hf_energy = r['e_hf']
+
+
+# ---- 1 ----------------------------------------------------------------------------------
+
+def test_cfg1_water_against_oracle():
+    print('cfg1: H2O / cc-pVDZ, Cholesky 1e-8', flush=True)
+    atoms = [(s, tuple(x)) for s, x in M.WATER]
+    with M.Molecule(atoms) as mol:
+        r = mol.run(cholesky_threshold=1e-8, scf_threshold=1e-10)
+    n_o, n_J = r['n_o'], r['n_J']
+    n_v = r['L_mo'].shape[1] - n_o
+    assert (n_o, n_o + n_v) == (5, 24)
+    ref = fe.molecular_inputs(dict(geometry_angstrom=[(s, *x) for s, x in atoms], basis=M.load_basis(),
+                                   bohr_to_angstrom=M.BOHR_TO_ANGSTROM, cholesky_threshold=1e-8))
+    assert ref['n_J'] == n_J
+    assert abs(r['e_nuc'] - ref['e_nuc']) < 1e-11
+    assert abs(r['e_hf'] - ref['e_hf']) < 1e-10
+    with ccsd.CCSD(n_o, n_v, n_J) as wf:
+        load(wf, r)
+        e_gpu = solvers.DiisCCGroundState(wf, residual_threshold=1e-10).run()
+    _, _, e_corr, _ = O.solve_ground_state(ref['L_mo'], ref['h_mo'], ref['eps'], n_o, residual_threshold=1e-10)
+    print('cfg1: E_HF %.12f (oracle %.12f), E_CCSD %.12f (oracle %.12f)'
+          % (r['e_hf'], ref['e_hf'], e_gpu, ref['e_hf'] + e_corr), flush=True)
+    assert abs(e_gpu - (ref['e_hf'] + e_corr)) < 1e-10
+
+
+# ---- 2 and 4: (H2O)_8 -------------------------------------------------------------------
+
+@pytest.fixture(scope='module')
+def octamer():
+    print('(H2O)8 producer', flush=True)
+    with M.Molecule(M.water_lattice((2, 2, 2), 2.9, 0)) as mol:
+        r = mol.run(cholesky_threshold=1e-8, scf_threshold=1e-10)
+    assert (r['n_o'], r['L_mo'].shape[1]) == (40, 192)
+    return r
+
+
+def test_cfg2_octamer_ground_state_against_oracle(octamer):
+    r = octamer
+    n_o, n_J = r['n_o'], r['n_J']
+    n_v = r['L_mo'].shape[1] - n_o
+    n_t1 = n_o * n_v
+    with ccsd.CCSD(n_o, n_v, n_J) as wf:
+        load(wf, r)
+        wf.set_initial_amplitudes_guess()
+        wf.construct_fock()
+        om_guess = wf.construct_omega(ccsd.FP64)
+        t_guess = wf.get_amplitudes()
+        solvers.DiisCCGroundState(wf, residual_threshold=1e-10).run()
+        t = wf.get_amplitudes()
+        e_gpu = wf.correlation_energy
+    print('cfg2: GPU E_corr %.12f; oracle residuals on the host' % e_gpu, flush=True)
+    L, h, eps = r['L_mo'], r['h_mo'], r['eps']
+    # the MP2 guess: elementwise
+    t1 = t_guess[:n_t1].reshape(n_v, n_o, order='F')
+    Lt = O.t1_cholesky(L, t1, n_o)
+    ref = O.omega(L, Lt, O.fock_t1(h, Lt, t1, n_o), t1, t_guess[n_t1:], n_o)
+    assert np.abs(om_guess - ref).max() <= 1e-12 * np.abs(ref).max()
+    # the converged amplitudes
+    t1 = t[:n_t1].reshape(n_v, n_o, order='F')
+    Lt = O.t1_cholesky(L, t1, n_o)
+    om = O.omega(L, Lt, O.fock_t1(h, Lt, t1, n_o), t1, t[n_t1:], n_o)
+    e = O.correlation_energy(Lt, t1, t[n_t1:], n_o)
+    print('cfg2: oracle E_corr %.12f, ||Omega_oracle(t_GPU)|| %.3e' % (e, np.linalg.norm(om)), flush=True)
+    assert abs(e - e_gpu) <= 1e-10
+    assert np.linalg.norm(om) <= 1e-9
+
+
+def test_cfg4_octamer_eom_12_roots(octamer):
+    r = octamer
+    n_o, n_J = r['n_o'], r['n_J']
+    n_v = r['L_mo'].shape[1] - n_o
+    threshold = 1e-5
+    roots = {}
+    with ccsd.CCSD(n_o, n_v, n_J) as wf:
+        load(wf, r)
+        solvers.DiisCCGroundState(wf, residual_threshold=1e-10).run()
+        for side in ('right', 'left'):
+            print('cfg4: %s EOM, 12 roots, mixed precision' % side, flush=True)
+            sv = solvers.MixedPrecisionDavidson(wf, 12, side, residual_threshold=threshold)
+            w = np.asarray(sv.run())
+            X = sv.eigenvectors
+            print('cfg4: phases %s' % [(str(p), k, st, '%.2e' % r) for p, k, st, r in sv.phases], flush=True)
+            # an FP64 transformation of each returned vector, outside the solver
+            AX = wf.jacobian_transformations(X, side, ccsd.FP64)
+            res = np.linalg.norm(AX - X * w[None, :], axis=0) / np.linalg.norm(X, axis=0)
+            print('cfg4: %s roots (eV) %s; max residual %.3e' % (side, np.round(w * HARTREE_TO_EV, 6), res.max()),
+                  flush=True)
+            assert res.max() <= threshold
+            roots[side] = w
+    assert np.abs(roots['right'] - roots['left']).max() * HARTREE_TO_EV <= 1e-5
+
+
+# ---- 3 ------------------------------------------------------------------------------------
+
+def test_cfg3_full_size_residual_properties():
+    n_o, n_v, n_J, seed = 48, 400, 1600, 0
+    print('cfg3: o=%d v=%d J=%d seed %d' % (n_o, n_v, n_J, seed), flush=True)
+    L = syn.cholesky_vectors(seed, n_o + n_v, n_J)
+    eps = syn.orbital_energies(seed, n_o, n_v)
+    h = syn.canonical_core_hamiltonian(L, eps, n_o)
+    t2 = syn.doubles_amplitudes(seed, n_o, n_v, sigma=1e-3)
+    N = n_o * n_v
+    with ccsd.CCSD(n_o, n_v, n_J) as wf:
+        wf.set_cholesky(L)
+        wf.set_oei(h)
+        wf.set_orbital_energies(eps)
+        # t = 0: Omega_ai = F_ai = 0 (canonical), Omega_aibj = g_aibj with 1/2 on the diagonal
+        wf.set_amplitudes(np.zeros(wf.n_gs_amplitudes))
+        wf.construct_t1_cholesky()
+        wf.construct_fock()
+        om0 = wf.construct_omega(ccsd.FP64)
+        Lvo = np.ascontiguousarray(L[:, n_o:, :n_o]).reshape(n_J, N, order='F')
+        g = Lvo.T @ Lvo                                          # g_(ai),(bj)
+        g[np.diag_indices(N)] *= 0.5
+        assert np.abs(om0[:N]).max() < 1e-12
+        assert np.abs(om0[N:] - O.packin(g)).max() <= 1e-12 * np.abs(g).max()
+        del g, Lvo
+        t = np.zeros(wf.n_gs_amplitudes)
+        t[N:] = t2
+        wf.set_amplitudes(t)
+        wf.construct_t1_cholesky()
+        wf.construct_fock()
+        ref = wf.construct_omega(ccsd.FP64)
+        assert np.isfinite(ref).all()
+        scale = np.abs(ref).max()
+        wf.set_ladder_batch(57)                                   # another g_acbd block plan
+        other = wf.construct_omega(ccsd.FP64)
+        assert np.abs(other - ref).max() <= 1e-13 * scale
+        for p, bound in ((ccsd.FP32, 2e-5), (ccsd.BF16X3, 1e-4)):
+            om = wf.construct_omega(p)
+            err = np.abs(om - ref).max() / scale
+            print('cfg3: precision %d max rel error %.3e' % (p, err), flush=True)
+            assert np.isfinite(om).all() and 0.0 < err <= bound
+
+
+# ---- 5 ------------------------------------------------------------------------------------
+
+def test_cfg5_hexadecamer_ground_state_multipliers_dipole():
+    print('cfg5: (H2O)16 producer', flush=True)
+    from et_amd import properties
+    mol = M.Molecule(M.water_lattice((4, 2, 2), 2.9, 0))
+    try:
+        r = mol.run(cholesky_threshold=1e-8, scf_threshold=1e-10)
+        n_o, n_J = r['n_o'], r['n_J']
+        n_v = r['L_mo'].shape[1] - n_o
+        assert (n_o, n_o + n_v) == (80, 384)
+        with ccsd.CCSD(n_o, n_v, n_J) as wf:
+            load(wf, r)
+            del r['L_mo']
+            print('cfg5: mixed-precision ground state', flush=True)
+            solvers.MixedPrecisionGroundState(wf, residual_threshold=1e-10).run()
+            wf.construct_fock()
+            wf.construct_omega(ccsd.FP64, copy_out=False)
+            gs_norm = wf.omega_norm()
+            print('cfg5: ||Omega_FP64|| %.3e, E_corr %.12f; mixed-precision multipliers'
+                  % (gs_norm, wf.correlation_energy), flush=True)
+            assert gs_norm <= 1e-10
+            solvers.MixedPrecisionMultipliers(wf, residual_threshold=1e-8).run()
+            wf.construct_multiplier_equation(copy_out=False, precision=ccsd.FP64)
+            m_norm = wf.multiplier_residual_norm()
+            print('cfg5: ||A^T tbar + eta|| %.3e; dipole' % m_norm, flush=True)
+            assert m_norm <= 1e-8
+            mu = properties.dipole_moment(wf, mol, r['C'])
+            D = mol.dipole_integrals()
+            forward = np.array([-wf.one_electron_expectation(r['C'].T @ D[c] @ r['C']) for c in range(3)])
+            print('cfg5: electronic dipole (density) %s, (forward replays) %s'
+                  % (mu['electronic'], forward), flush=True)
+            assert np.abs(forward - mu['electronic']).max() <= 1e-9 * max(1.0, np.abs(forward).max())
+    finally:
+        mol.close()

@@ -162,3 +162,19 @@ def test_jacobian_diagonal_convention_on_determinant_space():
     t2c[k - n_t1] += 2j * step       # amplitude change 2 eps <-> operator E_ai E_ai with weight eps
     direct = O.residual(L, h, t1 + 0j, t2c, n_o).imag / step
     np.testing.assert_allclose(rho, direct, rtol=0, atol=1e-13)
+
+
+@pytest.mark.parametrize('n_o,n_v,n_J,seed', [(2, 3, 6, 1), (4, 7, 12, 3), (5, 4, 10, 4), (6, 11, 20, 6)])
+def test_batched_ladder_equals_single_batch(n_o, n_v, n_J, seed):
+    """eT's (a-batch, b-batch) ladder with the C gathers and scatter (oracle/ladder_pm.c)
+    equals the single-batch +/- form for every batch length, including 1 and n_v."""
+    s = syn.system(seed, n_o, n_v, n_J, t2_sigma=0.03, t1_sigma=0.03, J_decay=6.0, L_scale=0.3)
+    Lt = O.t1_cholesky(s['L'], s['t1'], n_o)
+    ref = O.omega_a2_plus_minus(Lt, s['t2'], n_o, n_v)
+    for batch in sorted({1, 2, max(1, n_v // 2), n_v}):
+        got = O.omega_a2_batched(Lt, s['t2'], n_o, n_v, batch)
+        assert np.abs(got - ref).max() <= 1e-14 * np.abs(ref).max(), batch
+    F = O.fock_t1(s['h'], Lt, s['t1'], n_o)
+    full = O.omega(s['L'], Lt, F, s['t1'], s['t2'], n_o)
+    assert np.abs(O.omega(s['L'], Lt, F, s['t1'], s['t2'], n_o, ladder_batch=2) - full).max() \
+        <= 1e-14 * np.abs(full).max()
