@@ -37,9 +37,24 @@ def sources():
     return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith('.hip'))
 
 
+# The split-bf16 product keeps its 256 FP32 accumulators per lane in the
+# accumulation registers (the default MFMA form): its fragments fill the VGPRs.
+FILE_FLAGS = {'gemm_bf16.hip': ['-mllvm', '-amdgpu-mfma-vgpr-form=0']}
+
+
+def _flags(src):
+    extra = FILE_FLAGS.get(os.path.basename(src))
+    if not extra:
+        return HIPFLAGS
+    flags = list(HIPFLAGS)
+    i = flags.index('-amdgpu-mfma-vgpr-form=1')
+    del flags[i - 1:i + 1]
+    return flags + extra
+
+
 def _compile(src):
     obj = os.path.join(OBJDIR, os.path.basename(src) + '.o')
-    cmd = [hipcc(), *HIPFLAGS, '-c', src, '-o', obj]
+    cmd = [hipcc(), *_flags(src), '-c', src, '-o', obj]
     res = subprocess.run(cmd, capture_output=True, text=True)
     msgs = [ln for ln in (res.stdout + res.stderr).splitlines()
             if 'packed-fp32-ops' not in ln]
