@@ -72,6 +72,8 @@ _SIGNATURES = {
                                 _i64, ctypes.c_int, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p,
                                 ctypes.POINTER(_i64)],
     'etg_test_reduced_eigen': [_i64, _c_dbl_p, _i64, _c_dbl_p, _c_dbl_p, _c_dbl_p],
+    'etg_reduced_eigen_noalloc': [_i64, ctypes.c_void_p, _i64, _i64, ctypes.c_void_p, ctypes.c_void_p,
+                                  ctypes.c_void_p, _i64],
     'etg_set_davidson_start_precision': [ctypes.c_void_p, ctypes.c_int],
     'etg_diis_initialize': [ctypes.c_void_p, _i64],
     'etg_quasi_newton_update': [ctypes.c_void_p],

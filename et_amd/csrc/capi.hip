@@ -406,6 +406,15 @@ int etg_test_reduced_eigen(int64_t k, const double* G, int64_t n_want, double* w
   });
 }
 
+int etg_reduced_eigen_noalloc(int64_t k, const double* G_device, int64_t ldg, int64_t n_want, double* wr,
+                              double* wi, double* X_device, int64_t ldx) {
+  return guarded([&] {
+    if (G_device == nullptr || wr == nullptr || wi == nullptr || (n_want > 0 && X_device == nullptr))
+      etg::fail("etg_reduced_eigen_noalloc: null pointer");
+    etg::reduced_eigen_solve_noalloc(nullptr, G_device, k, ldg, n_want, wr, wi, X_device, ldx);
+  });
+}
+
 // Times `reps` products C = A B (op N,N unless ta/tb say otherwise) on
 // device-resident operands filled with a fixed pattern; returns seconds per product.
 int etg_bench_gemm(char ta, char tb, int64_t m, int64_t n, int64_t k, int precision, int reps,

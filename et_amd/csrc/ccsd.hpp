@@ -16,9 +16,13 @@ namespace etg {
 // wavefront; dimension <= 128): G (device, ld ldg) -> wr, wi (host, k values by
 // ascending real part), X (device, k x n_want, ld ldx).  Throws on complex
 // wanted roots or QR non-convergence.
-void reduced_eigen_solve(hipStream_t stream, DeviceBuffer& scratch, const double* G_dev, int64_t k,
+void reduced_eigen_solve(hipStream_t stream, DeviceBuffer& workspace, const double* G_dev, int64_t k,
                          int64_t ldg, int64_t n_want, double* wr, double* wi, double* X_dev,
                          int64_t ldx);
+// The same solve with its workspace in a static device array (no allocation;
+// dimension <= 128).  Calls that overlap on one GPU share that array and race.
+void reduced_eigen_solve_noalloc(hipStream_t stream, const double* G_dev, int64_t k, int64_t ldg,
+                                 int64_t n_want, double* wr, double* wi, double* X_dev, int64_t ldx);
 
 // One DIIS space in HBM (diis_tool_class.F90): up to `dim` records of an error
 // vector and a parameter vector of length n, the matrix of error overlaps and

@@ -21,6 +21,8 @@
 // real and imaginary residuals, as eT's Davidson tool does.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "ccsd.hpp"
@@ -275,14 +277,25 @@ __global__ void precondition_kernel(const double* __restrict__ r, const double* 
 
 }  // namespace
 
-void reduced_eigen_solve(hipStream_t stream, DeviceBuffer& scratch, const double* G_dev, int64_t k,
-                         int64_t ldg, int64_t n_want, double* wr, double* wi, double* X_dev,
-                         int64_t ldx) {
+namespace {
+
+// Doubles of the reduced eigensolver's device workspace: eigenvalues (2 x 128),
+// the real 2k x 2k systems of complex pairs (4 x 128^2), the status flag.
+constexpr size_t kEigenWork = 2 * kMaxRed + 4 * kMaxRed * kMaxRed + 1;
+
+// The same workspace as a file-scope array, for the entry point that allocates
+// nothing (one copy per GPU, 526 KB, held until the process ends).  Host code
+// uses only the address hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) double scratch[kEigenWork];
+
+// Launches the solve with its workspace at `base` (kEigenWork doubles) and
+// returns wr, wi to the host; throws on QR non-convergence.
+void reduced_eigen_launch(hipStream_t stream, double* base, const double* G_dev, int64_t k, int64_t ldg,
+                          int64_t n_want, double* wr, double* wi, double* X_dev, int64_t ldx) {
   if (k < 1 || k > kMaxRed) fail("reduced eigenproblem: dimension must be in [1, 128]");
   if (n_want < 0 || n_want > k) fail("reduced eigenproblem: invalid number of wanted roots");
   if (ldg < k || ldx < k) fail("reduced eigenproblem: leading dimension below the dimension");
-  scratch.resize(sizeof(double) * (2 * kMaxRed + 4 * kMaxRed * kMaxRed) + sizeof(int));
-  double* dwr = scratch.as<double>();
+  double* dwr = base;
   double* dwi = dwr + kMaxRed;
   double* work = dwi + kMaxRed;             // (2k)^2: complex-pair systems
   int* dinfo = reinterpret_cast<int*>(work + 4 * kMaxRed * kMaxRed);
@@ -295,6 +308,43 @@ void reduced_eigen_solve(hipStream_t stream, DeviceBuffer& scratch, const double
   ETG_HIP(hipMemcpyAsync(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost, stream));
   ETG_HIP(hipStreamSynchronize(stream));
   if (info == 1) fail("reduced eigenproblem: QR iteration did not converge");
+}
+
+// Device address of `scratch` on the current device, looked up once per device.
+double* scratch_address() {
+  static std::mutex mu;
+  static std::vector<double*> addr;
+  int dev = 0;
+  ETG_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(mu);
+  if (size_t(dev) >= addr.size()) addr.resize(size_t(dev) + 1, nullptr);
+  if (addr[size_t(dev)] == nullptr) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr)
+      fail(std::string("reduced eigenproblem: no device address for the static workspace: ") +
+           hipGetErrorString(e));
+    addr[size_t(dev)] = static_cast<double*>(p);
+  }
+  return addr[size_t(dev)];
+}
+
+}  // namespace
+
+void reduced_eigen_solve(hipStream_t stream, DeviceBuffer& workspace, const double* G_dev, int64_t k,
+                         int64_t ldg, int64_t n_want, double* wr, double* wi, double* X_dev,
+                         int64_t ldx) {
+  if (k < 1 || k > kMaxRed) fail("reduced eigenproblem: dimension must be in [1, 128]");
+  workspace.resize(sizeof(double) * kEigenWork);
+  reduced_eigen_launch(stream, workspace.as<double>(), G_dev, k, ldg, n_want, wr, wi, X_dev, ldx);
+}
+
+void reduced_eigen_solve_noalloc(hipStream_t stream, const double* G_dev, int64_t k, int64_t ldg,
+                                 int64_t n_want, double* wr, double* wi, double* X_dev, int64_t ldx) {
+  if (k < 1 || k > kMaxRed)
+    fail("reduced eigenproblem (no-allocation entry): dimension " + std::to_string(k) +
+         " outside [1, 128], the size of its static workspace");
+  reduced_eigen_launch(stream, scratch_address(), G_dev, k, ldg, n_want, wr, wi, X_dev, ldx);
 }
 
 void CCSD::reduced_eigen(const double* G_dev, int64_t k, int64_t ldg, int64_t n_want, double* wr,

@@ -526,6 +526,16 @@ module etgpu_interface
          real(c_double), dimension(*), intent(out) :: wr, wi, X
       end function etg_test_reduced_eigen
 !
+!     The same solve on device memory with a static workspace: allocates nothing
+!     (for callers that cannot allocate; k <= 128).  G and X are device addresses.
+      integer(c_int) function etg_reduced_eigen_noalloc(k, G, ldg, n_want, wr, wi, X, ldx) &
+                                                        bind(C, name='etg_reduced_eigen_noalloc')
+         import :: c_int, c_int64_t, c_double, c_ptr
+         integer(c_int64_t), value :: k, ldg, n_want, ldx
+         type(c_ptr), value :: G, X
+         real(c_double), dimension(*), intent(out) :: wr, wi
+      end function etg_reduced_eigen_noalloc
+!
    end interface
 !
 contains

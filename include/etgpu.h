@@ -387,6 +387,18 @@ int etg_test_gemm(char ta, char tb, int64_t m, int64_t n, int64_t k, double alph
 int etg_test_reduced_eigen(int64_t k, const double* G, int64_t n_want, double* wr, double* wi,
                            double* X);
 
+/* The reduced-space eigensolver of etg_test_reduced_eigen on caller-owned
+ * device memory, allocating nothing: G (device, k x k, ld ldg) -> wr, wi (host,
+ * k values by ascending real part), X (device, k x n_want, ld ldx).  Its
+ * workspace (eigenvalue staging, the complex-pair systems, the status flag;
+ * 526 KB) is a static device array, one per GPU, held until the process ends;
+ * k above 128 (the array's size) is rejected with an error.  Calls that overlap
+ * on one GPU share that array and race.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight solver or serving loops).  Runs on the null stream and synchronises it. */
+int etg_reduced_eigen_noalloc(int64_t k, const double* G_device, int64_t ldg, int64_t n_want,
+                              double* wr, double* wi, double* X_device, int64_t ldx);
+
 int etg_bench_gemm(char ta, char tb, int64_t m, int64_t n, int64_t k, int precision, int reps,
                    double* seconds);
 
