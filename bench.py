@@ -169,6 +169,51 @@ def jacobian_throughput(wf):
     return out
 
 
+def jacobian_sweep(r, n_vecs=(1, 7, 12, 24, 36)):
+    """Batched Jacobian transformations on the system r after an FP64 ground state:
+    for each side, precision and trial-vector count (7: an uneven group), the wall
+    time of one etg_jacobian_transformations call (graph replays; host vectors, so
+    two PCIe copies per vector are included), and from the per-kernel ledger of an
+    eager pass of the same call the products' FLOPs per transformation and their
+    rate against the matrix-core roofline of the precision."""
+    from et_amd import solvers
+    n_o = r['n_o']
+    n_v = r['L_mo'].shape[1] - n_o
+    out = {'workload': 'h2o8_ccpvdz_seed0', 'n_o': n_o, 'n_v': n_v, 'n_J': r['n_J'], 'rows': []}
+    with ccsd.CCSD(n_o, n_v, r['n_J']) as wf:
+        load(wf, r)
+        solvers.DiisCCGroundState(wf, residual_threshold=1e-8).run()
+        rng = np.random.default_rng(11)
+        X = rng.standard_normal((wf.n_gs_amplitudes, max(n_vecs)))
+        for side in ('right', 'left'):
+            wf.prepare_for_jacobian(side)
+            for prec, pname in ((ccsd.FP64, 'fp64'), (ccsd.BF16X3, 'bf16x3')):
+                for nv in n_vecs:
+                    wf.jacobian_transformations(X[:, :nv], side, prec)       # sizes, captures
+                    wf.synchronize()
+                    t0 = time.perf_counter()
+                    wf.jacobian_transformations(X[:, :nv], side, prec)
+                    wf.synchronize()
+                    dt = time.perf_counter() - t0
+                    wf.set_kernel_ledger(True)
+                    wf.jacobian_transformations(X[:, :nv], side, prec)
+                    ledger = wf.kernel_ledger()
+                    wf.set_kernel_ledger(False)
+                    fl = sum(e[3] for e in ledger)
+                    sec = sum(e[2] for e in ledger)
+                    row = {'side': side, 'precision': pname, 'n_vec': nv,
+                           'transforms_per_s': nv / dt, 'ms_per_transform': dt / nv * 1e3,
+                           'product_flops_per_transform': fl / nv,
+                           'product_tflops': fl / sec / 1e12 if sec > 0 else None,
+                           'product_frac': fl / sec / PEAKS[pname] if sec > 0 else None,
+                           'product_ms_per_transform': sec / nv * 1e3}
+                    out['rows'].append(row)
+                    log('jacobian %s %s n_vec %d: %.1f ms per transform, products %.1f TF/s (%.0f %% of peak)'
+                        % (side, pname, nv, row['ms_per_transform'], row['product_tflops'] or 0,
+                           100 * (row['product_frac'] or 0)))
+    return out
+
+
 def water_octamer(seed=0, cholesky_threshold=1e-8):
     """(H2O)_8 / cc-pVDZ (o = 40, v = 152): eight waters on a 2 x 2 x 2 lattice of
     2.9 Angstrom with seeded uniform rotations (1.5 Angstrom rejection), through
@@ -526,6 +571,8 @@ def main():
         octamer = water_octamer()
         extra['time_to_solution'] = time_to_solution(octamer)
         extra['eom_time_to_solution'] = eom_time_to_solution(octamer)
+        if not args.no_jacobian:
+            extra['jacobian_sweep'] = jacobian_sweep(octamer)
         del octamer
 
     cpu = finish_cpu_baseline(*cpu_job) if cpu_job else None

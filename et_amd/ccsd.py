@@ -75,6 +75,7 @@ _SIGNATURES = {
     'etg_reduced_eigen_noalloc': [_i64, ctypes.c_void_p, _i64, _i64, ctypes.c_void_p, ctypes.c_void_p,
                                   ctypes.c_void_p, _i64],
     'etg_set_davidson_start_precision': [ctypes.c_void_p, ctypes.c_int],
+    'etg_set_davidson_space': [ctypes.c_void_p, ctypes.c_int],
     'etg_diis_initialize': [ctypes.c_void_p, _i64],
     'etg_quasi_newton_update': [ctypes.c_void_p],
     'etg_diis_update': [ctypes.c_void_p],
@@ -465,6 +466,10 @@ class CCSD:
     def set_davidson_start_precision(self, precision=None):
         """Precision of the start block's transformation in davidson() (None: the solve's)."""
         _check(self._lib.etg_set_davidson_start_precision(self._h, -1 if precision is None else int(precision)))
+
+    def set_davidson_space(self, where='auto'):
+        """Placement of davidson()'s V and A V: 'auto', 'hbm' or 'host' (pinned host memory)."""
+        _check(self._lib.etg_set_davidson_space(self._h, {'auto': 0, 'hbm': 1, 'host': 2}[where]))
 
     def linear_davidson(self, rhs, side='right', start=None, residual_threshold=1e-6, max_iterations=100,
                         max_dim=50, precision=FP64):

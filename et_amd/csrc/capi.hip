@@ -200,6 +200,10 @@ int etg_davidson(etg_handle* h, int side, int64_t n_states, double residual_thre
   });
 }
 
+int etg_set_davidson_space(etg_handle* h, int mode) {
+  return guarded([&] { wf_of(h).set_davidson_space(mode); });
+}
+
 int etg_set_davidson_start_precision(etg_handle* h, int precision) {
   return guarded([&] {
     if (precision >= 0) (void)prec_of(precision);

@@ -153,6 +153,12 @@ class CCSD {
   // block plus reduced-precision corrections of the size of the start residuals.
   // start_precision < 0: the solve's precision.
   void set_davidson_start_precision(int start_precision) { dav_start_prec_ = start_precision; }
+  // Where the Davidson space V, A V lives: 0 automatic (HBM when both fit in
+  // half of the free HBM, else pinned host memory), 1 HBM, 2 pinned host memory.
+  void set_davidson_space(int mode) {
+    if (mode < 0 || mode > 2) fail("davidson space: mode must be 0 (automatic), 1 (HBM) or 2 (host)");
+    dav_space_ = mode;
+  }
   // lowest n_want eigenpairs of the k x k matrix G (device, ld ldg): wr, wi (host,
   // k, ascending real part), X (device, k x n_want, ld ldx)
   void reduced_eigen(const double* G_dev, int64_t k, int64_t ldg, int64_t n_want, double* wr,
@@ -239,6 +245,7 @@ class CCSD {
   // Davidson space (V, AV), Ritz block, reduced matrices
   DeviceBuffer dav_V_, dav_AV_, dav_Y_, dav_small_, red_eig_;
   int dav_start_prec_ = -1;                       // < 0: the solve's precision
+  int dav_space_ = 0;                             // V, AV placement (set_davidson_space)
   DeviceBuffer cvs_core_;                         // n_o flags: core occupied MOs
   bool cvs_ = false, remove_core_ = false, ionization_ = false;
 

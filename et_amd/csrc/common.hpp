@@ -37,7 +37,7 @@ class DeviceBuffer {
   ~DeviceBuffer() { release(); }
   DeviceBuffer(const DeviceBuffer&) = delete;
   DeviceBuffer& operator=(const DeviceBuffer&) = delete;
-  DeviceBuffer(DeviceBuffer&& o) noexcept : ptr_(o.ptr_), bytes_(o.bytes_) {
+  DeviceBuffer(DeviceBuffer&& o) noexcept : ptr_(o.ptr_), bytes_(o.bytes_), host_(o.host_) {
     o.ptr_ = nullptr;
     o.bytes_ = 0;
   }
@@ -46,6 +46,7 @@ class DeviceBuffer {
       release();
       ptr_ = o.ptr_;
       bytes_ = o.bytes_;
+      host_ = o.host_;
       o.ptr_ = nullptr;
       o.bytes_ = 0;
     }
@@ -54,27 +55,37 @@ class DeviceBuffer {
   // Grows the allocation (contents are not preserved).  Work queued on any
   // stream may still use the old block, so the device is drained first.
   // Grows to at least `bytes` (contents are not kept); true when the memory moved.
-  bool resize(size_t bytes) {
-    if (bytes <= bytes_) return false;
-    if (ptr_) ETG_HIP(hipDeviceSynchronize());
-    release();
-    if (bytes == 0) return false;
-    ETG_HIP(hipMalloc(&ptr_, bytes));
-    bytes_ = bytes;
-    return true;
-  }
+  bool resize(size_t bytes) { return place(bytes, false); }
+  // The same in pinned host memory that kernels read and write over PCIe with
+  // the same address (for arrays larger than the free HBM).
+  bool resize_host(size_t bytes) { return place(bytes, true); }
   void release() {
-    if (ptr_) (void)hipFree(ptr_);
+    if (ptr_) (void)(host_ ? hipHostFree(ptr_) : hipFree(ptr_));
     ptr_ = nullptr;
     bytes_ = 0;
   }
   template <typename T = double>
   T* as() const { return static_cast<T*>(ptr_); }
   size_t bytes() const { return bytes_; }
+  bool on_host() const { return host_; }
 
  private:
+  bool place(size_t bytes, bool host) {
+    if (bytes <= bytes_ && host == host_) return false;
+    if (ptr_) ETG_HIP(hipDeviceSynchronize());
+    release();
+    host_ = host;
+    if (bytes == 0) return false;
+    if (host)
+      ETG_HIP(hipHostMalloc(&ptr_, bytes, hipHostMallocDefault));
+    else
+      ETG_HIP(hipMalloc(&ptr_, bytes));
+    bytes_ = bytes;
+    return true;
+  }
   void* ptr_ = nullptr;
   size_t bytes_ = 0;
+  bool host_ = false;
 };
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }

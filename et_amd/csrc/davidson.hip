@@ -265,8 +265,9 @@ __global__ void ritz_residual_kernel(const double* __restrict__ AY, const double
 }
 
 // x_i = r_i / (w - d_i), guarded where the denominator vanishes (diagonal preconditioner)
-__global__ void precondition_kernel(const double* __restrict__ r, const double* __restrict__ d,
-                                    double w, int64_t n, double* __restrict__ x) {
+// x = r / (w - d) elementwise (x may be r itself)
+__global__ void precondition_kernel(const double* r, const double* __restrict__ d, double w, int64_t n,
+                                    double* x) {
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += int64_t(gridDim.x) * blockDim.x) {
     double den = w - d[i];
@@ -367,11 +368,33 @@ int64_t CCSD::davidson(int side, int64_t n_states, double threshold, int64_t max
   const double lindep = std::min(1e-11, 0.1 * threshold);
   const int64_t s = n_states, s1 = n_states + 1;    // a complex pair may straddle the last root
 
-  // HBM: V, AV (n x max_dim), Y, AY, R (n x s1), G (max_dim^2), Xr (max_dim x s1), coefficients
-  dav_V_.resize(sizeof(double) * size_t(n) * size_t(max_dim));
-  dav_AV_.resize(sizeof(double) * size_t(n) * size_t(max_dim));
+  // HBM: Y, AY, R (n x s1), G (max_dim^2), Xr and the projection coefficients
+  // (max_dim x s1 each), [w | c | partner] (3 s1), Gram-Schmidt coefficients.
+  // V and AV (n x max_dim): in HBM when both fit in half of the free memory
+  // (the transformation passes size their vector groups from the rest),
+  // otherwise AV, and then V as well, in pinned host memory that the kernels
+  // read and write over PCIe (etg_set_davidson_space overrides the choice).
   dav_Y_.resize(sizeof(double) * size_t(n) * size_t(3 * s1));
-  dav_small_.resize(sizeof(double) * size_t(max_dim * max_dim + max_dim * s1 + 3 * s1 + max_dim));
+  dav_small_.resize(sizeof(double) * size_t(max_dim * max_dim + 2 * max_dim * s1 + 3 * s1 + max_dim));
+  {
+    const size_t space = sizeof(double) * size_t(n) * size_t(max_dim);
+    size_t free_b = 0, total_b = 0;
+    ETG_HIP(hipMemGetInfo(&free_b, &total_b));
+    const size_t held = (dav_V_.on_host() ? 0 : dav_V_.bytes()) + (dav_AV_.on_host() ? 0 : dav_AV_.bytes());
+    const double avail = 0.5 * double(free_b + held);
+    bool v_host = false, av_host = false;
+    if (dav_space_ == 2) {
+      v_host = av_host = true;
+    } else if (dav_space_ == 0) {
+      av_host = 2.0 * double(space) > avail;
+      v_host = double(space) > avail;
+    }
+    // release first so that the other buffer can take the memory
+    if (dav_V_.on_host() != v_host) dav_V_.release();
+    if (dav_AV_.on_host() != av_host) dav_AV_.release();
+    if (av_host) dav_AV_.resize_host(space); else dav_AV_.resize(space);
+    if (v_host) dav_V_.resize_host(space); else dav_V_.resize(space);
+  }
   double* V = dav_V_.as<double>();
   double* AV = dav_AV_.as<double>();
   double* Y = dav_Y_.as<double>();
@@ -379,7 +402,8 @@ int64_t CCSD::davidson(int side, int64_t n_states, double threshold, int64_t max
   double* Rb = Y + 2 * n * s1;
   double* G = dav_small_.as<double>();
   double* Xr = G + max_dim * max_dim;
-  double* coef = Xr + max_dim * s1;          // [w | c | partner] (3 s1)
+  double* cmat = Xr + max_dim * s1;          // projection coefficients (max_dim x s1)
+  double* coef = cmat + max_dim * s1;        // [w | c | partner] (3 s1)
   double* cvec = coef + 3 * s1;              // Gram-Schmidt coefficients (max_dim)
   diff_.resize(sizeof(double) * n);
   double* d = diff_.as<double>();
@@ -395,14 +419,16 @@ int64_t CCSD::davidson(int side, int64_t n_states, double threshold, int64_t max
   };
   if (start) {
     // caller's start vectors (e.g. a reduced-precision solution), orthonormalised
-    ETG_HIP(hipMemcpyAsync(V, start, sizeof(double) * n * s, hipMemcpyDefault, stream_));
+    // in HBM (the residual block) and then placed in the space
+    ETG_HIP(hipMemcpyAsync(Rb, start, sizeof(double) * n * s, hipMemcpyDefault, stream_));
     for (int64_t j = 0; j < s; ++j) {
       for (int pass = 0; pass < 2; ++pass)
-        for (int64_t i = 0; i < j; ++i) axpy(stream_, n, -dot_host(V + i * n, V + j * n), V + i * n, V + j * n);
-      const double nv = std::sqrt(dot_host(V + j * n, V + j * n));
+        for (int64_t i = 0; i < j; ++i) axpy(stream_, n, -dot_host(Rb + i * n, Rb + j * n), Rb + i * n, Rb + j * n);
+      const double nv = std::sqrt(dot_host(Rb + j * n, Rb + j * n));
       if (!(nv > 1e-10)) fail("davidson: start vectors are linearly dependent");
-      scale(stream_, V + j * n, n, 1.0 / nv);
+      scale(stream_, Rb + j * n, n, 1.0 / nv);
     }
+    ETG_HIP(hipMemcpyAsync(V, Rb, sizeof(double) * n * s, hipMemcpyDefault, stream_));
   } else {
     // unit vectors on the lowest orbital differences
     std::vector<double> hd(n);
@@ -412,10 +438,11 @@ int64_t CCSD::davidson(int side, int64_t n_states, double threshold, int64_t max
     for (int64_t i = 0; i < n; ++i) idx[i] = i;
     std::partial_sort(idx.begin(), idx.begin() + s, idx.end(),
                       [&](int64_t a, int64_t b) { return hd[a] < hd[b] || (hd[a] == hd[b] && a < b); });
-    fill(stream_, V, n * s, 0.0);
+    fill(stream_, Rb, n * s, 0.0);
     const double one = 1.0;
     for (int64_t j = 0; j < s; ++j)
-      ETG_HIP(hipMemcpyAsync(V + j * n + idx[j], &one, sizeof(double), hipMemcpyHostToDevice, stream_));
+      ETG_HIP(hipMemcpyAsync(Rb + j * n + idx[j], &one, sizeof(double), hipMemcpyHostToDevice, stream_));
+    ETG_HIP(hipMemcpyAsync(V, Rb, sizeof(double) * n * s, hipMemcpyDefault, stream_));
   }
   synchronize();
   int64_t k = 0, m = s;                   // space dimension, new vectors at V[:, k..k+m)
@@ -506,44 +533,62 @@ int64_t CCSD::davidson(int side, int64_t n_states, double threshold, int64_t max
     int64_t n_new = 0;
     for (int64_t j = 0; j < se; ++j) n_new += norms[j] >= threshold;
     if (k + n_new > max_dim) {
-      ETG_HIP(hipMemcpyAsync(V, Y, sizeof(double) * n * se, hipMemcpyDeviceToDevice, stream_));
-      ETG_HIP(hipMemcpyAsync(AV, AY, sizeof(double) * n * se, hipMemcpyDeviceToDevice, stream_));
-      // orthonormalise the Ritz vectors (and their transforms alike)
+      // orthonormalise the Ritz vectors (and their transforms alike) in HBM,
+      // then they become the space
       for (int64_t j = 0; j < se; ++j) {
         for (int pass = 0; pass < 2; ++pass)
           for (int64_t i = 0; i < j; ++i) {
-            const double c = dot_host(V + i * n, V + j * n);
-            axpy(stream_, n, -c, V + i * n, V + j * n);
-            axpy(stream_, n, -c, AV + i * n, AV + j * n);
+            const double c = dot_host(Y + i * n, Y + j * n);
+            axpy(stream_, n, -c, Y + i * n, Y + j * n);
+            axpy(stream_, n, -c, AY + i * n, AY + j * n);
           }
-        const double nv = std::sqrt(dot_host(V + j * n, V + j * n));
-        scale(stream_, V + j * n, n, 1.0 / nv);
-        scale(stream_, AV + j * n, n, 1.0 / nv);
+        const double nv = std::sqrt(dot_host(Y + j * n, Y + j * n));
+        scale(stream_, Y + j * n, n, 1.0 / nv);
+        scale(stream_, AY + j * n, n, 1.0 / nv);
       }
-      run_gemm(make_gemm('T', 'N', se, se, n, 1.0, V, n, AV, n, 0.0, G, max_dim), Prec::F64);
+      run_gemm(make_gemm('T', 'N', se, se, n, 1.0, Y, n, AY, n, 0.0, G, max_dim), Prec::F64);
+      ETG_HIP(hipMemcpyAsync(V, Y, sizeof(double) * n * se, hipMemcpyDefault, stream_));
+      ETG_HIP(hipMemcpyAsync(AV, AY, sizeof(double) * n * se, hipMemcpyDefault, stream_));
       k = se;
     }
-    // preconditioned residuals, orthonormalised against the space (two passes)
-    m = 0;
+    // preconditioned residuals of the unconverged roots, normalised, in the
+    // first columns of the residual block (column j moves to nc <= j, already read)
+    int64_t nc = 0;
     for (int64_t j = 0; j < se; ++j) {
       if (norms[j] < threshold) continue;
-      double* x = V + (k + m) * n;
+      double* x = Rb + nc * n;
       precondition_kernel<<<grid_for_d(n), kBlock, 0, stream_>>>(Rb + j * n, d, hcoef[j], n, x);
       ETG_LAUNCH("precondition");
-      double nx = std::sqrt(dot_host(x, x));
+      const double nx = std::sqrt(dot_host(x, x));
       if (nx == 0.0) continue;
       scale(stream_, x, n, 1.0 / nx);
-      const int64_t kk = k + m;
+      ++nc;
+    }
+    // orthogonalised against the space as one block (two passes: one read of V
+    // per product however many vectors are new) ...
+    if (k > 0 && nc > 0)
       for (int pass = 0; pass < 2; ++pass) {
-        run_gemm(make_gemm('T', 'N', kk, 1, n, 1.0, V, n, x, n, 0.0, cvec, max_dim), Prec::F64);
-        run_gemm(make_gemm('N', 'N', n, 1, kk, -1.0, V, n, cvec, max_dim, 1.0, x, n), Prec::F64);
+        run_gemm(make_gemm('T', 'N', k, nc, n, 1.0, V, n, Rb, n, 0.0, cmat, max_dim), Prec::F64);
+        run_gemm(make_gemm('N', 'N', n, nc, k, -1.0, V, n, cmat, max_dim, 1.0, Rb, n), Prec::F64);
       }
-      nx = std::sqrt(dot_host(x, x));
+    // ... then against each other (two passes), dropping linearly dependent ones
+    m = 0;
+    for (int64_t c = 0; c < nc; ++c) {
+      double* x = Rb + c * n;
+      if (m > 0)
+        for (int pass = 0; pass < 2; ++pass) {
+          run_gemm(make_gemm('T', 'N', m, 1, n, 1.0, Rb, n, x, n, 0.0, cvec, max_dim), Prec::F64);
+          run_gemm(make_gemm('N', 'N', n, 1, m, -1.0, Rb, n, cvec, max_dim, 1.0, x, n), Prec::F64);
+        }
+      const double nx = std::sqrt(dot_host(x, x));
       if (nx < lindep) continue;
       scale(stream_, x, n, 1.0 / nx);
+      if (c != m)
+        ETG_HIP(hipMemcpyAsync(Rb + m * n, x, sizeof(double) * n, hipMemcpyDeviceToDevice, stream_));
       ++m;
     }
     if (m == 0) fail("davidson: no new trial vectors (linear dependence) before convergence");
+    ETG_HIP(hipMemcpyAsync(V + k * n, Rb, sizeof(double) * n * m, hipMemcpyDefault, stream_));
   }
   fail("davidson: did not converge in the maximum number of iterations");
 }

@@ -451,6 +451,13 @@ module etgpu_interface
          integer(c_int), value :: precision
       end function etg_set_davidson_start_precision
 !
+!     placement of the Davidson space: 0 automatic, 1 HBM, 2 pinned host memory
+      integer(c_int) function etg_set_davidson_space(handle, mode) bind(C, name='etg_set_davidson_space')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: handle
+         integer(c_int), value :: mode
+      end function etg_set_davidson_space
+!
 !     virtual batch length of the ladder term's g_acbd blocks
       integer(c_int) function etg_set_ladder_batch(handle, batch) bind(C, name='etg_set_ladder_batch')
          import :: c_int, c_ptr, c_int64_t

@@ -193,6 +193,16 @@ int etg_davidson(etg_handle* handle, int side, int64_t n_states, double residual
  * precision schedule of the excited-state solve, set before etg_davidson. */
 int etg_set_davidson_start_precision(etg_handle* handle, int precision);
 
+/* Where etg_davidson keeps its trial space V and transforms A V (n x max_dim
+ * each): 0 (the default) in HBM when both fit in half of the free device
+ * memory, otherwise A V and, if needed, V as well in pinned host memory that the
+ * kernels read and write over PCIe; 1 always HBM; 2 always host memory.
+ * Results do not depend on the placement.
+ * Replaces: the records_in_memory choice of eT's Davidson tool
+ * (davidson_tool_class.F90:150, record_storer_class.F90), which keeps the
+ * trial and transformed vectors in memory or on disk. */
+int etg_set_davidson_space(etg_handle* handle, int mode);
+
 /* A X = F (side 0) or A^T X = F (side 1) by eT's linear Davidson solver, on
  * the device: start from start_vector (may be NULL: the preconditioner guess
  * F / eps), one new trial per iteration (-R / eps of the residual R = A X - F,

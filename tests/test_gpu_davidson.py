@@ -83,3 +83,31 @@ def test_davidson_lowest_roots_match_dense_jacobian(side):
     M = A if side == 'right' else A.T
     for j in range(3):
         assert np.linalg.norm(M @ X[:, j] - w[j] * X[:, j]) < 1e-8
+
+
+@pytest.mark.parametrize('side', ['right', 'left'])
+def test_davidson_space_in_host_memory(side):
+    """V and A V in pinned host memory (read and written by the kernels over PCIe)
+    give the roots of the HBM placement, with restarts of the space (max_dim 14,
+    four new vectors per iteration) and the block Gram-Schmidt of the new trial
+    vectors."""
+    n_o, n_v, n_J = 3, 6, 12
+    s = syn.system(8, n_o, n_v, n_J, t2_sigma=0.03, t1_sigma=0.03, J_decay=6.0, L_scale=0.3)
+    A = O.jacobian_matrix(s['L'], s['h'], s['t1'], s['t2'], n_o)
+    ref = np.sort(np.linalg.eigvals(A).real)[:3]
+    out = {}
+    with ccsd.CCSD(n_o, n_v, n_J) as wf:
+        wf.set_cholesky(s['L'])
+        wf.set_oei(s['h'])
+        wf.set_orbital_energies(s['eps'])
+        wf.set_amplitudes(np.concatenate([s['t1'].ravel(order='F'), s['t2']]))
+        wf.construct_t1_cholesky()
+        for where in ('hbm', 'host'):
+            wf.set_davidson_space(where)
+            print('space', where, flush=True)
+            dav = solvers.EigenDavidson(wf, 3, side, residual_threshold=1e-8, max_dim=14)
+            out[where] = (np.asarray(dav.run()), len(dav.history))
+        wf.set_davidson_space('auto')
+    np.testing.assert_allclose(out['hbm'][0], ref, rtol=0, atol=1e-8)
+    assert np.abs(out['host'][0] - out['hbm'][0]).max() < 1e-11
+    assert out['host'][1] == out['hbm'][1]
