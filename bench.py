@@ -297,6 +297,7 @@ def eom_time_to_solution(r, n_states=12, threshold=1e-5):
             res = {}
             for name in ('fp64', 'mixed'):
                 wf.synchronize()
+                wf.davidson_timing(reset=True)
                 t0 = time.perf_counter()
                 if name == 'fp64':
                     sv = solvers.EigenDavidson(wf, n_states, side, residual_threshold=threshold)
@@ -313,12 +314,15 @@ def eom_time_to_solution(r, n_states=12, threshold=1e-5):
                               for p, k, st, rr in sv.phases]
                 wf.synchronize()
                 dt = time.perf_counter() - t0
-                res[name] = {'seconds': dt, 'iterations': iters, 'energies_ev': [float(x) * HARTREE_TO_EV for x in w]}
+                tr_s, dav_s, n_tr = wf.davidson_timing(reset=True)
+                res[name] = {'seconds': dt, 'iterations': iters, 'energies_ev': [float(x) * HARTREE_TO_EV for x in w],
+                             'davidson_seconds': dav_s, 'transformation_seconds': tr_s, 'transformations': n_tr}
                 if name == 'mixed':
                     res[name]['phases'] = phases
                     res[name]['noise'] = {names[p]: v for p, v in sv.noise.items()}
                     res[name]['fp64_checks_max_residual'] = [float(c.max()) for c in sv.checks]
-                log('EOM %s %s: %.2f s, %s' % (side, name, dt, iters))
+                log('EOM %s %s: %.2f s, %s; inside the Davidson solves %.2f s, of which %d transformations %.2f s'
+                    % (side, name, dt, iters, dav_s, n_tr, tr_s))
             res['mixed']['max_abs_err_vs_fp64_gpu_ev'] = float(
                 np.abs(np.array(res['mixed']['energies_ev']) - np.array(res['fp64']['energies_ev'])).max())
             res['speedup_mixed_vs_fp64'] = res['fp64']['seconds'] / res['mixed']['seconds']

@@ -76,6 +76,7 @@ _SIGNATURES = {
                                   ctypes.c_void_p, _i64],
     'etg_set_davidson_start_precision': [ctypes.c_void_p, ctypes.c_int],
     'etg_set_davidson_space': [ctypes.c_void_p, ctypes.c_int],
+    'etg_davidson_timing': [ctypes.c_void_p, ctypes.c_int, _c_dbl_p, _c_dbl_p, ctypes.POINTER(_i64)],
     'etg_diis_initialize': [ctypes.c_void_p, _i64],
     'etg_quasi_newton_update': [ctypes.c_void_p],
     'etg_diis_update': [ctypes.c_void_p],
@@ -470,6 +471,14 @@ class CCSD:
     def set_davidson_space(self, where='auto'):
         """Placement of davidson()'s V and A V: 'auto', 'hbm' or 'host' (pinned host memory)."""
         _check(self._lib.etg_set_davidson_space(self._h, {'auto': 0, 'hbm': 1, 'host': 2}[where]))
+
+    def davidson_timing(self, reset=True):
+        """(seconds in transformations, seconds in davidson() in total, vectors transformed)
+        summed over the solves since the last reset."""
+        a, b, c = ctypes.c_double(), ctypes.c_double(), _i64()
+        _check(self._lib.etg_davidson_timing(self._h, 1 if reset else 0, ctypes.byref(a), ctypes.byref(b),
+                                             ctypes.byref(c)))
+        return a.value, b.value, c.value
 
     def linear_davidson(self, rhs, side='right', start=None, residual_threshold=1e-6, max_iterations=100,
                         max_dim=50, precision=FP64):

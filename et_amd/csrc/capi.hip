@@ -200,6 +200,17 @@ int etg_davidson(etg_handle* h, int side, int64_t n_states, double residual_thre
   });
 }
 
+int etg_davidson_timing(etg_handle* h, int reset, double* transform_seconds, double* total_seconds,
+                        int64_t* transforms) {
+  return guarded([&] {
+    double* t = wf_of(h).dav_timing_;
+    if (transform_seconds) *transform_seconds = t[0];
+    if (total_seconds) *total_seconds = t[1];
+    if (transforms) *transforms = int64_t(t[2]);
+    if (reset) t[0] = t[1] = t[2] = 0.0;
+  });
+}
+
 int etg_set_davidson_space(etg_handle* h, int mode) {
   return guarded([&] { wf_of(h).set_davidson_space(mode); });
 }

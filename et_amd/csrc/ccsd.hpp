@@ -246,6 +246,11 @@ class CCSD {
   DeviceBuffer dav_V_, dav_AV_, dav_Y_, dav_small_, red_eig_;
   int dav_start_prec_ = -1;                       // < 0: the solve's precision
   int dav_space_ = 0;                             // V, AV placement (set_davidson_space)
+ public:
+  // seconds in the transformations, seconds in davidson() altogether, vectors
+  // transformed, summed over the solves since the last reset
+  double dav_timing_[3] = {0.0, 0.0, 0.0};
+ private:
   DeviceBuffer cvs_core_;                         // n_o flags: core occupied MOs
   bool cvs_ = false, remove_core_ = false, ionization_ = false;
 

@@ -20,6 +20,7 @@
 // (for a complex pair a +- ib, the real 2k system of u + i v).  Complex pairs are carried as (u, v) column pairs with
 // real and imaginary residuals, as eT's Davidson tool does.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <mutex>
 #include <string>
@@ -359,6 +360,11 @@ int64_t CCSD::davidson(int side, int64_t n_states, double threshold, int64_t max
   require(tape_ != nullptr, "davidson: prepare_for_jacobian has not been called");
   require(have_eps_, "davidson: orbital energies required");
   require(side == 0 || side == 1, "davidson: side must be 0 (right) or 1 (left)");
+  struct Clock {                          // wall time of the solve (davidson_timing)
+    double& total;
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    ~Clock() { total += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
+  } clock{dav_timing_[1]};
   const int64_t n = n_t1 + n_t2;
   require(n_states >= 1 && n_states <= std::min<int64_t>(n - 1, 62), "davidson: n_states must be in [1, 62]");
   if (max_dim <= 0) max_dim = std::max<int64_t>(100, 10 * n_states);
@@ -452,7 +458,10 @@ int64_t CCSD::davidson(int side, int64_t n_states, double threshold, int64_t max
   const Prec start_prec = dav_start_prec_ < 0 ? prec : static_cast<Prec>(dav_start_prec_);
   for (int64_t it = 1; it <= max_iterations; ++it) {
     // transforms of the new trial vectors (the start block in its own precision)
+    const auto tt0 = std::chrono::steady_clock::now();
     transform_batch(side, m, V + k * n, AV + k * n, it == 1 ? start_prec : prec);
+    dav_timing_[0] += std::chrono::duration<double>(std::chrono::steady_clock::now() - tt0).count();
+    dav_timing_[2] += double(m);
     project_cvs(AV + k * n, m);              // eigen_davidson_solver_class.F90:187
     // G = V^T AV: new columns for all rows, new rows for the old columns
     const int64_t kn = k + m;

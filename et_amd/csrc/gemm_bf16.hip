@@ -41,34 +41,44 @@ constexpr int kGroupM = 8;
 //   kcontig:  element (r, k) = X[k + r ld]     (A with op = T, B with op = N)
 //   else:     element (r, k) = X[r + k ld]
 // ---------------------------------------------------------------------------
-__global__ void pack_split_bf16_kernel(const double* __restrict__ X, int64_t ld, int64_t strideX,
-                                       int kcontig, int64_t rows, int64_t K, int64_t rows_p, int64_t Kp,
-                                       __bf16* __restrict__ P) {
-  const int64_t nchunk = Kp / 8;       // units of 8 k
+// One workgroup packs a tile of 64 rows (4 row blocks) x 32 k (one step): each
+// thread reads 8 consecutive k of one row (kcontig: 4 threads span a row's 256
+// B; otherwise 64 threads span 64 rows of one k), splits them and places hi and
+// lo in the tile's four chunks in LDS; the chunks then leave as contiguous
+// 2 KB blocks.
+__global__ __launch_bounds__(256) void pack_split_bf16_kernel(const double* __restrict__ X, int64_t ld,
+                                                              int64_t strideX, int kcontig, int64_t rows,
+                                                              int64_t K, int64_t rows_p, int64_t Kp,
+                                                              __bf16* __restrict__ P) {
+  __shared__ u4 tile[4 * 128];       // 4 chunks of 2 KB
   const int64_t KS = Kp / kDepth;
-  const int64_t total = rows_p * nchunk;
+  const int64_t ks = blockIdx.x;
+  const int64_t rb0 = int64_t(blockIdx.y) * 4;
   const int64_t bz = blockIdx.z;
   const double* Xb = X + bz * strideX;
-  __bf16* Pb = P + bz * rows_p * Kp * 2;
-  for (int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; idx < total;
-       idx += int64_t(gridDim.x) * blockDim.x) {
-    int64_t r, kc;
-    if (kcontig) { kc = idx % nchunk; r = idx / nchunk; } else { r = idx % rows_p; kc = idx / rows_p; }
-    bf8 h, l;
+  const int t = threadIdx.x;
+  const int row = kcontig ? (t >> 2) : (t & 63);
+  const int kq = kcontig ? (t & 3) : (t >> 6);
+  const int64_t r = rb0 * 16 + row;
+  bf8 h, l;
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const int64_t k = 8 * kc + e;
-      double x = 0.0;
-      if (r < rows && k < K) x = kcontig ? Xb[k + r * ld] : Xb[r + k * ld];
-      const __bf16 xh = __bf16(float(x));
-      h[e] = xh;
-      l[e] = __bf16(float(x - double(float(xh))));
-    }
-    const int64_t rb = r >> 4, ks = kc >> 2;
-    const int lane = int(r & 15) + 16 * int(kc & 3);
-    __bf16* dst = Pb + (rb * KS + ks) * 1024 + lane * 8;
-    *reinterpret_cast<bf8*>(dst) = h;
-    *reinterpret_cast<bf8*>(dst + 512) = l;
+  for (int e = 0; e < 8; ++e) {
+    const int64_t k = ks * kDepth + 8 * kq + e;
+    double x = 0.0;
+    if (r < rows && k < K) x = kcontig ? Xb[k + r * ld] : Xb[r + k * ld];
+    const __bf16 xh = __bf16(float(x));
+    h[e] = xh;
+    l[e] = __bf16(float(x - double(float(xh))));
+  }
+  const int lane = (row & 15) + 16 * kq;
+  tile[(row >> 4) * 128 + lane] = __builtin_bit_cast(u4, h);
+  tile[(row >> 4) * 128 + 64 + lane] = __builtin_bit_cast(u4, l);
+  __syncthreads();
+  u4* Pb = reinterpret_cast<u4*>(P + bz * rows_p * Kp * 2);
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const int q = t + 256 * i;                     // u4 unit of the tile: chunk q / 128
+    Pb[((rb0 + (q >> 7)) * KS + ks) * 128 + (q & 127)] = tile[q];
   }
 }
 
@@ -242,8 +252,9 @@ __global__ void splitk_reduce_bf(const double* __restrict__ W, int splits, int64
 
 void pack(hipStream_t s, const double* X, int64_t ld, int64_t strideX, bool kcontig, int64_t rows,
           int64_t K, int64_t rows_p, int64_t Kp, int64_t batch, __bf16* P) {
-  const int64_t total = rows_p * (Kp / 8);
-  dim3 grid(unsigned(std::min<int64_t>((total + 255) / 256, 1 << 16)), 1, unsigned(batch));
+  if (rows_p % 64 != 0 || rows_p / 64 > 65535 || batch > 65535 || Kp / kDepth > 0x7fffffff)
+    fail("split-bf16 pack: operand shape outside the pack grid");
+  dim3 grid(unsigned(Kp / kDepth), unsigned(rows_p / 64), unsigned(batch));
   pack_split_bf16_kernel<<<grid, 256, 0, s>>>(X, ld, strideX, kcontig ? 1 : 0, rows, K, rows_p, Kp, P);
   ETG_LAUNCH("pack_split_bf16");
 }

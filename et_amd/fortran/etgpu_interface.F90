@@ -458,6 +458,16 @@ module etgpu_interface
          integer(c_int), value :: mode
       end function etg_set_davidson_space
 !
+!     diagnostic: seconds in the transformations and in total, vectors transformed
+      integer(c_int) function etg_davidson_timing(handle, reset, transform_seconds, total_seconds, &
+                                                  transforms) bind(C, name='etg_davidson_timing')
+         import :: c_int, c_ptr, c_double, c_int64_t
+         type(c_ptr), value :: handle
+         integer(c_int), value :: reset
+         real(c_double), intent(out) :: transform_seconds, total_seconds
+         integer(c_int64_t), intent(out) :: transforms
+      end function etg_davidson_timing
+!
 !     virtual batch length of the ladder term's g_acbd blocks
       integer(c_int) function etg_set_ladder_batch(handle, batch) bind(C, name='etg_set_ladder_batch')
          import :: c_int, c_ptr, c_int64_t

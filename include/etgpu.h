@@ -203,6 +203,12 @@ int etg_set_davidson_start_precision(etg_handle* handle, int precision);
  * trial and transformed vectors in memory or on disk. */
 int etg_set_davidson_space(etg_handle* handle, int mode);
 
+/* Diagnostic: wall seconds etg_davidson spent in its transformation passes and
+ * in total, and the number of vectors it transformed, summed over the solves
+ * since the last reset (reset != 0 zeroes them after reading). */
+int etg_davidson_timing(etg_handle* handle, int reset, double* transform_seconds, double* total_seconds,
+                        int64_t* transforms);
+
 /* A X = F (side 0) or A^T X = F (side 1) by eT's linear Davidson solver, on
  * the device: start from start_vector (may be NULL: the preconditioner guess
  * F / eps), one new trial per iteration (-R / eps of the residual R = A X - F,
