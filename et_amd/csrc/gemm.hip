@@ -525,66 +525,95 @@ GemmLaunch launch_typed(hipStream_t stream, const Gemm& g, GemmWorkspace* ws) {
 // than the 31-bit per-thread buffer offsets of the tile kernel span over 16
 // columns.  Its products are tall and skinny and bound by HBM, so two plain
 // FP64 kernels with 64-bit addressing serve them:
-//   gemm_tall_tn: C (m x n) = alpha A^T B + beta C, A (K x m), B (K x n), K long:
-//     per-block partial sums over row chunks (A and B rows staged through LDS),
-//     then a deterministic sum over the blocks;
+//   gemm_tall_tn: C (m x n) = alpha A^T B + beta C, A (K x m), B (K x n), K long,
+//     m <= 128, n <= 16: each block takes a range of rows in chunks of 32,
+//     staged through LDS row-major (a chunk row holds the m values of A and
+//     the n of B); wave w accumulates columns 4w .. 4w+3 of C and lane l rows
+//     2l, 2l+1, so per row an FMA-pair reads 16 B of A (lanes contiguous, no
+//     bank conflict) and a broadcast of B; the next chunk's loads are in flight
+//     while the current one is multiplied.  Per-block partial sums, then a
+//     deterministic sum over the blocks;
 //   gemm_tall_nn: C (M x n) = alpha A B + beta C, A (M x k), B (k x n) small:
 //     B in LDS, one row of C per thread, columns of A read coalesced.
 constexpr int kTallRows = 32;
 constexpr int kTallThreads = 256;
-constexpr int kTallOut = 16384;                    // m n <= 16384 (64 outputs per thread)
+constexpr int kTallM = 128;                        // rows of C (A columns)
+constexpr int kTallN = 16;                         // columns of C (B columns)
+constexpr int kTallLdA = kTallM + 2;               // LDS row stride of A (doubles; keeps 16-B alignment)
 
 __global__ __launch_bounds__(kTallThreads) void gemm_tall_tn_kernel(
     const double* __restrict__ A, int64_t lda, const double* __restrict__ B, int64_t ldb, int64_t K, int m,
     int n, int64_t rows_per_block, double* __restrict__ part) {
-  extern __shared__ double sm[];
-  double* As = sm;                                  // [m][kTallRows]
-  double* Bs = sm + int64_t(m) * kTallRows;         // [n][kTallRows]
-  const int tid = threadIdx.x, nout = m * n;
+  __shared__ double As[kTallRows * kTallLdA];      // [row][i]
+  __shared__ double Bs[kTallRows * kTallN];        // [row][j]
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int64_t r0 = int64_t(blockIdx.x) * rows_per_block;
   const int64_t r1 = min(K, r0 + rows_per_block);
-  double acc[kTallOut / kTallThreads];
+  // loads of one chunk: thread t takes row t % 32 of columns t / 32 + 8 q
+  const int lr = tid & (kTallRows - 1), lc = tid >> 5;
+  double ra[kTallM / 8], rb[kTallN / 8];
+  auto load = [&](int64_t rb0) {
+    const int64_t row = rb0 + lr;
+    const bool ok = row < r1;
 #pragma unroll
-  for (int q = 0; q < kTallOut / kTallThreads; ++q) acc[q] = 0.0;
-  for (int64_t rb = r0; rb < r1; rb += kTallRows) {
-    const int nr = int(min<int64_t>(kTallRows, r1 - rb));
-    for (int e = tid; e < kTallRows * m; e += kTallThreads) {
-      const int rr = e % kTallRows, i = e / kTallRows;
-      As[e] = rr < nr ? A[(rb + rr) + int64_t(i) * lda] : 0.0;
+    for (int q = 0; q < kTallM / 8; ++q) {
+      const int i = lc + 8 * q;
+      ra[q] = (ok && i < m) ? A[row + int64_t(i) * lda] : 0.0;
     }
-    for (int e = tid; e < kTallRows * n; e += kTallThreads) {
-      const int rr = e % kTallRows, j = e / kTallRows;
-      Bs[e] = rr < nr ? B[(rb + rr) + int64_t(j) * ldb] : 0.0;
+#pragma unroll
+    for (int q = 0; q < kTallN / 8; ++q) {
+      const int j = lc + 8 * q;
+      rb[q] = (ok && j < n) ? B[row + int64_t(j) * ldb] : 0.0;
     }
+  };
+  auto store = [&] {
+#pragma unroll
+    for (int q = 0; q < kTallM / 8; ++q) As[lr * kTallLdA + lc + 8 * q] = ra[q];
+#pragma unroll
+    for (int q = 0; q < kTallN / 8; ++q) Bs[lr * kTallN + lc + 8 * q] = rb[q];
+  };
+  double acc[2][4];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+  if (r0 < r1) load(r0);
+  for (int64_t rb0 = r0; rb0 < r1; rb0 += kTallRows) {
+    __syncthreads();                               // the previous chunk's reads are done
+    store();
     __syncthreads();
-#pragma unroll
-    for (int q = 0; q < kTallOut / kTallThreads; ++q) {
-      const int o = tid + q * kTallThreads;
-      if (o < nout) {
-        const double* a = As + (o % m) * kTallRows;
-        const double* b = Bs + (o / m) * kTallRows;
-        double s = 0.0;
+    if (rb0 + kTallRows < r1) load(rb0 + kTallRows);
 #pragma unroll 8
-        for (int rr = 0; rr < kTallRows; ++rr) s += a[rr] * b[rr];
-        acc[q] += s;
+    for (int rr = 0; rr < kTallRows; ++rr) {
+      const double2 a = *reinterpret_cast<const double2*>(&As[rr * kTallLdA + 2 * lane]);
+      const double* b = &Bs[rr * kTallN + 4 * wave];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        acc[0][c] += a.x * b[c];
+        acc[1][c] += a.y * b[c];
       }
     }
-    __syncthreads();
   }
+  const int nout = m * n;
 #pragma unroll
-  for (int q = 0; q < kTallOut / kTallThreads; ++q) {
-    const int o = tid + q * kTallThreads;
-    if (o < nout) part[int64_t(blockIdx.x) * nout + o] = acc[q];
-  }
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int i = 2 * lane + a, j = 4 * wave + c;
+      if (i < m && j < n) part[int64_t(blockIdx.x) * nout + i + j * m] = acc[a][c];
+    }
 }
 
+// C(i, j) (or C(j, i) with transpose) = alpha sum_b part_b(i, j) + beta C
 __global__ void gemm_tall_reduce_kernel(const double* __restrict__ part, int64_t nblocks, int m, int n,
-                                        double alpha, double beta, double* __restrict__ C, int64_t ldc) {
+                                        double alpha, double beta, double* __restrict__ C, int64_t ldc,
+                                        int transpose) {
   const int o = blockIdx.x * blockDim.x + threadIdx.x;
   if (o >= m * n) return;
   double s = 0.0;
   for (int64_t b = 0; b < nblocks; ++b) s += part[b * m * n + o];
-  double* c = C + (o % m) + int64_t(o / m) * ldc;
+  const int i = o % m, j = o / m;
+  double* c = transpose ? C + j + int64_t(i) * ldc : C + i + int64_t(j) * ldc;
   *c = (beta == 0.0) ? alpha * s : alpha * s + beta * *c;
 }
 
@@ -617,19 +646,30 @@ __global__ __launch_bounds__(kTallThreads) void gemm_tall_nn_kernel(
 
 GemmLaunch gemm_tall(hipStream_t stream, const Gemm& g, GemmWorkspace* ws) {
   if (g.batch != 1) fail("gemm: leading dimension too large for a batched product");
-  if (g.ta && !g.tb && g.m * g.n <= kTallOut) {
+  if (g.ta && !g.tb) {
     if (ws == nullptr) fail("gemm: tall products need a workspace");
-    const int64_t nblocks = std::max<int64_t>(1, std::min<int64_t>(1024, ceil_div(g.k, 4096)));
-    const int64_t rows = ceil_div(ceil_div(g.k, nblocks), kTallRows) * kTallRows;
-    const int64_t nb = ceil_div(g.k, rows);
-    if (ws->buf.resize(sizeof(double) * size_t(nb * g.m * g.n))) ++ws->gen;
-    const size_t lds = sizeof(double) * size_t(kTallRows * (g.m + g.n));
-    gemm_tall_tn_kernel<<<unsigned(nb), kTallThreads, lds, stream>>>(g.A, g.lda, g.B, g.ldb, g.k, int(g.m),
-                                                                      int(g.n), rows, ws->buf.as<double>());
-    ETG_LAUNCH("gemm_tall_tn");
-    gemm_tall_reduce_kernel<<<unsigned(ceil_div(g.m * g.n, 256)), 256, 0, stream>>>(
-        ws->buf.as<double>(), nb, int(g.m), int(g.n), g.alpha, g.beta, g.C, g.ldc);
-    ETG_LAUNCH("gemm_tall_reduce");
+    // the kernel takes at most 128 x 16 outputs: the longer side of C goes to its
+    // 128-row side (C^T = B^T A when n > m), larger C in 128 x 16 blocks
+    const bool swap = g.n > g.m;
+    const double* P = swap ? g.B : g.A;
+    const double* Q = swap ? g.A : g.B;
+    const int64_t ldp = swap ? g.ldb : g.lda, ldq = swap ? g.lda : g.ldb;
+    const int64_t rows = swap ? g.n : g.m, cols = swap ? g.m : g.n;
+    const int64_t nblocks = std::max<int64_t>(1, std::min<int64_t>(2048, ceil_div(g.k, 4096)));
+    const int64_t per = ceil_div(ceil_div(g.k, nblocks), kTallRows) * kTallRows;
+    const int64_t nb = ceil_div(g.k, per);
+    if (ws->buf.resize(sizeof(double) * size_t(nb * kTallM * kTallN))) ++ws->gen;
+    for (int64_t j0 = 0; j0 < cols; j0 += kTallN)
+      for (int64_t i0 = 0; i0 < rows; i0 += kTallM) {
+        const int mi = int(std::min<int64_t>(kTallM, rows - i0)), nj = int(std::min<int64_t>(kTallN, cols - j0));
+        gemm_tall_tn_kernel<<<unsigned(nb), kTallThreads, 0, stream>>>(P + i0 * ldp, ldp, Q + j0 * ldq, ldq, g.k,
+                                                                        mi, nj, per, ws->buf.as<double>());
+        ETG_LAUNCH("gemm_tall_tn");
+        double* C0 = swap ? g.C + j0 + i0 * g.ldc : g.C + i0 + j0 * g.ldc;
+        gemm_tall_reduce_kernel<<<unsigned(ceil_div(int64_t(mi) * nj, 256)), 256, 0, stream>>>(
+            ws->buf.as<double>(), nb, mi, nj, g.alpha, g.beta, C0, g.ldc, swap ? 1 : 0);
+        ETG_LAUNCH("gemm_tall_reduce");
+      }
     return GemmLaunch{"gemm_tall_tn", int(nb)};
   }
   if (!g.ta && !g.tb && g.k * g.n <= 8192) {
