@@ -331,13 +331,18 @@ def eom_time_to_solution(r, n_states=12, threshold=1e-5):
     return out
 
 
-def transforms_profile(side, precision, copies=4, n_vec=12):
-    """One group of n_vec right or left transformations of `copies` H2O-He copies
-    (eager passes, graphs off), after a short FP64 ground state: the launch
-    sequence the EOM Davidson replays, for rocprofv3 kernel traces."""
+def transforms_profile(side, precision, system='h2o_he_x4', n_vec=12):
+    """One group of n_vec right or left transformations (eager passes, graphs off)
+    after a short FP64 ground state, timed after a first group that sizes the
+    buffers: the launch sequence the EOM Davidson replays, for rocprofv3 kernel
+    traces.  system: 'h2o_he_x4' (four H2O-He copies from the committed inputs) or
+    'h2o8' ((H2O)_8 / cc-pVDZ from the device producer)."""
     from et_amd import solvers, workloads
     prec = ccsd.PRECISIONS[precision]
-    r = workloads.replicate(workloads.h2o_he_inputs(), copies)
+    if system == 'h2o8':
+        r = water_octamer()
+    else:
+        r = workloads.replicate(workloads.h2o_he_inputs(), 4)
     n_o = r['n_o']
     n_v = r['L_mo'].shape[1] - n_o
     with ccsd.CCSD(n_o, n_v, r['n_J']) as wf:
@@ -355,8 +360,34 @@ def transforms_profile(side, precision, copies=4, n_vec=12):
         wf.jacobian_transformations(X, side, prec)
         wf.synchronize()
         dt = time.perf_counter() - t0
-    print(json.dumps({'transforms_profile': side, 'precision': precision, 'copies': copies, 'n_vec': n_vec,
+    print(json.dumps({'transforms_profile': side, 'precision': precision, 'system': system, 'n_vec': n_vec,
                       'ms_per_transform': dt / n_vec * 1e3}), flush=True)
+
+
+def eom_profile(side, precision, n_states=12, threshold=1e-5):
+    """One 12-root EOM solve on (H2O)_8 / cc-pVDZ after an FP64 ground state, with the
+    transformation passes launched eagerly (graphs off) so that a kernel trace sees
+    every kernel: FP64 Davidson ('fp64') or the mixed-precision solver ('mixed')."""
+    from et_amd import solvers
+    r = water_octamer()
+    n_o = r['n_o']
+    n_v = r['L_mo'].shape[1] - n_o
+    with ccsd.CCSD(n_o, n_v, r['n_J']) as wf:
+        load(wf, r)
+        del r
+        solvers.DiisCCGroundState(wf, residual_threshold=1e-10).run()
+        wf.set_transform_graphs(False)
+        wf.prepare_for_jacobian(side)
+        wf.davidson_timing(reset=True)
+        t0 = time.perf_counter()
+        if precision == 'mixed':
+            solvers.MixedPrecisionDavidson(wf, n_states, side, residual_threshold=threshold).run()
+        else:
+            solvers.EigenDavidson(wf, n_states, side, residual_threshold=threshold).run()
+        dt = time.perf_counter() - t0
+        tr_s, dav_s, n_tr = wf.davidson_timing()
+    print(json.dumps({'eom_profile': side, 'precision': precision, 'seconds': dt, 'davidson_seconds': dav_s,
+                      'transformation_seconds': tr_s, 'transformations': n_tr}), flush=True)
 
 
 def lambda_dipole(shape=(4, 2, 2), seed=0, gs_threshold=1e-10, lambda_threshold=1e-8, fp64=False):
@@ -460,6 +491,10 @@ def main():
     ap.add_argument('--transforms-profile', nargs=2, metavar=('SIDE', 'PRECISION'),
                     help='only run two groups of 12 EOM transformations (right|left, fp64|fp32|bf16x3) '
                          'on four H2O-He copies, graphs off (for kernel traces)')
+    ap.add_argument('--transforms-system', choices=['h2o_he_x4', 'h2o8'], default='h2o_he_x4',
+                    help='system of --transforms-profile')
+    ap.add_argument('--eom-profile', nargs=2, metavar=('SIDE', 'PRECISION'),
+                    help='one 12-root EOM solve on (H2O)_8 with eager passes (fp64 or mixed), for kernel traces')
     ap.add_argument('--lambda-dipole', action='store_true',
                     help='only run the (H2O)16 / cc-pVDZ ground state, Lambda equations and dipole '
                          '(mixed precision; a few minutes)')
@@ -479,7 +514,10 @@ def main():
         copy_calibration()
         return
     if args.transforms_profile:
-        transforms_profile(*args.transforms_profile)
+        transforms_profile(*args.transforms_profile, system=args.transforms_system)
+        return
+    if args.eom_profile:
+        eom_profile(*args.eom_profile)
         return
     if args.lambda_dipole:
         print(json.dumps({'lambda_dipole': lambda_dipole(fp64=args.fp64_reference)}), flush=True)

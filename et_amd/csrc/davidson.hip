@@ -22,6 +22,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -45,8 +47,10 @@ struct WaveTeam {
   __device__ void sync() const { __syncthreads(); }
 };
 
-// H(i, j) of the k x k working matrix in LDS, column-major with ld kMaxRed.
-#define HH(i, j) H[(i) + (j) * kMaxRed]
+// H(i, j) of the k x k working matrix in LDS, column-major with the odd leading
+// dimension kLdH: lanes that walk a row touch distinct banks.
+constexpr int kLdH = kMaxRed + 1;
+#define HH(i, j) H[(i) + (j) * kLdH]
 
 // Inverse iteration on the nm x nm matrix M (ld ldm; LDS or global): LU with
 // partial pivoting in place, then three solves M y = y starting from a fixed
@@ -84,23 +88,28 @@ __device__ void inverse_iteration(double* M, int ldm, int nm, double anorm, int*
   for (int i = lane; i < nm; i += kLanes) y[i] = 1.0 + 0.01 * ((i * 7 + seed * 3) % 11);
   __syncthreads();
   for (int iter = 0; iter < 3; ++iter) {
-    if (lane == 0) {
+    if (lane == 0)
       for (int c = 0; c < nm; ++c) {
         const int p = piv[c];
         const double tmp = y[c];
         y[c] = y[p];
         y[p] = tmp;
       }
-      for (int c = 0; c < nm; ++c) {
-        double s = y[c];
-        for (int j = 0; j < c; ++j) s -= MM(c, j) * y[j];
-        y[c] = s;
-      }
-      for (int c = nm - 1; c >= 0; --c) {
-        double s = y[c];
-        for (int j = c + 1; j < nm; ++j) s -= MM(c, j) * y[j];
-        y[c] = s / MM(c, c);
-      }
+    __syncthreads();
+    // triangular solves column by column, the updates of each column across the lanes
+    for (int c = 0; c < nm; ++c) {                     // unit lower L
+      const double yc = y[c];
+      for (int r = c + 1 + lane; r < nm; r += kLanes) y[r] -= MM(r, c) * yc;
+      __syncthreads();
+    }
+    for (int c = nm - 1; c >= 0; --c) {                // upper U
+      const double yc = y[c] / MM(c, c);
+      for (int r = lane; r < c; r += kLanes) y[r] -= MM(r, c) * yc;
+      __syncthreads();
+      if (lane == 0) y[c] = yc;
+      __syncthreads();
+    }
+    if (lane == 0) {
       double nrm = 0.0, big = 0.0;
       int ib = 0;
       for (int i = 0; i < nm; ++i) {
@@ -123,7 +132,7 @@ __global__ __launch_bounds__(kLanes) void reduced_eigen_kernel(const double* __r
                                                                double* __restrict__ X, int ldx,
                                                                double* __restrict__ work,
                                                                int* __restrict__ info) {
-  __shared__ double H[kMaxRed * kMaxRed];
+  __shared__ double H[kLdH * kMaxRed];
   __shared__ double wr[kMaxRed], wi[kMaxRed], ort[kMaxRed], y[2 * kMaxRed];
   __shared__ int order[kMaxRed], piv[2 * kMaxRed];
   __shared__ int status;
@@ -168,7 +177,7 @@ __global__ __launch_bounds__(kLanes) void reduced_eigen_kernel(const double* __r
   // ---- eigenvalues of the Hessenberg matrix (implicit double-shift QR) ----
   double anorm = 0.0;
   {
-    const int st = francis::hessenberg_eigenvalues(WaveTeam(), H, kMaxRed, k, wr, wi, &anorm);
+    const int st = francis::hessenberg_eigenvalues(WaveTeam(), H, kLdH, k, wr, wi, &anorm);
     if (lane == 0 && st != 0) status = st;
   }
   __syncthreads();
@@ -204,7 +213,7 @@ __global__ __launch_bounds__(kLanes) void reduced_eigen_kernel(const double* __r
         HH(i, j) = G[i + j * int64_t(ldg)] - (i == j ? shift : 0.0);
       }
       __syncthreads();
-      inverse_iteration(H, kMaxRed, k, anorm, piv, y, w, lane);
+      inverse_iteration(H, kLdH, k, anorm, piv, y, w, lane);
       if (lane == 0) {
         // orthogonal to earlier vectors of (numerically) the same eigenvalue
         for (int u = 0; u < w; ++u) {
@@ -456,6 +465,11 @@ int64_t CCSD::davidson(int side, int64_t n_states, double threshold, int64_t max
   double best = HUGE_VAL;                 // reduced precision: smallest largest-residual so far
   int64_t since_best = 0;
   const Prec start_prec = dav_start_prec_ < 0 ? prec : static_cast<Prec>(dav_start_prec_);
+  // ETG_DAVIDSON_LOG=1: one line per iteration on stderr (progress of long solves)
+  static const bool log_iterations = [] {
+    const char* e = std::getenv("ETG_DAVIDSON_LOG");
+    return e != nullptr && e[0] == '1';
+  }();
   for (int64_t it = 1; it <= max_iterations; ++it) {
     // transforms of the new trial vectors (the start block in its own precision)
     const auto tt0 = std::chrono::steady_clock::now();
@@ -515,6 +529,13 @@ int64_t CCSD::davidson(int side, int64_t n_states, double threshold, int64_t max
     for (int64_t j = 0; j < s; ++j) {
       energies[j] = hcoef[j];
       if (residual_norms) residual_norms[(it - 1) * s + j] = norms[j];
+    }
+    if (log_iterations) {
+      double worst = 0.0;
+      for (int64_t j = 0; j < s; ++j) worst = std::max(worst, norms[j]);
+      std::fprintf(stderr, "davidson: iteration %lld, space %lld, %lld transformed, max residual %.3e\n",
+                   (long long)it, (long long)k, (long long)m, worst);
+      std::fflush(stderr);
     }
     // With reduced-precision transformations the residuals bottom out at the
     // transforms' rounding level: the solve also ends once the largest residual

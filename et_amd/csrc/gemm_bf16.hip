@@ -53,32 +53,36 @@ __global__ __launch_bounds__(256) void pack_split_bf16_kernel(const double* __re
   __shared__ u4 tile[4 * 128];       // 4 chunks of 2 KB
   const int64_t KS = Kp / kDepth;
   const int64_t ks = blockIdx.x;
-  const int64_t rb0 = int64_t(blockIdx.y) * 4;
   const int64_t bz = blockIdx.z;
   const double* Xb = X + bz * strideX;
+  u4* Pb = reinterpret_cast<u4*>(P + bz * rows_p * Kp * 2);
   const int t = threadIdx.x;
   const int row = kcontig ? (t >> 2) : (t & 63);
   const int kq = kcontig ? (t & 3) : (t >> 6);
-  const int64_t r = rb0 * 16 + row;
-  bf8 h, l;
-#pragma unroll
-  for (int e = 0; e < 8; ++e) {
-    const int64_t k = ks * kDepth + 8 * kq + e;
-    double x = 0.0;
-    if (r < rows && k < K) x = kcontig ? Xb[k + r * ld] : Xb[r + k * ld];
-    const __bf16 xh = __bf16(float(x));
-    h[e] = xh;
-    l[e] = __bf16(float(x - double(float(xh))));
-  }
   const int lane = (row & 15) + 16 * kq;
-  tile[(row >> 4) * 128 + lane] = __builtin_bit_cast(u4, h);
-  tile[(row >> 4) * 128 + 64 + lane] = __builtin_bit_cast(u4, l);
-  __syncthreads();
-  u4* Pb = reinterpret_cast<u4*>(P + bz * rows_p * Kp * 2);
+  // tiles of 64 rows: blockIdx.y, then every gridDim.y-th one (more than 65535 tiles)
+  for (int64_t tr = blockIdx.y; tr < rows_p / 64; tr += gridDim.y) {
+    const int64_t rb0 = tr * 4;
+    const int64_t r = rb0 * 16 + row;
+    bf8 h, l;
 #pragma unroll
-  for (int i = 0; i < 2; ++i) {
-    const int q = t + 256 * i;                     // u4 unit of the tile: chunk q / 128
-    Pb[((rb0 + (q >> 7)) * KS + ks) * 128 + (q & 127)] = tile[q];
+    for (int e = 0; e < 8; ++e) {
+      const int64_t k = ks * kDepth + 8 * kq + e;
+      double x = 0.0;
+      if (r < rows && k < K) x = kcontig ? Xb[k + r * ld] : Xb[r + k * ld];
+      const __bf16 xh = __bf16(float(x));
+      h[e] = xh;
+      l[e] = __bf16(float(x - double(float(xh))));
+    }
+    __syncthreads();                             // the previous tile has left LDS
+    tile[(row >> 4) * 128 + lane] = __builtin_bit_cast(u4, h);
+    tile[(row >> 4) * 128 + 64 + lane] = __builtin_bit_cast(u4, l);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const int q = t + 256 * i;                 // u4 unit of the tile: chunk q / 128
+      Pb[((rb0 + (q >> 7)) * KS + ks) * 128 + (q & 127)] = tile[q];
+    }
   }
 }
 
@@ -252,9 +256,9 @@ __global__ void splitk_reduce_bf(const double* __restrict__ W, int splits, int64
 
 void pack(hipStream_t s, const double* X, int64_t ld, int64_t strideX, bool kcontig, int64_t rows,
           int64_t K, int64_t rows_p, int64_t Kp, int64_t batch, __bf16* P) {
-  if (rows_p % 64 != 0 || rows_p / 64 > 65535 || batch > 65535 || Kp / kDepth > 0x7fffffff)
+  if (rows_p % 64 != 0 || batch > 65535 || Kp / kDepth > 0x7fffffff)
     fail("split-bf16 pack: operand shape outside the pack grid");
-  dim3 grid(unsigned(Kp / kDepth), unsigned(rows_p / 64), unsigned(batch));
+  dim3 grid(unsigned(Kp / kDepth), unsigned(std::min<int64_t>(rows_p / 64, 65535)), unsigned(batch));
   pack_split_bf16_kernel<<<grid, 256, 0, s>>>(X, ld, strideX, kcontig ? 1 : 0, rows, K, rows_p, Kp, P);
   ETG_LAUNCH("pack_split_bf16");
 }

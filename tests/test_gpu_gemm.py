@@ -90,6 +90,21 @@ def test_bf16x3_gemm(ta, tb, m, n, k):
     assert np.all(np.abs(out - ref) <= bound)
 
 
+@pytest.mark.parametrize('ta,tb,m,n,k', [('N', 'N', (1 << 22) + 1, 3, 5), ('T', 'N', 3, (1 << 22) + 1, 5)])
+def test_bf16x3_gemm_with_more_than_65535_row_tiles(ta, tb, m, n, k):
+    """An operand of more than 2^22 rows: the pack walks its 64-row tiles beyond the
+    65535 of one grid dimension (the T1 part of the Jacobian products at o=48, v=400)."""
+    rng = np.random.default_rng(7)
+    A = rng.normal(size=(k, m) if ta == 'T' else (m, k))
+    B = rng.normal(size=(k, n))
+    C = np.zeros((m, n))
+    opA = A.T if ta == 'T' else A
+    out = ccsd.device_gemm(ta, tb, 1.0, A, B, 0.0, C, precision=ccsd.BF16X3)
+    ref = opA @ B
+    bound = (4 * 2.0 ** -16 + k * 2.0 ** -24) * (np.abs(opA) @ np.abs(B)) + 1e-300
+    assert np.all(np.abs(out - ref) <= bound)
+
+
 @pytest.mark.parametrize('ta,tb,m,n,k', [('N', 'N', 48, 48, 4000), ('T', 'N', 300, 200, 500),
                                          ('N', 'T', 640, 520, 130), ('T', 'T', 129, 257, 3000)])
 def test_measured_tile_choice_keeps_results(ta, tb, m, n, k):
