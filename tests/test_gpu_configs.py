@@ -14,7 +14,8 @@
 4. 12-root EOM on (H2O)_8, right and left, split-bf16 Davidson then the refined
    phase (FP64 start block, split-bf16 expansions): every root re-checked with a
    fresh FP64 transformation of its vector (||A x - w x|| / ||x|| below the
-   threshold), right and left energies equal to 1e-5 eV.
+   threshold), right and left energies equal to 1e-5 eV, and each within 1e-5 eV of
+   the two-sided Rayleigh quotients of the right and left vectors.
 5. (H2O)_16 / cc-pVDZ (4x2x2 lattice) mixed-precision ground state, multipliers
    and dipole: ||Omega_FP64|| <= 1e-10, ||A^T t-bar + eta|| <= 1e-8, and the
    dipole from the density of one backward replay equals <r_c> from forward
@@ -113,7 +114,7 @@ def test_cfg4_octamer_eom_12_roots(octamer):
     n_o, n_J = r['n_o'], r['n_J']
     n_v = r['L_mo'].shape[1] - n_o
     threshold = 1e-5
-    roots = {}
+    roots, vecs = {}, {}
     with ccsd.CCSD(n_o, n_v, n_J) as wf:
         load(wf, r)
         solvers.DiisCCGroundState(wf, residual_threshold=1e-10).run()
@@ -130,7 +131,16 @@ def test_cfg4_octamer_eom_12_roots(octamer):
                   flush=True)
             assert res.max() <= threshold
             roots[side] = w
+            vecs[side] = (X, AX)
     assert np.abs(roots['right'] - roots['left']).max() * HARTREE_TO_EV <= 1e-5
+    # two-sided Rayleigh quotients l^T A r / l^T r: second order in both residuals,
+    # so they stand for the FP64 eigenvalues; each side's roots within 1e-5 eV of them
+    (R, AR), (Lv, _) = vecs['right'], vecs['left']
+    w2 = np.einsum('ij,ij->j', Lv, AR) / np.einsum('ij,ij->j', Lv, R)
+    for side in ('right', 'left'):
+        err = np.abs(roots[side] - w2).max() * HARTREE_TO_EV
+        print('cfg4: %s roots vs two-sided quotients: %.2e eV' % (side, err), flush=True)
+        assert err <= 1e-5
 
 
 # ---- 3 ------------------------------------------------------------------------------------
