@@ -76,6 +76,7 @@ _SIGNATURES = {
                                   ctypes.c_void_p, _i64],
     'etg_set_davidson_start_precision': [ctypes.c_void_p, ctypes.c_int],
     'etg_set_davidson_space': [ctypes.c_void_p, ctypes.c_int],
+    'etg_diis_finalize': [ctypes.c_void_p],
     'etg_davidson_timing': [ctypes.c_void_p, ctypes.c_int, _c_dbl_p, _c_dbl_p, ctypes.POINTER(_i64)],
     'etg_diis_initialize': [ctypes.c_void_p, _i64],
     'etg_quasi_newton_update': [ctypes.c_void_p],
@@ -529,6 +530,10 @@ class CCSD:
             _check(self._lib.etg_diis_initialize_crop(self._h, int(dimension)))
         else:
             _check(self._lib.etg_diis_initialize(self._h, int(dimension)))
+
+    def diis_finalize(self):
+        """Releases the DIIS records (diis%finalize, diis_tool_class.F90:280-299)."""
+        _check(self._lib.etg_diis_finalize(self._h))
 
     def quasi_newton_update(self):
         """Device quasi-Newton step t += -omega/eps (quasi_newton_updater_class.F90:81-118)."""

@@ -265,6 +265,10 @@ int etg_diis_excited_states(etg_handle* h, int side, int64_t n_states, double re
   });
 }
 
+int etg_diis_finalize(etg_handle* h) {
+  return guarded([&] { wf_of(h).diis_finalize(); });
+}
+
 int etg_diis_initialize(etg_handle* h, int64_t dimension) {
   return guarded([&] { wf_of(h).diis_initialize(dimension); });
 }

@@ -64,6 +64,7 @@ class CCSD {
   // DIIS extrapolation over records kept in HBM
   static constexpr int kMaxDiis = 32;
   void diis_initialize(int64_t dimension, bool crop = false);
+  void diis_finalize();
   void quasi_newton_update();                     // e = -omega/eps (doubles diagonal x2); t += e
   void diis_update();                             // records (e, t); t = DIIS-extrapolated t
 

@@ -407,8 +407,22 @@ int64_t CCSD::davidson(int side, int64_t n_states, double threshold, int64_t max
     // release first so that the other buffer can take the memory
     if (dav_V_.on_host() != v_host) dav_V_.release();
     if (dav_AV_.on_host() != av_host) dav_AV_.release();
-    if (av_host) dav_AV_.resize_host(space); else dav_AV_.resize(space);
-    if (v_host) dav_V_.resize_host(space); else dav_V_.resize(space);
+    // pinned host memory can be refused (the system's limit on locked pages): the
+    // buffer then goes to HBM after all
+    auto place = [&](DeviceBuffer& b, bool host) {
+      if (host) {
+        try {
+          b.resize_host(space);
+          return;
+        } catch (const Error&) {
+          (void)hipGetLastError();
+          b.release();
+        }
+      }
+      b.resize(space);
+    };
+    place(dav_AV_, av_host);
+    place(dav_V_, v_host);
   }
   double* V = dav_V_.as<double>();
   double* AV = dav_AV_.as<double>();

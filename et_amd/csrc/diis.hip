@@ -162,6 +162,15 @@ void CCSD::diis_initialize(int64_t dimension, bool crop) {
   diis_space_initialize(stream_, diis_, n_t1 + n_t2, dimension, crop);
 }
 
+// finalize_diis_tool (diis_tool_class.F90:280-299): the records are released
+void CCSD::diis_finalize() {
+  synchronize();
+  diis_.e.release();
+  diis_.x.release();
+  diis_.dim = 0;
+  diis_.count = 0;
+}
+
 void CCSD::quasi_newton_update() {
   require(have_eps_ && have_t_, "quasi_newton_update: orbital energies and amplitudes required");
   require(have_omega_, "quasi_newton_update: the residual has not been constructed");

@@ -194,6 +194,12 @@ module etgpu_interface
          integer(c_int64_t), intent(out) :: iterations
       end function etg_diis_excited_states
 !
+!     diis%finalize (diis_tool_class.F90:280-299): the records are released
+      integer(c_int) function etg_diis_finalize(handle) bind(C, name='etg_diis_finalize')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: handle
+      end function etg_diis_finalize
+!
       integer(c_int) function etg_diis_initialize(handle, dimension) bind(C, name='etg_diis_initialize')
          import :: c_int, c_ptr, c_int64_t
          type(c_ptr), value :: handle

@@ -59,6 +59,7 @@ class DiisCCGroundState:
             if self.log:
                 self.log('%3d %25.12f %11.4e %11.4e' % (it, wf.energy, norm, abs(wf.energy - previous)))
             if norm < self.residual_threshold:
+                wf.diis_finalize()
                 return wf.energy
             wf.quasi_newton_update()
             wf.diis_update()
@@ -122,6 +123,7 @@ class MixedPrecisionGroundState:
             if self.log:
                 self.log('%3d  %-6s %25.12f %11.4e' % (it, _PREC_NAMES[precision], wf.energy, norm))
             if norm < threshold and precision == _ccsd.FP64:
+                wf.diis_finalize()
                 return wf.energy
             if norm < 0.5 * best:
                 best, since_best = norm, 0
@@ -183,6 +185,7 @@ class NewtonRaphsonCCGroundState:
                 self.log('%3d %25.12f %11.4e %11.4e' % (it, wf.energy, norm, de))
             energy_ok = self.energy_threshold is None or de < self.energy_threshold or it == 1
             if norm < self.residual_threshold and energy_ok:
+                wf.diis_finalize()
                 return wf.energy
             wf.prepare_for_jacobian()
             self.micro.append(list(wf.newton_raphson_update(self.rel_threshold, self.max_micro_iterations,
@@ -224,7 +227,9 @@ class DiisCCMultipliers:
             if self.log:
                 self.log('%3d %11.4e' % (it, norm))
             if norm < self.residual_threshold:
-                return wf.get_multipliers()
+                tbar = wf.get_multipliers()
+                wf.diis_finalize()
+                return tbar
             wf.multiplier_quasi_newton_update()
             wf.multiplier_diis_update()
         raise _ccsd.ETGError('was not able to converge the equations in the given number of '
@@ -309,7 +314,9 @@ class MixedPrecisionMultipliers:
             if self.log:
                 self.log('%3d  %-6s %11.4e' % (it, _PREC_NAMES[precision], norm))
             if norm < threshold and precision == _ccsd.FP64:
-                return wf.get_multipliers()
+                tbar = wf.get_multipliers()
+                wf.diis_finalize()
+                return tbar
             if norm < 0.5 * best:
                 best, since_best = norm, 0
             else:

@@ -134,6 +134,12 @@ int etg_construct_omega_resident(etg_handle* handle, int precision);
  * as run_diis_cc_gs calls it (src/solvers/cc/diis_cc_gs_class.F90:230). */
 int etg_diis_initialize(etg_handle* handle, int64_t dimension);
 
+/* Releases the DIIS records (their HBM returns to the device) at the end of a
+ * solve; etg_diis_initialize allocates them again.
+ * Replaces: diis%finalize (finalize_diis_tool, diis_tool_class.F90:280-299),
+ * called by eT's solvers when they finish (diis_cc_gs_class.F90:313). */
+int etg_diis_finalize(etg_handle* handle);
+
 /* etg_diis_initialize with eT's CROP variant: after each extrapolation the newest
  * records become the extrapolated parameters <x> and error <e> = sum_i w_i e_i.
  * Replaces: diis_tool(..., crop=.true.) (diis_tool_class.F90:150-210, 763-805) as
