@@ -13,6 +13,7 @@
 #include <cmath>
 
 #include "kernels.hpp"
+#include "ladder_order.hpp"
 
 namespace etg {
 
@@ -577,8 +578,9 @@ void CCSD::build_ladder_plan() {
   upload(pcol_b_, cb);
   const int64_t op = n_o * (n_o + 1) / 2, om = n_o * (n_o - 1) / 2;
   const int64_t vp = v * (v + 1) / 2, vm = v * (v - 1) / 2;
-  lsz_tp_ = op * vp;
-  lsz_tm_ = std::max<int64_t>(1, om * vm);
+  // t+- hold either order of the contraction index (triangle or tile-blocked)
+  lsz_tp_ = op * std::max(vp, ladder_blocked_len(v, true));
+  lsz_tm_ = std::max<int64_t>(1, om * std::max(vm, ladder_blocked_len(v, false)));
   lsz_wp_ = op * vp;
   lsz_wm_ = std::max<int64_t>(1, om * vp);
   tp_.resize(sizeof(double) * lsz_tp_);
@@ -588,28 +590,63 @@ void CCSD::build_ladder_plan() {
   gblk_.resize(sizeof(double) * v * v * std::min(nb, v) * std::min(nb, v));
   gp_.resize(sizeof(double) * vp * maxpairs);
   gm_.resize(sizeof(double) * std::max<int64_t>(1, vm * maxpairs));
+  ladder_maxpairs_ = maxpairs;
+  gpk_.release();                       // sized by the first split-bf16 ladder
+  gmk_.release();
   reserved_nb_ = 0;                     // per-vector sizes changed: size the batches again
   nb_cap_ = 0;
 }
 
+// The split-bf16 packs of g+- for one block (grown to the plan's largest block
+// on first use; true when they moved).
+bool CCSD::reserve_ladder_packs() {
+  const int64_t v = n_v;
+  bool moved = gpk_.resize(2 * size_t(split_bf16_pack_elems(ladder_maxpairs_, ladder_blocked_len(v, true))));
+  moved |= gmk_.resize(2 * size_t(split_bf16_pack_elems(ladder_maxpairs_, ladder_blocked_len(v, false))));
+  return moved;
+}
+
+// Per block: G, then g+- and the products w+- = 1/2 t+- g+-.  The split-bf16
+// form writes g+- straight as packs in the tile-blocked (c, d) order, with t+-
+// in the same order; the FP64 and FP32 forms use FP64 g+- in the triangle order.
 void CCSD::ladder_primal(const double* t2, const double* LT, double* om2, Prec prec) {
   const int64_t o = n_o, v = n_v, J = n_J;
   const int64_t op = o * (o + 1) / 2, om = o * (o - 1) / 2;
-  const int64_t vp = v * (v + 1) / 2, vm = v * (v - 1) / 2;
+  const bool split = prec == Prec::BF16X3;
+  const int64_t kp = split ? ladder_blocked_len(v, true) : v * (v + 1) / 2;
+  const int64_t km = split ? ladder_blocked_len(v, false) : v * (v - 1) / 2;
+  const bool minus = om > 0 && v > 1;
   hipStream_t s = stream_;
+  if (split && reserve_ladder_packs()) drop_transform_graphs();
   const uint64_t gen = ++ladder_gen_ << 3;     // t+/- are rebuilt below: new pack tags
-  ladder_t_pm(s, t2, tp_.as<double>(), tm_.as<double>(), o, v);
+  if (split)
+    ladder_t_pm_blocked(s, t2, tp_.as<double>(), tm_.as<double>(), o, v);
+  else
+    ladder_t_pm(s, t2, tp_.as<double>(), tm_.as<double>(), o, v);
   for (const Block& b : blocks_) {
-    run_gemm(tagged(make_gemm('T', 'N', v * b.la, v * b.lb, J, 1.0, LT + J * v * b.a0, J,
-                              LT + J * v * b.b0, J, 0.0, gblk_.as<double>(), v * b.la), gen | 7, pack_cache_),
-             prec);
-    ladder_g_pm(s, gblk_.as<double>(), v, b.la, b.lb, pair_a_.as<int32_t>() + b.col0,
-                pair_b_.as<int32_t>() + b.col0, b.npairs, gp_.as<double>(), gm_.as<double>());
-    run_gemm(tagged(make_gemm('N', 'N', op, b.npairs, vp, 0.5, tp_.as<double>(), op, gp_.as<double>(),
-                              vp, 0.0, wp_.as<double>() + op * b.col0, op), gen | 1, pack_cache_), prec);
-    if (om > 0 && vm > 0)
-      run_gemm(tagged(make_gemm('N', 'N', om, b.npairs, vm, 0.5, tm_.as<double>(), om, gm_.as<double>(),
-                                vm, 0.0, wm_.as<double>() + om * b.col0, om), gen | 2, pack_cache_), prec);
+    Gemm gg = make_gemm('T', 'N', v * b.la, v * b.lb, J, 1.0, LT + J * v * b.a0, J, LT + J * v * b.b0, J, 0.0,
+                        gblk_.as<double>(), v * b.la);
+    if (split) {                        // G in FP32
+      gg.C = nullptr;
+      gg.Cf = gblk_.as<float>();
+    }
+    run_gemm(tagged(gg, gen | 7, pack_cache_), prec);
+    Gemm gpl = make_gemm('N', 'N', op, b.npairs, kp, 0.5, tp_.as<double>(), op, gp_.as<double>(), kp, 0.0,
+                         wp_.as<double>() + op * b.col0, op);
+    Gemm gmi = make_gemm('N', 'N', om, b.npairs, km, 0.5, tm_.as<double>(), om, gm_.as<double>(), km, 0.0,
+                         wm_.as<double>() + om * b.col0, om);
+    if (split) {
+      ladder_g_pm_split(s, gblk_.as<float>(), v, b.la, pair_a_.as<int32_t>() + b.col0,
+                        pair_b_.as<int32_t>() + b.col0, b.npairs, gpk_.as<__bf16>(), gmk_.as<__bf16>());
+      gpl.B = gmi.B = nullptr;
+      gpl.b_packed = gpk_.as<__bf16>();
+      gmi.b_packed = gmk_.as<__bf16>();
+    } else {
+      ladder_g_pm(s, gblk_.as<double>(), v, b.la, b.lb, pair_a_.as<int32_t>() + b.col0,
+                  pair_b_.as<int32_t>() + b.col0, b.npairs, gp_.as<double>(), gm_.as<double>());
+    }
+    run_gemm(tagged(gpl, gen | 1, pack_cache_), prec);
+    if (minus) run_gemm(tagged(gmi, gen | 2, pack_cache_), prec);
   }
   ladder_gather(s, wp_.as<double>(), wm_.as<double>(), pair_col_.as<int32_t>(), om2, o, v);
 }
@@ -684,9 +721,13 @@ void CCSD::ladder_tangent(const double* t2, const double* dt2, int64_t s_dt2, co
     fail("ladder tangent: the record's ladder operands are required");
   const int64_t o = n_o, v = n_v, J = n_J, N = n_t1;
   const int64_t op = o * (o + 1) / 2, om = o * (o - 1) / 2;
-  const int64_t vp = v * (v + 1) / 2, vm = v * (v - 1) / 2;
+  const bool split = prec == Prec::BF16X3;
+  const int64_t kp = split ? ladder_blocked_len(v, true) : v * (v + 1) / 2;
+  const int64_t km = split ? ladder_blocked_len(v, false) : v * (v - 1) / 2;
+  const bool minus = om > 0 && v > 1;
   hipStream_t s = stream_;
   reserve_ladder_vectors(nb);
+  if (split) reserve_ladder_packs();    // sized before any capture by transform_batch
   DirectOps X(*this);
   double* dtp = dtp_.as<double>();
   double* dtm = dtm_.as<double>();
@@ -699,20 +740,37 @@ void CCSD::ladder_tangent(const double* t2, const double* dt2, int64_t s_dt2, co
     g.strideC = sC;
     run_gemm(g, prec);
   };
-  ladder_t_pm(s, t2, tp_.as<double>(), tm_.as<double>(), o, v);
-  for (int64_t k = 0; k < nb; ++k)
-    ladder_t_pm(s, dt2 + k * s_dt2, dtp + k * lsz_tp_, dtm + k * lsz_tm_, o, v);
+  for (int64_t k = 0; k < nb; ++k) {
+    if (split)
+      ladder_t_pm_blocked(s, dt2 + k * s_dt2, dtp + k * lsz_tp_, dtm + k * lsz_tm_, o, v);
+    else
+      ladder_t_pm(s, dt2 + k * s_dt2, dtp + k * lsz_tp_, dtm + k * lsz_tm_, o, v);
+  }
   for (const Block& b : blocks_) {
     const double* LA = LT + J * v * b.a0;
     const double* LB = LT + J * v * b.b0;
-    X.gemm('T', 'N', v * b.la, v * b.lb, J, 1.0, LA, J, LB, J, 0.0, gblk_.as<double>(), v * b.la, prec);
-    ladder_g_pm(s, gblk_.as<double>(), v, b.la, b.lb, pair_a_.as<int32_t>() + b.col0,
-                pair_b_.as<int32_t>() + b.col0, b.npairs, gp_.as<double>(), gm_.as<double>());
-    batched(make_gemm('N', 'N', op, b.npairs, vp, 0.5, dtp, op, gp_.as<double>(), vp, 0.0,
-                      dwp + op * b.col0, op), lsz_tp_, 0, lsz_wp_);
-    if (om > 0 && vm > 0)
-      batched(make_gemm('N', 'N', om, b.npairs, vm, 0.5, dtm, om, gm_.as<double>(), vm, 0.0,
-                        dwm + om * b.col0, om), lsz_tm_, 0, lsz_wm_);
+    Gemm gg = make_gemm('T', 'N', v * b.la, v * b.lb, J, 1.0, LA, J, LB, J, 0.0, gblk_.as<double>(), v * b.la);
+    if (split) {                        // G in FP32
+      gg.C = nullptr;
+      gg.Cf = gblk_.as<float>();
+    }
+    run_gemm(gg, prec);
+    Gemm gpl = make_gemm('N', 'N', op, b.npairs, kp, 0.5, dtp, op, gp_.as<double>(), kp, 0.0,
+                         dwp + op * b.col0, op);
+    Gemm gmi = make_gemm('N', 'N', om, b.npairs, km, 0.5, dtm, om, gm_.as<double>(), km, 0.0,
+                         dwm + om * b.col0, om);
+    if (split) {
+      ladder_g_pm_split(s, gblk_.as<float>(), v, b.la, pair_a_.as<int32_t>() + b.col0,
+                        pair_b_.as<int32_t>() + b.col0, b.npairs, gpk_.as<__bf16>(), gmk_.as<__bf16>());
+      gpl.B = gmi.B = nullptr;
+      gpl.b_packed = gpk_.as<__bf16>();
+      gmi.b_packed = gmk_.as<__bf16>();
+    } else {
+      ladder_g_pm(s, gblk_.as<double>(), v, b.la, b.lb, pair_a_.as<int32_t>() + b.col0,
+                  pair_b_.as<int32_t>() + b.col0, b.npairs, gp_.as<double>(), gm_.as<double>());
+    }
+    batched(gpl, lsz_tp_, 0, lsz_wp_);
+    if (minus) batched(gmi, lsz_tm_, 0, lsz_wm_);
   }
   for (int64_t k = 0; k < nb; ++k)
     ladder_gather(s, dwp + k * lsz_wp_, dwm + k * lsz_wm_, pair_col_.as<int32_t>(), dom2 + k * s_dom2,
@@ -1025,6 +1083,7 @@ void CCSD::transform_batch(int side, int64_t n_vec, const double* X, double* R, 
     if (moved) drop_transform_graphs();
     reserved_nb_ = group;
   }
+  if (prec == Prec::BF16X3 && reserve_ladder_packs()) drop_transform_graphs();
   const bool graphs = use_graphs_ && !ledger_on_;
   pack_cache_ = false;                  // packs go to fixed workspaces inside the passes
   for (int64_t k0 = 0; k0 < n_vec; k0 += group) {

@@ -229,8 +229,11 @@ class CCSD {
   DeviceBuffer tp_, tm_, wp_, wm_, gblk_, gp_, gm_;
   DeviceBuffer dtp_, dtm_, dwp_, dwm_, lz1_, lz2_;   // per derivative vector, x batch
   int64_t lsz_tp_ = 0, lsz_tm_ = 0, lsz_wp_ = 0, lsz_wm_ = 0;
+  int64_t ladder_maxpairs_ = 0;
+  DeviceBuffer gpk_, gmk_;                          // split-bf16 packs of g+- (one block)
   int64_t ladder_vector_elems() const;
   bool reserve_ladder_vectors(int64_t nb);
+  bool reserve_ladder_packs();
   // W_kbij = sum_cd (sum_J L_kc L_t1,bd) t_cidj of the current record (T1 part of
   // the ladder derivative)
   DeviceBuffer lw_;

@@ -694,6 +694,12 @@ int64_t gemm_tuned_shapes() {
 GemmLaunch gemm(hipStream_t stream, const Gemm& g, Prec prec, GemmWorkspace* ws) {
   if (g.m < 0 || g.n < 0 || g.k < 0 || g.batch < 1) fail("gemm: negative dimension");
   if (g.m == 0 || g.n == 0) return GemmLaunch{};
+  if (g.b_packed != nullptr && (prec != Prec::BF16X3 || g.tb || g.k == 0 ||
+                                std::max(g.lda, g.ldb) >= (int64_t(1) << 24)))
+    fail("gemm: a prepacked B operand needs a split-bf16 product with op(B) = B");
+  if (g.Cf != nullptr && (prec != Prec::BF16X3 || g.beta != 0.0 || g.k == 0 ||
+                          std::max(g.lda, g.ldb) >= (int64_t(1) << 24)))
+    fail("gemm: an FP32 output needs a split-bf16 product with beta = 0");
   const int64_t rowsA = g.ta ? g.k : g.m;
   const int64_t rowsB = g.tb ? g.n : g.k;
   if (g.lda < std::max<int64_t>(1, rowsA)) fail("gemm: lda too small");

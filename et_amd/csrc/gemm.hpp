@@ -32,6 +32,13 @@ struct Gemm {
   // Nonzero: the caller guarantees that A holds the same data whenever it passes
   // this tag, so the split-bf16 path may reuse the packed A of an earlier call.
   uint64_t a_tag = 0;
+  // Split-bf16 products only: op(B) given as the pack of op(B)^T (n rows, k
+  // columns, split_bf16_pack_elems(n, k) bf16 values, fragment order), shared by
+  // the batch; B and ldb are then not read.
+  const void* b_packed = nullptr;
+  // Split-bf16 products only: C written in FP32 (the products' accumulation
+  // type) to Cf instead of C; beta must be 0.
+  float* Cf = nullptr;
 };
 
 // A packed split-bf16 operand kept for reuse under a caller tag.
@@ -75,6 +82,17 @@ GemmLaunch gemm(hipStream_t stream, const Gemm& g, Prec prec, GemmWorkspace* ws)
 
 // The split-bf16 path (packs both operands into ws, then the bf16 matrix-core kernel).
 GemmLaunch gemm_bf16x3(hipStream_t stream, const Gemm& g, GemmWorkspace* ws);
+
+// bf16 values of the split-bf16 pack of a rows x k operand (both splits, padded
+// to the product kernel's 256-row tile and 32-deep step).
+int64_t split_bf16_pack_elems(int64_t rows, int64_t k);
+
+// The ladder's g+- of one G block as split-bf16 packs for the products
+// w+- = t+- g+- (B operands: rows = the block's npairs pairs, k = the tile-blocked
+// (c, d) index of ladder_order.hpp, ladder_blocked_len(n_v, +-) long).  G is the
+// (n_v lenA) x (n_v lenB) block in FP32, pair_a / pair_b the pairs' local a and b.
+void ladder_g_pm_split(hipStream_t s, const float* G, int64_t n_v, int64_t lenA, const int32_t* pair_a,
+                       const int32_t* pair_b, int64_t npairs, __bf16* Pp, __bf16* Pm);
 
 // Convenience: C = alpha op(A) op(B) + beta C, no batching.
 inline GemmLaunch gemm(hipStream_t s, char ta, char tb, int64_t m, int64_t n, int64_t k,

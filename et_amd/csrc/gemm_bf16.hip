@@ -17,6 +17,7 @@
 #include <string>
 
 #include "gemm.hpp"
+#include "ladder_order.hpp"
 
 namespace etg {
 namespace {
@@ -86,6 +87,85 @@ __global__ __launch_bounds__(256) void pack_split_bf16_kernel(const double* __re
   }
 }
 
+// ---------------------------------------------------------------------------
+// The ladder's g+- written straight as split-bf16 packs.  For the pairs k =
+// (a, b) of a G block, M_k(c, d) = G(c + v a, d + v b) gives
+// g+_cd = M(c, d) + M(d, c) and g-_cd = M(c, d) - M(d, c); the packs are the B
+// operands (rows = pairs, k = the tile-blocked (c, d) index of
+// ladder_order.hpp) of the products w+- = t+- g+-.  One workgroup per (16 x 16
+// tile pair ti <= tj of the (c, d) plane, 16 pairs): it stages the mirrored
+// tile of its 16 pairs in LDS (reads along the contiguous index), reads the
+// tile itself straight from G, then writes whole 2 KB chunks of both packs: a
+// tile is a whole number of 32-deep steps, and the 16 pairs are one 16-row
+// block.  G comes from a split-bf16
+// product in FP32 (its accumulation type, so nothing is lost); the splits are
+// those of the pack kernel applied to the FP64 sums.
+// ---------------------------------------------------------------------------
+constexpr int kPairTile = 16 * 17 + 1;   // floats per staged pair tile (odd strides)
+
+__global__ __launch_bounds__(256) void ladder_g_pm_split_kernel(
+    const float* __restrict__ G, int64_t n_v, int64_t ldG, const int32_t* __restrict__ pa,
+    const int32_t* __restrict__ pb, int64_t npairs, int64_t KSP, int64_t KSM, int64_t row_blocks,
+    u4* __restrict__ Pp, u4* __restrict__ Pm) {
+  // T[p][x][y] = M_k(d0 + y, c0 + x): the mirrored tile (for a diagonal tile, the tile itself)
+  __shared__ float T[16 * kPairTile];
+  const int t = threadIdx.x;
+  const int tj = small_tri_root(int(blockIdx.x)), ti = int(blockIdx.x) - tj * (tj + 1) / 2;
+  const int64_t c0 = int64_t(kLadderTile) * ti, d0 = int64_t(kLadderTile) * tj;
+  const bool diag = ti == tj;
+  const int w = int(ladder_col_width(tj, n_v));
+  const int64_t ksp0 = ladder_tile_offset(ti, tj, n_v, true) / kDepth;
+  const int64_t ksm0 = ladder_tile_offset(ti, tj, n_v, false) / kDepth;
+  // 32-deep steps of the tile in g+ and g-
+  const int nsp = int(diag ? ladder_diag_size(w, true) / kDepth : w / 2);
+  const int nsm = int(diag ? ladder_diag_size(w, false) / kDepth : w / 2);
+  for (int64_t rb = blockIdx.y; rb < row_blocks; rb += gridDim.y) {
+    __syncthreads();                     // the previous row block has left LDS
+    {
+      const int x = t >> 4, y = t & 15;
+#pragma unroll 4
+      for (int p = 0; p < 16; ++p) {
+        const int64_t k = rb * 16 + p;
+        float val = 0.0f;
+        if (k < npairs && d0 + y < n_v && c0 + x < n_v)
+          val = G[n_v * pa[k] + ldG * n_v * pb[k] + (d0 + y) + ldG * (c0 + x)];
+        T[p * kPairTile + x * 17 + y] = val;
+      }
+    }
+    __syncthreads();
+    // item: (sign, step ks of the tile, lane slot ls = p + 16 q): 8 consecutive
+    // blocked indices 32 ks + 8 q + e of pair p, one 16 B lane of the chunk;
+    // M(c, d) of an off-diagonal tile is read from G directly (8 consecutive c)
+    for (int i = t; i < (nsp + nsm) * 64; i += 256) {
+      const bool plus = i < nsp * 64;
+      const int j = plus ? i : i - nsp * 64;
+      const int ks = j >> 6, ls = j & 63, p = ls & 15, q = ls >> 4;
+      const int64_t k = rb * 16 + p;
+      const bool row_ok = k < npairs;
+      const float* M = row_ok ? G + n_v * pa[k] + ldG * n_v * pb[k] : G;   // M(c, d) = M[c + ldG d]
+      bf8 h, l;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        int cl, dl;
+        const bool ok = ladder_tile_pair(32 * ks + 8 * q + e, diag, w, plus, cl, dl);
+        double g = 0.0;
+        if (ok && row_ok && c0 + cl < n_v && d0 + dl < n_v) {
+          const double x1 = diag ? T[p * kPairTile + dl * 17 + cl] : M[(c0 + cl) + ldG * (d0 + dl)];
+          const double x2 = T[p * kPairTile + cl * 17 + dl];   // M(d, c)
+          g = plus ? x1 + x2 : x1 - x2;
+        }
+        const __bf16 gh = __bf16(float(g));
+        h[e] = gh;
+        l[e] = __bf16(float(g - double(float(gh))));
+      }
+      u4* P = plus ? Pp : Pm;
+      const int64_t chunk = rb * (plus ? KSP : KSM) + (plus ? ksp0 : ksm0) + ks;
+      P[chunk * 128 + ls] = __builtin_bit_cast(u4, h);
+      P[chunk * 128 + 64 + ls] = __builtin_bit_cast(u4, l);
+    }
+  }
+}
+
 struct Params {
   const __bf16 *A, *B;               // fragment-ordered packs
   int64_t KS;                        // k steps (of 32) in a packed row block
@@ -93,6 +173,7 @@ struct Params {
   int64_t m, n;
   double alpha, beta;
   double* C;
+  float* Cf;                         // C in FP32 instead (beta = 0)
   int64_t ldc, strideC;
   double* W;                         // split-K partials (m x n per split and batch entry)
   int64_t ks_per_split;
@@ -139,17 +220,18 @@ __global__ __launch_bounds__(kThreads, 1) void gemm_bf16x3_kernel(Params p) {
   const int64_t astride = p.KS * 128;          // u4 units from one row block to the next
 
   // the 16 chunks this wave copies per step: c = 16 wave + q; c < 32: A row block
-  // c / 2, split c % 2; otherwise B column block (c - 32) / 2, split c % 2
+  // c / 2, split c % 2; otherwise B column block (c - 32) / 2, split c % 2.
+  // Chunk addresses: a wave-uniform base (scalar registers) + 16 B per lane.
   const u4* src = reinterpret_cast<const u4*>(wave < 2 ? p.A + bz * p.strideAp : p.B + bz * p.strideBp) +
-                  int64_t(wave < 2 ? tile_m : tile_n) * 16 * astride + lane;
+                  int64_t(wave < 2 ? tile_m : tile_n) * 16 * astride;
   const int cbase = 16 * (wave & 1);           // chunk index within the operand's 32
   auto load = [&](int buf, int64_t ks) {
     u4* dst = &lds[buf][(wave < 2 ? 0 : 2048) + cbase * 64];
+    const u4* base = src + (cbase >> 1) * astride + ks * 128;
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
-      const int c = cbase + q;                 // operand chunk: row block c / 2, split c % 2
-      const u4* g = src + (c >> 1) * astride + ks * 128 + (c & 1) * 64;
-      __builtin_amdgcn_global_load_lds(reinterpret_cast<const void*>(g),
+      const u4* g = base + (q >> 1) * astride + (q & 1) * 64;
+      __builtin_amdgcn_global_load_lds(reinterpret_cast<const void*>(g + lane),
                                        (lds_void*)(dst + q * 64), 16, 0, 0);
     }
   };
@@ -160,22 +242,30 @@ __global__ __launch_bounds__(kThreads, 1) void gemm_bf16x3_kernel(Params p) {
 #pragma unroll
     for (int j = 0; j < 8; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
 
-  // A fragments first, then B fragments two column blocks ahead of the products
-  // that use them; the scheduling groups keep that order so the LDS reads run
-  // under the matrix-core work.
+  // LDS reads in the order the products consume them: hi of B column 0 and A,
+  // lo of B column 0 and A, then B one column ahead.  The scheduling groups
+  // start the first products after two reads and keep every read ahead of its
+  // first use, so the step's LDS reads run under its matrix-core work.
   auto compute = [&](int buf) {
     const u4* As = &lds[buf][0];
     const u4* Bs = &lds[buf][2048];
     bf8 ah[8], al[8], bh[8], bl[8];
+    auto rd_b = [&](int j, int sp) {
+      return __builtin_bit_cast(bf8, Bs[((wn * 8 + j) * 2 + sp) * 64 + lane]);
+    };
+    auto rd_a = [&](int i, int sp) {
+      return __builtin_bit_cast(bf8, As[((wm * 8 + i) * 2 + sp) * 64 + lane]);
+    };
+    bh[0] = rd_b(0, 0);
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      ah[i] = __builtin_bit_cast(bf8, As[((wm * 8 + i) * 2) * 64 + lane]);
-      al[i] = __builtin_bit_cast(bf8, As[((wm * 8 + i) * 2 + 1) * 64 + lane]);
-    }
+    for (int i = 0; i < 8; ++i) ah[i] = rd_a(i, 0);
+    bl[0] = rd_b(0, 1);
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      bh[j] = __builtin_bit_cast(bf8, Bs[((wn * 8 + j) * 2) * 64 + lane]);
-      bl[j] = __builtin_bit_cast(bf8, Bs[((wn * 8 + j) * 2 + 1) * 64 + lane]);
+    for (int i = 0; i < 8; ++i) al[i] = rd_a(i, 1);
+#pragma unroll
+    for (int j = 1; j < 8; ++j) {
+      bh[j] = rd_b(j, 0);
+      bl[j] = rd_b(j, 1);
     }
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
@@ -186,9 +276,26 @@ __global__ __launch_bounds__(kThreads, 1) void gemm_bf16x3_kernel(Params p) {
 #pragma unroll
       for (int i = 0; i < 8; ++i) acc[i][j] = mfma(al[i], bh[j], acc[i][j]);
     }
-    __builtin_amdgcn_sched_group_barrier(0x100, 20, 0);
+    // reads: bh0 ah0..7 | bl0 al0..7 | bh1 bl1 ... bh7 bl7
+    __builtin_amdgcn_sched_group_barrier(0x100, 3, 0);   // bh0 ah0 ah1
 #pragma unroll
-    for (int j = 0; j < 6; ++j) {
+    for (int i = 0; i < 6; ++i) {                          // hi x hi of column 0 under ah2..ah7
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+      __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+    }
+    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+    __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);     // bl0
+    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {                          // hi x lo of column 0 under al0..al7
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+      __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+    }
+    __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);     // bh1 bl1
+    __builtin_amdgcn_sched_group_barrier(0x008, 8, 0);     // lo x hi of column 0
+    __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);     // column 2
+#pragma unroll
+    for (int j = 1; j < 6; ++j) {                          // column j under the reads of column j + 2
       __builtin_amdgcn_sched_group_barrier(0x008, 24, 0);
       __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
     }
@@ -222,6 +329,20 @@ __global__ __launch_bounds__(kThreads, 1) void gemm_bf16x3_kernel(Params p) {
         }
     return;
   }
+  if (p.Cf) {
+    float* C = p.Cf + bz * p.strideC;
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int64_t row = r0 + i * 16 + 4 * lrow + r;
+          const int64_t col = c0 + j * 16 + lcol;
+          if (row < p.m && col < p.n) C[row + col * p.ldc] = float(p.alpha * double(acc[i][j][r]));
+        }
+    return;
+  }
   double* C = p.C + bz * p.strideC;
 #pragma unroll
   for (int i = 0; i < 8; ++i)
@@ -241,7 +362,8 @@ __global__ __launch_bounds__(kThreads, 1) void gemm_bf16x3_kernel(Params p) {
 
 // C = alpha * sum_s W_s + beta * C
 __global__ void splitk_reduce_bf(const double* __restrict__ W, int splits, int64_t m, int64_t n,
-                                 double alpha, double beta, double* C, int64_t ldc, int64_t strideC) {
+                                 double alpha, double beta, double* C, float* Cf, int64_t ldc,
+                                 int64_t strideC) {
   const int64_t bz = blockIdx.z;
   const int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (idx >= m * n) return;
@@ -249,8 +371,12 @@ __global__ void splitk_reduce_bf(const double* __restrict__ W, int splits, int64
   const double* w = W + bz * splits * m * n + idx;
   double s = 0.0;
   for (int q = 0; q < splits; ++q) s += w[int64_t(q) * m * n];
-  double* c = C + bz * strideC + row + col * ldc;
   const double v = alpha * s;
+  if (Cf) {
+    Cf[bz * strideC + row + col * ldc] = float(v);
+    return;
+  }
+  double* c = C + bz * strideC + row + col * ldc;
   *c = (beta == 0.0) ? v : v + beta * *c;
 }
 
@@ -293,6 +419,23 @@ __bf16* packed_a(hipStream_t stream, const Gemm& g, GemmWorkspace* ws, int64_t r
 
 }  // namespace
 
+int64_t split_bf16_pack_elems(int64_t rows, int64_t k) {
+  return 2 * ceil_div(rows, kTile) * kTile * ceil_div(k, kDepth) * kDepth;
+}
+
+void ladder_g_pm_split(hipStream_t s, const float* G, int64_t n_v, int64_t lenA, const int32_t* pair_a,
+                       const int32_t* pair_b, int64_t npairs, __bf16* Pp, __bf16* Pm) {
+  if (npairs <= 0) return;
+  const int64_t nt = ladder_tiles(n_v);
+  const int64_t row_blocks = ceil_div(npairs, kTile) * kTile / 16;
+  dim3 grid(unsigned(nt * (nt + 1) / 2), unsigned(std::min<int64_t>(row_blocks, 65535)));
+  ladder_g_pm_split_kernel<<<grid, 256, 0, s>>>(G, n_v, n_v * lenA, pair_a, pair_b, npairs,
+                                                ladder_blocked_len(n_v, true) / kDepth,
+                                                ladder_blocked_len(n_v, false) / kDepth, row_blocks,
+                                                reinterpret_cast<u4*>(Pp), reinterpret_cast<u4*>(Pm));
+  ETG_LAUNCH("ladder_g_pm_split");
+}
+
 GemmLaunch gemm_bf16x3(hipStream_t stream, const Gemm& g, GemmWorkspace* ws) {
   if (ws == nullptr) fail("gemm: split-bf16 products need a workspace");
   const int64_t m_p = ceil_div(g.m, kTile) * kTile, n_p = ceil_div(g.n, kTile) * kTile;
@@ -300,10 +443,16 @@ GemmLaunch gemm_bf16x3(hipStream_t stream, const Gemm& g, GemmWorkspace* ws) {
   const int64_t KS = Kp / kDepth;
   const int64_t b_batch = g.strideB == 0 ? 1 : g.batch;
   __bf16* Ap = packed_a(stream, g, ws, m_p, Kp);
-  const size_t b_elems = size_t(b_batch) * size_t(n_p) * size_t(Kp);
-  if (ws->packB.resize(2 * b_elems * sizeof(__bf16))) ++ws->gen;
-  __bf16* Bp = ws->packB.as<__bf16>();
-  pack(stream, g.B, g.ldb, g.strideB, !g.tb, g.n, g.k, n_p, Kp, b_batch, Bp);
+  const __bf16* Bp = static_cast<const __bf16*>(g.b_packed);
+  if (Bp == nullptr) {
+    const size_t b_elems = size_t(b_batch) * size_t(n_p) * size_t(Kp);
+    if (ws->packB.resize(2 * b_elems * sizeof(__bf16))) ++ws->gen;
+    __bf16* P = ws->packB.as<__bf16>();
+    pack(stream, g.B, g.ldb, g.strideB, !g.tb, g.n, g.k, n_p, Kp, b_batch, P);
+    Bp = P;
+  } else if (g.strideB != 0) {
+    fail("gemm: a prepacked B operand is shared by the whole batch");
+  }
 
   Params p{};
   p.A = Ap; p.B = Bp;
@@ -311,7 +460,7 @@ GemmLaunch gemm_bf16x3(hipStream_t stream, const Gemm& g, GemmWorkspace* ws) {
   p.strideAp = g.strideA == 0 ? 0 : 2 * m_p * Kp;
   p.strideBp = g.strideB == 0 ? 0 : 2 * n_p * Kp;
   p.m = g.m; p.n = g.n; p.alpha = g.alpha; p.beta = g.beta;
-  p.C = g.C; p.ldc = g.ldc; p.strideC = g.strideC;
+  p.C = g.C; p.Cf = g.Cf; p.ldc = g.ldc; p.strideC = g.strideC;
   p.tiles_m = int(m_p / kTile);
   p.tiles_n = int(n_p / kTile);
   p.ks_per_split = KS;
@@ -346,7 +495,7 @@ GemmLaunch gemm_bf16x3(hipStream_t stream, const Gemm& g, GemmWorkspace* ws) {
   if (splits > 1) {
     const int64_t mn = g.m * g.n;
     dim3 rgrid(unsigned(ceil_div(mn, 256)), 1, unsigned(g.batch));
-    splitk_reduce_bf<<<rgrid, 256, 0, stream>>>(p.W, splits, g.m, g.n, g.alpha, g.beta, g.C, g.ldc,
+    splitk_reduce_bf<<<rgrid, 256, 0, stream>>>(p.W, splits, g.m, g.n, g.alpha, g.beta, g.C, g.Cf, g.ldc,
                                                 g.strideC);
     ETG_LAUNCH("splitk_reduce");
   }

@@ -1,5 +1,6 @@
 // Bandwidth-bound kernels of the CCSD residual (see kernels.hpp).
 #include "kernels.hpp"
+#include "ladder_order.hpp"
 
 #include <cmath>
 #include <cstring>
@@ -156,6 +157,33 @@ __global__ void ladder_t_pm_kernel(const double* __restrict__ t2, double* __rest
     }
   }
   (void)vm;
+}
+
+// t+ (op x KP) and t- (om x KM) with the contraction index in the tile-blocked
+// order of ladder_order.hpp (zero at padding positions).
+__global__ void ladder_t_pm_blocked_kernel(const double* __restrict__ t2, double* __restrict__ tp,
+                                           double* __restrict__ tm, int64_t n_o, int64_t n_v, int64_t KP,
+                                           int64_t KM) {
+  const int64_t op = n_o * (n_o + 1) / 2, om = n_o * (n_o - 1) / 2;
+  const int64_t np = op * KP, n = np + om * KM;
+  for (int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; idx < n;
+       idx += int64_t(gridDim.x) * blockDim.x) {
+    const bool plus = idx < np;
+    const int64_t q = plus ? idx : idx - np;
+    const int64_t nij = plus ? op : om;
+    const int64_t rij = q % nij, kk = q / nij;
+    // plus: i <= j at j (j + 1) / 2 + i; minus: i < j at j (j - 1) / 2 + i
+    const int64_t j = plus ? tri_root(rij) : tri_root(rij) + 1;
+    const int64_t i = rij - (plus ? j * (j + 1) / 2 : j * (j - 1) / 2);
+    int64_t c = 0, d = 0;
+    double val = 0.0;
+    if (ladder_pair_of(kk, n_v, plus, c, d)) {
+      const double x1 = t2[pidx(c + n_v * i, d + n_v * j)];
+      const double x2 = t2[pidx(c + n_v * j, d + n_v * i)];
+      val = plus ? (c == d ? 0.5 : 1.0) * (x1 + x2) : x1 - x2;
+    }
+    (plus ? tp : tm)[q] = val;
+  }
 }
 
 // g+ (v(v+1)/2 x npairs) and g- (v(v-1)/2 x npairs) of a G block.  For pair k = (a, b) the v x v block
@@ -403,6 +431,14 @@ void ladder_t_pm(hipStream_t s, const double* t2, double* tp, double* tm, int64_
   const int64_t n = (n_o * (n_o + 1) / 2) * (n_v * (n_v + 1) / 2);
   ladder_t_pm_kernel<<<grid_for(n), kBlock, 0, s>>>(t2, tp, tm, n_o, n_v);
   ETG_LAUNCH("ladder_t_pm");
+}
+
+void ladder_t_pm_blocked(hipStream_t s, const double* t2, double* tp, double* tm, int64_t n_o,
+                         int64_t n_v) {
+  const int64_t KP = ladder_blocked_len(n_v, true), KM = ladder_blocked_len(n_v, false);
+  const int64_t n = (n_o * (n_o + 1) / 2) * KP + (n_o * (n_o - 1) / 2) * KM;
+  ladder_t_pm_blocked_kernel<<<grid_for(n), kBlock, 0, s>>>(t2, tp, tm, n_o, n_v, KP, KM);
+  ETG_LAUNCH("ladder_t_pm_blocked");
 }
 
 void ladder_g_pm(hipStream_t s, const double* G, int64_t n_v, int64_t lenA, int64_t lenB,

@@ -39,6 +39,10 @@ void symmetrize_and_pack(hipStream_t s, const double* x, double* packed, int64_t
 // t+ (o+ x v+): (1 - d_cd/2)(t_cidj + t_cjdi), rows ij (i <= j), cols cd (c <= d)
 // t- (o- x v-): t_cidj - t_cjdi, rows i < j, cols c < d
 void ladder_t_pm(hipStream_t s, const double* t2, double* tp, double* tm, int64_t n_o, int64_t n_v);
+// the same in the tile-blocked (c, d) order of the split-bf16 ladder (ladder_order.hpp):
+// tp is op x ladder_blocked_len(n_v, true), tm is om x ladder_blocked_len(n_v, false)
+void ladder_t_pm_blocked(hipStream_t s, const double* t2, double* tp, double* tm, int64_t n_o,
+                         int64_t n_v);
 
 // From G(c, a', d, b') = sum_J L_ac L_bd of one (A, B) block build
 // g+ (v+ x npairs) = G(c,a,d,b) + G(d,a,c,b) and g- (v- x npairs) = G(c,a,d,b) - G(d,a,c,b)

@@ -94,6 +94,22 @@ def test_adjoint_identity_larger_system():
         assert rel(Ac, ref) < 1e-11
 
 
+def test_split_bf16_transformations_over_ladder_tiles():
+    """Split-bf16 right and left transformations of a system whose ladder spans several
+    16 x 16 tiles of the (c, d) plane (n_v = 37: a narrow last tile column) and several
+    ladder blocks stay within the products' rounding of the FP64 ones."""
+    n_o, n_v, n_J = 4, 37, 40
+    s = make(n_o, n_v, n_J, 11)
+    n = n_o * n_v + O.n_t2(n_o, n_v)
+    X = np.random.default_rng(12).normal(size=(n, 3))
+    with load(s, 13) as wf:
+        wf.prepare_for_jacobian()
+        for side in ('right', 'left'):
+            R = wf.jacobian_transformations(X, side)
+            Rp = wf.jacobian_transformations(X, side, precision=ccsd.BF16X3)
+            assert 0 < rel(Rp, R) < 2e-4, side
+
+
 def test_jacobian_requires_preparation():
     s = make(2, 3, 5, 4)
     with load(s) as wf:
