@@ -197,20 +197,23 @@ __device__ inline f4 mfma(bf8 a, bf8 b, f4 c) {
 }
 
 // ---------------------------------------------------------------------------
-// Workgroup tile 256 x 256, 4 waves in 2 x 2, each 128 x 128 of the output
-// (8 x 8 blocks; FP32 accumulators in the accumulation registers).  Both
-// operand slabs of a 32-k step (16 row blocks x 2 splits each: 64 chunks of
-// 1 KB) go from global memory straight into LDS (global_load_lds_dwordx4: lane
-// l's 16 B land at the chunk's LDS base + 16 l), double buffered (128 KB).  A
-// wave's step is 192 MFMAs (3072 cycles) against 32 KB of LDS fragment reads;
-// per output element and k the workgroup moves 1/64 operand bytes.
+// Workgroup tile 256 x 256, 8 waves in 2 x 4 (two per SIMD), each 128 x 64 of
+// the output (8 x 4 blocks; FP32 accumulators in the accumulation registers).
+// Both operand slabs of a 32-k step (16 row blocks x 2 splits each: 64 chunks
+// of 1 KB) go from global memory straight into LDS (global_load_lds_dwordx4:
+// lane l's 16 B land at the chunk's LDS base + 16 l), double buffered (128 KB).
+// A wave's step is 96 MFMAs (1536 cycles) against 24 KB of LDS fragment reads;
+// the two waves of a SIMD cover each other's LDS waits.  Per output element and
+// k the workgroup moves 1/64 operand bytes.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kThreads, 1) void gemm_bf16x3_kernel(Params p) {
+constexpr int kProductThreads = 512;
+
+__global__ __launch_bounds__(kProductThreads, 1) void gemm_bf16x3_kernel(Params p) {
   __shared__ u4 lds[2][4096];        // per buffer: A chunks 0..31, B chunks 32..63 (1 KB each)
 
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int wm = wave & 1, wn = wave >> 1;
+  const int wm = wave & 1, wn = wave >> 1;     // rows 128 wm.., columns 64 wn..
   int tile_m, tile_n;
   map_tile(blockIdx.x, p.tiles_m, p.tiles_n, tile_m, tile_n);
   const int64_t bz = blockIdx.z, split = blockIdx.y;
@@ -219,39 +222,40 @@ __global__ __launch_bounds__(kThreads, 1) void gemm_bf16x3_kernel(Params p) {
   const int64_t nsteps = ks1 - ks0;
   const int64_t astride = p.KS * 128;          // u4 units from one row block to the next
 
-  // the 16 chunks this wave copies per step: c = 16 wave + q; c < 32: A row block
-  // c / 2, split c % 2; otherwise B column block (c - 32) / 2, split c % 2.
-  // Chunk addresses: a wave-uniform base (scalar registers) + 16 B per lane.
-  const u4* src = reinterpret_cast<const u4*>(wave < 2 ? p.A + bz * p.strideAp : p.B + bz * p.strideBp) +
-                  int64_t(wave < 2 ? tile_m : tile_n) * 16 * astride;
-  const int cbase = 16 * (wave & 1);           // chunk index within the operand's 32
+  // waves 0-3 copy A, waves 4-7 copy B: 8 chunks each per step, c = 8 (wave % 4)
+  // + q: row block c / 2, split c % 2 of the operand's 32 chunks.  Chunk
+  // addresses: a wave-uniform base (scalar registers) + 16 B per lane.
+  const bool copies_a = wave < 4;
+  const int cbase = 8 * (wave & 3);
+  const u4* src = reinterpret_cast<const u4*>(copies_a ? p.A + bz * p.strideAp : p.B + bz * p.strideBp) +
+                  int64_t(copies_a ? tile_m : tile_n) * 16 * astride + (cbase >> 1) * astride;
   auto load = [&](int buf, int64_t ks) {
-    u4* dst = &lds[buf][(wave < 2 ? 0 : 2048) + cbase * 64];
-    const u4* base = src + (cbase >> 1) * astride + ks * 128;
+    u4* dst = &lds[buf][(copies_a ? 0 : 2048) + cbase * 64];
+    const u4* base = src + ks * 128;
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
+    for (int q = 0; q < 8; ++q) {
       const u4* g = base + (q >> 1) * astride + (q & 1) * 64;
       __builtin_amdgcn_global_load_lds(reinterpret_cast<const void*>(g + lane),
                                        (lds_void*)(dst + q * 64), 16, 0, 0);
     }
   };
 
-  f4 acc[8][8];
+  f4 acc[8][4];
 #pragma unroll
   for (int i = 0; i < 8; ++i)
 #pragma unroll
-    for (int j = 0; j < 8; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+    for (int j = 0; j < 4; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
 
   // LDS reads in the order the products consume them: hi of B column 0 and A,
-  // lo of B column 0 and A, then B one column ahead.  The scheduling groups
-  // start the first products after two reads and keep every read ahead of its
-  // first use, so the step's LDS reads run under its matrix-core work.
+  // lo of B column 0 and A, then B one column ahead; the scheduling groups
+  // start the first products after a few reads and keep every read ahead of
+  // its first use.
   auto compute = [&](int buf) {
     const u4* As = &lds[buf][0];
     const u4* Bs = &lds[buf][2048];
-    bf8 ah[8], al[8], bh[8], bl[8];
+    bf8 ah[8], al[8], bh[4], bl[4];
     auto rd_b = [&](int j, int sp) {
-      return __builtin_bit_cast(bf8, Bs[((wn * 8 + j) * 2 + sp) * 64 + lane]);
+      return __builtin_bit_cast(bf8, Bs[((wn * 4 + j) * 2 + sp) * 64 + lane]);
     };
     auto rd_a = [&](int i, int sp) {
       return __builtin_bit_cast(bf8, As[((wm * 8 + i) * 2 + sp) * 64 + lane]);
@@ -263,12 +267,12 @@ __global__ __launch_bounds__(kThreads, 1) void gemm_bf16x3_kernel(Params p) {
 #pragma unroll
     for (int i = 0; i < 8; ++i) al[i] = rd_a(i, 1);
 #pragma unroll
-    for (int j = 1; j < 8; ++j) {
+    for (int j = 1; j < 4; ++j) {
       bh[j] = rd_b(j, 0);
       bl[j] = rd_b(j, 1);
     }
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
+    for (int j = 0; j < 4; ++j) {
 #pragma unroll
       for (int i = 0; i < 8; ++i) acc[i][j] = mfma(ah[i], bh[j], acc[i][j]);
 #pragma unroll
@@ -276,7 +280,7 @@ __global__ __launch_bounds__(kThreads, 1) void gemm_bf16x3_kernel(Params p) {
 #pragma unroll
       for (int i = 0; i < 8; ++i) acc[i][j] = mfma(al[i], bh[j], acc[i][j]);
     }
-    // reads: bh0 ah0..7 | bl0 al0..7 | bh1 bl1 ... bh7 bl7
+    // reads: bh0 ah0..7 | bl0 al0..7 | bh1 bl1 bh2 bl2 bh3 bl3
     __builtin_amdgcn_sched_group_barrier(0x100, 3, 0);   // bh0 ah0 ah1
 #pragma unroll
     for (int i = 0; i < 6; ++i) {                          // hi x hi of column 0 under ah2..ah7
@@ -291,15 +295,12 @@ __global__ __launch_bounds__(kThreads, 1) void gemm_bf16x3_kernel(Params p) {
       __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
       __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
     }
-    __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);     // bh1 bl1
+    __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);     // column 1
     __builtin_amdgcn_sched_group_barrier(0x008, 8, 0);     // lo x hi of column 0
     __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);     // column 2
-#pragma unroll
-    for (int j = 1; j < 6; ++j) {                          // column j under the reads of column j + 2
-      __builtin_amdgcn_sched_group_barrier(0x008, 24, 0);
-      __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
-    }
-    __builtin_amdgcn_sched_group_barrier(0x008, 48, 0);
+    __builtin_amdgcn_sched_group_barrier(0x008, 24, 0);    // column 1
+    __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);     // column 3
+    __builtin_amdgcn_sched_group_barrier(0x008, 48, 0);    // columns 2 and 3
   };
 
   // Step s computes from buffer s % 2 while the slabs of step s + 1 land in the
@@ -314,13 +315,13 @@ __global__ __launch_bounds__(kThreads, 1) void gemm_bf16x3_kernel(Params p) {
   }
 
   const int lrow = lane >> 4, lcol = lane & 15;
-  const int64_t r0 = int64_t(tile_m) * kTile + wm * 128, c0 = int64_t(tile_n) * kTile + wn * 128;
+  const int64_t r0 = int64_t(tile_m) * kTile + wm * 128, c0 = int64_t(tile_n) * kTile + wn * 64;
   if (p.W) {
     double* W = p.W + (split + bz * gridDim.y) * p.m * p.n;
 #pragma unroll
     for (int i = 0; i < 8; ++i)
 #pragma unroll
-      for (int j = 0; j < 8; ++j)
+      for (int j = 0; j < 4; ++j)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int64_t row = r0 + i * 16 + 4 * lrow + r;
@@ -334,7 +335,7 @@ __global__ __launch_bounds__(kThreads, 1) void gemm_bf16x3_kernel(Params p) {
 #pragma unroll
     for (int i = 0; i < 8; ++i)
 #pragma unroll
-      for (int j = 0; j < 8; ++j)
+      for (int j = 0; j < 4; ++j)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int64_t row = r0 + i * 16 + 4 * lrow + r;
@@ -347,7 +348,7 @@ __global__ __launch_bounds__(kThreads, 1) void gemm_bf16x3_kernel(Params p) {
 #pragma unroll
   for (int i = 0; i < 8; ++i)
 #pragma unroll
-    for (int j = 0; j < 8; ++j)
+    for (int j = 0; j < 4; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int64_t row = r0 + i * 16 + 4 * lrow + r;
@@ -490,7 +491,7 @@ GemmLaunch gemm_bf16x3(hipStream_t stream, const Gemm& g, GemmWorkspace* ws) {
     p.W = ws->buf.as<double>();
   }
   dim3 grid(unsigned(tiles / g.batch), unsigned(splits), unsigned(g.batch));
-  gemm_bf16x3_kernel<<<grid, kThreads, 0, stream>>>(p);
+  gemm_bf16x3_kernel<<<grid, kProductThreads, 0, stream>>>(p);
   ETG_LAUNCH("gemm_bf16x3");
   if (splits > 1) {
     const int64_t mn = g.m * g.n;
