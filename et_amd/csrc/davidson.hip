@@ -390,7 +390,7 @@ int64_t CCSD::davidson(int side, int64_t n_states, double threshold, int64_t max
   // otherwise AV, and then V as well, in pinned host memory that the kernels
   // read and write over PCIe (etg_set_davidson_space overrides the choice).
   dav_Y_.resize(sizeof(double) * size_t(n) * size_t(3 * s1));
-  dav_small_.resize(sizeof(double) * size_t(max_dim * max_dim + 2 * max_dim * s1 + 3 * s1 + max_dim));
+  dav_small_.resize(sizeof(double) * size_t(max_dim * max_dim + 2 * max_dim * s1 + 3 * s1 + max_dim + 2 * s1 * s1));
   {
     const size_t space = sizeof(double) * size_t(n) * size_t(max_dim);
     size_t free_b = 0, total_b = 0;
@@ -434,6 +434,8 @@ int64_t CCSD::davidson(int side, int64_t n_states, double threshold, int64_t max
   double* cmat = Xr + max_dim * s1;          // projection coefficients (max_dim x s1)
   double* coef = cmat + max_dim * s1;        // [w | c | partner] (3 s1)
   double* cvec = coef + 3 * s1;              // Gram-Schmidt coefficients (max_dim)
+  double* gram = cvec + max_dim;             // Gram matrix of a block (s1 x s1)
+  double* tmat = gram + s1 * s1;             // inverse Cholesky factor of a block (s1 x s1)
   diff_.resize(sizeof(double) * n);
   double* d = diff_.as<double>();
   orbital_differences(stream_, eps_.as<double>(), d, n_o, n_v, n);
@@ -445,6 +447,41 @@ int64_t CCSD::davidson(int side, int64_t n_states, double threshold, int64_t max
     ETG_HIP(hipMemcpyAsync(&v, res, sizeof(double), hipMemcpyDeviceToHost, stream_));
     synchronize();
     return v;
+  };
+  // Cholesky QR of q columns at X: W = X T with T = R^-1, R^T R = X^T X (and
+  // WA = XA T alongside when XA is given).  It gives the Q of Gram-Schmidt when
+  // every column keeps more than 1e-4 of its norm against the ones before it
+  // (Cholesky pivots above 1e-8 of the diagonal); false otherwise, with nothing
+  // written.
+  auto cholqr = [&](const double* X, const double* XA, int64_t q, double* W, double* WA) {
+    run_gemm(make_gemm('T', 'N', q, q, n, 1.0, X, n, X, n, 0.0, gram, q), Prec::F64);
+    std::vector<double> hg(size_t(q * q)), R(size_t(q * q), 0.0), T(size_t(q * q), 0.0);
+    ETG_HIP(hipMemcpyAsync(hg.data(), gram, sizeof(double) * q * q, hipMemcpyDeviceToHost, stream_));
+    synchronize();
+    for (int64_t c = 0; c < q; ++c) {
+      for (int64_t i = 0; i < c; ++i) {
+        double v = hg[i + c * q];
+        for (int64_t l = 0; l < i; ++l) v -= R[l + i * q] * R[l + c * q];
+        R[i + c * q] = v / R[i + i * q];
+      }
+      double dcc = hg[c + c * q];
+      for (int64_t l = 0; l < c; ++l) dcc -= R[l + c * q] * R[l + c * q];
+      if (!(dcc > 1e-8 * hg[c + c * q])) return false;
+      R[c + c * q] = std::sqrt(dcc);
+    }
+    for (int64_t c = 0; c < q; ++c) {
+      T[c + c * q] = 1.0 / R[c + c * q];
+      for (int64_t i = c - 1; i >= 0; --i) {
+        double v = 0.0;
+        for (int64_t l = i + 1; l <= c; ++l) v += R[i + l * q] * T[l + c * q];
+        T[i + c * q] = -v / R[i + i * q];
+      }
+    }
+    ETG_HIP(hipMemcpyAsync(tmat, T.data(), sizeof(double) * q * q, hipMemcpyHostToDevice, stream_));
+    run_gemm(make_gemm('N', 'N', n, q, q, 1.0, X, n, tmat, q, 0.0, W, n), Prec::F64);
+    if (XA) run_gemm(make_gemm('N', 'N', n, q, q, 1.0, XA, n, tmat, q, 0.0, WA, n), Prec::F64);
+    synchronize();                        // T (host) is read by the copy above
+    return true;
   };
   if (start) {
     // caller's start vectors (e.g. a reduced-precision solution), orthonormalised
@@ -577,8 +614,17 @@ int64_t CCSD::davidson(int side, int64_t n_states, double threshold, int64_t max
     int64_t n_new = 0;
     for (int64_t j = 0; j < se; ++j) n_new += norms[j] >= threshold;
     if (k + n_new > max_dim) {
-      // orthonormalise the Ritz vectors (and their transforms alike) in HBM,
-      // then they become the space
+      // orthonormalise the Ritz vectors (and their transforms alike), then they
+      // become the space: Cholesky QR twice (through V and AV, which are
+      // replaced anyway), or one vector at a time in HBM
+      if (se > 1 && cholqr(Y, AY, se, V, AV) && cholqr(V, AV, se, Y, AY)) {
+        run_gemm(make_gemm('T', 'N', se, se, n, 1.0, Y, n, AY, n, 0.0, G, max_dim), Prec::F64);
+        ETG_HIP(hipMemcpyAsync(V, Y, sizeof(double) * n * se, hipMemcpyDefault, stream_));
+        ETG_HIP(hipMemcpyAsync(AV, AY, sizeof(double) * n * se, hipMemcpyDefault, stream_));
+        k = se;
+      }
+    }
+    if (k + n_new > max_dim) {
       for (int64_t j = 0; j < se; ++j) {
         for (int pass = 0; pass < 2; ++pass)
           for (int64_t i = 0; i < j; ++i) {
@@ -615,9 +661,13 @@ int64_t CCSD::davidson(int side, int64_t n_states, double threshold, int64_t max
         run_gemm(make_gemm('T', 'N', k, nc, n, 1.0, V, n, Rb, n, 0.0, cmat, max_dim), Prec::F64);
         run_gemm(make_gemm('N', 'N', n, nc, k, -1.0, V, n, cmat, max_dim, 1.0, Rb, n), Prec::F64);
       }
-    // ... then against each other (two passes), dropping linearly dependent ones
+    // ... then against each other: Cholesky QR twice when no vector comes near
+    // the span of the ones before it (through Y, free until the next Ritz
+    // vectors), otherwise one vector at a time (two passes), dropping linearly
+    // dependent ones
     m = 0;
-    for (int64_t c = 0; c < nc; ++c) {
+    if (nc > 1 && cholqr(Rb, nullptr, nc, Y, nullptr) && cholqr(Y, nullptr, nc, Rb, nullptr)) m = nc;
+    for (int64_t c = m; c < nc; ++c) {
       double* x = Rb + c * n;
       if (m > 0)
         for (int pass = 0; pass < 2; ++pass) {

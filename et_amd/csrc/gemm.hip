@@ -532,7 +532,8 @@ GemmLaunch launch_typed(hipStream_t stream, const Gemm& g, GemmWorkspace* ws) {
 //     2l, 2l+1, so per row an FMA-pair reads 16 B of A (lanes contiguous, no
 //     bank conflict) and a broadcast of B; the next chunk's loads are in flight
 //     while the current one is multiplied.  Per-block partial sums, then a
-//     deterministic sum over the blocks;
+//     deterministic sum over the blocks (for m <= 16 a kernel with one
+//     output per thread instead);
 //   gemm_tall_nn: C (M x n) = alpha A B + beta C, A (M x k), B (k x n) small:
 //     B in LDS, one row of C per thread, columns of A read coalesced.
 constexpr int kTallRows = 32;
@@ -604,6 +605,48 @@ __global__ __launch_bounds__(kTallThreads) void gemm_tall_tn_kernel(
     }
 }
 
+// The same partial sums for m, n <= 16: a chunk of 32 rows of A and B staged
+// in LDS, thread t accumulating C(t % 16, t / 16) over the chunk (one FMA per
+// row and thread; the 128 x 16 kernel would spend 2048 per row).
+__global__ __launch_bounds__(kTallThreads) void gemm_tall_tn_small_kernel(
+    const double* __restrict__ A, int64_t lda, const double* __restrict__ B, int64_t ldb, int64_t K, int m,
+    int n, int64_t rows_per_block, double* __restrict__ part) {
+  __shared__ double As[kTallRows * 17];            // [row][i]
+  __shared__ double Bs[kTallRows * 17];            // [row][j]
+  const int tid = threadIdx.x;
+  const int ci = tid & 15, cj = tid >> 4;
+  const int64_t r0 = int64_t(blockIdx.x) * rows_per_block;
+  const int64_t r1 = min(K, r0 + rows_per_block);
+  double ra[2], rb[2];
+  auto load = [&](int64_t rb0) {
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int e = tid + kTallThreads * q;       // row e % 32, column e / 32
+      const int64_t row = rb0 + (e & (kTallRows - 1));
+      const int col = e / kTallRows;
+      const bool ok = row < r1;
+      ra[q] = (ok && col < m) ? A[row + int64_t(col) * lda] : 0.0;
+      rb[q] = (ok && col < n) ? B[row + int64_t(col) * ldb] : 0.0;
+    }
+  };
+  double acc = 0.0;
+  if (r0 < r1) load(r0);
+  for (int64_t rb0 = r0; rb0 < r1; rb0 += kTallRows) {
+    __syncthreads();                               // the previous chunk's reads are done
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int e = tid + kTallThreads * q;
+      As[(e & (kTallRows - 1)) * 17 + e / kTallRows] = ra[q];
+      Bs[(e & (kTallRows - 1)) * 17 + e / kTallRows] = rb[q];
+    }
+    __syncthreads();
+    if (rb0 + kTallRows < r1) load(rb0 + kTallRows);
+#pragma unroll 8
+    for (int rr = 0; rr < kTallRows; ++rr) acc += As[rr * 17 + ci] * Bs[rr * 17 + cj];
+  }
+  if (ci < m && cj < n) part[int64_t(blockIdx.x) * (m * n) + ci + cj * m] = acc;
+}
+
 // C(i, j) (or C(j, i) with transpose) = alpha sum_b part_b(i, j) + beta C
 __global__ void gemm_tall_reduce_kernel(const double* __restrict__ part, int64_t nblocks, int m, int n,
                                         double alpha, double beta, double* __restrict__ C, int64_t ldc,
@@ -662,8 +705,12 @@ GemmLaunch gemm_tall(hipStream_t stream, const Gemm& g, GemmWorkspace* ws) {
     for (int64_t j0 = 0; j0 < cols; j0 += kTallN)
       for (int64_t i0 = 0; i0 < rows; i0 += kTallM) {
         const int mi = int(std::min<int64_t>(kTallM, rows - i0)), nj = int(std::min<int64_t>(kTallN, cols - j0));
-        gemm_tall_tn_kernel<<<unsigned(nb), kTallThreads, 0, stream>>>(P + i0 * ldp, ldp, Q + j0 * ldq, ldq, g.k,
-                                                                        mi, nj, per, ws->buf.as<double>());
+        if (mi <= 16)
+          gemm_tall_tn_small_kernel<<<unsigned(nb), kTallThreads, 0, stream>>>(
+              P + i0 * ldp, ldp, Q + j0 * ldq, ldq, g.k, mi, nj, per, ws->buf.as<double>());
+        else
+          gemm_tall_tn_kernel<<<unsigned(nb), kTallThreads, 0, stream>>>(P + i0 * ldp, ldp, Q + j0 * ldq, ldq,
+                                                                          g.k, mi, nj, per, ws->buf.as<double>());
         ETG_LAUNCH("gemm_tall_tn");
         double* C0 = swap ? g.C + j0 + i0 * g.ldc : g.C + i0 + j0 * g.ldc;
         gemm_tall_reduce_kernel<<<unsigned(ceil_div(int64_t(mi) * nj, 256)), 256, 0, stream>>>(

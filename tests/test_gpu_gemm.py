@@ -132,6 +132,7 @@ def test_measured_tile_choice_keeps_results(ta, tb, m, n, k):
 
 
 @pytest.mark.parametrize('ta,tb,m,n,k', [('T', 'N', 3, 2, (1 << 24) + 3),       # inner products
+                                         ('T', 'N', 16, 16, (1 << 24) + 3),     # one output per thread
                                          ('T', 'N', 20, 17, (1 << 24) + 3),     # two 16-column blocks
                                          ('T', 'N', 5, 40, (1 << 24) + 3),      # C^T = B^T A
                                          ('N', 'N', (1 << 24) + 3, 2, 3)])      # tall output
