@@ -146,36 +146,73 @@ def finish_cpu_baseline(proc, threads):
     return out
 
 
+class DeviceVectors:
+    """Column-major float64 arrays in HBM through the HIP runtime the library loaded
+    (the same process-wide instance), for calls on device-resident trial vectors."""
+
+    def __init__(self, X):
+        import ctypes
+        ccsd.load_library()
+        with open('/proc/self/maps') as f:
+            paths = {ln.split()[-1] for ln in f if 'libamdhip64' in ln and ln.split()[-1].startswith('/')}
+        self._hip = hip = ctypes.CDLL(sorted(paths)[0])
+        hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        hip.hipFree.argtypes = [ctypes.c_void_p]
+        X = np.asfortranarray(X, dtype=np.float64)
+        self.x, self.r = ctypes.c_void_p(), ctypes.c_void_p()
+        if hip.hipMalloc(ctypes.byref(self.x), X.nbytes) != 0 or hip.hipMalloc(ctypes.byref(self.r), X.nbytes) != 0:
+            raise RuntimeError('hipMalloc of the trial vectors failed')
+        if hip.hipMemcpy(self.x, X.ctypes.data, X.nbytes, 1) != 0:
+            raise RuntimeError('hipMemcpy of the trial vectors failed')
+
+    def close(self):
+        self._hip.hipFree(self.x)
+        self._hip.hipFree(self.r)
+
+
 def jacobian_throughput(wf):
     """Right and left Jacobian transformations of one trial vector on the same workload.
 
     The residual program is recorded once (etg_prepare_for_jacobian); a right
     transformation replays it forward, a left one backward with the g_acbd
-    blocks recomputed.  The trial vector and result are host arrays here, so
-    each rate includes two PCIe copies of n_t1 + n_t2 doubles.
+    blocks recomputed.  Each rate is the second call of its (side, precision) on a
+    trial vector resident in HBM (the first sizes the buffers and captures the
+    pass); the same call on host arrays (two PCIe copies of n_t1 + n_t2 doubles
+    of pageable memory) is reported beside it.
     """
     t0 = time.perf_counter()
     wf.prepare_for_jacobian()
     prep = time.perf_counter() - t0
     X = np.random.default_rng(5).standard_normal((wf.n_gs_amplitudes, 1))
+    dv = DeviceVectors(X)
     out = {'prepare_s': prep}
     for side in ('right', 'left'):
         for prec, pname in ((ccsd.FP64, 'fp64'), (ccsd.BF16X3, 'split-bf16x3')):
+            wf.jacobian_transformations_at(dv.x.value, dv.r.value, 1, side, prec)
+            wf.synchronize()
+            t0 = time.perf_counter()
+            wf.jacobian_transformations_at(dv.x.value, dv.r.value, 1, side, prec)
+            wf.synchronize()
+            dt = time.perf_counter() - t0
             t0 = time.perf_counter()
             wf.jacobian_transformations(X, side, precision=prec)
-            dt = time.perf_counter() - t0
+            dt_host = time.perf_counter() - t0
             out['%s_%s_transforms_per_s' % (side, pname)] = 1.0 / dt
-            log('%s transformation (%s): %.3f s' % (side, pname, dt))
+            out['%s_%s_transforms_per_s_host_vectors' % (side, pname)] = 1.0 / dt_host
+            log('%s transformation (%s): %.3f s (%.3f s with host vectors)' % (side, pname, dt, dt_host))
+    dv.close()
     return out
 
 
 def jacobian_sweep(r, n_vecs=(1, 7, 12, 24, 36)):
     """Batched Jacobian transformations on the system r after an FP64 ground state:
     for each side, precision and trial-vector count (7: an uneven group), the wall
-    time of one etg_jacobian_transformations call (graph replays; host vectors, so
-    two PCIe copies per vector are included), and from the per-kernel ledger of an
-    eager pass of the same call the products' FLOPs per transformation and their
-    rate against the matrix-core roofline of the precision."""
+    time of one etg_jacobian_transformations call on trial vectors resident in HBM
+    (graph replays; as the Davidson solver calls it) and on host vectors (two PCIe
+    copies of pageable memory per vector included), and from the per-kernel ledger
+    of an eager pass of the same call the products' FLOPs per transformation and
+    their rate against the matrix-core roofline of the precision."""
     from et_amd import solvers
     n_o = r['n_o']
     n_v = r['L_mo'].shape[1] - n_o
@@ -185,32 +222,40 @@ def jacobian_sweep(r, n_vecs=(1, 7, 12, 24, 36)):
         solvers.DiisCCGroundState(wf, residual_threshold=1e-8).run()
         rng = np.random.default_rng(11)
         X = rng.standard_normal((wf.n_gs_amplitudes, max(n_vecs)))
+        dv = DeviceVectors(X)                                  # the same vectors in HBM
         for side in ('right', 'left'):
             wf.prepare_for_jacobian(side)
             for prec, pname in ((ccsd.FP64, 'fp64'), (ccsd.BF16X3, 'bf16x3')):
                 for nv in n_vecs:
-                    wf.jacobian_transformations(X[:, :nv], side, prec)       # sizes, captures
+                    wf.jacobian_transformations_at(dv.x.value, dv.r.value, nv, side, prec)  # sizes, captures
                     wf.synchronize()
+                    t0 = time.perf_counter()
+                    wf.jacobian_transformations_at(dv.x.value, dv.r.value, nv, side, prec)
+                    wf.synchronize()
+                    dt = time.perf_counter() - t0
                     t0 = time.perf_counter()
                     wf.jacobian_transformations(X[:, :nv], side, prec)
                     wf.synchronize()
-                    dt = time.perf_counter() - t0
+                    dt_host = time.perf_counter() - t0
                     wf.set_kernel_ledger(True)
-                    wf.jacobian_transformations(X[:, :nv], side, prec)
+                    wf.jacobian_transformations_at(dv.x.value, dv.r.value, nv, side, prec)
                     ledger = wf.kernel_ledger()
                     wf.set_kernel_ledger(False)
                     fl = sum(e[3] for e in ledger)
                     sec = sum(e[2] for e in ledger)
                     row = {'side': side, 'precision': pname, 'n_vec': nv,
                            'transforms_per_s': nv / dt, 'ms_per_transform': dt / nv * 1e3,
+                           'ms_per_transform_host_vectors': dt_host / nv * 1e3,
                            'product_flops_per_transform': fl / nv,
                            'product_tflops': fl / sec / 1e12 if sec > 0 else None,
                            'product_frac': fl / sec / PEAKS[pname] if sec > 0 else None,
                            'product_ms_per_transform': sec / nv * 1e3}
                     out['rows'].append(row)
-                    log('jacobian %s %s n_vec %d: %.1f ms per transform, products %.1f TF/s (%.0f %% of peak)'
-                        % (side, pname, nv, row['ms_per_transform'], row['product_tflops'] or 0,
-                           100 * (row['product_frac'] or 0)))
+                    log('jacobian %s %s n_vec %d: %.1f ms per transform (%.1f with host vectors), products '
+                        '%.1f TF/s (%.0f %% of peak)'
+                        % (side, pname, nv, row['ms_per_transform'], row['ms_per_transform_host_vectors'],
+                           row['product_tflops'] or 0, 100 * (row['product_frac'] or 0)))
+        dv.close()
     return out
 
 

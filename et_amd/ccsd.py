@@ -442,6 +442,13 @@ class CCSD:
                                                       _ptr(X), _ptr(R), int(precision)))
         return R
 
+    def jacobian_transformations_at(self, x_addr, r_addr, n_vec, side='right', precision=FP64):
+        """The same on caller-owned (n_t1 + n_t2) x n_vec column-major arrays at raw
+        addresses, host or device (device-resident vectors skip the PCIe copies)."""
+        _check(self._lib.etg_jacobian_transformations(self._h, 0 if side == 'right' else 1, n_vec,
+                                                      ctypes.cast(x_addr, _c_dbl_p),
+                                                      ctypes.cast(r_addr, _c_dbl_p), int(precision)))
+
     def davidson(self, n_states, side='right', residual_threshold=1e-6, max_iterations=100,
                  max_dim=0, precision=FP64, vectors=False, start=None):
         """Device Davidson (eigen_davidson_solver_class.F90:122-318): lowest roots of A or A^T.
