@@ -61,9 +61,26 @@ __device__ void inverse_iteration(double* M, int ldm, int nm, double anorm, int*
                                   int seed, int lane) {
 #define MM(i, j) M[(i) + (j) * ldm]
   for (int c = 0; c < nm; ++c) {
+    // pivot: the first row of largest magnitude, found across the lanes (each
+    // lane scans every kLanes-th row, then the wave keeps the larger value and,
+    // on a tie, the smaller row)
     int p = c;
-    for (int r = c + 1; r < nm; ++r)
-      if (fabs(MM(r, c)) > fabs(MM(p, c))) p = r;
+    double best = fabs(MM(c, c));
+    for (int r = c + 1 + lane; r < nm; r += kLanes) {
+      const double v = fabs(MM(r, c));
+      if (v > best) {
+        best = v;
+        p = r;
+      }
+    }
+    for (int off = kLanes / 2; off > 0; off >>= 1) {
+      const double ob = __shfl_xor(best, off, kLanes);
+      const int op = __shfl_xor(p, off, kLanes);
+      if (ob > best || (ob == best && op < p)) {
+        best = ob;
+        p = op;
+      }
+    }
     __syncthreads();
     if (lane == 0) {
       piv[c] = p;

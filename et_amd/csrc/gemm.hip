@@ -647,14 +647,24 @@ __global__ __launch_bounds__(kTallThreads) void gemm_tall_tn_small_kernel(
   if (ci < m && cj < n) part[int64_t(blockIdx.x) * (m * n) + ci + cj * m] = acc;
 }
 
-// C(i, j) (or C(j, i) with transpose) = alpha sum_b part_b(i, j) + beta C
-__global__ void gemm_tall_reduce_kernel(const double* __restrict__ part, int64_t nblocks, int m, int n,
-                                        double alpha, double beta, double* __restrict__ C, int64_t ldc,
-                                        int transpose) {
-  const int o = blockIdx.x * blockDim.x + threadIdx.x;
-  if (o >= m * n) return;
+// C(i, j) (or C(j, i) with transpose) = alpha sum_b part_b(i, j) + beta C.
+// 16 outputs per workgroup, each summed by 16 threads over every 16th block
+// and the 16 partial sums then added in a fixed order (deterministic).
+__global__ __launch_bounds__(256) void gemm_tall_reduce_kernel(const double* __restrict__ part, int64_t nblocks,
+                                                               int m, int n, double alpha, double beta,
+                                                               double* __restrict__ C, int64_t ldc, int transpose) {
+  __shared__ double red[16][17];
+  const int lo = threadIdx.x & 15, q = threadIdx.x >> 4;
+  const int o = blockIdx.x * 16 + lo;
+  const int mn = m * n;
   double s = 0.0;
-  for (int64_t b = 0; b < nblocks; ++b) s += part[b * m * n + o];
+  if (o < mn)
+    for (int64_t b = q; b < nblocks; b += 16) s += part[b * mn + o];
+  red[q][lo] = s;
+  __syncthreads();
+  if (q != 0 || o >= mn) return;
+  s = 0.0;
+  for (int r = 0; r < 16; ++r) s += red[r][lo];
   const int i = o % m, j = o / m;
   double* c = transpose ? C + j + int64_t(i) * ldc : C + i + int64_t(j) * ldc;
   *c = (beta == 0.0) ? alpha * s : alpha * s + beta * *c;
@@ -673,7 +683,19 @@ __global__ __launch_bounds__(kTallThreads) void gemm_tall_nn_kernel(
       double acc[16];
 #pragma unroll
       for (int jj = 0; jj < 16; ++jj) acc[jj] = 0.0;
-      for (int i = 0; i < k; ++i) {
+      // eight columns of A per round: their loads are in flight together
+      int i = 0;
+      for (; i + 8 <= k; i += 8) {
+        double a[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) a[u] = A[row + int64_t(i + u) * lda];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+#pragma unroll
+          for (int jj = 0; jj < 16; ++jj)
+            if (jj < nj) acc[jj] += a[u] * Bs[i + u + k * (j0 + jj)];
+      }
+      for (; i < k; ++i) {
         const double a = A[row + int64_t(i) * lda];
 #pragma unroll
         for (int jj = 0; jj < 16; ++jj)
@@ -713,7 +735,7 @@ GemmLaunch gemm_tall(hipStream_t stream, const Gemm& g, GemmWorkspace* ws) {
                                                                           g.k, mi, nj, per, ws->buf.as<double>());
         ETG_LAUNCH("gemm_tall_tn");
         double* C0 = swap ? g.C + j0 + i0 * g.ldc : g.C + i0 + j0 * g.ldc;
-        gemm_tall_reduce_kernel<<<unsigned(ceil_div(int64_t(mi) * nj, 256)), 256, 0, stream>>>(
+        gemm_tall_reduce_kernel<<<unsigned(ceil_div(int64_t(mi) * nj, 16)), 256, 0, stream>>>(
             ws->buf.as<double>(), nb, mi, nj, g.alpha, g.beta, C0, g.ldc, swap ? 1 : 0);
         ETG_LAUNCH("gemm_tall_reduce");
       }

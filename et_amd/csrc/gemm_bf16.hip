@@ -247,9 +247,8 @@ __global__ __launch_bounds__(kProductThreads, 1) void gemm_bf16x3_kernel(Params 
     for (int j = 0; j < 4; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
 
   // LDS reads in the order the products consume them: hi of B column 0 and A,
-  // lo of B column 0 and A, then B one column ahead; the scheduling groups
-  // start the first products after a few reads and keep every read ahead of
-  // its first use.
+  // lo of B column 0 and A, then the other B columns.  With two waves per SIMD
+  // the compiler's own schedule beats pinned scheduling groups (measured).
   auto compute = [&](int buf) {
     const u4* As = &lds[buf][0];
     const u4* Bs = &lds[buf][2048];
@@ -280,27 +279,6 @@ __global__ __launch_bounds__(kProductThreads, 1) void gemm_bf16x3_kernel(Params 
 #pragma unroll
       for (int i = 0; i < 8; ++i) acc[i][j] = mfma(al[i], bh[j], acc[i][j]);
     }
-    // reads: bh0 ah0..7 | bl0 al0..7 | bh1 bl1 bh2 bl2 bh3 bl3
-    __builtin_amdgcn_sched_group_barrier(0x100, 3, 0);   // bh0 ah0 ah1
-#pragma unroll
-    for (int i = 0; i < 6; ++i) {                          // hi x hi of column 0 under ah2..ah7
-      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-      __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
-    }
-    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-    __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);     // bl0
-    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {                          // hi x lo of column 0 under al0..al7
-      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-      __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
-    }
-    __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);     // column 1
-    __builtin_amdgcn_sched_group_barrier(0x008, 8, 0);     // lo x hi of column 0
-    __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);     // column 2
-    __builtin_amdgcn_sched_group_barrier(0x008, 24, 0);    // column 1
-    __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);     // column 3
-    __builtin_amdgcn_sched_group_barrier(0x008, 48, 0);    // columns 2 and 3
   };
 
   // Step s computes from buffer s % 2 while the slabs of step s + 1 land in the
