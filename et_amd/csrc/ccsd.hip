@@ -606,11 +606,36 @@ bool CCSD::reserve_ladder_packs() {
   return moved;
 }
 
+// G of ladder block b: G(c + v aa, d + v bb) = sum_J L_t1(J, c, a0 + aa)
+// L_t1(J, d, b0 + bb), in FP32 when `fp32` (the split-bf16 path).  The pairs
+// of a diagonal block (a0 == b0) have aa >= bb, so it is formed in column strips
+// of 8 b's, each from the row block of its first b on (a little over half of the
+// full block's work).  `tag` names the full block's A operand for the pack cache
+// (0: no tag).
+void CCSD::ladder_G(const Block& b, const double* LT, Prec prec, bool fp32, uint64_t tag) {
+  const int64_t v = n_v, J = n_J, ldg = v * b.la;
+  auto product = [&](int64_t r0, int64_t c0, int64_t nc, uint64_t t) {
+    Gemm gg = make_gemm('T', 'N', v * (b.la - r0), v * nc, J, 1.0, LT + J * v * (b.a0 + r0), J,
+                        LT + J * v * (b.b0 + c0), J, 0.0, gblk_.as<double>() + v * r0 + ldg * v * c0, ldg);
+    if (fp32) {
+      gg.C = nullptr;
+      gg.Cf = gblk_.as<float>() + v * r0 + ldg * v * c0;
+    }
+    run_gemm(tagged(gg, t, t != 0 && pack_cache_), prec);
+  };
+  if (b.a0 != b.b0) {
+    product(0, 0, b.lb, tag);
+    return;
+  }
+  constexpr int64_t kStrip = 8;
+  for (int64_t c0 = 0; c0 < b.lb; c0 += kStrip) product(c0, c0, std::min(kStrip, b.lb - c0), 0);
+}
+
 // Per block: G, then g+- and the products w+- = 1/2 t+- g+-.  The split-bf16
 // form writes g+- straight as packs in the tile-blocked (c, d) order, with t+-
 // in the same order; the FP64 and FP32 forms use FP64 g+- in the triangle order.
 void CCSD::ladder_primal(const double* t2, const double* LT, double* om2, Prec prec) {
-  const int64_t o = n_o, v = n_v, J = n_J;
+  const int64_t o = n_o, v = n_v;
   const int64_t op = o * (o + 1) / 2, om = o * (o - 1) / 2;
   const bool split = prec == Prec::BF16X3;
   const int64_t kp = split ? ladder_blocked_len(v, true) : v * (v + 1) / 2;
@@ -624,13 +649,7 @@ void CCSD::ladder_primal(const double* t2, const double* LT, double* om2, Prec p
   else
     ladder_t_pm(s, t2, tp_.as<double>(), tm_.as<double>(), o, v);
   for (const Block& b : blocks_) {
-    Gemm gg = make_gemm('T', 'N', v * b.la, v * b.lb, J, 1.0, LT + J * v * b.a0, J, LT + J * v * b.b0, J, 0.0,
-                        gblk_.as<double>(), v * b.la);
-    if (split) {                        // G in FP32
-      gg.C = nullptr;
-      gg.Cf = gblk_.as<float>();
-    }
-    run_gemm(tagged(gg, gen | 7, pack_cache_), prec);
+    ladder_G(b, LT, prec, split, gen | 7);
     Gemm gpl = make_gemm('N', 'N', op, b.npairs, kp, 0.5, tp_.as<double>(), op, gp_.as<double>(), kp, 0.0,
                          wp_.as<double>() + op * b.col0, op);
     Gemm gmi = make_gemm('N', 'N', om, b.npairs, km, 0.5, tm_.as<double>(), om, gm_.as<double>(), km, 0.0,
@@ -719,7 +738,7 @@ void CCSD::ladder_tangent(const double* t2, const double* dt2, int64_t s_dt2, co
   if (dt2 == nullptr || dom2 == nullptr) fail("ladder tangent: missing derivative");
   if (t2 != t_.as<double>() + n_t1 || LT != Lt_vvT_.as<double>() || !have_lw_)
     fail("ladder tangent: the record's ladder operands are required");
-  const int64_t o = n_o, v = n_v, J = n_J, N = n_t1;
+  const int64_t o = n_o, v = n_v, N = n_t1;
   const int64_t op = o * (o + 1) / 2, om = o * (o - 1) / 2;
   const bool split = prec == Prec::BF16X3;
   const int64_t kp = split ? ladder_blocked_len(v, true) : v * (v + 1) / 2;
@@ -747,14 +766,7 @@ void CCSD::ladder_tangent(const double* t2, const double* dt2, int64_t s_dt2, co
       ladder_t_pm(s, dt2 + k * s_dt2, dtp + k * lsz_tp_, dtm + k * lsz_tm_, o, v);
   }
   for (const Block& b : blocks_) {
-    const double* LA = LT + J * v * b.a0;
-    const double* LB = LT + J * v * b.b0;
-    Gemm gg = make_gemm('T', 'N', v * b.la, v * b.lb, J, 1.0, LA, J, LB, J, 0.0, gblk_.as<double>(), v * b.la);
-    if (split) {                        // G in FP32
-      gg.C = nullptr;
-      gg.Cf = gblk_.as<float>();
-    }
-    run_gemm(gg, prec);
+    ladder_G(b, LT, prec, split, 0);
     Gemm gpl = make_gemm('N', 'N', op, b.npairs, kp, 0.5, dtp, op, gp_.as<double>(), kp, 0.0,
                          dwp + op * b.col0, op);
     Gemm gmi = make_gemm('N', 'N', om, b.npairs, km, 0.5, dtm, om, gm_.as<double>(), km, 0.0,
@@ -797,7 +809,7 @@ void CCSD::ladder_adjoint(const double* t2, double* t2b, int64_t s_t2b, const do
   if (t2b == nullptr || om2b == nullptr) fail("ladder adjoint: missing derivative");
   if (t2 != t_.as<double>() + n_t1 || LT != Lt_vvT_.as<double>() || !have_lw_)
     fail("ladder adjoint: the record's ladder operands are required");
-  const int64_t o = n_o, v = n_v, J = n_J, N = n_t1;
+  const int64_t o = n_o, v = n_v, N = n_t1;
   const int64_t op = o * (o + 1) / 2, om = o * (o - 1) / 2;
   const int64_t vp = v * (v + 1) / 2, vm = v * (v - 1) / 2;
   hipStream_t s = stream_;
@@ -821,9 +833,7 @@ void CCSD::ladder_adjoint(const double* t2, double* t2b, int64_t s_t2b, const do
   fill(s, tpb, nb * lsz_tp_, 0.0);
   if (om > 0 && vm > 0) fill(s, tmb, nb * lsz_tm_, 0.0);
   for (const Block& b : blocks_) {
-    const double* LA = LT + J * v * b.a0;
-    const double* LB = LT + J * v * b.b0;
-    X.gemm('T', 'N', v * b.la, v * b.lb, J, 1.0, LA, J, LB, J, 0.0, gblk_.as<double>(), v * b.la, prec);
+    ladder_G(b, LT, prec, false, 0);
     ladder_g_pm(s, gblk_.as<double>(), v, b.la, b.lb, pair_a_.as<int32_t>() + b.col0,
                 pair_b_.as<int32_t>() + b.col0, b.npairs, gp_.as<double>(), gm_.as<double>());
     // w+ = 1/2 t+ g+  ->  t+_bar += 1/2 w+_bar g+^T

@@ -234,6 +234,7 @@ class CCSD {
   int64_t ladder_vector_elems() const;
   bool reserve_ladder_vectors(int64_t nb);
   bool reserve_ladder_packs();
+  void ladder_G(const Block& b, const double* LT, Prec prec, bool fp32, uint64_t tag);
   // W_kbij = sum_cd (sum_J L_kc L_t1,bd) t_cidj of the current record (T1 part of
   // the ladder derivative)
   DeviceBuffer lw_;

@@ -27,7 +27,6 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 typedef unsigned int u4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(3))) void lds_void;
 
-constexpr int kThreads = 256;
 constexpr int kTile = 256;      // output tile (rows and columns)
 constexpr int kDepth = 32;      // k per step (one MFMA step)
 constexpr int kXcds = 8;
