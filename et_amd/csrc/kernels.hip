@@ -30,17 +30,18 @@ __device__ inline int64_t pidx(int64_t x, int64_t y) {
   return hi * (hi + 1) / 2 + lo;
 }
 
-__global__ void permute4_kernel(const double* __restrict__ x, double* __restrict__ y, int64_t e0,
-                                int64_t e1, int64_t e2, int64_t e3, int64_t s0, int64_t s1,
-                                int64_t s2, int64_t s3, double alpha, double beta) {
-  const int64_t n = e0 * e1 * e2 * e3;
-  for (int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; idx < n;
-       idx += int64_t(gridDim.x) * blockDim.x) {
-    int64_t r = idx;
-    const int64_t o0 = r % e0; r /= e0;
-    const int64_t o1 = r % e1; r /= e1;
-    const int64_t o2 = r % e2;
-    const int64_t o3 = r / e2;
+// I: index type (32-bit when the array has fewer than 2^32 elements: the
+// element-wise divisions then cost a fraction of the 64-bit ones)
+template <typename I>
+__global__ void permute4_kernel(const double* __restrict__ x, double* __restrict__ y, I e0, I e1, I e2,
+                                I e3, I s0, I s1, I s2, I s3, double alpha, double beta) {
+  const I n = e0 * e1 * e2 * e3;
+  for (I idx = I(blockIdx.x) * blockDim.x + threadIdx.x; idx < n; idx += I(gridDim.x) * blockDim.x) {
+    I r = idx;
+    const I o0 = r % e0; r /= e0;
+    const I o1 = r % e1; r /= e1;
+    const I o2 = r % e2;
+    const I o3 = r / e2;
     const double v = alpha * x[o0 * s0 + o1 * s1 + o2 * s2 + o3 * s3];
     y[idx] = (beta == 0.0) ? v : v + beta * y[idx];
   }
@@ -380,8 +381,14 @@ void permute4(hipStream_t s, const double* x, double* y, const int64_t e[4], con
     ETG_LAUNCH("permute4_tiled");
     return;
   }
-  permute4_kernel<<<grid_for(n), kBlock, 0, s>>>(x, y, e[0], e[1], e[2], e[3], st[0], st[1], st[2],
-                                                 st[3], alpha, beta);
+  // (32-bit indices: the grid-stride step stays below 2^32 - n as well)
+  if (n + int64_t(kBlock) * grid_for(n) <= int64_t(UINT32_MAX))
+    permute4_kernel<uint32_t><<<grid_for(n), kBlock, 0, s>>>(
+        x, y, uint32_t(e[0]), uint32_t(e[1]), uint32_t(e[2]), uint32_t(e[3]), uint32_t(st[0]), uint32_t(st[1]),
+        uint32_t(st[2]), uint32_t(st[3]), alpha, beta);
+  else
+    permute4_kernel<int64_t><<<grid_for(n), kBlock, 0, s>>>(x, y, e[0], e[1], e[2], e[3], st[0], st[1], st[2],
+                                                            st[3], alpha, beta);
   ETG_LAUNCH("permute4");
 }
 
