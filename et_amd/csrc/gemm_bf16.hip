@@ -92,13 +92,12 @@ __global__ __launch_bounds__(256) void pack_split_bf16_kernel(const double* __re
 // g+_cd = M(c, d) + M(d, c) and g-_cd = M(c, d) - M(d, c); the packs are the B
 // operands (rows = pairs, k = the tile-blocked (c, d) index of
 // ladder_order.hpp) of the products w+- = t+- g+-.  One workgroup per (16 x 16
-// tile pair ti <= tj of the (c, d) plane, 16 pairs): it stages the mirrored
-// tile of its 16 pairs in LDS (reads along the contiguous index), reads the
-// tile itself straight from G, then writes whole 2 KB chunks of both packs: a
-// tile is a whole number of 32-deep steps, and the 16 pairs are one 16-row
-// block.  G comes from a split-bf16
-// product in FP32 (its accumulation type, so nothing is lost); the splits are
-// those of the pack kernel applied to the FP64 sums.
+// tile pair ti <= tj of the (c, d) plane, 16 pairs): it stages the tile and
+// its mirror for its 16 pairs in LDS (both read along the contiguous index),
+// then writes whole 2 KB chunks of both packs: a tile is a whole number of
+// 32-deep steps, and the 16 pairs are one 16-row block.  G comes from a
+// split-bf16 product in FP32 (its accumulation type, so nothing is lost); the
+// splits are those of the pack kernel applied to the FP64 sums.
 // ---------------------------------------------------------------------------
 constexpr int kPairTile = 16 * 17 + 1;   // floats per staged pair tile (odd strides)
 
@@ -106,8 +105,10 @@ __global__ __launch_bounds__(256) void ladder_g_pm_split_kernel(
     const float* __restrict__ G, int64_t n_v, int64_t ldG, const int32_t* __restrict__ pa,
     const int32_t* __restrict__ pb, int64_t npairs, int64_t KSP, int64_t KSM, int64_t row_blocks,
     u4* __restrict__ Pp, u4* __restrict__ Pm) {
-  // T[p][x][y] = M_k(d0 + y, c0 + x): the mirrored tile (for a diagonal tile, the tile itself)
+  // T[p][x][y] = M_k(d0 + y, c0 + x): the mirrored tile (for a diagonal tile, the tile itself);
+  // D[p][x][y] = M_k(c0 + y, d0 + x): the tile (off-diagonal tiles only)
   __shared__ float T[16 * kPairTile];
+  __shared__ float D[16 * kPairTile];
   const int t = threadIdx.x;
   const int tj = small_tri_root(int(blockIdx.x)), ti = int(blockIdx.x) - tj * (tj + 1) / 2;
   const int64_t c0 = int64_t(kLadderTile) * ti, d0 = int64_t(kLadderTile) * tj;
@@ -125,23 +126,25 @@ __global__ __launch_bounds__(256) void ladder_g_pm_split_kernel(
 #pragma unroll 4
       for (int p = 0; p < 16; ++p) {
         const int64_t k = rb * 16 + p;
-        float val = 0.0f;
-        if (k < npairs && d0 + y < n_v && c0 + x < n_v)
-          val = G[n_v * pa[k] + ldG * n_v * pb[k] + (d0 + y) + ldG * (c0 + x)];
+        float val = 0.0f, vd = 0.0f;
+        if (k < npairs) {
+          const float* M = G + n_v * pa[k] + ldG * n_v * pb[k];   // M(c, d) = M[c + ldG d]
+          if (d0 + y < n_v && c0 + x < n_v) val = M[(d0 + y) + ldG * (c0 + x)];
+          if (!diag && c0 + y < n_v && d0 + x < n_v) vd = M[(c0 + y) + ldG * (d0 + x)];
+        }
         T[p * kPairTile + x * 17 + y] = val;
+        if (!diag) D[p * kPairTile + x * 17 + y] = vd;
       }
     }
     __syncthreads();
     // item: (sign, step ks of the tile, lane slot ls = p + 16 q): 8 consecutive
-    // blocked indices 32 ks + 8 q + e of pair p, one 16 B lane of the chunk;
-    // M(c, d) of an off-diagonal tile is read from G directly (8 consecutive c)
+    // blocked indices 32 ks + 8 q + e of pair p, one 16 B lane of the chunk
+    const float* Dx = diag ? T : D;      // x1 = M(c, d) = Dx[p][d'][c']
     for (int i = t; i < (nsp + nsm) * 64; i += 256) {
       const bool plus = i < nsp * 64;
       const int j = plus ? i : i - nsp * 64;
       const int ks = j >> 6, ls = j & 63, p = ls & 15, q = ls >> 4;
-      const int64_t k = rb * 16 + p;
-      const bool row_ok = k < npairs;
-      const float* M = row_ok ? G + n_v * pa[k] + ldG * n_v * pb[k] : G;   // M(c, d) = M[c + ldG d]
+      const bool row_ok = rb * 16 + p < npairs;
       bf8 h, l;
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
@@ -149,7 +152,7 @@ __global__ __launch_bounds__(256) void ladder_g_pm_split_kernel(
         const bool ok = ladder_tile_pair(32 * ks + 8 * q + e, diag, w, plus, cl, dl);
         double g = 0.0;
         if (ok && row_ok && c0 + cl < n_v && d0 + dl < n_v) {
-          const double x1 = diag ? T[p * kPairTile + dl * 17 + cl] : M[(c0 + cl) + ldG * (d0 + dl)];
+          const double x1 = Dx[p * kPairTile + dl * 17 + cl];  // M(c, d)
           const double x2 = T[p * kPairTile + cl * 17 + dl];   // M(d, c)
           g = plus ? x1 + x2 : x1 - x2;
         }
