@@ -89,6 +89,28 @@ __global__ void squareup_kernel(const double* __restrict__ packed, double* __res
   }
 }
 
+// The same through 32 x 32 LDS tiles: one workgroup per tile pair (Y block <=
+// X block) reads packed elements (X, Y), Y <= X, along Y and writes full(Y, X)
+// along Y and its mirror full(X, Y) along X.
+__global__ __launch_bounds__(256) void squareup_tiled_kernel(const double* __restrict__ packed,
+                                                             double* __restrict__ full, int64_t N) {
+  __shared__ double P[32][33];                          // P[xl][yl] = packed(X0 + xl, Y0 + yl)
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int64_t xb = tri_root(blockIdx.x), yb = int64_t(blockIdx.x) - xb * (xb + 1) / 2;
+  const int64_t X0 = xb * 32, Y0 = yb * 32;
+  for (int k = ty; k < 32; k += 8) {
+    const int64_t X = X0 + k, Y = Y0 + tx;
+    P[k][tx] = (X < N && Y <= X) ? packed[X * (X + 1) / 2 + Y] : 0.0;
+  }
+  __syncthreads();
+  for (int k = ty; k < 32; k += 8) {
+    const int64_t X = X0 + k, Y = Y0 + tx;              // full(Y, X), along Y
+    if (X < N && Y <= X) full[Y + N * X] = P[k][tx];
+    const int64_t Xr = X0 + tx, Yc = Y0 + k;            // full(Xr, Yc), along Xr
+    if (Xr < N && Yc <= Xr) full[Xr + N * Yc] = P[tx][k];
+  }
+}
+
 // packed element k = (X, Y), X >= Y, reads full(Y, X) (column X is contiguous in Y)
 __global__ void packin_kernel(const double* __restrict__ full, double* __restrict__ packed,
                               int64_t N, double alpha, double beta, double diag_scale) {
@@ -117,6 +139,53 @@ __global__ void symmetric_sum_kernel(const double* __restrict__ x, double* __res
   for (int k = ty; k < 32; k += 8) {
     const int64_t r = bx * 32 + tx, c = by * 32 + k;
     if (r < N && c < N) y[r + N * c] = x[r + N * c] + tile[tx][k];
+  }
+}
+
+// The same through 32 x 32 LDS tiles: one workgroup per tile pair (Y block <=
+// X block) stages x(Y, X) (read along Y), x(X, Y) (along X) and, when f2 != 0,
+// the two exchange terms x(p + v s, r + v q) (along p) and x(r + v q, p + v s)
+// (along r) for Y = p + v q, X = r + v s, then adds to packed element
+// X (X + 1) / 2 + Y, contiguous in Y for each X.
+__global__ __launch_bounds__(256) void symmetrize_and_pack_tiled_kernel(const double* __restrict__ x,
+                                                                        double* __restrict__ y, int64_t n_v,
+                                                                        int64_t N, double alpha, double f1,
+                                                                        double f2) {
+  __shared__ double A[32][33], B[32][33];              // A[xl][yl] = x(Y, X), B[yl][xl] = x(X, Y)
+  __shared__ double C[32][33], D[32][33];              // C[xl][yl] = x(p + v s, r + v q), D[yl][xl] = mirror
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int64_t xb = tri_root(blockIdx.x), yb = int64_t(blockIdx.x) - xb * (xb + 1) / 2;
+  const int64_t X0 = xb * 32, Y0 = yb * 32;
+  const bool exch = f2 != 0.0;
+  for (int k = ty; k < 32; k += 8) {
+    const int64_t Yr = Y0 + tx, Xc = X0 + k;            // A[k][tx] = x(Y0 + tx, X0 + k)
+    const bool okA = Yr < N && Xc < N;
+    A[k][tx] = okA ? x[Yr + N * Xc] : 0.0;
+    const int64_t Xr = X0 + tx, Yc = Y0 + k;            // B[k][tx] = x(X0 + tx, Y0 + k)
+    const bool okB = Xr < N && Yc < N;
+    B[k][tx] = okB ? x[Xr + N * Yc] : 0.0;
+    if (exch) {
+      // C[k][tx]: Y = Yr = p + v q, X = Xc = r + v s  ->  x(p + v s, r + v q)
+      const int64_t p = Yr % n_v, q = Yr / n_v, r = Xc % n_v, sx = Xc / n_v;
+      C[k][tx] = okA ? x[(p + n_v * sx) + N * (r + n_v * q)] : 0.0;
+      // D[k][tx]: X = Xr = r' + v s', Y = Yc = p' + v q'  ->  x(r' + v q', p' + v s')
+      const int64_t p2 = Yc % n_v, q2 = Yc / n_v, r2 = Xr % n_v, s2 = Xr / n_v;
+      D[k][tx] = okB ? x[(r2 + n_v * q2) + N * (p2 + n_v * s2)] : 0.0;
+    }
+  }
+  __syncthreads();
+  for (int k = ty; k < 32; k += 8) {
+    const int64_t X = X0 + k, Y = Y0 + tx;
+    if (X >= N || Y > X) continue;
+    double v;
+    if (X == Y) {
+      v = alpha * A[k][tx];
+    } else {
+      v = f1 * (A[k][tx] + B[tx][k]);
+      if (exch) v += f2 * (C[k][tx] + D[tx][k]);
+      v *= alpha;
+    }
+    y[X * (X + 1) / 2 + Y] += v;
   }
 }
 
@@ -409,6 +478,12 @@ void sort4(hipStream_t s, const double* x, const int64_t d[4], const char* perm,
 }
 
 void squareup(hipStream_t s, const double* packed, double* full, int64_t N) {
+  if (N >= 32) {
+    const int64_t nb = ceil_div(N, 32);
+    squareup_tiled_kernel<<<unsigned(nb * (nb + 1) / 2), kBlock, 0, s>>>(packed, full, N);
+    ETG_LAUNCH("squareup_tiled");
+    return;
+  }
   squareup_kernel<<<grid_for(N * N), kBlock, 0, s>>>(packed, full, N);
   ETG_LAUNCH("squareup");
 }
@@ -429,6 +504,13 @@ void symmetric_sum(hipStream_t s, const double* x, double* y, int64_t N) {
 void symmetrize_and_pack(hipStream_t s, const double* x, double* packed, int64_t n_v, int64_t n_o,
                          double alpha, double f1, double f2) {
   const int64_t N = n_v * n_o;
+  if (N >= 32) {
+    const int64_t nb = ceil_div(N, 32);
+    symmetrize_and_pack_tiled_kernel<<<unsigned(nb * (nb + 1) / 2), kBlock, 0, s>>>(x, packed, n_v, N, alpha,
+                                                                                   f1, f2);
+    ETG_LAUNCH("symmetrize_and_pack_tiled");
+    return;
+  }
   symmetrize_and_pack_kernel<<<grid_for(N * (N + 1) / 2), kBlock, 0, s>>>(x, packed, n_v, n_o,
                                                                           alpha, f1, f2);
   ETG_LAUNCH("symmetrize_and_pack");
