@@ -693,6 +693,29 @@ __global__ void sympack_adjoint_kernel(const double* __restrict__ p, double* __r
   }
 }
 
+// The f2 = 0 form through 32 x 32 LDS tiles (as squareup: one workgroup per
+// tile pair reads packed (X, Y), Y <= X, along Y and updates x(Y, X) along Y
+// and x(X, Y) along X).
+__global__ __launch_bounds__(256) void sympack_adjoint_tiled_kernel(const double* __restrict__ p,
+                                                                    double* __restrict__ x, int64_t N,
+                                                                    double alpha, double f1) {
+  __shared__ double P[32][33];                          // P[xl][yl] = p(X0 + xl, Y0 + yl)
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int64_t xb = tri_root2(blockIdx.x), yb = int64_t(blockIdx.x) - xb * (xb + 1) / 2;
+  const int64_t X0 = xb * 32, Y0 = yb * 32;
+  for (int k = ty; k < 32; k += 8) {
+    const int64_t X = X0 + k, Y = Y0 + tx;
+    P[k][tx] = (X < N && Y <= X) ? p[X * (X + 1) / 2 + Y] * (X == Y ? alpha : alpha * f1) : 0.0;
+  }
+  __syncthreads();
+  for (int k = ty; k < 32; k += 8) {
+    const int64_t X = X0 + k, Y = Y0 + tx;              // x(Y, X), along Y
+    if (X < N && Y <= X) x[Y + N * X] += P[k][tx];
+    const int64_t Xr = X0 + tx, Yc = Y0 + k;            // x(Xr, Yc), along Xr (the diagonal once)
+    if (Xr < N && Yc < Xr) x[Xr + N * Yc] += P[tx][k];
+  }
+}
+
 // t2[{(c,i),(d,j)}] += coef * tp[ij, cd] + sgn * tm[ij, cd]
 __global__ void ladder_t_pm_adjoint_kernel(const double* __restrict__ tp, const double* __restrict__ tm,
                                            double* __restrict__ t2, int64_t n_o, int64_t n_v) {
@@ -874,6 +897,12 @@ void unpack_upper_add(hipStream_t s, const double* packed, double* full, int64_t
 void symmetrize_and_pack_adjoint(hipStream_t s, const double* packed, double* x, int64_t n_v,
                                  int64_t n_o, double alpha, double f1, double f2) {
   const int64_t N = n_v * n_o;
+  if (f2 == 0.0 && N >= 32) {
+    const int64_t nb = ceil_div(N, 32);
+    sympack_adjoint_tiled_kernel<<<unsigned(nb * (nb + 1) / 2), kBlock, 0, s>>>(packed, x, N, alpha, f1);
+    ETG_LAUNCH("sympack_adjoint_tiled");
+    return;
+  }
   sympack_adjoint_kernel<<<g2(N * N), kBlock, 0, s>>>(packed, x, n_v, n_o, alpha, f1, f2);
   ETG_LAUNCH("sympack_adjoint");
 }
