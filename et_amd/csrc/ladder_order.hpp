@@ -17,41 +17,48 @@
 // so the padding adds nothing to the products.
 #pragma once
 
-#include <hip/hip_runtime.h>
-
+#include <cmath>
 #include <cstdint>
+
+#ifndef ETG_HD
+#if defined(__HIPCC__)
+#define ETG_HD __host__ __device__
+#else
+#define ETG_HD
+#endif
+#endif
 
 namespace etg {
 
 constexpr int kLadderTile = 16;
 
-__host__ __device__ inline int64_t ladder_pad32(int64_t x) { return (x + 31) / 32 * 32; }
+ETG_HD inline int64_t ladder_pad32(int64_t x) { return (x + 31) / 32 * 32; }
 
-__host__ __device__ inline int64_t ladder_tiles(int64_t n_v) { return (n_v + kLadderTile - 1) / kLadderTile; }
+ETG_HD inline int64_t ladder_tiles(int64_t n_v) { return (n_v + kLadderTile - 1) / kLadderTile; }
 
 // d-width of tile column tj (even)
-__host__ __device__ inline int64_t ladder_col_width(int64_t tj, int64_t n_v) {
+ETG_HD inline int64_t ladder_col_width(int64_t tj, int64_t n_v) {
   const int64_t w = n_v - kLadderTile * tj;
   return w >= kLadderTile ? kLadderTile : w + (w & 1);
 }
 
 // entries of a diagonal tile of width w (padded)
-__host__ __device__ inline int64_t ladder_diag_size(int64_t w, bool plus) {
+ETG_HD inline int64_t ladder_diag_size(int64_t w, bool plus) {
   return ladder_pad32(plus ? w * (w + 1) / 2 : w * (w - 1) / 2);
 }
 
 // first index of tile column tj (all of its tiles full width below the last column)
-__host__ __device__ inline int64_t ladder_col_offset(int64_t tj, bool plus) {
+ETG_HD inline int64_t ladder_col_offset(int64_t tj, bool plus) {
   return 128 * tj * (tj - 1) + ladder_diag_size(kLadderTile, plus) * tj;
 }
 
 // first index of tile (ti, tj), ti <= tj
-__host__ __device__ inline int64_t ladder_tile_offset(int64_t ti, int64_t tj, int64_t n_v, bool plus) {
+ETG_HD inline int64_t ladder_tile_offset(int64_t ti, int64_t tj, int64_t n_v, bool plus) {
   return ladder_col_offset(tj, plus) + kLadderTile * ladder_col_width(tj, n_v) * ti;
 }
 
 // length of the blocked index (a multiple of 32)
-__host__ __device__ inline int64_t ladder_blocked_len(int64_t n_v, bool plus) {
+ETG_HD inline int64_t ladder_blocked_len(int64_t n_v, bool plus) {
   const int64_t L = ladder_tiles(n_v) - 1;
   if (L < 0) return 0;
   const int64_t w = ladder_col_width(L, n_v);
@@ -59,7 +66,7 @@ __host__ __device__ inline int64_t ladder_blocked_len(int64_t n_v, bool plus) {
 }
 
 // largest r with r (r + 1) / 2 <= x, for small x >= 0
-__host__ __device__ inline int small_tri_root(int x) {
+ETG_HD inline int small_tri_root(int x) {
   int r = int((sqrt(8.0 * double(x) + 1.0) - 1.0) * 0.5);
   while (r * (r + 1) / 2 > x) --r;
   while ((r + 1) * (r + 2) / 2 <= x) ++r;
@@ -68,7 +75,7 @@ __host__ __device__ inline int small_tri_root(int x) {
 
 // position loc inside a tile -> (c', d'); false for a padding position of a
 // diagonal tile of width w
-__host__ __device__ inline bool ladder_tile_pair(int loc, bool diag, int w, bool plus, int& cl, int& dl) {
+ETG_HD inline bool ladder_tile_pair(int loc, bool diag, int w, bool plus, int& cl, int& dl) {
   if (!diag) {
     cl = loc % kLadderTile;
     dl = loc / kLadderTile;
@@ -85,7 +92,7 @@ __host__ __device__ inline bool ladder_tile_pair(int loc, bool diag, int w, bool
 }
 
 // blocked index kk -> (c, d); false for a padding position
-__host__ __device__ inline bool ladder_pair_of(int64_t kk, int64_t n_v, bool plus, int64_t& c, int64_t& d) {
+ETG_HD inline bool ladder_pair_of(int64_t kk, int64_t n_v, bool plus, int64_t& c, int64_t& d) {
   const int64_t L = ladder_tiles(n_v) - 1;
   int64_t tj;
   if (kk >= ladder_col_offset(L, plus)) {
