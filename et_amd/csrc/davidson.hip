@@ -98,7 +98,17 @@ __device__ void inverse_iteration(double* M, int ldm, int nm, double anorm, int*
     for (int r = c + 1 + lane; r < nm; r += kLanes) {
       const double f = MM(r, c) / d;
       MM(r, c) = f;
-      for (int j = c + 1; j < nm; ++j) MM(r, j) -= f * MM(c, j);
+      // four columns per round: their loads go out before the stores
+      int j = c + 1;
+      for (; j + 4 <= nm; j += 4) {
+        const double p0 = MM(c, j), p1 = MM(c, j + 1), p2 = MM(c, j + 2), p3 = MM(c, j + 3);
+        const double a0 = MM(r, j), a1 = MM(r, j + 1), a2 = MM(r, j + 2), a3 = MM(r, j + 3);
+        MM(r, j) = a0 - f * p0;
+        MM(r, j + 1) = a1 - f * p1;
+        MM(r, j + 2) = a2 - f * p2;
+        MM(r, j + 3) = a3 - f * p3;
+      }
+      for (; j < nm; ++j) MM(r, j) -= f * MM(c, j);
     }
     __syncthreads();
   }
