@@ -235,9 +235,22 @@ __global__ __launch_bounds__(kLanes) void reduced_eigen_kernel(const double* __r
     const double lam = wr[order[w]], mu = wi[order[w]];
     if (mu == 0.0) {
       const double shift = lam + 1e-13 * fmax(anorm / k, fabs(lam));
-      for (int q = lane; q < k * k; q += kLanes) {
-        const int i = q % k, j = q / k;
-        HH(i, j) = G[i + j * int64_t(ldg)] - (i == j ? shift : 0.0);
+      // eight elements per lane and round: their global loads are in flight together
+      for (int q0 = 0; q0 < k * k; q0 += 8 * kLanes) {
+        double g[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int q = q0 + u * kLanes + lane;
+          g[u] = q < k * k ? G[q % k + (q / k) * int64_t(ldg)] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int q = q0 + u * kLanes + lane;
+          if (q < k * k) {
+            const int i = q % k, j = q / k;
+            HH(i, j) = g[u] - (i == j ? shift : 0.0);
+          }
+        }
       }
       __syncthreads();
       inverse_iteration(H, kLdH, k, anorm, piv, y, w, lane);
