@@ -20,6 +20,7 @@ LIB_PATH = os.path.join(_HERE, 'libetgpu.so')
 FP64 = 0
 FP32 = 1
 BF16X3 = 2
+BF16X3_ANY_SHAPE = 3    # etg_test_gemm / etg_bench_gemm: the split-bf16 kernel for every shape
 PRECISIONS = {'fp64': FP64, 'fp32': FP32, 'bf16x3': BF16X3}
 
 

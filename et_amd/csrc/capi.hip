@@ -402,8 +402,10 @@ int etg_test_gemm(char ta, char tb, int64_t m, int64_t n, int64_t k, double alph
     ETG_HIP(hipMemcpy(d.b.as<double>(), B, sizeof(double) * ldb * colsB, hipMemcpyHostToDevice));
     ETG_HIP(hipMemcpy(d.c.as<double>(), C, sizeof(double) * ldc * n, hipMemcpyHostToDevice));
     etg::GemmWorkspace ws;
-    etg::gemm(nullptr, ta, tb, m, n, k, alpha, d.a.as<double>(), lda, d.b.as<double>(), ldb, beta,
-              d.c.as<double>(), ldc, prec_of(precision), &ws);
+    etg::Gemm g = etg::make_gemm(ta, tb, m, n, k, alpha, d.a.as<double>(), lda, d.b.as<double>(), ldb, beta,
+                                 d.c.as<double>(), ldc);
+    g.split_any_shape = precision == ETG_BF16X3_ANY_SHAPE;
+    etg::gemm(nullptr, g, g.split_any_shape ? etg::Prec::BF16X3 : prec_of(precision), &ws);
     ETG_HIP(hipDeviceSynchronize());
     ETG_HIP(hipMemcpy(C, d.c.as<double>(), sizeof(double) * ldc * n, hipMemcpyDeviceToHost));
   });
@@ -449,10 +451,11 @@ int etg_bench_gemm(char ta, char tb, int64_t m, int64_t n, int64_t k, int precis
     ETG_HIP(hipMemset(d.a.as<double>(), 0, sizeof(double) * m * k));
     ETG_HIP(hipMemset(d.b.as<double>(), 0, sizeof(double) * k * n));
     etg::GemmWorkspace ws;
-    auto run = [&] {
-      etg::gemm(nullptr, ta, tb, m, n, k, 1.0, d.a.as<double>(), lda, d.b.as<double>(), ldb, 0.0,
-                d.c.as<double>(), m, prec_of(precision), &ws);
-    };
+    etg::Gemm g = etg::make_gemm(ta, tb, m, n, k, 1.0, d.a.as<double>(), lda, d.b.as<double>(), ldb, 0.0,
+                                 d.c.as<double>(), m);
+    g.split_any_shape = precision == ETG_BF16X3_ANY_SHAPE;
+    const etg::Prec prec = g.split_any_shape ? etg::Prec::BF16X3 : prec_of(precision);
+    auto run = [&] { etg::gemm(nullptr, g, prec, &ws); };
     run();
     ETG_HIP(hipDeviceSynchronize());
     hipEvent_t e0, e1;

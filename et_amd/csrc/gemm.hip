@@ -785,7 +785,7 @@ GemmLaunch gemm(hipStream_t stream, const Gemm& g, Prec prec, GemmWorkspace* ws)
   // per-thread byte offsets inside a tile span 16 operand columns and are 31-bit:
   // longer leading dimensions go to the tall FP64 kernels (any precision asked)
   if (std::max(g.lda, g.ldb) >= (int64_t(1) << 24)) return gemm_tall(stream, g, ws);
-  if (prec == Prec::BF16X3) return gemm_bf16x3(stream, g, ws);
+  if (prec == Prec::BF16X3 && split_bf16_pays(g)) return gemm_bf16x3(stream, g, ws);
   if (prec == Prec::F32) return launch_typed<float>(stream, g, ws);
   return launch_typed<double>(stream, g, ws);
 }

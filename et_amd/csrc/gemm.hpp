@@ -39,7 +39,25 @@ struct Gemm {
   // Split-bf16 products only: C written in FP32 (the products' accumulation
   // type) to Cf instead of C; beta must be 0.
   float* Cf = nullptr;
+  // Split-bf16 products only: run the split-bf16 kernel whatever the shape
+  // (diagnostics; see split_bf16_pays).
+  bool split_any_shape = false;
 };
+
+// Whether a product asked for in split-bf16 runs on the split-bf16 kernel.  The
+// kernel first packs both operands (one pass over each).  A product with few
+// outputs per operand element, m n < 64 (m + n), is bound by HBM even on the
+// FP64 matrix cores (at most ~32 FLOP per operand byte); when its operands also
+// hold 2^24 elements or more (128 MB in FP64), where the pack passes cost more
+// than the whole FP64 product, it runs in FP64 instead: faster, and exact.
+// Smaller products keep the precision asked for.  Products with a prepacked B
+// or an FP32 output always take the split-bf16 kernel.
+inline bool split_bf16_pays(const Gemm& g) {
+  if (g.b_packed != nullptr || g.Cf != nullptr || g.split_any_shape) return true;
+  const double m = double(g.m), n = double(g.n);
+  const double operand_elems = (m + n) * double(g.k) * double(g.batch);
+  return m * n >= 64.0 * (m + n) || operand_elems < double(int64_t(1) << 24);
+}
 
 // A packed split-bf16 operand kept for reuse under a caller tag.
 struct PackedOperand {

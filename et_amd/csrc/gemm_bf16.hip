@@ -466,11 +466,11 @@ GemmLaunch gemm_bf16x3(hipStream_t stream, const Gemm& g, GemmWorkspace* ws) {
   p.ks_per_split = KS;
   // One workgroup per CU.  Split K when the output has few tiles: the smallest
   // split count whose rounds of 256 workgroups are at least 90 % full (at most
-  // 16 splits of at least 16 k steps each).
+  // 256 splits of at least 16 k steps each; partials of at most 256 tiles).
   const int64_t tiles = int64_t(p.tiles_m) * p.tiles_n * g.batch;
   int splits = 1;
   if (tiles < 2048 && KS >= 32) {
-    const int64_t cus = 256, max_s = std::min<int64_t>(16, KS / 16);
+    const int64_t cus = 256, max_s = std::min<int64_t>(256, KS / 16);
     int best = 1;
     double best_eff = 0.0;
     for (int64_t s = 1; s <= max_s; ++s) {

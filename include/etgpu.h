@@ -36,8 +36,13 @@ typedef struct etg_handle etg_handle;
 enum etg_precision {
   ETG_FP64 = 0,   /* FP64 matrix cores throughout */
   ETG_FP32 = 1,   /* FP32 matrix-core products, FP32 accumulation */
-  ETG_BF16X3 = 2  /* operands split into bf16 hi + lo, three bf16 matrix-core
-                     products per term (hi*hi + hi*lo + lo*hi), FP32 accumulation */
+  ETG_BF16X3 = 2, /* operands split into bf16 hi + lo, three bf16 matrix-core
+                     products per term (hi*hi + hi*lo + lo*hi), FP32 accumulation;
+                     a product with few outputs per operand element (m n below
+                     64 (m + n): bound by HBM even in FP64) and operands of 2^24
+                     elements or more runs in FP64 */
+  ETG_BF16X3_ANY_SHAPE = 3 /* etg_test_gemm and etg_bench_gemm only: the split-
+                     bf16 kernel for every shape */
 };
 
 /* Creates the device state for a CCSD wavefunction with n_o occupied and
@@ -401,8 +406,6 @@ int etg_test_gemm(char ta, char tb, int64_t m, int64_t n, int64_t k, double alph
                   const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
                   double* C, int64_t ldc, int precision);
 
-/* Seconds per product for `reps` back-to-back m x n x k products on
- * device-resident operands. */
 /* Diagnostic: lowest n_want eigenpairs of a k x k real matrix (host, column-
  * major) with the device reduced-space solver: wr, wi (k, ascending real
  * part), X (k x n_want, unit vectors).  Fails on complex wanted roots. */
@@ -421,6 +424,8 @@ int etg_test_reduced_eigen(int64_t k, const double* G, int64_t n_want, double* w
 int etg_reduced_eigen_noalloc(int64_t k, const double* G_device, int64_t ldg, int64_t n_want,
                               double* wr, double* wi, double* X_device, int64_t ldx);
 
+/* Seconds per product for `reps` back-to-back m x n x k products on
+ * device-resident operands. */
 int etg_bench_gemm(char ta, char tb, int64_t m, int64_t n, int64_t k, int precision, int reps,
                    double* seconds);
 

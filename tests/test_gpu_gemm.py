@@ -84,7 +84,7 @@ def test_bf16x3_gemm(ta, tb, m, n, k):
     C = rng.normal(size=(m, n))
     opA = A.T if ta == 'T' else A
     opB = B.T if tb == 'T' else B
-    out = ccsd.device_gemm(ta, tb, 0.7, A, B, -0.3, C, precision=ccsd.BF16X3)
+    out = ccsd.device_gemm(ta, tb, 0.7, A, B, -0.3, C, precision=ccsd.BF16X3_ANY_SHAPE)
     ref = 0.7 * opA @ opB - 0.3 * C
     bound = 0.7 * (4 * 2.0 ** -16 + k * 2.0 ** -24) * (np.abs(opA) @ np.abs(opB)) + 1e-15 * np.abs(C)
     assert np.all(np.abs(out - ref) <= bound)
@@ -99,10 +99,34 @@ def test_bf16x3_gemm_with_more_than_65535_row_tiles(ta, tb, m, n, k):
     B = rng.normal(size=(k, n))
     C = np.zeros((m, n))
     opA = A.T if ta == 'T' else A
-    out = ccsd.device_gemm(ta, tb, 1.0, A, B, 0.0, C, precision=ccsd.BF16X3)
+    out = ccsd.device_gemm(ta, tb, 1.0, A, B, 0.0, C, precision=ccsd.BF16X3_ANY_SHAPE)
     ref = opA @ B
     bound = (4 * 2.0 ** -16 + k * 2.0 ** -24) * (np.abs(opA) @ np.abs(B)) + 1e-300
     assert np.all(np.abs(out - ref) <= bound)
+
+
+@pytest.mark.parametrize('m,n,k,split', [(8, 8, 1 << 20, False), (40, 3, 400000, False),
+                                         (48, 48, 4000, True), (8, 8, (1 << 20) - 1, True),
+                                         (129, 257, 31, True), (300, 200, 500, True), (128, 128, 8000, True)])
+def test_split_bf16_products_bound_by_hbm_run_in_fp64(m, n, k, split):
+    """Products asked for in split-bf16 with m n < 64 (m + n) and operands of 2^24
+    elements or more run on the FP64 kernel (bit-identical to the FP64 product); the
+    others on the split-bf16 kernel (different from FP64 in the last bits, within the
+    split-bf16 bound)."""
+    assert (m * n >= 64 * (m + n) or (m + n) * k < (1 << 24)) == split
+    rng = np.random.default_rng(m + 7 * n + k)
+    A = rng.normal(size=(m, k))
+    B = rng.normal(size=(k, n))
+    C = rng.normal(size=(m, n))
+    ccsd.device_gemm('N', 'N', 0.9, A, B, 0.4, C, precision=ccsd.FP64)   # settles the measured tile choice
+    f64 = ccsd.device_gemm('N', 'N', 0.9, A, B, 0.4, C, precision=ccsd.FP64)
+    bf = ccsd.device_gemm('N', 'N', 0.9, A, B, 0.4, C, precision=ccsd.BF16X3)
+    if split:
+        assert not np.array_equal(bf, f64)
+        bound = 0.9 * (4 * 2.0 ** -16 + k * 2.0 ** -24) * (np.abs(A) @ np.abs(B)) + 1e-15 * np.abs(C)
+        assert np.all(np.abs(bf - (0.9 * A @ B + 0.4 * C)) <= bound)
+    else:
+        assert np.array_equal(bf, f64)
 
 
 @pytest.mark.parametrize('ta,tb,m,n,k', [('N', 'N', 48, 48, 4000), ('T', 'N', 300, 200, 500),
