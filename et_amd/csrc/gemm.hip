@@ -508,7 +508,7 @@ Choice tuned_choice(hipStream_t stream, const Gemm& g, GemmWorkspace* ws, Choice
 }
 
 template <typename T>
-GemmLaunch launch_typed(hipStream_t stream, const Gemm& g, GemmWorkspace* ws) {
+GemmLaunch launch_rows(hipStream_t stream, const Gemm& g, GemmWorkspace* ws) {
   // Tile size and split-K count from a cost model: a launch runs in rounds of
   // `slots` concurrent tiles (two 128 x 128 or four 64 x 64 workgroups per CU
   // on 256 CUs), each round at the tile's sustained matrix-core efficiency;
@@ -518,6 +518,26 @@ GemmLaunch launch_typed(hipStream_t stream, const Gemm& g, GemmWorkspace* ws) {
   Choice ch = choose_tiling(g, peak, ws != nullptr);
   ch = tuned_choice<T>(stream, g, ws, ch, peak);
   return run_choice<T>(stream, g, ws, ch);
+}
+
+// A product whose last 128-row tile would hold at most 64 rows (the ladder's
+// o(o+1)/2 = 1176 = 9 x 128 + 24 pairs, or n_J = 1600 = 12 x 128 + 64) would
+// spend a whole tile row on them: beyond 1e10 FLOP those rows run as a product
+// of their own, on the tiling chosen for that shape.
+template <typename T>
+GemmLaunch launch_typed(hipStream_t stream, const Gemm& g, GemmWorkspace* ws) {
+  const int64_t rest_rows = g.m % 128;
+  if (g.m > 128 && rest_rows != 0 && rest_rows <= 64 && gemm_flops(g) >= 1e10) {
+    Gemm top = g, rest = g;
+    top.m = g.m - rest_rows;
+    rest.m = rest_rows;
+    rest.A = g.ta ? g.A + top.m * g.lda : g.A + top.m;
+    rest.C = g.C + top.m;
+    const GemmLaunch info = launch_rows<T>(stream, top, ws);
+    launch_rows<T>(stream, rest, ws);
+    return info;
+  }
+  return launch_rows<T>(stream, g, ws);
 }
 
 // ---- tall products with leading dimensions beyond the matrix-core kernel ----

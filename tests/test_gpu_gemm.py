@@ -41,6 +41,25 @@ def test_fp32_gemm(m, n, k):
     assert np.abs(out - ref).max() < tol
 
 
+@pytest.mark.parametrize('ta', ['N', 'T'])
+@pytest.mark.parametrize('m', [152, 192, 130])
+def test_products_with_a_short_last_row_tile(ta, m):
+    """m % 128 <= 64 with 1e10 FLOP or more: the last rows run as a product of their own
+    (offset A rows or columns and C rows); FP64 and FP32 against numpy, beta != 0."""
+    n, k = 2560, 16384
+    assert 2 * m * n * k >= 1e10
+    rng = np.random.default_rng(m)
+    A = rng.normal(size=(k, m) if ta == 'T' else (m, k))
+    B = rng.normal(size=(k, n))
+    C = rng.normal(size=(m, n))
+    opA = A.T if ta == 'T' else A
+    ref = 0.6 * opA @ B + 0.25 * C
+    out = ccsd.device_gemm(ta, 'N', 0.6, A, B, 0.25, C)
+    assert np.abs(out - ref).max() < 1e-14 * np.sqrt(k) * 8
+    out32 = ccsd.device_gemm(ta, 'N', 0.6, A, B, 0.25, C, precision=ccsd.FP32)
+    assert np.abs(out32 - ref).max() < 4.0 * k * 2.0 ** -24 * np.abs(A).max() * np.abs(B).max()
+
+
 def test_zero_k_scales_c():
     C = np.arange(12.0).reshape(3, 4)
     out = ccsd.device_gemm('N', 'N', 1.0, np.zeros((3, 0)), np.zeros((0, 4)), 2.0, C)
