@@ -180,6 +180,7 @@ struct Params {
   double* W;                         // split-K partials (m x n per split and batch entry)
   int64_t ks_per_split;
   int tiles_m, tiles_n;
+  int vec_rows;                      // C / Cf rows written four at a time (alignment checked on the host)
 };
 
 __device__ inline void map_tile(int bid, int tiles_m, int tiles_n, int& tm, int& tn) {
@@ -329,6 +330,23 @@ __global__ __launch_bounds__(kProductThreads, 1) void gemm_bf16x3_kernel(Params 
   }
   if (p.Cf) {
     float* C = p.Cf + bz * p.strideC;
+    if (p.vec_rows) {
+      // a lane's four accumulators of a block are four consecutive rows of one column
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int64_t row = r0 + i * 16 + 4 * lrow;
+          const int64_t col = c0 + j * 16 + lcol;
+          if (row < p.m && col < p.n) {
+            f4 v;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) v[r] = float(p.alpha * double(acc[i][j][r]));
+            *reinterpret_cast<f4*>(C + row + col * p.ldc) = v;
+          }
+        }
+      return;
+    }
 #pragma unroll
     for (int i = 0; i < 8; ++i)
 #pragma unroll
@@ -342,6 +360,28 @@ __global__ __launch_bounds__(kProductThreads, 1) void gemm_bf16x3_kernel(Params 
     return;
   }
   double* C = p.C + bz * p.strideC;
+  if (p.vec_rows) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int64_t row = r0 + i * 16 + 4 * lrow;
+        const int64_t col = c0 + j * 16 + lcol;
+        if (row < p.m && col < p.n) {
+          double2* c = reinterpret_cast<double2*>(C + row + col * p.ldc);
+          double2 v0 = {p.alpha * double(acc[i][j][0]), p.alpha * double(acc[i][j][1])};
+          double2 v1 = {p.alpha * double(acc[i][j][2]), p.alpha * double(acc[i][j][3])};
+          if (p.beta != 0.0) {
+            const double2 o0 = c[0], o1 = c[1];
+            v0.x += p.beta * o0.x; v0.y += p.beta * o0.y;
+            v1.x += p.beta * o1.x; v1.y += p.beta * o1.y;
+          }
+          c[0] = v0;
+          c[1] = v1;
+        }
+      }
+    return;
+  }
 #pragma unroll
   for (int i = 0; i < 8; ++i)
 #pragma unroll
@@ -488,6 +528,13 @@ GemmLaunch gemm_bf16x3(hipStream_t stream, const Gemm& g, GemmWorkspace* ws) {
     if (ws->buf.resize(size_t(splits) * size_t(g.m) * size_t(g.n) * size_t(g.batch) * sizeof(double)))
       ++ws->gen;
     p.W = ws->buf.as<double>();
+  }
+  // rows four at a time: m, ldc and the batch stride multiples of 4 elements and
+  // the output 16 B (FP32) or 32 B (FP64) aligned
+  {
+    const uintptr_t base = g.Cf ? reinterpret_cast<uintptr_t>(g.Cf) : reinterpret_cast<uintptr_t>(g.C);
+    p.vec_rows = splits == 1 && g.m % 4 == 0 && g.ldc % 4 == 0 && (g.batch == 1 || g.strideC % 4 == 0) &&
+                 base % (g.Cf ? 16 : 32) == 0;
   }
   dim3 grid(unsigned(tiles / g.batch), unsigned(splits), unsigned(g.batch));
   gemm_bf16x3_kernel<<<grid, kProductThreads, 0, stream>>>(p);
