@@ -180,7 +180,7 @@ struct Params {
   double* W;                         // split-K partials (m x n per split and batch entry)
   int64_t ks_per_split;
   int tiles_m, tiles_n;
-  int vec_rows;                      // C / Cf rows written four at a time (alignment checked on the host)
+  int vec_rows;                      // W / C / Cf rows written four at a time (alignment checked on the host)
 };
 
 __device__ inline void map_tile(int bid, int tiles_m, int tiles_n, int& tm, int& tn) {
@@ -316,6 +316,21 @@ __global__ __launch_bounds__(kProductThreads, 1) void gemm_bf16x3_kernel(Params 
   const int64_t r0 = int64_t(tile_m) * kTile + wm * 128, c0 = int64_t(tile_n) * kTile + wn * 64;
   if (p.W) {
     double* W = p.W + (split + bz * gridDim.y) * p.m * p.n;
+    if (p.vec_rows) {
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int64_t row = r0 + i * 16 + 4 * lrow;
+          const int64_t col = c0 + j * 16 + lcol;
+          if (row < p.m && col < p.n) {
+            double2* w = reinterpret_cast<double2*>(W + row + col * p.m);
+            w[0] = double2{double(acc[i][j][0]), double(acc[i][j][1])};
+            w[1] = double2{double(acc[i][j][2]), double(acc[i][j][3])};
+          }
+        }
+      return;
+    }
 #pragma unroll
     for (int i = 0; i < 8; ++i)
 #pragma unroll
@@ -530,10 +545,13 @@ GemmLaunch gemm_bf16x3(hipStream_t stream, const Gemm& g, GemmWorkspace* ws) {
     p.W = ws->buf.as<double>();
   }
   // rows four at a time: m, ldc and the batch stride multiples of 4 elements and
-  // the output 16 B (FP32) or 32 B (FP64) aligned
-  {
+  // the output 16 B (FP32) or 32 B (FP64) aligned; split-K partials (ld m, in the
+  // 256 B aligned workspace) need m a multiple of 4
+  if (splits > 1) {
+    p.vec_rows = g.m % 4 == 0 && reinterpret_cast<uintptr_t>(p.W) % 32 == 0;
+  } else {
     const uintptr_t base = g.Cf ? reinterpret_cast<uintptr_t>(g.Cf) : reinterpret_cast<uintptr_t>(g.C);
-    p.vec_rows = splits == 1 && g.m % 4 == 0 && g.ldc % 4 == 0 && (g.batch == 1 || g.strideC % 4 == 0) &&
+    p.vec_rows = g.m % 4 == 0 && g.ldc % 4 == 0 && (g.batch == 1 || g.strideC % 4 == 0) &&
                  base % (g.Cf ? 16 : 32) == 0;
   }
   dim3 grid(unsigned(tiles / g.batch), unsigned(splits), unsigned(g.batch));
