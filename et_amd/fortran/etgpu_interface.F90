@@ -17,6 +17,7 @@ module etgpu_interface
    integer(c_int), parameter :: etg_fp64 = 0
    integer(c_int), parameter :: etg_fp32 = 1
    integer(c_int), parameter :: etg_bf16x3 = 2
+   integer(c_int), parameter :: etg_bf16x3_any_shape = 3   ! etg_test_gemm / etg_bench_gemm only
 !
    interface
 !
